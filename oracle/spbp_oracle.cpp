@@ -1,0 +1,756 @@
+// spbp_oracle.cpp -- CPU oracle for SPIDER's Fourier-gridding backprojection
+// (operations `BP 3F` and `BP 32F`).
+//
+// TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.  The
+// product path (spider_amd, libspbp) never links, loads or calls it.
+//
+// Two instantiations of every step:
+//   * "f32": a faithful restatement of the reference in single precision, in
+//     the reference's loop order and rounding (fp32 positions, NINT table
+//     index, fp32 accumulation).  Used to compare with the reference's own
+//     routines compiled from source (oracle/_ref) and as the CPU baseline.
+//   * "f64": the same discretised algorithm evaluated in double precision
+//     (exact pi, FP64 rotation, FP64 KB table, FP64 accumulation and FFTs).
+//     This is the "FP64 oracle" that GPU results are measured against.
+//
+// Citations are to files of the reference tree (spider-em/SPIDER, src/).
+// Parity pins (see tests/test_oracle_*.py):
+//   cang / kb table / asta mean / ONELINE / BESI1 / RIBSL  -> compared with the
+//       reference's own Fortran routines compiled by oracle/Makefile
+//   FFTs -> numpy.fft (DFT definition)
+//   SYMPLANE0 -> Hermitian invariant + brute-force pairing on tiny grids
+//   NRMW2 / WINDKB2A -> closed forms (I_{3/2}) + special cases
+//   end to end -> proc/testbp3f.tst known-answer range (12 x 75^2 MO T images)
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+constexpr int LN2 = 2;        // half width of the 5-tap KB kernel (bp32f.f:193-194)
+constexpr int LTAB = 4999;    // KB table length - 1 (bp32f.f:81, oneline.f:44)
+
+// ---------------------------------------------------------------- Bessel
+// Modified Bessel I1 by its power series; converges to double precision for
+// the arguments used here (0 <= x <= 6.5).  Replaces Cody's CALCI1/BESI1
+// (besi1.f:3-251), which evaluates the same function in single precision.
+double bessel_i1(double x) {
+    double h = 0.5 * x, term = h, sum = h, h2 = h * h;
+    for (int k = 1; k < 200; ++k) {
+        term *= h2 / (double(k) * double(k + 1));
+        sum += term;
+        if (term < 1e-18 * sum) break;
+    }
+    return sum;
+}
+// I_{3/2}(x) = sqrt(2/(pi x)) (cosh x - sinh x / x): closed form of the
+// half-integer order used by WINDKB2A via RIBSL (besi1.f:793-800).  Small x by
+// series to avoid cancellation.
+double bessel_i15(double x) {
+    if (x < 0.5) {
+        // I_v(x) = sum (x/2)^{2k+v} / (k! Gamma(k+v+1)), v=3/2
+        double h = 0.5 * x, sum = 0.0, term = std::pow(h, 1.5) / std::tgamma(2.5);
+        for (int k = 0; k < 60; ++k) {
+            sum += term;
+            term *= h * h / (double(k + 1) * (k + 2.5));
+        }
+        return sum;
+    }
+    return std::sqrt(2.0 / (M_PI * x)) * (std::cosh(x) - std::sinh(x) / x);
+}
+// J_{3/2}(x) = sqrt(2/(pi x)) (sin x / x - cos x) (RJBSL, besi1.f:276).  The
+// window never takes this branch inside the reconstruction sphere; kept so the
+// restatement covers every branch of bp32f.f:984-1000.
+double bessel_j15(double x) {
+    return std::sqrt(2.0 / (M_PI * x)) * (std::sin(x) / x - std::cos(x));
+}
+
+template <typename R> struct Num;
+template <> struct Num<float> {
+    static long nint(float v) { return lroundf(v); }
+    static float fabs_(float v) { return fabsf(v); }
+    static float sqrt_(float v) { return sqrtf(v); }
+    static float pow_(float a, float b) { return powf(a, b); }
+    static float i1(float x) { return (float)bessel_i1((double)x); }
+    static float i15(float x) { return (float)bessel_i15((double)x); }
+    static float j15(float x) { return (float)bessel_j15((double)x); }
+    static float twopi() { return 2.0f * (float)M_PI; }          // REAL parameter (bp32f.f:950-951)
+};
+template <> struct Num<double> {
+    static long nint(double v) { return lround(v); }
+    static double fabs_(double v) { return fabs(v); }
+    static double sqrt_(double v) { return sqrt(v); }
+    static double pow_(double a, double b) { return pow(a, b); }
+    static double i1(double x) { return bessel_i1(x); }
+    static double i15(double x) { return bessel_i15(x); }
+    static double j15(double x) { return bessel_j15(x); }
+    static double twopi() { return 2.0 * M_PI; }
+};
+
+// ---------------------------------------------------------------- KB table
+// bp32f.f:193-214 (== FILLBESSIL, fillbessi0.f:88-117):
+//   V = (LN-1)/2/N2, ALPHA = 6.5, AAAA = 0.9 V, B0 = sqrt(ALPHA) I1(ALPHA),
+//   FLTB = LTAB/(LN2+1), S = I/FLTB/N2,
+//   TABI(I) = S <= AAAA ? sqrt(ALPHA X) I1(ALPHA X) / B0 : 0, X = sqrt(1-(S/AAAA)^2)
+template <typename R>
+void kb_table(int n2, R* tab, R* fltb_out) {
+    typedef Num<R> M;
+    const int LN = 5;
+    R V = R(LN - 1) / R(2.0) / R(n2);
+    R alpha = R(6.5);
+    R aaaa = R(0.9) * V;
+    R b0 = M::sqrt_(alpha) * M::i1(alpha);
+    R fltb = R(LTAB) / R(LN2 + 1);
+    for (int i = 0; i <= LTAB; ++i) {
+        R s = R(i) / fltb / R(n2);
+        if (s <= aaaa) {
+            R q = s / aaaa;
+            R x = M::sqrt_(R(1.0) - q * q);
+            tab[i] = M::sqrt_(alpha * x) * M::i1(alpha * x) / b0;
+        } else {
+            tab[i] = R(0);
+        }
+    }
+    if (fltb_out) *fltb_out = fltb;
+}
+
+// ---------------------------------------------------------------- CANG
+// cang.f:44-82.  The degree->radian factor is a DOUBLE PRECISION parameter
+// built from a default-real literal, i.e. float32(pi)/180 evaluated in double
+// (cang.f:50-53).  cos/sin and products in double, stored as REAL.
+void cang_ref(float phi, float theta, float psi, float* dm) {
+    const double dgr = (double)(float)3.14159265358979323846 / 180.0;
+    double cphi = cos((double)phi * dgr), sphi = sin((double)phi * dgr);
+    double cthe = cos((double)theta * dgr), sthe = sin((double)theta * dgr);
+    double cpsi = cos((double)psi * dgr), spsi = sin((double)psi * dgr);
+    dm[0] = (float)(cphi * cthe * cpsi - sphi * spsi);
+    dm[1] = (float)(sphi * cthe * cpsi + cphi * spsi);
+    dm[2] = (float)(-sthe * cpsi);
+    dm[3] = (float)(-cphi * cthe * spsi - sphi * cpsi);
+    dm[4] = (float)(-sphi * cthe * spsi + cphi * cpsi);
+    dm[5] = (float)(sthe * spsi);
+    dm[6] = (float)(sthe * cphi);
+    dm[7] = (float)(sthe * sphi);
+    dm[8] = (float)(cthe);
+}
+// FP64 rotation; `dgr` = degree->radian factor: pi/180 (exact) or the
+// reference's float32(pi)/180 (cang.f:52-53).
+double deg2rad(int exact_pi) {
+    return exact_pi ? M_PI / 180.0 : (double)(float)3.14159265358979323846 / 180.0;
+}
+void cang_f64(double phi, double theta, double psi, double* dm, double dgr = M_PI / 180.0) {
+    double cphi = cos(phi * dgr), sphi = sin(phi * dgr);
+    double cthe = cos(theta * dgr), sthe = sin(theta * dgr);
+    double cpsi = cos(psi * dgr), spsi = sin(psi * dgr);
+    dm[0] = cphi * cthe * cpsi - sphi * spsi;
+    dm[1] = sphi * cthe * cpsi + cphi * spsi;
+    dm[2] = -sthe * cpsi;
+    dm[3] = -cphi * cthe * spsi - sphi * cpsi;
+    dm[4] = -sphi * cthe * spsi + cphi * cpsi;
+    dm[5] = sthe * spsi;
+    dm[6] = sthe * cphi;
+    dm[7] = sthe * sphi;
+    dm[8] = cthe;
+}
+
+// ---------------------------------------------------------------- FFT
+// Mixed-radix complex FFT (radix 4, 2, 3, 5, generic prime), double precision.
+// Defined by the DFT: y[k] = sum_n x[n] exp(sign * 2 pi i k n / N).
+struct FFTPlan {
+    int n = 0, sign = -1;
+    std::vector<int> radix;
+    std::vector<double> tw;   // exp(sign 2 pi i k / n), k = 0..n-1, interleaved
+};
+FFTPlan make_plan(int n, int sign) {
+    FFTPlan P;
+    P.n = n; P.sign = sign;
+    int m = n;
+    while (m % 4 == 0) { P.radix.push_back(4); m /= 4; }
+    while (m % 2 == 0) { P.radix.push_back(2); m /= 2; }
+    for (int f = 3; m > 1; f += 2) while (m % f == 0) { P.radix.push_back(f); m /= f; }
+    P.tw.resize(2 * (size_t)n);
+    for (int k = 0; k < n; ++k) {
+        double a = sign * 2.0 * M_PI * (double)k / (double)n;
+        P.tw[2 * k] = cos(a);
+        P.tw[2 * k + 1] = sin(a);
+    }
+    return P;
+}
+void fft_rec(const FFTPlan& P, int level, int n, const double* x, long xs, double* y, long twst) {
+    if (n == 1) { y[0] = x[0]; y[1] = x[1]; return; }
+    const int p = P.radix[level];
+    const int m = n / p;
+    for (int r = 0; r < p; ++r) fft_rec(P, level + 1, m, x + 2 * r * xs, xs * p, y + 2 * (long)r * m, twst * p);
+    const double* tw = P.tw.data();
+    const long N = P.n;
+    if (p == 2) {
+        for (int k = 0; k < m; ++k) {
+            double* a = y + 2 * k; double* b = y + 2 * (k + m);
+            long e = (long)k * twst;
+            double wr = tw[2 * e], wi = tw[2 * e + 1];
+            double br = b[0] * wr - b[1] * wi, bi = b[0] * wi + b[1] * wr;
+            double ar = a[0], ai = a[1];
+            a[0] = ar + br; a[1] = ai + bi; b[0] = ar - br; b[1] = ai - bi;
+        }
+        return;
+    }
+    if (p == 4) {
+        const double s = (double)P.sign;  // multiply by (sign * i)
+        for (int k = 0; k < m; ++k) {
+            double v[4][2];
+            for (int r = 0; r < 4; ++r) {
+                const double* src = y + 2 * ((long)r * m + k);
+                long e = ((long)r * k) * twst % N;
+                double wr = tw[2 * e], wi = tw[2 * e + 1];
+                v[r][0] = src[0] * wr - src[1] * wi;
+                v[r][1] = src[0] * wi + src[1] * wr;
+            }
+            double t0r = v[0][0] + v[2][0], t0i = v[0][1] + v[2][1];
+            double t1r = v[0][0] - v[2][0], t1i = v[0][1] - v[2][1];
+            double t2r = v[1][0] + v[3][0], t2i = v[1][1] + v[3][1];
+            double t3r = v[1][0] - v[3][0], t3i = v[1][1] - v[3][1];
+            // (sign*i) * t3
+            double ur = -s * t3i, ui = s * t3r;
+            double* o0 = y + 2 * (long)k; double* o1 = y + 2 * ((long)k + m);
+            double* o2 = y + 2 * ((long)k + 2 * m); double* o3 = y + 2 * ((long)k + 3 * m);
+            o0[0] = t0r + t2r; o0[1] = t0i + t2i;
+            o2[0] = t0r - t2r; o2[1] = t0i - t2i;
+            o1[0] = t1r + ur;  o1[1] = t1i + ui;
+            o3[0] = t1r - ur;  o3[1] = t1i - ui;
+        }
+        return;
+    }
+    std::vector<double> t(2 * p), o(2 * p);
+    for (int k = 0; k < m; ++k) {
+        for (int r = 0; r < p; ++r) {
+            const double* src = y + 2 * ((long)r * m + k);
+            long e = ((long)r * k % n) * twst;
+            double wr = tw[2 * e], wi = tw[2 * e + 1];
+            t[2 * r] = src[0] * wr - src[1] * wi;
+            t[2 * r + 1] = src[0] * wi + src[1] * wr;
+        }
+        for (int q = 0; q < p; ++q) {
+            double sr = 0, si = 0;
+            for (int r = 0; r < p; ++r) {
+                long e = ((long)r * q % p) * m * twst;
+                double wr = tw[2 * e], wi = tw[2 * e + 1];
+                sr += t[2 * r] * wr - t[2 * r + 1] * wi;
+                si += t[2 * r] * wi + t[2 * r + 1] * wr;
+            }
+            o[2 * q] = sr; o[2 * q + 1] = si;
+        }
+        for (int q = 0; q < p; ++q) {
+            y[2 * ((long)k + (long)q * m)] = o[2 * q];
+            y[2 * ((long)k + (long)q * m) + 1] = o[2 * q + 1];
+        }
+    }
+}
+// in-place transform of n complex values with element stride `stride` (complex units)
+void fft_strided(const FFTPlan& P, double* data, long stride, double* work_in, double* work_out) {
+    const int n = P.n;
+    for (int i = 0; i < n; ++i) { work_in[2 * i] = data[2 * i * stride]; work_in[2 * i + 1] = data[2 * i * stride + 1]; }
+    fft_rec(P, 0, n, work_in, 1, work_out, 1);
+    for (int i = 0; i < n; ++i) { data[2 * i * stride] = work_out[2 * i]; data[2 * i * stride + 1] = work_out[2 * i + 1]; }
+}
+
+// 2D real-to-complex, FFTW forward sign (e^{-i}), unnormalised.
+// in: N x N real (x fastest); out: (N/2+1) x N complex interleaved (kx fastest).
+void fft2_r2c(const double* in, int N, double* out) {
+    const int H = N / 2 + 1;
+    FFTPlan P = make_plan(N, -1);
+    std::vector<double> full(2 * (size_t)N * N);
+    #pragma omp parallel
+    {
+        std::vector<double> a(2 * N), b(2 * N);
+        #pragma omp for schedule(static)
+        for (int y = 0; y < N; ++y) {
+            for (int x = 0; x < N; ++x) { a[2 * x] = in[(size_t)y * N + x]; a[2 * x + 1] = 0.0; }
+            fft_rec(P, 0, N, a.data(), 1, b.data(), 1);
+            for (int x = 0; x < H; ++x) { full[2 * ((size_t)y * H + x)] = b[2 * x]; full[2 * ((size_t)y * H + x) + 1] = b[2 * x + 1]; }
+        }
+        #pragma omp for schedule(static)
+        for (int x = 0; x < H; ++x) fft_strided(P, full.data() + 2 * x, H, a.data(), b.data());
+    }
+    std::memcpy(out, full.data(), sizeof(double) * 2 * (size_t)H * N);
+}
+
+// 3D complex-to-real, FFTW backward sign (e^{+i}), unnormalised, with FFTW's
+// c2r semantics: result = Re sum_k c_kx X(k) e^{+2 pi i k.n/N}, c_0 = c_{N/2} = 1,
+// else 2 (equivalent to Hermitian-symmetrising the kx = 0 and kx = N/2 planes).
+// in: (N/2+1) x N x N complex (kx fastest); out: N x N x N real (x fastest).
+void fft3_c2r(const double* in, int N, double* out) {
+    const int H = N / 2 + 1;
+    FFTPlan P = make_plan(N, +1);
+    std::vector<double> a(in, in + 2 * (size_t)H * N * N);
+    #pragma omp parallel
+    {
+        std::vector<double> w1(2 * N), w2(2 * N);
+        // along z: for each (ky, kx)
+        #pragma omp for schedule(static)
+        for (long yx = 0; yx < (long)N * H; ++yx) fft_strided(P, a.data() + 2 * yx, (long)N * H, w1.data(), w2.data());
+        // along y: for each (z, kx)
+        #pragma omp for schedule(static)
+        for (long zx = 0; zx < (long)N * H; ++zx) {
+            long z = zx / H, x = zx % H;
+            fft_strided(P, a.data() + 2 * (z * N * H + x), H, w1.data(), w2.data());
+        }
+        // along x: Hermitian row, real output
+        #pragma omp for schedule(static)
+        for (long zy = 0; zy < (long)N * N; ++zy) {
+            const double* row = a.data() + 2 * zy * H;
+            for (int k = 0; k < H; ++k) { w1[2 * k] = row[2 * k]; w1[2 * k + 1] = row[2 * k + 1]; }
+            w1[1] = 0.0;
+            if (N % 2 == 0) w1[2 * (N / 2) + 1] = 0.0;
+            for (int k = 1; k < (N + 1) / 2; ++k) { w1[2 * (N - k)] = row[2 * k]; w1[2 * (N - k) + 1] = -row[2 * k + 1]; }
+            fft_rec(P, 0, N, w1.data(), 1, w2.data(), 1);
+            for (int x = 0; x < N; ++x) out[zy * N + x] = w2[2 * x];
+        }
+    }
+}
+
+// ---------------------------------------------------------------- PADD2
+// bp3f.f:1307-1343 with ASTA_D (asta.f:61-92): subtract the FP64 mean of the
+// pixels outside the circle of radius L/2 about (L/2+1, L/2+1) (1-based), then
+// place the image at offset IP = (N-L)/2 + mod(L,2) of an N x N zero buffer.
+double asta_mean(const float* img, int L) {
+    float R = (float)(L / 2);
+    float r2 = R * R;
+    int nc = L / 2 + 1;
+    double aba = 0.0, dklp = 0.0;
+    for (int J = 1; J <= L; ++J) {
+        float t = (float)(J - nc);
+        float xx = t * t;
+        for (int I = 1; I <= L; ++I) {
+            t = (float)(I - nc);
+            if (xx + t * t > r2) { aba += (double)img[(size_t)(J - 1) * L + (I - 1)]; dklp += 1.0; }
+        }
+    }
+    return aba / (double)(float)dklp;
+}
+template <typename R>
+void pad_image(const float* img, int L, int N, R* out) {
+    double qs = asta_mean(img, L);
+    std::fill(out, out + (size_t)N * N, R(0));
+    int ip = (N - L) / 2 + (L % 2);
+    for (int j = 0; j < L; ++j)
+        for (int i = 0; i < L; ++i)
+            out[(size_t)(ip + j) * N + ip + i] = (R)((double)img[(size_t)j * L + i] - qs);
+}
+
+// ---------------------------------------------------------------- spectrum
+// PADD2 -> FMRS_2 forward (FFTW r2c, then imag = -imag: fftw3.f:409-419) ->
+// checkerboard BI(I,J) *= (-1)^(I+J+1), J 1-based (bp32f.f:859-864).
+// Output BI(0:NX, 0:N-1) complex, kx fastest.
+template <typename R>
+void spectrum(const float* img, int nx, R* bi) {
+    const int N = 2 * nx, H = nx + 1;
+    std::vector<R> pad((size_t)N * N);
+    pad_image<R>(img, nx, N, pad.data());
+    std::vector<double> pd(pad.begin(), pad.end()), sp(2 * (size_t)H * N);
+    fft2_r2c(pd.data(), N, sp.data());
+    for (int j = 0; j < N; ++j)
+        for (int i = 0; i < H; ++i) {
+            size_t o = (size_t)j * H + i;
+            double s = ((i + j) & 1) ? -1.0 : 1.0;
+            bi[2 * o] = (R)(sp[2 * o] * s);
+            bi[2 * o + 1] = (R)(-sp[2 * o + 1] * s);
+        }
+}
+
+// ---------------------------------------------------------------- ONELINE
+// oneline.f:38-158, argument names as in the caller: N = padded size (N2 = 2 NX),
+// nx = image size.  X(0:NX,N,N) complex, W(0:NX,N,N) real, BI(0:NX,N) complex,
+// dm = first six entries of the CANG matrix.
+template <typename R>
+void oneline(int J, int N, int nx, R* X, R* W, const R* BI, const R* dm, const R* tab, R fltb) {
+    typedef Num<R> M;
+    const int H = nx + 1;
+    const int jp = (J >= 0) ? J : N + J;
+    for (int I = 0; I <= nx; ++I) {
+        if (!(((I * I + J * J) < (N * N / 4)) && !(I == 0 && J < 0))) continue;
+        R x = R(I) * dm[0] + R(J) * dm[3];
+        R y = R(I) * dm[1] + R(J) * dm[4];
+        R z = R(I) * dm[2] + R(J) * dm[5];
+        R vr = BI[2 * ((size_t)jp * H + I)], vi = BI[2 * ((size_t)jp * H + I) + 1];
+        if (x < R(0)) { x = -x; y = -y; z = -z; vi = -vi; }
+        const int ix = (int)(x + R(0.5) + R(N)) - N;
+        const int iy = (int)(y + R(0.5) + R(N)) - N;
+        const int iz = (int)(z + R(0.5) + R(N)) - N;
+        if (!(ix <= nx - LN2 - 1 && iy >= -nx + 2 + LN2 && iy <= nx - LN2 - 1 &&
+              iz >= -nx + 2 + LN2 && iz <= nx - LN2 - 1)) continue;
+        if (ix >= 0) {
+            const int lb = -std::min(ix, LN2);
+            for (int lz = -LN2; lz <= LN2; ++lz) {
+                const int izp = iz + lz;
+                const int iza = izp >= 0 ? izp : N + izp;
+                const R tz = tab[M::nint(M::fabs_(z - R(izp)) * fltb)];
+                if (tz == R(0)) continue;
+                for (int ly = -LN2; ly <= LN2; ++ly) {
+                    const int iyp = iy + ly;
+                    const int iya = iyp >= 0 ? iyp : N + iyp;
+                    const R ty = tab[M::nint(M::fabs_(y - R(iyp)) * fltb)] * tz;
+                    if (ty == R(0)) continue;
+                    for (int ixp = lb + ix; ixp <= LN2 + ix; ++ixp) {
+                        const R wg = tab[M::nint(M::fabs_(x - R(ixp)) * fltb)] * ty;
+                        if (wg != R(0)) {
+                            const size_t o = ((size_t)iza * N + iya) * H + ixp;
+                            X[2 * o] = X[2 * o] + vr * wg;
+                            X[2 * o + 1] = X[2 * o + 1] + vi * wg;
+                            W[o] = W[o] + wg;
+                        }
+                    }
+                }
+            }
+        }
+        // reflected points (oneline.f:118-152)
+        if (ix < LN2) {
+            for (int lz = -LN2; lz <= LN2; ++lz) {
+                const int izp = iz + lz;
+                int izt = -izp;
+                if (izp > 0) izt = N + izt;
+                const R tz = tab[M::nint(M::fabs_(z - R(izp)) * fltb)];
+                if (tz == R(0)) continue;
+                for (int ly = -LN2; ly <= LN2; ++ly) {
+                    const int iyp = iy + ly;
+                    int iyt = -iyp;
+                    if (iyp > 0) iyt = iyt + N;
+                    const R ty = tab[M::nint(M::fabs_(y - R(iyp)) * fltb)] * tz;
+                    if (ty == R(0)) continue;
+                    for (int ixp = ix - LN2; ixp <= -1; ++ixp) {
+                        const R wg = tab[M::nint(M::fabs_(x - R(ixp)) * fltb)] * ty;
+                        if (wg != R(0)) {
+                            const size_t o = ((size_t)izt * N + iyt) * H + (-ixp);
+                            X[2 * o] = X[2 * o] + vr * wg;
+                            X[2 * o + 1] = X[2 * o + 1] + (-vi) * wg;
+                            W[o] = W[o] + wg;
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+template <typename R>
+void insert_projection(int nx, R* X, R* W, const R* bi, const R* dm, const R* tab, R fltb) {
+    const int N = 2 * nx;
+    for (int J = -nx + 1; J <= nx; ++J) oneline<R>(J, N, nx, X, W, bi, dm, tab, fltb);
+}
+
+// ---------------------------------------------------------------- SYMPLANE0
+// bp32f.f:1051-1088 (argument N2 there is NX).  0-based: mirror of index a is N-a.
+template <typename R>
+void symplane0(int nx, R* X, R* W) {
+    const int N = 2 * nx, H = nx + 1;
+    auto at = [&](int iy, int iz) { return ((size_t)iz * N + iy) * H; };
+    auto addc = [&](size_t a, size_t b) {   // X(a) += conj(X(b)); W(a) += W(b); X(b) = conj(X(a)); W(b) = W(a)
+        X[2 * a] = X[2 * a] + X[2 * b];
+        X[2 * a + 1] = X[2 * a + 1] + (-X[2 * b + 1]);
+        W[a] = W[a] + W[b];
+        X[2 * b] = X[2 * a];
+        X[2 * b + 1] = -X[2 * a + 1];
+        W[b] = W[a];
+    };
+    for (int iz = 1; iz <= nx - 1; ++iz)
+        for (int iy = 1; iy <= nx - 1; ++iy) {
+            addc(at(iy, iz), at(N - iy, N - iz));
+            addc(at(N - iy, iz), at(iy, N - iz));
+        }
+    for (int iy = 1; iy <= nx - 1; ++iy) addc(at(iy, 0), at(N - iy, 0));
+    for (int iz = 1; iz <= nx - 1; ++iz) addc(at(0, iz), at(0, N - iz));
+}
+
+// ---------------------------------------------------------------- NRMW2 (weighting part)
+// bp3f.f:1355-1366: X = W > 0.1 ? X (-1)^(i+j+k) / W : 0 (0-based i,j,k here).
+template <typename R>
+void normalize(int nx, R* X, const R* W) {
+    const int N = 2 * nx, H = nx + 1;
+    for (int k = 0; k < N; ++k)
+        for (int j = 0; j < N; ++j)
+            for (int i = 0; i < H; ++i) {
+                size_t o = ((size_t)k * N + j) * H + i;
+                if (W[o] > (R)0.1f) {
+                    R s = ((i + j + k) & 1) ? R(-1) : R(1);
+                    X[2 * o] = (X[2 * o] * s) / W[o];
+                    X[2 * o + 1] = (X[2 * o + 1] * s) / W[o];
+                } else {
+                    X[2 * o] = R(0);
+                    X[2 * o + 1] = R(0);
+                }
+            }
+}
+
+// NRMW2 inverse part: FMRS_3 INV=-1 -> conj, c2r, x 1/N^3 (fftw3.f:430-451).
+template <typename R>
+void inverse3(int nx, const R* X, R* real) {
+    const int N = 2 * nx, H = nx + 1;
+    const size_t nh = (size_t)H * N * N, nr = (size_t)N * N * N;
+    std::vector<double> a(2 * nh), o(nr);
+    for (size_t q = 0; q < nh; ++q) { a[2 * q] = (double)X[2 * q]; a[2 * q + 1] = -(double)X[2 * q + 1]; }
+    fft3_c2r(a.data(), N, o.data());
+    if (sizeof(R) == sizeof(float)) {
+        const float pix = 1.0f / (float)((long)N * N * N);
+        for (size_t q = 0; q < nr; ++q) real[q] = (R)((float)o[q] * pix);
+    } else {
+        const double pix = 1.0 / ((double)N * N * N);
+        for (size_t q = 0; q < nr; ++q) real[q] = (R)(o[q] * pix);
+    }
+}
+
+// ---------------------------------------------------------------- WINDKB2A
+// bp32f.f:945-1045: crop the central L^3 at IP = (N-L)/2 + mod(L,2); inside the
+// sphere LR <= (L/2)^2 divide by the KB window transform
+//   WKB = ALPHA^1.5 / I_1.5(ALPHA) * I_1.5(ART) / ART^1.5, ART = sqrt(|QRT LR - ALPHA^2|),
+//   QRT = (2 pi AAAA)^2;
+// subtract the mean over the shell (L/2-1)^2 <= LR <= (L/2)^2; zero outside.
+template <typename R>
+void window(int nx, const R* real, R* out) {
+    typedef Num<R> M;
+    const int L = nx, N = 2 * nx;
+    const int ip = (N - L) / 2 + (L % 2);
+    for (int k = 0; k < L; ++k)
+        for (int j = 0; j < L; ++j)
+            for (int i = 0; i < L; ++i)
+                out[((size_t)k * L + j) * L + i] = real[((size_t)(ip + k) * N + (ip + j)) * N + (ip + i)];
+    const int L2 = (L / 2) * (L / 2);
+    const int L2P = (L / 2 - 1) * (L / 2 - 1);
+    const int c = L / 2;   // 0-based centre (1-based L/2+1)
+    const R alpha = R(6.5);
+    const R xnu = R(3) / R(2.0);
+    const R V = R(4) / R(2.0) / R(N);
+    const R aaaa = R(0.9) * V;
+    R ri = M::i15(alpha);
+    const R wkb0 = M::pow_(alpha, xnu) / ri;
+    const R tq = M::twopi() * aaaa;
+    const R qrt = tq * tq;
+    R tnr = R(0);
+    long m = 0;
+    for (int k = 0; k < L; ++k) {
+        const int kk = (k - c) * (k - c);
+        for (int j = 0; j < L; ++j) {
+            const int jj = (j - c) * (j - c) + kk;
+            for (int i = 0; i < L; ++i) {
+                const int lr = jj + (i - c) * (i - c);
+                if (lr > L2) continue;
+                const R sigma = qrt * R(lr) - alpha * alpha;
+                R wkb;
+                if (M::fabs_(sigma) < R(1.0e-7)) {
+                    wkb = R(1.0);
+                } else if (sigma > R(0)) {
+                    const R art = M::sqrt_(sigma);
+                    wkb = wkb0 * M::j15(art) / M::pow_(art, xnu);
+                } else {
+                    const R art = M::sqrt_(M::fabs_(sigma));
+                    wkb = wkb0 * M::i15(art) / M::pow_(art, xnu);
+                }
+                R& v = out[((size_t)k * L + j) * L + i];
+                v = v / M::fabs_(wkb);
+                if (lr >= L2P && lr <= L2) { tnr = tnr + v; ++m; }
+            }
+        }
+    }
+    tnr = tnr / R(m);
+    for (int k = 0; k < L; ++k) {
+        const int kk = (k - c) * (k - c);
+        for (int j = 0; j < L; ++j) {
+            const int jj = (j - c) * (j - c) + kk;
+            for (int i = 0; i < L; ++i) {
+                const int lr = jj + (i - c) * (i - c);
+                R& v = out[((size_t)k * L + j) * L + i];
+                v = (lr <= L2) ? v - tnr : R(0);
+            }
+        }
+    }
+}
+
+template <typename R>
+void finalize_volume(int nx, R* X, const R* W, R* out) {
+    const int N = 2 * nx;
+    std::vector<R> real((size_t)N * N * N);
+    normalize<R>(nx, X, W);
+    inverse3<R>(nx, X, real.data());
+    window<R>(nx, real.data(), out);
+}
+
+template <typename R> struct Rot;
+template <> struct Rot<float> {
+    static void make(const double* ang, float* dm, int) { cang_ref((float)ang[2], (float)ang[1], (float)ang[0], dm); }
+};
+template <> struct Rot<double> {
+    static void make(const double* ang, double* dm, int exact_pi) {
+        cang_f64(ang[2], ang[1], ang[0], dm, deg2rad(exact_pi));
+    }
+};
+
+// Whole operation (BP 3F when half == NULL, BP 32F otherwise).
+// angles: nimg x 3 doc-file registers (PSI, THETA, PHI) in degrees; the
+// rotation is CANG(PHI, THETA, PSI) (bp3f.f:1290-1291).
+// exact_pi (f64 only): 0 -> the reference's degree conversion (default), 1 -> exact pi.
+template <typename R>
+int bp_run(int nx, int nimg, const float* imgs, const double* angles, const int8_t* half,
+           R* out_full, R* out_h1, R* out_h2, R* xw_out, int exact_pi) {
+    if (nx < 2) return 101;
+    const int N = 2 * nx, H = nx + 1;
+    const size_t nh = (size_t)H * N * N;
+    const int nvol = half ? 2 : 1;
+    std::vector<R> X(2 * nh * nvol, R(0)), W(nh * nvol, R(0));
+    std::vector<R> tab(LTAB + 1);
+    R fltb;
+    kb_table<R>(N, tab.data(), &fltb);
+    std::vector<R> bi(2 * (size_t)H * N);
+    R dm[9];
+    for (int p = 0; p < nimg; ++p) {
+        Rot<R>::make(angles + 3 * p, dm, exact_pi);
+        spectrum<R>(imgs + (size_t)p * nx * nx, nx, bi.data());
+        const int v = half ? (half[p] ? 1 : 0) : 0;
+        insert_projection<R>(nx, X.data() + 2 * nh * v, W.data() + nh * v, bi.data(), dm, tab.data(), fltb);
+    }
+    for (int v = 0; v < nvol; ++v) symplane0<R>(nx, X.data() + 2 * nh * v, W.data() + nh * v);
+    if (nvol == 2) {
+        // full = NRMW2(XE + XO, WE + WO) (bp32f.f:345-351); halves from XE, XO
+        std::vector<R> Xs(2 * nh), Ws(nh);
+        for (size_t q = 0; q < 2 * nh; ++q) Xs[q] = X[q] + X[2 * nh + q];
+        for (size_t q = 0; q < nh; ++q) Ws[q] = W[q] + W[nh + q];
+        if (xw_out) { std::copy(Xs.begin(), Xs.end(), xw_out); std::copy(Ws.begin(), Ws.end(), xw_out + 2 * nh); }
+        if (out_full) finalize_volume<R>(nx, Xs.data(), Ws.data(), out_full);
+        if (out_h1) finalize_volume<R>(nx, X.data(), W.data(), out_h1);
+        if (out_h2) finalize_volume<R>(nx, X.data() + 2 * nh, W.data() + nh, out_h2);
+    } else {
+        if (xw_out) { std::copy(X.begin(), X.end(), xw_out); std::copy(W.begin(), W.end(), xw_out + 2 * nh); }
+        if (out_full) finalize_volume<R>(nx, X.data(), W.data(), out_full);
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+double or_bessel_i1(double x) { return bessel_i1(x); }
+double or_bessel_i15(double x) { return bessel_i15(x); }
+double or_bessel_j15(double x) { return bessel_j15(x); }
+
+void or_cang_ref(float phi, float theta, float psi, float* dm) { cang_ref(phi, theta, psi, dm); }
+void or_cang_f64(double phi, double theta, double psi, double* dm, int exact_pi) {
+    cang_f64(phi, theta, psi, dm, deg2rad(exact_pi));
+}
+
+void or_kb_table_f32(int n2, float* tab, float* fltb) { kb_table<float>(n2, tab, fltb); }
+void or_kb_table_f64(int n2, double* tab, double* fltb) { kb_table<double>(n2, tab, fltb); }
+
+double or_asta_mean(const float* img, int L) { return asta_mean(img, L); }
+void or_pad_f32(const float* img, int L, int N, float* out) { pad_image<float>(img, L, N, out); }
+void or_pad_f64(const float* img, int L, int N, double* out) { pad_image<double>(img, L, N, out); }
+
+void or_fft1(double* data, int n, int sign) {
+    FFTPlan P = make_plan(n, sign);
+    std::vector<double> a(2 * n), b(2 * n);
+    fft_strided(P, data, 1, a.data(), b.data());
+}
+void or_fft2_r2c(const double* in, int N, double* out) { fft2_r2c(in, N, out); }
+void or_fft3_c2r(const double* in, int N, double* out) { fft3_c2r(in, N, out); }
+
+void or_spectrum_f32(const float* img, int nx, float* bi) { spectrum<float>(img, nx, bi); }
+void or_spectrum_f64(const float* img, int nx, double* bi) { spectrum<double>(img, nx, bi); }
+
+void or_oneline_f32(int J, int nx, float* X, float* W, const float* bi, const float* dm, const float* tab, float fltb) {
+    oneline<float>(J, 2 * nx, nx, X, W, bi, dm, tab, fltb);
+}
+void or_insert_f32(int nx, float* X, float* W, const float* bi, const float* dm, const float* tab, float fltb) {
+    insert_projection<float>(nx, X, W, bi, dm, tab, fltb);
+}
+void or_insert_f64(int nx, double* X, double* W, const double* bi, const double* dm, const double* tab, double fltb) {
+    insert_projection<double>(nx, X, W, bi, dm, tab, fltb);
+}
+void or_symplane0_f32(int nx, float* X, float* W) { symplane0<float>(nx, X, W); }
+void or_symplane0_f64(int nx, double* X, double* W) { symplane0<double>(nx, X, W); }
+void or_normalize_f32(int nx, float* X, const float* W) { normalize<float>(nx, X, W); }
+void or_normalize_f64(int nx, double* X, const double* W) { normalize<double>(nx, X, W); }
+void or_inverse3_f32(int nx, const float* X, float* real) { inverse3<float>(nx, X, real); }
+void or_inverse3_f64(int nx, const double* X, double* real) { inverse3<double>(nx, X, real); }
+void or_window_f32(int nx, const float* real, float* out) { window<float>(nx, real, out); }
+void or_window_f64(int nx, const double* real, double* out) { window<double>(nx, real, out); }
+void or_finalize_f32(int nx, float* X, const float* W, float* out) { finalize_volume<float>(nx, X, W, out); }
+void or_finalize_f64(int nx, double* X, const double* W, double* out) { finalize_volume<double>(nx, X, W, out); }
+
+int or_bp_f32(int nx, int nimg, const float* imgs, const double* angles, const int8_t* half,
+              float* out_full, float* out_h1, float* out_h2, float* xw_out) {
+    return bp_run<float>(nx, nimg, imgs, angles, half, out_full, out_h1, out_h2, xw_out, 0);
+}
+int or_bp_f64(int nx, int nimg, const float* imgs, const double* angles, const int8_t* half,
+              double* out_full, double* out_h1, double* out_h2, double* xw_out, int exact_pi) {
+    return bp_run<double>(nx, nimg, imgs, angles, half, out_full, out_h1, out_h2, xw_out, exact_pi);
+}
+
+// Number of distinct half-grid voxels that receive a nonzero KB weight from one
+// projection (Friedel-reflected targets counted once): the unit U(NX) of the
+// algorithmic byte count.  Uses the f64 sample geometry of ONELINE.
+long or_count_distinct(int nx, const double* dm) {
+    const int N = 2 * nx, H = nx + 1;
+    const size_t nh = (size_t)H * N * N;
+    std::vector<double> X(2 * nh, 0.0), W(nh, 0.0), bi(2 * (size_t)H * N, 1.0);
+    std::vector<double> tab(LTAB + 1);
+    double fltb;
+    kb_table<double>(N, tab.data(), &fltb);
+    insert_projection<double>(nx, X.data(), W.data(), bi.data(), dm, tab.data(), fltb);
+    long c = 0;
+    for (size_t q = 0; q < nh; ++q) c += (W[q] != 0.0);
+    return c;
+}
+
+// Number of (voxel, weight) updates one projection performs (direct + reflected taps).
+long or_count_updates(int nx, const double* dm) {
+    const int N = 2 * nx, H = nx + 1;
+    const size_t nh = (size_t)H * N * N;
+    std::vector<double> X(2 * nh, 0.0), W(nh, 0.0), bi(2 * (size_t)H * N, 0.0);
+    std::vector<double> tab(LTAB + 1);
+    double fltb;
+    kb_table<double>(N, tab.data(), &fltb);
+    // count via unit weights: a table of ones where the KB table is nonzero
+    for (auto& t : tab) t = (t != 0.0) ? 1.0 : 0.0;
+    insert_projection<double>(nx, X.data(), W.data(), bi.data(), dm, tab.data(), fltb);
+    double s = 0.0;
+    for (size_t q = 0; q < nh; ++q) s += W[q];
+    return (long)llround(s);
+}
+
+// CPU baseline: the reference's production scheme (SP_MP build): per image,
+// spectrum, then six phases of an OpenMP loop over rows J = -NX+JT, ..., step 6
+// (bp32f.f:873-888, bp3f.f:468-474), FP32, unsynchronised like the reference.
+// dms: nimg x 9 rotation matrices (fp32).  Returns 0.
+int or_insert_omp_f32(int nx, int nimg, const float* imgs, const float* dms, float* X, float* W, int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+    const int N = 2 * nx, H = nx + 1;
+    std::vector<float> tab(LTAB + 1);
+    float fltb;
+    kb_table<float>(N, tab.data(), &fltb);
+    std::vector<float> bi(2 * (size_t)H * N);
+    for (int p = 0; p < nimg; ++p) {
+        spectrum<float>(imgs + (size_t)p * nx * nx, nx, bi.data());
+        const float* dm = dms + 9 * (size_t)p;
+        for (int jt = 1; jt <= 6; ++jt) {
+            #pragma omp parallel for schedule(static)
+            for (int J = -nx + jt; J <= nx; J += 6)
+                oneline<float>(J, N, nx, X, W, bi.data(), dm, tab.data(), fltb);
+        }
+    }
+    return 0;
+}
+
+int or_max_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+}  // extern "C"

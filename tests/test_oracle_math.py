@@ -1,0 +1,253 @@
+"""Pins the oracle where the reference cannot run here: the FFT boundary (by
+the DFT definition), SYMPLANE0 / NRMW2 / WINDKB2A (closed forms, invariants,
+special cases), and the whole operation (the reference's testbp3f KAT)."""
+import os
+
+import numpy as np
+import pytest
+import scipy.special as sp
+
+import oracle
+from spider_amd import synth
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.complex128)
+    b = np.asarray(b, np.complex128)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+# ------------------------------------------------------------------ FFT
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 7, 8, 12, 30, 75, 128, 150, 256])
+def test_fft1_vs_numpy(n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    np.testing.assert_allclose(oracle.fft1(x, -1), np.fft.fft(x), rtol=0, atol=1e-10 * max(1, n))
+    np.testing.assert_allclose(oracle.fft1(x, +1), np.fft.ifft(x) * n, rtol=0, atol=1e-10 * max(1, n))
+
+
+@pytest.mark.parametrize("n", [24, 30, 150])
+def test_fft2_r2c_vs_numpy(n):
+    x = np.random.default_rng(n).standard_normal((n, n))
+    np.testing.assert_allclose(oracle.fft2_r2c(x), np.fft.rfft2(x), atol=1e-9)
+
+
+@pytest.mark.parametrize("n", [12, 30])
+def test_fft3_c2r_vs_numpy(n):
+    """Unnormalised backward c2r with FFTW semantics, incl. non-Hermitian input
+    on the kx = 0 plane (numpy's irfftn has the same semantics)."""
+    rng = np.random.default_rng(n)
+    X = rng.standard_normal((n, n, n // 2 + 1)) + 1j * rng.standard_normal((n, n, n // 2 + 1))
+    ref = np.fft.irfftn(X, s=(n, n, n)) * n ** 3
+    np.testing.assert_allclose(oracle.fft3_c2r(X), ref, atol=1e-8 * n ** 3)
+
+
+# ------------------------------------------------------------------ Bessel / KB
+def test_bessel_vs_scipy():
+    for x in np.linspace(0.0, 7.0, 71):
+        assert oracle.bessel_i1(x) == pytest.approx(sp.iv(1, x), rel=1e-14, abs=1e-300)
+        if x > 0:
+            assert oracle.bessel_i15(x) == pytest.approx(sp.iv(1.5, x), rel=1e-12)
+            assert oracle.lib().or_bessel_j15(x) == pytest.approx(sp.jv(1.5, x), rel=1e-9, abs=1e-13)
+
+
+def test_kb_table_closed_form():
+    """TABI(k) = sqrt(a x) I1(a x) / (sqrt(a) I1(a)), x = sqrt(1 - (k/(FLTB*1.8))^2)."""
+    tab, fltb = oracle.kb_table(256, "f64")
+    k = np.arange(5000)
+    d = k / fltb
+    x = np.sqrt(np.clip(1 - (d / 1.8) ** 2, 0, None))
+    want = np.where(d <= 1.8, np.sqrt(6.5 * x) * sp.iv(1, 6.5 * x) / (np.sqrt(6.5) * sp.iv(1, 6.5)), 0.0)
+    np.testing.assert_allclose(tab, want, rtol=1e-12, atol=1e-15)
+    assert tab[0] == pytest.approx(1.0) and fltb == pytest.approx(4999 / 3)
+
+
+def test_cang_rotation_properties():
+    rng = np.random.default_rng(3)
+    for a in rng.uniform(0, 360, (50, 3)):
+        m = oracle.cang(*a, prec="f64").reshape(3, 3)
+        np.testing.assert_allclose(m @ m.T, np.eye(3), atol=1e-14)
+        assert np.linalg.det(m) == pytest.approx(1.0)
+        m32 = oracle.cang(*a, prec="f32").reshape(3, 3)
+        assert np.abs(m32 - m).max() < 3e-7 * (1 + np.abs(a).max() / 100)
+    # theta = 0: rotation about z by (phi + psi); rows are (cos, sin, 0), (-sin, cos, 0)
+    m = oracle.cang(20.0, 0.0, 30.0, prec="f64").reshape(3, 3)
+    c, s = np.cos(np.radians(50)), np.sin(np.radians(50))
+    np.testing.assert_allclose(m, [[c, s, 0], [-s, c, 0], [0, 0, 1]], atol=1e-15)
+    # the synth module's FP64 rotation is the same matrix
+    ang = rng.uniform(0, 360, (10, 3))
+    np.testing.assert_allclose(synth.rotations(ang), oracle.rot_from_doc(ang, "f64"), atol=1e-15)
+
+
+# ------------------------------------------------------------------ pad / spectrum
+def test_pad_constant_image_is_zero_and_offsets():
+    for L in (16, 17):
+        img = np.full((L, L), 3.25, np.float32)
+        assert np.all(oracle.pad(img, 2 * L, "f64") == 0)
+        img = np.zeros((L, L), np.float32)
+        img[L // 2, L // 2] = 1.0      # the centre pixel goes to the padded centre N/2
+        p = oracle.pad(img, 2 * L, "f64")
+        (j, i), = np.argwhere(p == p.max())
+        assert (i, j) == (L, L)
+
+
+@pytest.mark.parametrize("nx", [16, 15])
+def test_spectrum_vs_numpy(nx):
+    img = np.random.default_rng(nx).standard_normal((nx, nx)).astype(np.float32)
+    n = 2 * nx
+    padded = np.zeros((n, n))
+    ip = (n - nx) // 2 + nx % 2
+    padded[ip:ip + nx, ip:ip + nx] = img.astype(np.float64) - oracle.asta_mean(img)
+    F = np.conj(np.fft.rfft2(padded))               # SPIDER sign: conj of FFTW forward
+    j, i = np.meshgrid(np.arange(n), np.arange(nx + 1), indexing="ij")
+    F *= np.where((i + j) % 2 == 1, -1.0, 1.0)       # (-1)^(I+J+1) with J 1-based
+    np.testing.assert_allclose(oracle.spectrum(img, "f64"), F, atol=1e-10)
+    assert rel_l2(oracle.spectrum(img, "f32"), F) < 1e-6
+
+
+# ------------------------------------------------------------------ SYMPLANE0
+def test_symplane0_invariants():
+    nx = 10
+    n = 2 * nx
+    rng = np.random.default_rng(5)
+    X0 = (rng.standard_normal((n, n, nx + 1)) + 1j * rng.standard_normal((n, n, nx + 1)))
+    W0 = rng.uniform(0, 1, (n, n, nx + 1))
+    X, W = X0.copy(), W0.copy()
+    oracle.symplane0(X, W, "f64")
+    # only the kx = 0 plane changes
+    np.testing.assert_array_equal(X[:, :, 1:], X0[:, :, 1:])
+    np.testing.assert_array_equal(W[:, :, 1:], W0[:, :, 1:])
+    mirror = lambda a: (-a) % n
+    for kz in range(n):
+        for ky in range(n):
+            a, b = (kz, ky), (mirror(kz), mirror(ky))
+            nyq = kz == nx or ky == nx
+            self_conj = a == b
+            if nyq or self_conj:      # Nyquist row/column and self-conjugate points untouched
+                assert X[kz, ky, 0] == X0[kz, ky, 0] and W[kz, ky, 0] == W0[kz, ky, 0]
+                continue
+            # Hermitian afterwards ...
+            assert X[b[0], b[1], 0] == np.conj(X[kz, ky, 0]) and W[b[0], b[1], 0] == W[kz, ky, 0]
+            # ... and the sum of the pair: X(a) = X0(a) + conj(X0(-a)) up to conj
+            s = X0[kz, ky, 0] + np.conj(X0[b[0], b[1], 0])
+            assert X[kz, ky, 0] == pytest.approx(s) or X[kz, ky, 0] == pytest.approx(np.conj(s))
+            assert W[kz, ky, 0] == pytest.approx(W0[kz, ky, 0] + W0[b[0], b[1], 0])
+
+
+# ------------------------------------------------------------------ NRMW2 + inverse
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_normalize_inverse_spike(prec):
+    """X = W everywhere -> X/W * (-1)^(i+j+k) = checkerboard -> inverse gives a
+    unit spike at the padded centre (N/2, N/2, N/2): pins sign, scale, origin."""
+    nx = 8
+    n = 2 * nx
+    ct = np.complex64 if prec == "f32" else np.complex128
+    W = np.full((n, n, nx + 1), 2.0, np.float32 if prec == "f32" else np.float64)
+    X = W.astype(ct).copy()
+    W[3, 5, 2] = 0.05           # below the 0.1 threshold -> zeroed
+    oracle.normalize(X, W, prec)
+    assert X[3, 5, 2] == 0 and X[0, 0, 0] == 1 and X[0, 0, 1] == -1 and X[1, 1, 1] == -1
+    X[3, 5, 2] = (-1) ** (2 + 5 + 3)
+    real = oracle.inverse3(X, prec)
+    want = np.zeros((n, n, n))
+    want[nx, nx, nx] = 1.0
+    np.testing.assert_allclose(real, want, atol=1e-6 if prec == "f32" else 1e-12)
+
+
+# ------------------------------------------------------------------ WINDKB2A
+@pytest.mark.parametrize("nx", [16, 15])
+def test_window_closed_form(nx):
+    n, L = 2 * nx, nx
+    real = np.full((n, n, n), 1.5)
+    out = oracle.window(real, "f64")
+    alpha, a = 6.5, 1.8 / n
+    c = L // 2
+    z, y, x = np.meshgrid(*(np.arange(L) - c,) * 3, indexing="ij")
+    lr = x * x + y * y + z * z
+    inside = lr <= c * c
+    beta = np.sqrt(alpha ** 2 - (2 * np.pi * a) ** 2 * lr)
+    wkb = (alpha ** 1.5 / sp.iv(1.5, alpha)) * sp.iv(1.5, beta) / beta ** 1.5
+    v = 1.5 / wkb
+    shell = inside & (lr >= (c - 1) ** 2)
+    want = np.where(inside, v - v[shell].mean(), 0.0)
+    np.testing.assert_allclose(out, want, atol=1e-12)
+    # crop offset: padded centre -> output centre
+    real = np.zeros((n, n, n))
+    real[nx, nx, nx] = 1.0
+    out = oracle.window(real, "f64")
+    assert np.unravel_index(np.argmax(out), out.shape) == (c, c, c)
+    o32 = oracle.window(np.full((n, n, n), 1.5, np.float32), "f32")
+    np.testing.assert_allclose(o32, want, atol=1e-5)   # fp32 shell-mean accumulation
+
+
+# ------------------------------------------------------------------ whole operation
+def kat_inputs():
+    doc = synth.read_angle_doc(os.path.join(GOLDEN, "sav_bp_ang_doc.dat"))
+    ang = np.array([doc[k] for k in range(1, 13)])     # keys 1-12: PSI, THETA, PHI
+    img = synth.mo_t_image(75)
+    return np.repeat(img[None], 12, axis=0), ang
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_testbp3f_known_answer(prec):
+    """proc/testbp3f.tst: BP 3F of 12 copies of `MO T` 75x75 with savangbprp
+    keys 1-12 -> range -0.12581E-01 ... 0.13152E-01 (5 significant digits)."""
+    imgs, ang = kat_inputs()
+    v = oracle.bp(imgs, ang, prec=prec)["full"]
+    assert float("%.4E" % v.min()) == pytest.approx(-0.012581, abs=5e-7)
+    assert float("%.4E" % v.max()) == pytest.approx(0.013152, abs=5e-7)
+    # BP 32F full volume expects the same range
+    v32 = oracle.bp(imgs, ang, half=np.arange(12) % 2, prec=prec)["full"]
+    assert float("%.4E" % v32.min()) == pytest.approx(-0.012581, abs=5e-7)
+    assert float("%.4E" % v32.max()) == pytest.approx(0.013152, abs=5e-7)
+
+
+def test_bp32f_full_equals_bp3f_and_linearity():
+    rng = np.random.default_rng(11)
+    nx, m = 16, 30
+    ang = synth.uniform_angles(m, rng)
+    # integer-valued images so that 2a - 3b is exact in fp32
+    a = rng.integers(-100, 100, (m, nx, nx)).astype(np.float32)
+    b = rng.integers(-100, 100, (m, nx, nx)).astype(np.float32)
+    half = rng.integers(0, 2, m)
+    r3 = oracle.bp(a, ang, prec="f64", want_xw=True)
+    r32 = oracle.bp(a, ang, half=half, prec="f64", want_xw=True)
+    np.testing.assert_allclose(r32["full"], r3["full"], atol=1e-12 * np.abs(r3["full"]).max())
+    # the accumulation is linear in the images, W depends only on the angles
+    rb = oracle.bp(b, ang, prec="f64", want_xw=True)
+    rab = oracle.bp((2 * a - 3 * b).astype(np.float32), ang, prec="f64", want_xw=True)
+    np.testing.assert_allclose(rab["X"], 2 * r3["X"] - 3 * rb["X"], atol=1e-9 * np.abs(r3["X"]).max())
+    np.testing.assert_array_equal(rb["W"], r3["W"])
+    # FP32-faithful vs FP64 oracle (same degree conversion): precision only.
+    # fp32 sample positions move NINT(|d| FLTB) to the neighbouring table entry
+    # for ~0.2% of lookups (each worth ~6e-4 of a weight): ~1e-5..1e-4 rel-L2.
+    r3f = oracle.bp(a, ang, prec="f32")
+    assert rel_l2(r3f["full"], r3["full"]) < 1e-4
+    # the reference's float32(pi) degree conversion is a visible (angle-
+    # proportional) rotation error against exact pi
+    r3x = oracle.bp(a, ang, prec="f64", exact_pi=True)
+    assert 1e-6 < rel_l2(r3x["full"], r3["full"]) < 1e-3
+
+
+def test_constant_images_give_zero_volume():
+    rng = np.random.default_rng(12)
+    imgs = np.full((5, 16, 16), 7.0, np.float32)
+    v = oracle.bp(imgs, synth.uniform_angles(5, rng), prec="f64")["full"]
+    assert np.abs(v).max() < 1e-12
+
+
+def test_handedness_blob_lands_at_its_position():
+    """An off-centre Gaussian blob, projected analytically along CANG's r3, is
+    reconstructed at its own position: pins synth geometry and handedness."""
+    rng = np.random.default_rng(13)
+    nx = 32
+    ph = synth.Phantom(nx, np.array([[6.0, -4.0, 9.0]]), np.array([1.6]), np.array([1.0]))
+    ang = synth.uniform_angles(400, rng)
+    imgs = synth.project(ph, synth.rotations(ang)).numpy()
+    v = oracle.bp(imgs, ang, prec="f64")["full"]
+    kz, ky, kx = np.unravel_index(np.argmax(v), v.shape)
+    assert (kx - nx // 2, ky - nx // 2, kz - nx // 2) == (6, -4, 9)
+    vol = ph.volume()
+    assert np.corrcoef(v.ravel(), vol.ravel())[0, 1] > 0.9
