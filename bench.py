@@ -1,0 +1,187 @@
+#!/usr/bin/env python3
+"""Throughput of the Fourier-gridding backprojection (SPIDER `BP 3F`) on one GPU.
+
+A step = one whole reconstruction job over HBM-resident synthetic projections:
+reset the accumulators, insert every projection (pad + batched 2D FFT +
+KB scatter), finalise the volume (normalise + 3D inverse FFT + window).
+Default workload: 20,000 projections of 128x128 fp32 into the 256^3 padded
+grid (the recipe: 60-blob phantom, uniform orientations, SNR 0.1).
+
+Prints one JSON line.  `roofline` is for the insertion kernel: algorithmic
+bytes per projection B = 4 NX^2 + 24 U(NX) (image read + read/write of the
+complex64 value and fp32 weight of each distinct voxel touched) over the
+kernel's HIP-event-timed launch durations.  `cpu_baseline` times the CPU
+oracle's restatement of the reference's OpenMP insertion loop on a bounded
+sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "projections/s into 256³ volume (plain vs batched/BF16); rel-L2 vs FP64 oracle"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def distinct_voxels(bpmod, nx, ang, n=16):
+    """U(NX): mean count of distinct half-grid voxels with nonzero weight from
+    one projection, over n orientations."""
+    import torch
+    counts = []
+    img = torch.ones((1, nx, nx), device="cuda")
+    with bpmod.Backprojector(nx) as b:
+        for i in range(n):
+            b.reset()
+            b.insert(img, ang[i:i + 1])
+            _, W = b.export_raw(0)
+            counts.append(int((W > 0).sum()))
+    return sum(counts) / len(counts)
+
+
+def cpu_baseline(imgs_host, ang, seconds):
+    import numpy as np
+    import oracle
+    nx = imgs_host.shape[-1]
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    dms = oracle.rot_from_doc(ang, "f32")
+
+    def run(m):
+        X, W = oracle.empty_grid(nx)
+        t = time.perf_counter()
+        oracle.insert_omp_f32(imgs_host[:m], dms[:m], X, W, threads)
+        return time.perf_counter() - t
+
+    m0 = min(8, imgs_host.shape[0])
+    t0 = run(m0)
+    m = int(max(m0, min(imgs_host.shape[0], seconds / max(t0 / m0, 1e-6))))
+    t = run(m)
+    return {"value": m / t, "unit": "projections/s", "cores": threads, "kind": "port",
+            "sample": f"{m} of the workload's projections ({nx}x{nx} fp32), insertion only "
+                      f"(pad + FFT + ONELINE, OpenMP rows stride 6 as bp32f.f:873-888), "
+                      f"{t:.1f} s on {threads} threads; oracle FFT is its own double-precision FFT"}
+
+
+def parity_subset(bpmod, imgs, ang, m):
+    import numpy as np
+    import oracle
+    sub = imgs[:m].contiguous()
+    vol = bpmod.bp3f(sub, ang[:m]).cpu().numpy().astype(np.float64)
+    host = sub.cpu().numpy()
+    o64 = oracle.bp(host, ang[:m], prec="f64")["full"]
+    o32 = oracle.bp(host, ang[:m], prec="f32")["full"].astype(np.float64)
+    r64 = float(np.linalg.norm(vol - o64) / np.linalg.norm(o64))
+    r32 = float(np.linalg.norm(vol - o32) / np.linalg.norm(o32))
+    return r64, r32
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", type=int, default=2, choices=[1, 2, 5])
+    ap.add_argument("--nproj", type=int, default=None, help="override the projection count")
+    ap.add_argument("--batch", type=int, default=0, help="projections per internal chunk (0 = auto)")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--parity-proj", type=int, default=100)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-parity", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(projection sharding + one sum-reduction of the half grid is the design, not built).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from spider_amd import _lib, synth
+    from spider_amd import bp as bpmod
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+    rec = synth.CONFIGS[args.config]
+    nx = rec["nx"]
+    nproj = args.nproj or rec["nproj"]
+    log(f"bench: generating config {args.config}: {nproj} projections {nx}x{nx}")
+    ang, ph, imgs = synth.make_dataset(args.config, nproj=nproj, device="cuda")
+    torch.cuda.synchronize()
+
+    b = bpmod.Backprojector(nx, timing=True, batch=args.batch)
+
+    def step():
+        b.reset()
+        b.insert(imgs, ang)
+        return b.finalize()["full"]
+
+    for w in range(args.warmup):
+        step()
+        torch.cuda.synchronize()
+        log(f"bench: warmup step {w + 1} done")
+    b.reset_stats()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        vol = step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    log(f"bench: {args.steps} timed steps done")
+    dt = t1 - t0
+    value = nproj * args.steps / dt
+    ins_ms, launches, units = b.kernel_stats(_lib.K_INSERT)
+    pad_ms, _, _ = b.kernel_stats(_lib.K_PAD)
+    fft_ms, _, _ = b.kernel_stats(_lib.K_FFT2)
+    fin_ms, nfin, _ = b.kernel_stats(_lib.K_FINALIZE)
+    b.close()
+
+    U = distinct_voxels(bpmod, nx, synth.uniform_angles(16, np.random.default_rng(5)))
+    b_alg = 4.0 * nx * nx + 24.0 * U                 # bytes per projection
+    avg_launch_s = ins_ms / 1e3 / launches
+    per_launch = units / launches
+    achieved = b_alg * per_launch / avg_launch_s / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+            "kernel": "insert_direct_kernel", "bytes_per_unit": round(b_alg), "unit_of_work": "projection",
+            "units_per_launch": round(per_launch, 1), "avg_launch_ms": round(avg_launch_s * 1e3, 4),
+            "U_distinct_voxels": round(U, 1)}
+
+    out = {"metric": METRIC, "value": round(value, 2), "unit": "projections/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic (seeded blob phantom, analytic projections, white noise)",
+           "config": {"workload": f"BP 3F plain, {nx}x{nx} fp32 images, {nproj} projections, "
+                                  f"{2 * nx}^3 padded Fourier grid (config {args.config})",
+                      "nx": nx, "nproj": nproj, "grid": 2 * nx, "images_dtype": "f32"},
+           "breakdown_ms_per_step": {"pad": round(pad_ms / args.steps, 3), "fft2": round(fft_ms / args.steps, 3),
+                                     "insert": round(ins_ms / args.steps, 3),
+                                     "finalize": round(fin_ms / max(nfin, 1), 3)},
+           "roofline": roof}
+    if not args.no_parity:
+        log("bench: parity subset vs oracle")
+        r64, r32 = parity_subset(bpmod, imgs, ang, min(args.parity_proj, nproj))
+        out["rel_l2_vs_fp64"] = r64
+        out["rel_l2_vs_ref_f32"] = r32
+        out["parity_subset_proj"] = min(args.parity_proj, nproj)
+    if not args.no_cpu:
+        log("bench: cpu baseline")
+        m = min(nproj, 4096)
+        out["cpu_baseline"] = cpu_baseline(imgs[:m].cpu().numpy(), ang[:m], args.cpu_seconds)
+        out["gpu_over_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

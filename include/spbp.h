@@ -1,0 +1,130 @@
+/* spbp.h -- C ABI of libspbp, the MI355X (gfx950) Fourier-gridding
+ * backprojection behind SPIDER's `BP 3F` / `BP 32F` operations.
+ *
+ * The reference has no library boundary: the operations BP3F / BP32F
+ * (bp3f.f:79, bp32f.f:57) prompt for files and call the accumulation engine
+ * BP3FQ / BP32FQ (bp3f.f:298, bp32f.f:433), whose inner kernel is ONELINE
+ * (oneline.f:38), and then the finalisation NRMW2 (bp3f.f:1347) + WINDKB2A
+ * (bp32f.f:945).  These entry points replace that engine + finalisation seam;
+ * the caller (a Fortran `bind(C)` driver or the Python extension) keeps the
+ * operation's signature: image series + angle doc in, volume(s) out.
+ *
+ * Conventions
+ *   - Images: nimg x NX x NX, x fastest (= Fortran PROJ(NX,NX)), fp32 or bf16.
+ *   - Angles: nimg x 3 doc-file registers (PSI, THETA, PHI) in degrees; the
+ *     rotation is CANG(PHI, THETA, PSI) (bp3f.f:1290-1291, cang.f:44).
+ *   - Half-grid Fourier volume: complex64 X(0:NX, N, N) and fp32 W(0:NX, N, N),
+ *     N = 2 NX, kx fastest, ky/kz in FFT wrap order (Fortran X(0:NX,N2,N2)).
+ *   - Output volumes: NX^3 fp32, x fastest.
+ *   - `stream` is a hipStream_t (NULL = the legacy default stream).
+ *   - Device pointers unless the name ends in _host.
+ *   - Return codes mirror the reference's IRTFLG / ERRT numbers where one
+ *     exists; 0 = success.  spbp_last_error() gives the message.
+ *   - One context per stream; calls on one context are not re-entrant.
+ */
+#ifndef SPBP_H
+#define SPBP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* return codes */
+#define SPBP_OK             0
+#define SPBP_ERR_ALLOC      46   /* ERRT 46: allocation failed (bp32f.f:180-185)            */
+#define SPBP_ERR_SIZE      101   /* ERRT 101: ONLY WORKS ON SQUARE IMAGES / bad box size     */
+#define SPBP_ERR_ANGLE     102   /* ERRT 102: MISSING ANGLE FOR IMAGE (bp3f.f:1283-1288)     */
+#define SPBP_ERR_FFT       201   /* FFT plan / execution failure (fmrs.f:246-258)            */
+#define SPBP_ERR_DEVICE    202   /* HIP runtime error                                        */
+#define SPBP_ERR_ARG       203   /* invalid argument (NULL context, bad half flag, ...)      */
+#define SPBP_ERR_NONFINITE 205   /* NaN / Inf in the images (only with SPBP_CHECK_FINITE)    */
+
+/* image storage types */
+#define SPBP_F32  0
+#define SPBP_BF16 1
+
+/* option flags */
+#define SPBP_TWO_HALVES   0x01  /* BP 32F: two half-set accumulators (bp32f.f:175-178)         */
+#define SPBP_EXACT_PI     0x02  /* exact pi in CANG instead of float32(pi)/180 (cang.f:52-53)  */
+#define SPBP_TWOSUM_W     0x04  /* compensated (two-sum) accumulation of the weights           */
+#define SPBP_CHECK_FINITE 0x08  /* reject images holding NaN / Inf                             */
+#define SPBP_TIMING       0x10  /* time each kernel launch with HIP events (spbp_kernel_stats) */
+
+/* kernels reported by spbp_kernel_stats */
+#define SPBP_K_INSERT   0   /* ONELINE scatter (units: projections)       */
+#define SPBP_K_PAD      1   /* PADD2 (units: projections)                 */
+#define SPBP_K_FFT2     2   /* batched 2D forward FFT (units: projections) */
+#define SPBP_K_FINALIZE 3   /* one whole finalisation (units: volumes)    */
+
+typedef struct {
+    int nx;      /* image edge NX; volume NX^3, padded Fourier grid N = 2 NX          */
+    int flags;   /* SPBP_* option flags                                               */
+    int device;  /* HIP device ordinal, -1 = current device                           */
+    int batch;   /* projections per internal FFT/insert chunk, 0 = automatic          */
+} spbp_opts;
+
+typedef struct spbp_ctx spbp_ctx;
+
+/* Create a context: allocates and zeroes the accumulators (BP32F's
+ * ALLOCATE XE,WE,XO,WO + zeroing, bp32f.f:175-178, 490-500). */
+int spbp_create(const spbp_opts* opts, spbp_ctx** out);
+int spbp_destroy(spbp_ctx* ctx);
+/* Message of the last failure on ctx (ctx == NULL: last failure of this thread). */
+const char* spbp_last_error(const spbp_ctx* ctx);
+/* Zero the accumulators. */
+int spbp_reset(spbp_ctx* ctx, void* stream);
+
+/* Insert projections: PADD2 + forward FFT + checkerboard + ONELINE over all
+ * rows, per image (the BP32FQ / BP3FQ projection loop, bp32f.f:802-913,
+ * bp3f.f:403-493).  half[i] = 0 -> first sample volume (XE), 1 -> second
+ * (XO), as RANDLIST does (bp32f.f:544-553); half may be NULL (all in the
+ * first).  angles_deg and half are host arrays. */
+int spbp_insert(spbp_ctx* ctx, const void* imgs, int dtype, int nimg,
+                const double* angles_deg, const int8_t* half, void* stream);
+/* Same with precomputed rotations: dm is nimg x 9 fp32 CANG matrices (host). */
+int spbp_insert_dm(spbp_ctx* ctx, const void* imgs, int dtype, int nimg,
+                   const float* dm, const int8_t* half, void* stream);
+/* Same with host-resident images (staged through pinned memory). */
+int spbp_insert_host(spbp_ctx* ctx, const void* imgs_host, int dtype, int nimg,
+                     const double* angles_deg, const int8_t* half, void* stream);
+
+/* Finalise: SYMPLANE0 (bp32f.f:1051) + NRMW2 (bp3f.f:1347: divide by W,
+ * inverse 3D FFT) + WINDKB2A (bp32f.f:945) into NX^3 fp32 volumes.
+ * out_full = both halves together (BP 3F volume / BP 32F overall volume);
+ * out_h1, out_h2 = the two sample volumes (SPBP_TWO_HALVES only).  Any may
+ * be NULL.  The accumulators are left unchanged (more inserts may follow). */
+int spbp_finalize(spbp_ctx* ctx, float* out_full, float* out_h1, float* out_h2, void* stream);
+int spbp_finalize_host(spbp_ctx* ctx, float* out_full, float* out_h1, float* out_h2, void* stream);
+
+/* Partial volumes = `BP 3F P` / `BP 3F M` (bp3f.f:191-248, 531-739):
+ * export writes the post-SYMPLANE0 half-grid (X interleaved complex64,
+ * 2 (NX+1) N N floats; W (NX+1) N N floats) of `half` (0, 1, or -1 = sum);
+ * import adds (accumulate != 0) or sets such a half-grid into `half`. */
+int spbp_export_partial(spbp_ctx* ctx, int half, float* X, float* W, void* stream);
+int spbp_import_partial(spbp_ctx* ctx, int half, const float* X, const float* W,
+                        int accumulate, void* stream);
+
+/* Raw accumulators of `half` (0 or 1), before SYMPLANE0, same layout.  No
+ * counterpart in the reference (its accumulators are internal arrays); for
+ * diagnostics and tests. */
+int spbp_export_raw(spbp_ctx* ctx, int half, float* X, float* W, void* stream);
+
+/* With SPBP_TIMING: total device milliseconds, launch count and work units
+ * of `kernel` (SPBP_K_*) since the last spbp_reset_stats; synchronises on the
+ * recorded events.  No counterpart in the reference. */
+int spbp_kernel_stats(spbp_ctx* ctx, int kernel, double* total_ms, long* launches, long* units);
+int spbp_reset_stats(spbp_ctx* ctx);
+
+/* Sizes (elements) of one half-grid: complex X count = (NX+1) N N. */
+size_t spbp_grid_elems(const spbp_ctx* ctx);
+
+/* Library version string. */
+const char* spbp_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPBP_H */

@@ -1,0 +1,98 @@
+"""ctypes binding of libspbp.so (include/spbp.h).
+
+There is no CPU fallback: if the library is missing or no GPU is present the
+calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libspbp.so")
+HEADER = os.path.join(os.path.dirname(_HERE), "include", "spbp.h")
+
+OK = 0
+ERR_ALLOC = 46
+ERR_SIZE = 101
+ERR_ANGLE = 102
+ERR_FFT = 201
+ERR_DEVICE = 202
+ERR_ARG = 203
+ERR_NONFINITE = 205
+
+F32 = 0
+BF16 = 1
+
+TWO_HALVES = 0x01
+EXACT_PI = 0x02
+TWOSUM_W = 0x04
+CHECK_FINITE = 0x08
+TIMING = 0x10
+
+K_INSERT = 0
+K_PAD = 1
+K_FFT2 = 2
+K_FINALIZE = 3
+
+
+class SpbpError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"spbp error {code}: {msg}")
+        self.code = code
+
+
+class Opts(C.Structure):
+    _fields_ = [("nx", C.c_int), ("flags", C.c_int), ("device", C.c_int), ("batch", C.c_int)]
+
+
+_LIB = None
+vp = C.c_void_p
+
+
+def lib():
+    """Load libspbp.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'`")
+        L = C.CDLL(LIB_PATH)
+        L.spbp_version.restype = C.c_char_p
+        L.spbp_last_error.restype = C.c_char_p
+        L.spbp_last_error.argtypes = [vp]
+        L.spbp_grid_elems.restype = C.c_size_t
+        L.spbp_grid_elems.argtypes = [vp]
+        L.spbp_create.argtypes = [C.POINTER(Opts), C.POINTER(vp)]
+        L.spbp_destroy.argtypes = [vp]
+        L.spbp_reset.argtypes = [vp, vp]
+        L.spbp_insert.argtypes = [vp, vp, C.c_int, C.c_int, vp, vp, vp]
+        L.spbp_insert_dm.argtypes = [vp, vp, C.c_int, C.c_int, vp, vp, vp]
+        L.spbp_insert_host.argtypes = [vp, vp, C.c_int, C.c_int, vp, vp, vp]
+        L.spbp_finalize.argtypes = [vp, vp, vp, vp, vp]
+        L.spbp_finalize_host.argtypes = [vp, vp, vp, vp, vp]
+        L.spbp_export_partial.argtypes = [vp, C.c_int, vp, vp, vp]
+        L.spbp_import_partial.argtypes = [vp, C.c_int, vp, vp, C.c_int, vp]
+        L.spbp_export_raw.argtypes = [vp, C.c_int, vp, vp, vp]
+        L.spbp_kernel_stats.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_long), C.POINTER(C.c_long)]
+        L.spbp_reset_stats.argtypes = [vp]
+        for name in ("spbp_create", "spbp_destroy", "spbp_reset", "spbp_insert", "spbp_insert_dm",
+                     "spbp_insert_host", "spbp_finalize", "spbp_finalize_host", "spbp_export_partial",
+                     "spbp_import_partial", "spbp_export_raw", "spbp_kernel_stats", "spbp_reset_stats"):
+            getattr(L, name).restype = C.c_int
+        _LIB = L
+    return _LIB
+
+
+def declared_symbols() -> list[str]:
+    """Every function the C header declares."""
+    with open(HEADER) as fh:
+        text = fh.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(spbp_[a-z0-9_]+)\s*\(", text)))
+
+
+def check(rc: int, ctx=None) -> None:
+    if rc != OK:
+        msg = lib().spbp_last_error(ctx)
+        raise SpbpError(rc, msg.decode() if msg else "")

@@ -1,0 +1,221 @@
+"""GPU parity: libspbp (through its C ABI) against the CPU oracle on the same
+seeded inputs.
+
+Tolerances (rel-L2 = ||a-b|| / ||b||):
+  * vs the FP32-faithful oracle (the reference's arithmetic, its float32(pi)
+    degree conversion, serial ONELINE order): <= 1e-5 on volumes; <= 1e-6 on
+    the weights.  Only the atomic summation order, the FFT and the pad mean
+    rounding differ.
+  * vs the FP64 oracle (same algorithm in double): <= 1e-4 (fp32 sample
+    positions move ~0.2% of the NINT table indices; see test_oracle_math).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from spider_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def rel(a, b):
+    a = np.asarray(a).astype(np.complex128)
+    b = np.asarray(b).astype(np.complex128)
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+@pytest.fixture(scope="module")
+def bpmod():
+    if not torch.cuda.is_available():
+        pytest.fail("GPU tests need a ROCm GPU")
+    from spider_amd import bp
+    return bp
+
+
+def gpu(x, dtype=torch.float32):
+    return torch.as_tensor(np.ascontiguousarray(x)).to("cuda", dtype)
+
+
+def oracle_raw(imgs, ang, half=None):
+    """Raw (pre-SYMPLANE0) f32 accumulators the reference's projection loop builds."""
+    nx = imgs.shape[-1]
+    tab, fltb = oracle.kb_table(2 * nx, "f32")
+    grids = [oracle.empty_grid(nx), oracle.empty_grid(nx)]
+    for p in range(imgs.shape[0]):
+        dm = oracle.cang(ang[p][2], ang[p][1], ang[p][0], prec="f32")
+        bi = oracle.spectrum(imgs[p], "f32")
+        X, W = grids[0 if half is None else int(half[p])]
+        oracle.insert(X, W, bi, dm, tab, fltb)
+    return grids
+
+
+@pytest.mark.parametrize("nx,m", [(16, 40), (15, 40), (32, 60)])
+def test_bp3f_parity_random(bpmod, nx, m):
+    rng = np.random.default_rng(100 + nx)
+    imgs = rng.standard_normal((m, nx, nx)).astype(np.float32)
+    ang = synth.uniform_angles(m, rng)
+    print(f"case bp3f nx={nx} m={m}", flush=True)
+    with bpmod.Backprojector(nx) as b:
+        b.insert(gpu(imgs), ang)
+        vol = b.finalize()["full"].cpu().numpy()
+        Xr, Wr = b.export_raw(0)
+        Xp, Wp = b.export_partial(-1)
+        Xr, Wr, Xp, Wp = (t.cpu().numpy() for t in (Xr, Wr, Xp, Wp))
+    (Xo, Wo), _ = oracle_raw(imgs, ang)
+    assert rel(Wr, Wo) < 1e-6 and rel(Xr, Xo) < 1e-6
+    # weights depend only on the geometry: identical support
+    assert np.array_equal(Wr > 0, Wo > 0)
+    o32 = oracle.bp(imgs, ang, prec="f32", want_xw=True)
+    o64 = oracle.bp(imgs, ang, prec="f64")
+    assert rel(Wp, o32["W"]) < 1e-6 and rel(Xp, o32["X"]) < 1e-6
+    assert rel(vol, o32["full"]) < 1e-5
+    assert rel(vol, o64["full"]) < 1e-4
+
+
+def test_testbp3f_known_answer_on_gpu(bpmod):
+    """proc/testbp3f.tst on the GPU: range -0.12581E-01 ... 0.13152E-01 (odd box, NX=75)."""
+    doc = synth.read_angle_doc(os.path.join(GOLDEN, "sav_bp_ang_doc.dat"))
+    ang = np.array([doc[k] for k in range(1, 13)])
+    imgs = np.repeat(synth.mo_t_image(75)[None], 12, axis=0)
+    v = bpmod.bp3f(gpu(imgs), ang).cpu().numpy()
+    assert float("%.4E" % v.min()) == pytest.approx(-0.012581, abs=5e-7)
+    assert float("%.4E" % v.max()) == pytest.approx(0.013152, abs=5e-7)
+    full, h1, h2 = bpmod.bp32f(gpu(imgs), ang, half=np.arange(12) % 2)
+    full = full.cpu().numpy()
+    assert float("%.4E" % full.min()) == pytest.approx(-0.012581, abs=5e-7)
+    assert float("%.4E" % full.max()) == pytest.approx(0.013152, abs=5e-7)
+    assert rel(full, v) < 1e-6
+
+
+def test_bp32f_halves_parity(bpmod):
+    rng = np.random.default_rng(7)
+    nx, m = 24, 50
+    imgs = rng.standard_normal((m, nx, nx)).astype(np.float32)
+    ang = synth.uniform_angles(m, rng)
+    half = bpmod.random_halves(m, seed=3)
+    full, h1, h2 = bpmod.bp32f(gpu(imgs), ang, half=half)
+    o = oracle.bp(imgs, ang, half=half, prec="f32")
+    assert rel(full.cpu().numpy(), o["full"]) < 1e-5
+    assert rel(h1.cpu().numpy(), o["h1"]) < 1e-5
+    assert rel(h2.cpu().numpy(), o["h2"]) < 1e-5
+
+
+def test_bf16_images(bpmod):
+    rng = np.random.default_rng(9)
+    nx, m = 32, 40
+    imgs = rng.standard_normal((m, nx, nx)).astype(np.float32)
+    ang = synth.uniform_angles(m, rng)
+    t = gpu(imgs, torch.bfloat16)
+    vol = bpmod.bp3f(t, ang).cpu().numpy()
+    rounded = t.float().cpu().numpy()          # the stored bf16 values, exactly
+    o = oracle.bp(rounded, ang, prec="f32")
+    assert rel(vol, o["full"]) < 1e-5
+
+
+def test_partial_export_import_merge(bpmod):
+    """BP 3F P on two subsets + BP 3F M == BP 3F on all (bp3f.f:191-248, 531-739)."""
+    rng = np.random.default_rng(21)
+    nx, m = 20, 60
+    imgs = gpu(rng.standard_normal((m, nx, nx)).astype(np.float32))
+    ang = synth.uniform_angles(m, rng)
+    with bpmod.Backprojector(nx) as a, bpmod.Backprojector(nx) as b, bpmod.Backprojector(nx) as c, \
+            bpmod.Backprojector(nx) as ref:
+        a.insert(imgs[:25], ang[:25])
+        b.insert(imgs[25:], ang[25:])
+        ref.insert(imgs, ang)
+        Xa, Wa = a.export_partial()
+        Xb, Wb = b.export_partial()
+        c.import_partial(Xa, Wa, accumulate=True)
+        c.import_partial(Xb, Wb, accumulate=True)
+        vm = c.finalize()["full"].cpu().numpy()
+        vr = ref.finalize()["full"].cpu().numpy()
+        Xc, Wc = c.export_partial()
+        Xr, Wr = ref.export_partial()
+    assert rel(vm, vr) < 1e-6
+    assert rel(Wc.cpu().numpy(), Wr.cpu().numpy()) < 1e-6
+    assert rel(Xc.cpu().numpy(), Xr.cpu().numpy()) < 1e-6
+
+
+def test_insert_after_finalize_and_reset(bpmod):
+    rng = np.random.default_rng(22)
+    nx, m = 16, 30
+    imgs = gpu(rng.standard_normal((m, nx, nx)).astype(np.float32))
+    ang = synth.uniform_angles(m, rng)
+    with bpmod.Backprojector(nx, batch=7) as b:      # several chunks incl. a tail chunk
+        b.insert(imgs[:10], ang[:10])
+        b.finalize()
+        b.insert(imgs[10:], ang[10:])
+        v1 = b.finalize()["full"].cpu().numpy()
+        b.reset()
+        b.insert(imgs, ang)
+        v2 = b.finalize()["full"].cpu().numpy()
+    o = oracle.bp(imgs.cpu().numpy(), ang, prec="f32")["full"]
+    assert rel(v1, o) < 1e-5 and rel(v2, o) < 1e-5
+
+
+def test_error_codes(bpmod):
+    from spider_amd import _lib
+    rng = np.random.default_rng(23)
+    with bpmod.Backprojector(16, check_finite=True) as b:
+        with pytest.raises(_lib.SpbpError) as e:
+            b.insert(gpu(np.zeros((2, 16, 15), np.float32)), np.zeros((2, 3)))
+        assert e.value.code == _lib.ERR_SIZE
+        with pytest.raises(_lib.SpbpError) as e:
+            b.insert(gpu(np.zeros((2, 16, 16), np.float32)), np.zeros((1, 3)))
+        assert e.value.code == _lib.ERR_ANGLE
+        with pytest.raises(_lib.SpbpError) as e:
+            b.insert(gpu(np.zeros((2, 16, 16), np.float32)), np.full((2, 3), np.nan))
+        assert e.value.code == _lib.ERR_ANGLE
+        with pytest.raises(_lib.SpbpError) as e:
+            b.insert(gpu(np.zeros((2, 16, 16), np.float32)), np.zeros((2, 3)), half=[0, 1])
+        assert e.value.code == _lib.ERR_ARG
+        bad = rng.standard_normal((3, 16, 16)).astype(np.float32)
+        bad[1, 4, 4] = np.inf
+        with pytest.raises(_lib.SpbpError) as e:
+            b.insert(gpu(bad), np.zeros((3, 3)))
+        assert e.value.code == _lib.ERR_NONFINITE
+        # nothing was accumulated by the failed calls
+        _, W = b.export_raw(0)
+        assert float(W.abs().sum()) == 0.0
+        b.insert(gpu(np.zeros((0, 16, 16), np.float32)), np.zeros((0, 3)))   # empty list: no-op
+
+
+def test_config1_subset_blob_phantom(bpmod):
+    """Config-1 recipe (64^2, 30-blob phantom, SNR 0.5) on a 300-projection subset."""
+    ang, ph, imgs = synth.make_dataset(1, nproj=300, device="cuda")
+    print("case config1 subset 300", flush=True)
+    vol = bpmod.bp3f(imgs, ang).cpu().numpy()
+    host = imgs.cpu().numpy()
+    o32 = oracle.bp(host, ang, prec="f32")["full"]
+    o64 = oracle.bp(host, ang, prec="f64")["full"]
+    assert rel(vol, o32) < 1e-5
+    assert rel(vol, o64) < 1e-4
+    assert np.corrcoef(vol.ravel(), ph.volume().ravel())[0, 1] > 0.5
+
+
+def test_config2_shape_properties(bpmod):
+    """128^2 images into the 256^3 grid (config-2 shape), 3000 projections in
+    several chunks: linearity and geometry-only weights at full size, and
+    parity with the oracle on a 150-projection subset."""
+    ang, ph, imgs = synth.make_dataset(2, nproj=3000, device="cuda")
+    print("case config2 3000", flush=True)
+    with bpmod.Backprojector(128, batch=1024) as b:
+        b.insert(imgs, ang)
+        va = b.finalize()["full"]
+        _, Wa = b.export_raw(0)
+        b.reset()
+        b.insert(2.0 * imgs, ang)
+        vb = b.finalize()["full"]
+        _, Wb = b.export_raw(0)
+    assert torch.equal(Wa > 0, Wb > 0)
+    assert float((Wa - Wb).abs().max()) <= 1e-5 * float(Wa.abs().max())
+    assert float((vb - 2 * va).norm() / (2 * va).norm()) < 1e-5
+    sub = slice(0, 150)
+    vs = bpmod.bp3f(imgs[sub].contiguous(), ang[sub]).cpu().numpy()
+    o32 = oracle.bp(imgs[sub].cpu().numpy(), ang[sub], prec="f32")["full"]
+    assert rel(vs, o32) < 1e-5
