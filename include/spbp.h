@@ -58,12 +58,20 @@ extern "C" {
 #define SPBP_K_PAD      1   /* PADD2 (units: projections)                 */
 #define SPBP_K_FFT2     2   /* batched 2D forward FFT (units: projections) */
 #define SPBP_K_FINALIZE 3   /* one whole finalisation (units: volumes)    */
+#define SPBP_K_BIN      4   /* brick work-list build (units: projections) */
+
+/* insertion strategies */
+#define SPBP_STRAT_AUTO   0   /* the library's choice                                     */
+#define SPBP_STRAT_DIRECT 1   /* one thread per sample, FP32 atomics into the half grid   */
+#define SPBP_STRAT_BRICK  2   /* LDS-binned bricks: workgroup owns a brick of the grid     */
 
 typedef struct {
     int nx;      /* image edge NX; volume NX^3, padded Fourier grid N = 2 NX          */
     int flags;   /* SPBP_* option flags                                               */
     int device;  /* HIP device ordinal, -1 = current device                           */
     int batch;   /* projections per internal FFT/insert chunk, 0 = automatic          */
+    int strategy;    /* SPBP_STRAT_*                                                  */
+    int brick_proj;  /* brick strategy: projections per work item, 0 = automatic      */
 } spbp_opts;
 
 typedef struct spbp_ctx spbp_ctx;

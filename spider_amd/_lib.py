@@ -35,6 +35,11 @@ K_INSERT = 0
 K_PAD = 1
 K_FFT2 = 2
 K_FINALIZE = 3
+K_BIN = 4
+
+STRAT_AUTO = 0
+STRAT_DIRECT = 1
+STRAT_BRICK = 2
 
 
 class SpbpError(RuntimeError):
@@ -44,7 +49,8 @@ class SpbpError(RuntimeError):
 
 
 class Opts(C.Structure):
-    _fields_ = [("nx", C.c_int), ("flags", C.c_int), ("device", C.c_int), ("batch", C.c_int)]
+    _fields_ = [("nx", C.c_int), ("flags", C.c_int), ("device", C.c_int), ("batch", C.c_int),
+                ("strategy", C.c_int), ("brick_proj", C.c_int)]
 
 
 _LIB = None

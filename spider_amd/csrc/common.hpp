@@ -1,0 +1,101 @@
+// common.hpp -- shared device definitions of libspbp (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+namespace spbp {
+
+constexpr int LN2 = 2;       // half width of the 5-tap kernel (bp32f.f:193-194)
+constexpr int LTAB = 4999;   // KB table length - 1 (oneline.f:44)
+constexpr int TPB = 256;
+
+
+struct Geom {        // per projection: CANG's matrix DM(1..9) + half flag
+    float r[9];
+    int half;
+    int pad_[2];
+};
+
+
+// ------------------------------------------------------------------ device helpers
+__device__ inline float load_img(const float* p, size_t i) { return p[i]; }
+__device__ inline float load_img(const uint16_t* p, size_t i) {
+    return __uint_as_float(((unsigned)p[i]) << 16);   // bf16 -> fp32 (exact)
+}
+
+template <typename T>
+__device__ inline T wave_sum(T v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// Block reduction of (double, unsigned) pairs; result valid in thread 0.
+__device__ inline void block_sum2(double& s, unsigned& c) {
+    __shared__ double ss[TPB / 64];
+    __shared__ unsigned sc[TPB / 64];
+    s = wave_sum(s);
+    c = wave_sum(c);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { ss[w] = s; sc[w] = c; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < (int)(blockDim.x >> 6); ++k) { s += ss[k]; c += sc[k]; }
+    }
+}
+
+
+__global__ __launch_bounds__(TPB) void fill_kernel(float* __restrict__ p, size_t n, float v) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < n) p[idx] = v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(TPB) void finite_kernel(const T* __restrict__ p, size_t n, int* __restrict__ flag) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < n && !isfinite(load_img(p, idx))) atomicOr(flag, 1);
+}
+
+inline unsigned nblocks(size_t n) { return (unsigned)((n + TPB - 1) / TPB); }
+
+// ------------------------------------------------------------------ ONELINE sample geometry
+// Shared by every insertion strategy so that all of them compute the sample
+// positions, nearest voxels and table indices with the same round-to-nearest
+// fp32 operations as the reference (oneline.f:56-105).
+
+// Position of lattice sample (I, J): I*DM(1:3) + J*DM(4:6).
+__device__ inline void sample_pos(const Geom& g, int I, int J, float& x, float& y, float& z) {
+    const float fI = (float)I, fJ = (float)J;
+    x = __fadd_rn(__fmul_rn(fI, g.r[0]), __fmul_rn(fJ, g.r[3]));
+    y = __fadd_rn(__fmul_rn(fI, g.r[1]), __fmul_rn(fJ, g.r[4]));
+    z = __fadd_rn(__fmul_rn(fI, g.r[2]), __fmul_rn(fJ, g.r[5]));
+}
+
+// IFIX(V + 0.5 + N) - N (oneline.f:69-71)
+__device__ inline int nearest_voxel(float v, int N) {
+    return (int)__fadd_rn(__fadd_rn(v, 0.5f), (float)N) - N;
+}
+
+// oneline.f:73-75
+__device__ inline bool in_bounds(int ix, int iy, int iz, int nx) {
+    return ix <= nx - LN2 - 1 && iy >= -nx + 2 + LN2 && iy <= nx - LN2 - 1 &&
+           iz >= -nx + 2 + LN2 && iz <= nx - LN2 - 1;
+}
+
+// TABI(NINT(ABS(D) * FLTB))
+__device__ inline float kb_weight(const float* __restrict__ tab, float d, float fltb) {
+    return tab[(int)roundf(__fmul_rn(fabsf(d), fltb))];
+}
+
+// BI(I, JP) of the raw R2C output with the SPIDER sign (conjugate,
+// fftw3.f:411-419) and the checkerboard (-1)^(I+J+1), J 1-based (bp32f.f:859-864).
+__device__ inline float2 spectrum_value(const float2* __restrict__ spec_p, int I, int J, int N, int H) {
+    const int jp = J >= 0 ? J : N + J;
+    const float2 s = spec_p[(size_t)jp * H + I];
+    const float sg = ((I + jp) & 1) ? -1.0f : 1.0f;
+    return make_float2(s.x * sg, -s.y * sg);
+}
+
+}  // namespace spbp

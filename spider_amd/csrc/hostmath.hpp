@@ -1,0 +1,77 @@
+// hostmath.hpp -- FP64 host-side tables: KB interpolant, window, rotations.
+#pragma once
+#include <cmath>
+#include <vector>
+
+namespace spbp {
+
+// ------------------------------------------------------------------ host math
+// Modified Bessel I1 by its power series (converges for the 0..6.5 used here).
+double bessel_i1(double x) {
+    double h = 0.5 * x, term = h, sum = h, h2 = h * h;
+    for (int k = 1; k < 200; ++k) {
+        term *= h2 / (double(k) * double(k + 1));
+        sum += term;
+        if (term < 1e-18 * sum) break;
+    }
+    return sum;
+}
+// I_{3/2}(x) = sqrt(2/(pi x)) (cosh x - sinh x / x), x > 0.
+double bessel_i15(double x) { return std::sqrt(2.0 / (M_PI * x)) * (std::cosh(x) - std::sinh(x) / x); }
+double bessel_j15(double x) { return std::sqrt(2.0 / (M_PI * x)) * (std::sin(x) / x - std::cos(x)); }
+
+// KB interpolation table TABI(0:4999) (bp32f.f:193-214), evaluated in FP64.
+void kb_table(int n2, std::vector<float>& tab, float& fltb_f) {
+    const double alpha = 6.5, V = 4.0 / 2.0 / (double)n2, aaaa = 0.9 * V;
+    const double b0 = std::sqrt(alpha) * bessel_i1(alpha);
+    const double fltb = (double)LTAB / (double)(LN2 + 1);
+    tab.assign(LTAB + 1, 0.0f);
+    for (int i = 0; i <= LTAB; ++i) {
+        const double s = (double)i / fltb / (double)n2;
+        if (s <= aaaa) {
+            const double q = s / aaaa, x = std::sqrt(1.0 - q * q);
+            tab[i] = (float)(std::sqrt(alpha * x) * bessel_i1(alpha * x) / b0);
+        }
+    }
+    fltb_f = (float)LTAB / (float)(LN2 + 1);   // REAL(LTAB)/REAL(LN2+1)
+}
+
+// |WKB| of WINDKB2A per integer squared radius LR = 0..(L/2)^2, FP64.
+void wkb_table(int nx, int N, std::vector<float>& t) {
+    const int cc = nx / 2;
+    const double alpha = 6.5, xnu = 1.5, aaaa = 0.9 * (4.0 / 2.0 / (double)N);
+    const double wkb0 = std::pow(alpha, xnu) / bessel_i15(alpha);
+    const double qrt = (2.0 * M_PI * aaaa) * (2.0 * M_PI * aaaa);
+    t.assign((size_t)cc * cc + 1, 1.0f);
+    for (int lr = 0; lr <= cc * cc; ++lr) {
+        const double sigma = qrt * lr - alpha * alpha;
+        double wkb;
+        if (std::fabs(sigma) < 1.0e-7) wkb = 1.0;
+        else if (sigma > 0.0) { const double art = std::sqrt(sigma); wkb = wkb0 * bessel_j15(art) / std::pow(art, xnu); }
+        else { const double art = std::sqrt(-sigma); wkb = wkb0 * bessel_i15(art) / std::pow(art, xnu); }
+        t[lr] = (float)std::fabs(wkb);
+    }
+}
+
+// CANG (cang.f:44-82) in FP64 from REAL doc registers, stored as REAL.  The
+// degree conversion is float32(pi)/180 evaluated in double (cang.f:50-53)
+// unless exact_pi.
+void cang(float phi, float theta, float psi, bool exact_pi, float* dm) {
+    const double dgr = exact_pi ? M_PI / 180.0 : (double)(float)3.14159265358979323846 / 180.0;
+    const double cphi = std::cos((double)phi * dgr), sphi = std::sin((double)phi * dgr);
+    const double cthe = std::cos((double)theta * dgr), sthe = std::sin((double)theta * dgr);
+    const double cpsi = std::cos((double)psi * dgr), spsi = std::sin((double)psi * dgr);
+    dm[0] = (float)(cphi * cthe * cpsi - sphi * spsi);
+    dm[1] = (float)(sphi * cthe * cpsi + cphi * spsi);
+    dm[2] = (float)(-sthe * cpsi);
+    dm[3] = (float)(-cphi * cthe * spsi - sphi * cpsi);
+    dm[4] = (float)(-sphi * cthe * spsi + cphi * cpsi);
+    dm[5] = (float)(sthe * spsi);
+    dm[6] = (float)(sthe * cphi);
+    dm[7] = (float)(sthe * sphi);
+    dm[8] = (float)(cthe);
+}
+
+
+
+}  // namespace spbp

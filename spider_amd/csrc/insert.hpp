@@ -1,0 +1,434 @@
+// insert.hpp -- PADD2 and the ONELINE scatter kernels.
+#pragma once
+#include "common.hpp"
+
+namespace spbp {
+
+// ------------------------------------------------------------------ PADD2
+template <typename T>
+__global__ __launch_bounds__(TPB) void pad_kernel(const T* __restrict__ imgs, int nx, int N,
+                                                  float* __restrict__ out) {
+    const int b = blockIdx.x;
+    const T* img = imgs + (size_t)b * nx * nx;
+    const int c = nx / 2;                 // NC - 1 (asta.f:75)
+    const float r2 = (float)c * (float)c; // RI = L/2 (bp3f.f:1316)
+    double s = 0.0;
+    unsigned cnt = 0;
+    for (int idx = threadIdx.x; idx < nx * nx; idx += blockDim.x) {
+        const int j = idx / nx, i = idx - j * nx;
+        const float tj = (float)(j - c), ti = (float)(i - c);
+        if (tj * tj + ti * ti > r2) { s += (double)load_img(img, idx); ++cnt; }
+    }
+    block_sum2(s, cnt);
+    __shared__ double qs_sh;
+    if (threadIdx.x == 0) qs_sh = s / (double)cnt;
+    __syncthreads();
+    const double qs = qs_sh;
+    const int ip = (N - nx) / 2 + (nx % 2);   // bp3f.f:1336
+    float* o = out + (size_t)b * N * N;
+    for (int idx = threadIdx.x; idx < N * N; idx += blockDim.x) {
+        const int y = idx / N, x = idx - y * N;
+        const int jj = y - ip, ii = x - ip;
+        float v = 0.0f;
+        if ((unsigned)jj < (unsigned)nx && (unsigned)ii < (unsigned)nx)
+            v = (float)((double)load_img(img, (size_t)jj * nx + ii) - qs);
+        o[idx] = v;
+    }
+}
+
+// ------------------------------------------------------------------ strategy 1: direct scatter
+// One thread per sample point (I, J) of one projection, FP32 atomics straight
+// into the half grid (the reference's ONELINE loop, oneline.f:53-156).
+__global__ __launch_bounds__(TPB) void insert_direct_kernel(
+    const float2* __restrict__ spec, const Geom* __restrict__ geom, int nx, int N,
+    const float* __restrict__ tab, float fltb,
+    float2* __restrict__ X0, float* __restrict__ W0, float2* __restrict__ X1, float* __restrict__ W1) {
+    const int H = nx + 1;
+    const int p = blockIdx.y;
+    const int lin = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lin >= H * N) return;
+    const int jrow = lin / H;
+    const int I = lin - jrow * H;
+    const int J = jrow - nx + 1;                                   // J = -NX+1 .. NX
+    if (!(I * I + J * J < (N * N) / 4) || (I == 0 && J < 0)) return;   // oneline.f:53-55
+    const Geom g = geom[p];
+    float x, y, z;
+    sample_pos(g, I, J, x, y, z);
+    float2 v = spectrum_value(spec + (size_t)p * N * H, I, J, N, H);
+    if (x < 0.0f) { x = -x; y = -y; z = -z; v.y = -v.y; }          // oneline.f:60-67
+    const int ix = nearest_voxel(x, N), iy = nearest_voxel(y, N), iz = nearest_voxel(z, N);
+    if (!in_bounds(ix, iy, iz, nx)) return;
+    float tx[5], ty[5], tz[5];
+#pragma unroll
+    for (int l = 0; l < 5; ++l) {
+        tx[l] = kb_weight(tab, __fsub_rn(x, (float)(ix + l - LN2)), fltb);
+        ty[l] = kb_weight(tab, __fsub_rn(y, (float)(iy + l - LN2)), fltb);
+        tz[l] = kb_weight(tab, __fsub_rn(z, (float)(iz + l - LN2)), fltb);
+    }
+    float2* __restrict__ X = g.half ? X1 : X0;
+    float* __restrict__ W = g.half ? W1 : W0;
+    const size_t plane = (size_t)N * H;
+    for (int lz = 0; lz < 5; ++lz) {
+        if (tz[lz] == 0.0f) continue;
+        const int izp = iz + lz - LN2;
+        const size_t za = (size_t)(izp >= 0 ? izp : N + izp) * plane;
+        const size_t zt = (size_t)(izp <= 0 ? -izp : N - izp) * plane;   // wrap(-izp)
+        for (int ly = 0; ly < 5; ++ly) {
+            const float tyz = __fmul_rn(ty[ly], tz[lz]);
+            if (tyz == 0.0f) continue;
+            const int iyp = iy + ly - LN2;
+            const size_t ya = (size_t)(iyp >= 0 ? iyp : N + iyp) * H;
+            const size_t yt = (size_t)(iyp <= 0 ? -iyp : N - iyp) * H;
+#pragma unroll
+            for (int lx = 0; lx < 5; ++lx) {
+                const float wg = __fmul_rn(tx[lx], tyz);
+                if (wg == 0.0f) continue;
+                const int px = ix + lx - LN2;
+                if (px >= 0) {                                   // direct taps
+                    const size_t o = za + ya + px;
+                    unsafeAtomicAdd(&X[o].x, __fmul_rn(v.x, wg));
+                    unsafeAtomicAdd(&X[o].y, __fmul_rn(v.y, wg));
+                    unsafeAtomicAdd(&W[o], wg);
+                } else {                                         // reflected taps, conj (oneline.f:118-152)
+                    const size_t o = zt + yt + (size_t)(-px);
+                    unsafeAtomicAdd(&X[o].x, __fmul_rn(v.x, wg));
+                    unsafeAtomicAdd(&X[o].y, __fmul_rn(-v.y, wg));
+                    unsafeAtomicAdd(&W[o], wg);
+                }
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ strategy 2: brick-binned
+// The half grid (signed kx in [0, NX), ky, kz in [-NX+1, NX)) is cut into
+// bricks of BX x BY x BZ voxels.  A work item is (brick, half, up to C
+// projections whose central plane crosses the brick).  Its workgroup
+// accumulates, in LDS, every tap of those projections that lands in the
+// brick, then adds the brick to the half grid once.  A sample (I, J) with
+// effective position E (after the Friedel flip) writes a tap p of its 5x5x5
+// window at p when p_x >= 0 and, conjugated, at -p when p_x < 0
+// (oneline.f:77-152); its taps with nonzero weight satisfy |E - p| <= 1.8
+// per axis.  So the samples feeding brick B are those with E within 1.8 of
+// B (all their taps are checked against B) plus, for bricks at kx < 3,
+// those with E within 1.8 of -B (mirrored taps only).
+
+constexpr float BRICK_MARGIN = 1.86f;   // > 1.8: support of the KB table in voxels
+
+struct BrickGrid {
+    int bx, by, bz;      // brick edge (voxels)
+    int nbx, nby, nbz;   // bricks per axis
+    int nb;              // nbx * nby * nbz
+};
+
+__device__ inline void brick_origin(const BrickGrid& bg, int b, int nx, int& x0, int& y0, int& z0) {
+    const int i = b % bg.nbx, j = (b / bg.nbx) % bg.nby, k = b / (bg.nbx * bg.nby);
+    x0 = i * bg.bx;
+    y0 = -nx + 1 + j * bg.by;
+    z0 = -nx + 1 + k * bg.bz;
+}
+
+// Does the plane through the origin with normal n (DM(7:9)) pass through the
+// brick's expanded box, inside the sample disk of radius NX?
+__device__ inline bool brick_hit(const BrickGrid& bg, int b, int nx, const Geom& g) {
+    int x0, y0, z0;
+    brick_origin(bg, b, nx, x0, y0, z0);
+    const float hx = 0.5f * (bg.bx - 1) + BRICK_MARGIN, hy = 0.5f * (bg.by - 1) + BRICK_MARGIN,
+                hz = 0.5f * (bg.bz - 1) + BRICK_MARGIN;
+    const float cx = x0 + 0.5f * (bg.bx - 1), cy = y0 + 0.5f * (bg.by - 1), cz = z0 + 0.5f * (bg.bz - 1);
+    const float d = fabsf(g.r[6] * cx + g.r[7] * cy + g.r[8] * cz);
+    const float reach = hx * fabsf(g.r[6]) + hy * fabsf(g.r[7]) + hz * fabsf(g.r[8]) + 0.02f;
+    if (d > reach) return false;
+    const float qx = fmaxf(fabsf(cx) - hx, 0.f), qy = fmaxf(fabsf(cy) - hy, 0.f), qz = fmaxf(fabsf(cz) - hz, 0.f);
+    const float r = (float)nx + 0.02f;
+    return qx * qx + qy * qy + qz * qz <= r * r;
+}
+
+// Count (brick, half) hits.  Block: one brick x 1024 projections.
+__global__ __launch_bounds__(TPB) void brick_count_kernel(const Geom* __restrict__ geom, int nproj, BrickGrid bg, int nx,
+                                                          int* __restrict__ cnt) {
+    const int b = blockIdx.x;
+    const int p0 = blockIdx.y * 1024;
+    unsigned c0 = 0, c1 = 0;
+    for (int p = p0 + threadIdx.x; p < min(nproj, p0 + 1024); p += TPB) {
+        const Geom g = geom[p];
+        if (brick_hit(bg, b, nx, g)) { if (g.half) ++c1; else ++c0; }
+    }
+    double d0 = c0;
+    block_sum2(d0, c1);
+    __shared__ unsigned sc0;
+    if (threadIdx.x == 0) sc0 = (unsigned)d0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (sc0) atomicAdd(&cnt[b], (int)sc0);
+        if (c1) atomicAdd(&cnt[b + bg.nb], (int)c1);
+    }
+}
+
+// Exclusive scans of the counts and of the work items per key; zero cursors.
+// One block of 1024 threads.  item_off[nkeys] = number of work items.
+__global__ __launch_bounds__(1024) void brick_scan_kernel(const int* __restrict__ cnt, int nkeys, int C,
+                                                          int* __restrict__ off, int* __restrict__ item_off,
+                                                          int* __restrict__ cursor, int* __restrict__ next_item) {
+    __shared__ int s_off[1024], s_it[1024];
+    const int per = (nkeys + 1023) / 1024;
+    const int k0 = threadIdx.x * per, k1 = min(nkeys, k0 + per);
+    int so = 0, si = 0;
+    for (int k = k0; k < k1; ++k) { so += cnt[k]; si += (cnt[k] + C - 1) / C; }
+    s_off[threadIdx.x] = so;
+    s_it[threadIdx.x] = si;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {   // inclusive Hillis-Steele scan
+        int a = 0, b = 0;
+        if ((int)threadIdx.x >= d) { a = s_off[threadIdx.x - d]; b = s_it[threadIdx.x - d]; }
+        __syncthreads();
+        s_off[threadIdx.x] += a;
+        s_it[threadIdx.x] += b;
+        __syncthreads();
+    }
+    int ro = s_off[threadIdx.x] - so, ri = s_it[threadIdx.x] - si;
+    for (int k = k0; k < k1; ++k) {
+        off[k] = ro;
+        item_off[k] = ri;
+        cursor[k] = 0;
+        ro += cnt[k];
+        ri += (cnt[k] + C - 1) / C;
+    }
+    if (threadIdx.x == 1023) { off[nkeys] = s_off[1023]; item_off[nkeys] = s_it[1023]; }
+    if (threadIdx.x == 0) *next_item = 0;
+}
+
+// Write the projection lists of every (brick, half) key.
+__global__ __launch_bounds__(TPB) void brick_fill_kernel(const Geom* __restrict__ geom, int nproj, BrickGrid bg, int nx,
+                                                         const int* __restrict__ off, int* __restrict__ cursor,
+                                                         int* __restrict__ list) {
+    const int b = blockIdx.x;
+    const int p0 = blockIdx.y * 1024;
+    for (int p = p0 + threadIdx.x; p < min(nproj, p0 + 1024); p += TPB) {
+        const Geom g = geom[p];
+        if (brick_hit(bg, b, nx, g)) {
+            const int key = b + (g.half ? bg.nb : 0);
+            list[off[key] + atomicAdd(&cursor[key], 1)] = p;
+        }
+    }
+}
+
+template <int BX, int BY, int BZ>
+struct BrickLds {
+    // the lattice lines crossing the expanded box, (max edge + 2 margins) * sqrt(3) + 2,
+    // must fit the 64-entry line table
+    static_assert(BX <= 31 && BY <= 31 && BZ <= 31, "brick edge too long for the line table");
+    float re[BX * BY * BZ], im[BX * BY * BZ], w[BX * BY * BZ];
+    int line_lo[64];
+    int line_pref[65];
+    int item;
+};
+
+// Taps of one effective sample into the LDS brick [x0,x0+BX) x [y0,..) x [z0,..).
+template <int BX, int BY, int BZ>
+__device__ inline void sample_taps_to_brick(BrickLds<BX, BY, BZ>& L, int x0, int y0, int z0,
+                                            float x, float y, float z, int ix, int iy, int iz, float2 v,
+                                            const float* __restrict__ tab, float fltb) {
+    float tx[5], ty[5], tz[5];
+#pragma unroll
+    for (int l = 0; l < 5; ++l) {
+        tx[l] = kb_weight(tab, __fsub_rn(x, (float)(ix + l - LN2)), fltb);
+        ty[l] = kb_weight(tab, __fsub_rn(y, (float)(iy + l - LN2)), fltb);
+        tz[l] = kb_weight(tab, __fsub_rn(z, (float)(iz + l - LN2)), fltb);
+    }
+    // direct taps p in B, p_x >= 0
+    {
+        const int zl = max(iz - LN2, z0), zh = min(iz + LN2, z0 + BZ - 1);
+        const int yl = max(iy - LN2, y0), yh = min(iy + LN2, y0 + BY - 1);
+        const int xl = max(max(ix - LN2, 0), x0), xh = min(ix + LN2, x0 + BX - 1);
+        for (int pz = zl; pz <= zh; ++pz) {
+            const float t_z = tz[pz - iz + LN2];
+            if (t_z == 0.0f) continue;
+            for (int py = yl; py <= yh; ++py) {
+                const float tyz = __fmul_rn(ty[py - iy + LN2], t_z);
+                if (tyz == 0.0f) continue;
+                const int base = ((pz - z0) * BY + (py - y0)) * BX - x0;
+                for (int px = xl; px <= xh; ++px) {
+                    const float wg = __fmul_rn(tx[px - ix + LN2], tyz);
+                    if (wg == 0.0f) continue;
+                    const int o = base + px;
+                    atomicAdd(&L.re[o], __fmul_rn(v.x, wg));
+                    atomicAdd(&L.im[o], __fmul_rn(v.y, wg));
+                    atomicAdd(&L.w[o], wg);
+                }
+            }
+        }
+    }
+    // mirrored taps: p_x < 0 written conjugated at -p, -p in B
+    if (ix < LN2 && x0 <= LN2) {
+        const int zl = max(iz - LN2, -(z0 + BZ - 1)), zh = min(iz + LN2, -z0);
+        const int yl = max(iy - LN2, -(y0 + BY - 1)), yh = min(iy + LN2, -y0);
+        const int xl = max(ix - LN2, -(x0 + BX - 1)), xh = min(-1, -x0);
+        for (int pz = zl; pz <= zh; ++pz) {
+            const float t_z = tz[pz - iz + LN2];
+            if (t_z == 0.0f) continue;
+            for (int py = yl; py <= yh; ++py) {
+                const float tyz = __fmul_rn(ty[py - iy + LN2], t_z);
+                if (tyz == 0.0f) continue;
+                const int base = ((-pz - z0) * BY + (-py - y0)) * BX - x0;
+                for (int px = xl; px <= xh; ++px) {
+                    const float wg = __fmul_rn(tx[px - ix + LN2], tyz);
+                    if (wg == 0.0f) continue;
+                    const int o = base - px;
+                    atomicAdd(&L.re[o], __fmul_rn(v.x, wg));
+                    atomicAdd(&L.im[o], __fmul_rn(-v.y, wg));
+                    atomicAdd(&L.w[o], wg);
+                }
+            }
+        }
+    }
+}
+
+// Enumerate the effective lattice samples of projection g whose E lies in
+// the box [lo, hi] (and, with `exclude`, not in [elo, ehi]); feed their taps
+// into the brick.  Lines are rows J of the lattice (b), runs are I (a), so
+// neighbouring lanes read neighbouring spectrum entries.
+template <int BX, int BY, int BZ>
+__device__ void brick_project(BrickLds<BX, BY, BZ>& L, const Geom& g, const float2* __restrict__ spec_p,
+                              int nx, int N, int x0, int y0, int z0,
+                              const float lo[3], const float hi[3], bool exclude, const float elo[3], const float ehi[3],
+                              const float* __restrict__ tab, float fltb) {
+    const int H = nx + 1;
+    // b range: extent of r2 . E over the box
+    float bmn = 0.f, bmx = 0.f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float c = g.r[3 + k];
+        bmn += fminf(c * lo[k], c * hi[k]);
+        bmx += fmaxf(c * lo[k], c * hi[k]);
+    }
+    const int b0 = (int)ceilf(bmn - 0.01f), b1 = (int)floorf(bmx + 0.01f);
+    const int nl = min(b1 - b0 + 1, 64);
+    // per line b: a interval with lo <= a r1 + b r2 <= hi
+    if ((int)threadIdx.x < 64) {
+        int cnt = 0, alo = 0;
+        const int t = threadIdx.x;
+        if (t < nl) {
+            const float bb = (float)(b0 + t);
+            float amin = -1e30f, amax = 1e30f;
+            bool empty = false;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const float r1 = g.r[k], off = bb * g.r[3 + k];
+                if (fabsf(r1) > 1e-6f) {
+                    const float u = (lo[k] - off) / r1, w = (hi[k] - off) / r1;
+                    amin = fmaxf(amin, fminf(u, w));
+                    amax = fminf(amax, fmaxf(u, w));
+                } else if (off < lo[k] - 0.01f || off > hi[k] + 0.01f) {
+                    empty = true;
+                }
+            }
+            if (!empty) {
+                const int a0 = (int)ceilf(amin - 0.01f), a1 = (int)floorf(amax + 0.01f);
+                if (a1 >= a0) { alo = a0; cnt = a1 - a0 + 1; }
+            }
+        }
+        // inclusive wave scan of cnt over lanes 0..63
+        int s = cnt;
+        for (int d = 1; d < 64; d <<= 1) {
+            const int u = __shfl_up(s, d, 64);
+            if (t >= d) s += u;
+        }
+        L.line_lo[t] = alo;
+        L.line_pref[t + 1] = s;
+        if (t == 0) L.line_pref[0] = 0;
+    }
+    __syncthreads();
+    const int total = L.line_pref[64];
+    for (int q = threadIdx.x; q < total; q += blockDim.x) {
+        // line of candidate q: largest l with line_pref[l] <= q
+        int lo_i = 0, hi_i = 63;
+        while (lo_i < hi_i) {
+            const int mid = (lo_i + hi_i + 1) >> 1;
+            if (L.line_pref[mid] <= q) lo_i = mid; else hi_i = mid - 1;
+        }
+        const int bb = b0 + lo_i;
+        const int aa = L.line_lo[lo_i] + (q - L.line_pref[lo_i]);
+        // canonical sample (I, J) of lattice point (aa, bb): the half plane I > 0 or I = 0, J >= 0
+        const bool canon = aa > 0 || (aa == 0 && bb >= 0);
+        const int I = canon ? aa : -aa, J = canon ? bb : -bb;
+        if (I > nx || J < -nx + 1 || J > nx || !(I * I + J * J < nx * nx)) continue;   // oneline.f:53-55
+        float x, y, z;
+        sample_pos(g, I, J, x, y, z);
+        const bool flip = x < 0.0f;                                    // oneline.f:60-67
+        if (flip == canon) continue;   // (aa, bb) must be the sample's effective lattice point
+        if (flip) { x = -x; y = -y; z = -z; }
+        if (x < lo[0] || x > hi[0] || y < lo[1] || y > hi[1] || z < lo[2] || z > hi[2]) continue;
+        if (exclude && x >= elo[0] && x <= ehi[0] && y >= elo[1] && y <= ehi[1] && z >= elo[2] && z <= ehi[2]) continue;
+        const int ix = nearest_voxel(x, N), iy = nearest_voxel(y, N), iz = nearest_voxel(z, N);
+        if (!in_bounds(ix, iy, iz, nx)) continue;
+        float2 v = spectrum_value(spec_p, I, J, N, H);
+        if (flip) v.y = -v.y;
+        sample_taps_to_brick<BX, BY, BZ>(L, x0, y0, z0, x, y, z, ix, iy, iz, v, tab, fltb);
+    }
+    __syncthreads();
+}
+
+// Persistent workgroups pull (key, projection slice) items from a queue.
+template <int BX, int BY, int BZ>
+__global__ __launch_bounds__(TPB) void insert_brick_kernel(
+    const float2* __restrict__ spec, const Geom* __restrict__ geom, int nx, int N,
+    const float* __restrict__ tab, float fltb, BrickGrid bg, int C,
+    const int* __restrict__ cnt, const int* __restrict__ off, const int* __restrict__ item_off,
+    const int* __restrict__ list, int* __restrict__ next_item,
+    float2* __restrict__ X0, float* __restrict__ W0, float2* __restrict__ X1, float* __restrict__ W1) {
+    constexpr int NV = BX * BY * BZ;
+    __shared__ BrickLds<BX, BY, BZ> L;
+    const int nkeys = 2 * bg.nb;
+    const int total = item_off[nkeys];
+    const int H = nx + 1;
+    for (;;) {
+        if (threadIdx.x == 0) L.item = atomicAdd(next_item, 1);
+        __syncthreads();
+        const int item = L.item;
+        __syncthreads();
+        if (item >= total) break;
+        // key: largest k with item_off[k] <= item
+        int lo_k = 0, hi_k = nkeys - 1;
+        while (lo_k < hi_k) {
+            const int mid = (lo_k + hi_k + 1) >> 1;
+            if (item_off[mid] <= item) lo_k = mid; else hi_k = mid - 1;
+        }
+        const int key = lo_k;
+        const int b = key % bg.nb, half = key / bg.nb;
+        const int s0 = (item - item_off[key]) * C;
+        const int ns = min(C, cnt[key] - s0);
+        int x0, y0, z0;
+        brick_origin(bg, b, nx, x0, y0, z0);
+        for (int i = threadIdx.x; i < NV; i += blockDim.x) { L.re[i] = 0.f; L.im[i] = 0.f; L.w[i] = 0.f; }
+        __syncthreads();
+        const float lo[3] = {x0 - BRICK_MARGIN, y0 - BRICK_MARGIN, z0 - BRICK_MARGIN};
+        const float hi[3] = {x0 + BX - 1 + BRICK_MARGIN, y0 + BY - 1 + BRICK_MARGIN, z0 + BZ - 1 + BRICK_MARGIN};
+        const float mlo[3] = {-hi[0], -hi[1], -hi[2]};
+        const float mhi[3] = {-lo[0], -lo[1], -lo[2]};
+        for (int s = 0; s < ns; ++s) {
+            const int p = list[off[key] + s0 + s];
+            const Geom g = geom[p];
+            const float2* spec_p = spec + (size_t)p * N * H;
+            brick_project<BX, BY, BZ>(L, g, spec_p, nx, N, x0, y0, z0, lo, hi, false, lo, hi, tab, fltb);
+            if (x0 <= LN2)
+                brick_project<BX, BY, BZ>(L, g, spec_p, nx, N, x0, y0, z0, mlo, mhi, true, lo, hi, tab, fltb);
+        }
+        // add the brick to the half grid
+        float2* __restrict__ X = half ? X1 : X0;
+        float* __restrict__ W = half ? W1 : W0;
+        for (int i = threadIdx.x; i < NV; i += blockDim.x) {
+            const float w = L.w[i];
+            if (w == 0.0f) continue;
+            const int lx = i % BX, ly = (i / BX) % BY, lz = i / (BX * BY);
+            const int kx = x0 + lx, ky = y0 + ly, kz = z0 + lz;
+            const size_t o = ((size_t)(kz >= 0 ? kz : N + kz) * N + (ky >= 0 ? ky : N + ky)) * H + kx;
+            unsafeAtomicAdd(&X[o].x, L.re[i]);
+            unsafeAtomicAdd(&X[o].y, L.im[i]);
+            unsafeAtomicAdd(&W[o], w);
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace spbp

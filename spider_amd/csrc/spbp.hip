@@ -35,382 +35,16 @@
 
 #include "spbp.h"
 
+
+#include "common.hpp"
+#include "finalize.hpp"
+#include "hostmath.hpp"
+#include "insert.hpp"
+
+using namespace spbp;
+
 namespace {
-
-constexpr int LN2 = 2;       // half width of the 5-tap kernel (bp32f.f:193-194)
-constexpr int LTAB = 4999;   // KB table length - 1 (oneline.f:44)
-constexpr int TPB = 256;
-
 thread_local std::string g_last_error;
-
-struct Geom {        // per projection: first two rows of CANG's matrix + half flag
-    float r[6];
-    int half;
-    int pad_;
-};
-
-// ------------------------------------------------------------------ device helpers
-__device__ inline float load_img(const float* p, size_t i) { return p[i]; }
-__device__ inline float load_img(const uint16_t* p, size_t i) {
-    return __uint_as_float(((unsigned)p[i]) << 16);   // bf16 -> fp32 (exact)
-}
-
-template <typename T>
-__device__ inline T wave_sum(T v) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    return v;
-}
-
-// Block reduction of (double, unsigned) pairs; result valid in thread 0.
-__device__ inline void block_sum2(double& s, unsigned& c) {
-    __shared__ double ss[TPB / 64];
-    __shared__ unsigned sc[TPB / 64];
-    s = wave_sum(s);
-    c = wave_sum(c);
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) { ss[w] = s; sc[w] = c; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int k = 1; k < (int)(blockDim.x >> 6); ++k) { s += ss[k]; c += sc[k]; }
-    }
-}
-
-// ------------------------------------------------------------------ PADD2
-template <typename T>
-__global__ __launch_bounds__(TPB) void pad_kernel(const T* __restrict__ imgs, int nx, int N,
-                                                  float* __restrict__ out) {
-    const int b = blockIdx.x;
-    const T* img = imgs + (size_t)b * nx * nx;
-    const int c = nx / 2;                 // NC - 1 (asta.f:75)
-    const float r2 = (float)c * (float)c; // RI = L/2 (bp3f.f:1316)
-    double s = 0.0;
-    unsigned cnt = 0;
-    for (int idx = threadIdx.x; idx < nx * nx; idx += blockDim.x) {
-        const int j = idx / nx, i = idx - j * nx;
-        const float tj = (float)(j - c), ti = (float)(i - c);
-        if (tj * tj + ti * ti > r2) { s += (double)load_img(img, idx); ++cnt; }
-    }
-    block_sum2(s, cnt);
-    __shared__ double qs_sh;
-    if (threadIdx.x == 0) qs_sh = s / (double)cnt;
-    __syncthreads();
-    const double qs = qs_sh;
-    const int ip = (N - nx) / 2 + (nx % 2);   // bp3f.f:1336
-    float* o = out + (size_t)b * N * N;
-    for (int idx = threadIdx.x; idx < N * N; idx += blockDim.x) {
-        const int y = idx / N, x = idx - y * N;
-        const int jj = y - ip, ii = x - ip;
-        float v = 0.0f;
-        if ((unsigned)jj < (unsigned)nx && (unsigned)ii < (unsigned)nx)
-            v = (float)((double)load_img(img, (size_t)jj * nx + ii) - qs);
-        o[idx] = v;
-    }
-}
-
-// ------------------------------------------------------------------ ONELINE scatter
-// One thread per sample point (I, J) of one projection.  Positions, rounding
-// and table indices use round-to-nearest fp32 operations in the reference's
-// order (no contraction), so the geometry matches ONELINE bit for bit.
-__global__ __launch_bounds__(TPB) void insert_direct_kernel(
-    const float2* __restrict__ spec, const Geom* __restrict__ geom, int nx, int N,
-    const float* __restrict__ tab, float fltb,
-    float2* __restrict__ X0, float* __restrict__ W0, float2* __restrict__ X1, float* __restrict__ W1) {
-    const int H = nx + 1;
-    const int p = blockIdx.y;
-    const int lin = blockIdx.x * blockDim.x + threadIdx.x;
-    if (lin >= H * N) return;
-    const int jrow = lin / H;
-    const int I = lin - jrow * H;
-    const int J = jrow - nx + 1;                                   // J = -NX+1 .. NX
-    if (!(I * I + J * J < (N * N) / 4) || (I == 0 && J < 0)) return;   // oneline.f:53-55
-    const Geom g = geom[p];
-    const float fI = (float)I, fJ = (float)J;
-    float x = __fadd_rn(__fmul_rn(fI, g.r[0]), __fmul_rn(fJ, g.r[3]));   // oneline.f:56-58
-    float y = __fadd_rn(__fmul_rn(fI, g.r[1]), __fmul_rn(fJ, g.r[4]));
-    float z = __fadd_rn(__fmul_rn(fI, g.r[2]), __fmul_rn(fJ, g.r[5]));
-    const int jp = J >= 0 ? J : N + J;
-    const float2 s = spec[((size_t)p * N + jp) * H + I];
-    const float sg = ((I + jp) & 1) ? -1.0f : 1.0f;   // (-1)^(I+J+1), J 1-based
-    float vr = s.x * sg;
-    float vi = -s.y * sg;                             // SPIDER sign: conjugate
-    if (x < 0.0f) { x = -x; y = -y; z = -z; vi = -vi; }   // oneline.f:60-67
-    const float fN = (float)N;
-    const int ix = (int)__fadd_rn(__fadd_rn(x, 0.5f), fN) - N;   // IFIX(X+0.5+N)-N
-    const int iy = (int)__fadd_rn(__fadd_rn(y, 0.5f), fN) - N;
-    const int iz = (int)__fadd_rn(__fadd_rn(z, 0.5f), fN) - N;
-    if (!(ix <= nx - LN2 - 1 && iy >= -nx + 2 + LN2 && iy <= nx - LN2 - 1 &&
-          iz >= -nx + 2 + LN2 && iz <= nx - LN2 - 1)) return;      // oneline.f:73-75
-    float tx[5], ty[5], tz[5];
-#pragma unroll
-    for (int l = 0; l < 5; ++l) {
-        tx[l] = tab[(int)roundf(__fmul_rn(fabsf(__fsub_rn(x, (float)(ix + l - LN2))), fltb))];
-        ty[l] = tab[(int)roundf(__fmul_rn(fabsf(__fsub_rn(y, (float)(iy + l - LN2))), fltb))];
-        tz[l] = tab[(int)roundf(__fmul_rn(fabsf(__fsub_rn(z, (float)(iz + l - LN2))), fltb))];
-    }
-    float2* __restrict__ X = g.half ? X1 : X0;
-    float* __restrict__ W = g.half ? W1 : W0;
-    const size_t plane = (size_t)N * H;
-    for (int lz = 0; lz < 5; ++lz) {
-        if (tz[lz] == 0.0f) continue;
-        const int izp = iz + lz - LN2;
-        const size_t za = (size_t)(izp >= 0 ? izp : N + izp) * plane;
-        const size_t zt = (size_t)(izp <= 0 ? -izp : N - izp) * plane;   // wrap(-izp)
-        for (int ly = 0; ly < 5; ++ly) {
-            const float tyz = __fmul_rn(ty[ly], tz[lz]);
-            if (tyz == 0.0f) continue;
-            const int iyp = iy + ly - LN2;
-            const size_t ya = (size_t)(iyp >= 0 ? iyp : N + iyp) * H;
-            const size_t yt = (size_t)(iyp <= 0 ? -iyp : N - iyp) * H;
-#pragma unroll
-            for (int lx = 0; lx < 5; ++lx) {
-                const float wg = __fmul_rn(tx[lx], tyz);
-                if (wg == 0.0f) continue;
-                const int px = ix + lx - LN2;
-                if (px >= 0) {                                   // direct taps
-                    const size_t o = za + ya + px;
-                    unsafeAtomicAdd(&X[o].x, __fmul_rn(vr, wg));
-                    unsafeAtomicAdd(&X[o].y, __fmul_rn(vi, wg));
-                    unsafeAtomicAdd(&W[o], wg);
-                } else {                                         // reflected taps, conj
-                    const size_t o = zt + yt + (size_t)(-px);
-                    unsafeAtomicAdd(&X[o].x, __fmul_rn(vr, wg));
-                    unsafeAtomicAdd(&X[o].y, __fmul_rn(-vi, wg));
-                    unsafeAtomicAdd(&W[o], wg);
-                }
-            }
-        }
-    }
-}
-
-// ------------------------------------------------------------------ accumulator views
-struct Acc {
-    const float2* X[2];
-    const float* W[2];
-    int n;   // halves summed: 1 or 2
-};
-
-// (j, k) of the kx = 0 plane is paired by SYMPLANE0 with (N-j, N-k) unless it
-// lies on the Nyquist row/column or is the origin (bp32f.f:1057-1087).
-__device__ inline bool sym_pair(int j, int k, int nx) {
-    return j != nx && k != nx && !(j == 0 && k == 0);
-}
-
-// Value of SYMPLANE0(half) at (i, j, k), halves added after (bp32f.f:925-930, 345-346).
-__device__ inline void load_sym(const Acc& a, int nx, int N, int i, int j, int k, float2& x, float& w) {
-    const int H = nx + 1;
-    const size_t o = ((size_t)k * N + j) * H + i;
-    const bool pr = (i == 0) && sym_pair(j, k, nx);
-    const size_t om = pr ? ((size_t)((N - k) % N) * N + (N - j) % N) * H : 0;
-    x = make_float2(0.f, 0.f);
-    w = 0.f;
-    for (int h = 0; h < a.n; ++h) {
-        float2 xh = a.X[h][o];
-        float wh = a.W[h][o];
-        if (pr) {
-            const float2 xm = a.X[h][om];
-            xh.x = xh.x + xm.x;
-            xh.y = xh.y + (-xm.y);
-            wh = wh + a.W[h][om];
-        }
-        if (h == 0) { x = xh; w = wh; }
-        else { x.x = x.x + xh.x; x.y = x.y + xh.y; w = w + wh; }
-    }
-}
-
-__device__ inline float2 norm_val(const Acc& a, int nx, int N, int i, int j, int k) {
-    float2 x;
-    float w;
-    load_sym(a, nx, N, i, j, k, x, w);
-    if (w > 0.1f) {                                    // NRMW2 (bp3f.f:1359)
-        const float s = ((i + j + k) & 1) ? -1.0f : 1.0f;
-        return make_float2(__fdiv_rn(x.x * s, w), __fdiv_rn(x.y * s, w));
-    }
-    return make_float2(0.f, 0.f);
-}
-
-__global__ __launch_bounds__(TPB) void normalize_kernel(Acc a, int nx, int N, float2* __restrict__ out) {
-    const int H = nx + 1;
-    const size_t total = (size_t)H * N * N;
-    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= total) return;
-    const int i = (int)(idx % H);
-    const size_t r = idx / H;
-    const int j = (int)(r % N), k = (int)(r / N);
-    float2 v = norm_val(a, nx, N, i, j, k);
-    if (i == 0 || i == nx) {
-        // c2r reads the kx = 0 and kx = N/2 planes as their Hermitian part
-        const float2 m = norm_val(a, nx, N, i, (N - j) % N, (N - k) % N);
-        v = make_float2((v.x + m.x) * 0.5f, (v.y - m.y) * 0.5f);
-    }
-    out[idx] = make_float2(v.x, -v.y);                 // SPIDER inverse: conjugate first
-}
-
-// ------------------------------------------------------------------ WINDKB2A
-__global__ __launch_bounds__(TPB) void window_kernel(const float* __restrict__ real, float* __restrict__ out,
-                                                     const float* __restrict__ wkb, int nx, int N, float pix,
-                                                     double* __restrict__ psum, unsigned* __restrict__ pcnt) {
-    const int L = nx;
-    const size_t total = (size_t)L * L * L;
-    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    double s = 0.0;
-    unsigned c = 0;
-    if (idx < total) {
-        const int i = (int)(idx % L);
-        const size_t t = idx / L;
-        const int j = (int)(t % L), k = (int)(t / L);
-        const int ip = (N - L) / 2 + (L % 2);
-        const int cc = L / 2;
-        float v = real[((size_t)(ip + k) * N + (ip + j)) * N + (ip + i)] * pix;   // FMRS_3 scale
-        const int lr = (i - cc) * (i - cc) + (j - cc) * (j - cc) + (k - cc) * (k - cc);
-        if (lr <= cc * cc) {
-            v = __fdiv_rn(v, wkb[lr]);
-            if (lr >= (cc - 1) * (cc - 1)) { s = v; c = 1; }
-        } else {
-            v = 0.0f;
-        }
-        out[idx] = v;
-    }
-    block_sum2(s, c);
-    if (threadIdx.x == 0) { psum[blockIdx.x] = s; pcnt[blockIdx.x] = c; }
-}
-
-__global__ __launch_bounds__(TPB) void shell_mean_kernel(const double* __restrict__ psum, const unsigned* __restrict__ pcnt,
-                                                         int nb, float* __restrict__ mean) {
-    double s = 0.0;
-    unsigned c = 0;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) { s += psum[b]; c += pcnt[b]; }
-    block_sum2(s, c);
-    if (threadIdx.x == 0) mean[0] = (float)(s / (double)(c ? c : 1));
-}
-
-__global__ __launch_bounds__(TPB) void subtract_kernel(float* __restrict__ out, const float* __restrict__ mean, int nx) {
-    const int L = nx;
-    const size_t total = (size_t)L * L * L;
-    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= total) return;
-    const int i = (int)(idx % L);
-    const size_t t = idx / L;
-    const int j = (int)(t % L), k = (int)(t / L);
-    const int cc = L / 2;
-    const int lr = (i - cc) * (i - cc) + (j - cc) * (j - cc) + (k - cc) * (k - cc);
-    if (lr <= cc * cc) out[idx] = out[idx] - mean[0];
-}
-
-// ------------------------------------------------------------------ partial volumes
-__global__ __launch_bounds__(TPB) void export_kernel(Acc a, int nx, int N, float2* __restrict__ X, float* __restrict__ W) {
-    const int H = nx + 1;
-    const size_t total = (size_t)H * N * N;
-    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= total) return;
-    const int i = (int)(idx % H);
-    const size_t r = idx / H;
-    float2 x;
-    float w;
-    load_sym(a, nx, N, i, (int)(r % N), (int)(r / N), x, w);
-    X[idx] = x;
-    W[idx] = w;
-}
-
-// Adds (or sets) a post-SYMPLANE0 half-grid into raw accumulators.  Entries of
-// SYMPLANE0 pairs are halved (exactly) so that SYMPLANE0 of the result gives
-// back the partial: P(a)/2 + conj(P(-a)/2) = P(a) for Hermitian P.
-__global__ __launch_bounds__(TPB) void import_kernel(const float2* __restrict__ Xp, const float* __restrict__ Wp, int nx, int N,
-                                                     float2* __restrict__ X, float* __restrict__ W, int accumulate) {
-    const int H = nx + 1;
-    const size_t total = (size_t)H * N * N;
-    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= total) return;
-    const int i = (int)(idx % H);
-    const size_t r = idx / H;
-    const int j = (int)(r % N), k = (int)(r / N);
-    float2 x = Xp[idx];
-    float w = Wp[idx];
-    if (i == 0 && sym_pair(j, k, nx)) { x.x *= 0.5f; x.y *= 0.5f; w *= 0.5f; }
-    if (accumulate) { x.x += X[idx].x; x.y += X[idx].y; w += W[idx]; }
-    X[idx] = x;
-    W[idx] = w;
-}
-
-__global__ __launch_bounds__(TPB) void fill_kernel(float* __restrict__ p, size_t n, float v) {
-    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx < n) p[idx] = v;
-}
-
-template <typename T>
-__global__ __launch_bounds__(TPB) void finite_kernel(const T* __restrict__ p, size_t n, int* __restrict__ flag) {
-    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx < n && !isfinite(load_img(p, idx))) atomicOr(flag, 1);
-}
-
-// ------------------------------------------------------------------ host math
-// Modified Bessel I1 by its power series (converges for the 0..6.5 used here).
-double bessel_i1(double x) {
-    double h = 0.5 * x, term = h, sum = h, h2 = h * h;
-    for (int k = 1; k < 200; ++k) {
-        term *= h2 / (double(k) * double(k + 1));
-        sum += term;
-        if (term < 1e-18 * sum) break;
-    }
-    return sum;
-}
-// I_{3/2}(x) = sqrt(2/(pi x)) (cosh x - sinh x / x), x > 0.
-double bessel_i15(double x) { return std::sqrt(2.0 / (M_PI * x)) * (std::cosh(x) - std::sinh(x) / x); }
-double bessel_j15(double x) { return std::sqrt(2.0 / (M_PI * x)) * (std::sin(x) / x - std::cos(x)); }
-
-// KB interpolation table TABI(0:4999) (bp32f.f:193-214), evaluated in FP64.
-void kb_table(int n2, std::vector<float>& tab, float& fltb_f) {
-    const double alpha = 6.5, V = 4.0 / 2.0 / (double)n2, aaaa = 0.9 * V;
-    const double b0 = std::sqrt(alpha) * bessel_i1(alpha);
-    const double fltb = (double)LTAB / (double)(LN2 + 1);
-    tab.assign(LTAB + 1, 0.0f);
-    for (int i = 0; i <= LTAB; ++i) {
-        const double s = (double)i / fltb / (double)n2;
-        if (s <= aaaa) {
-            const double q = s / aaaa, x = std::sqrt(1.0 - q * q);
-            tab[i] = (float)(std::sqrt(alpha * x) * bessel_i1(alpha * x) / b0);
-        }
-    }
-    fltb_f = (float)LTAB / (float)(LN2 + 1);   // REAL(LTAB)/REAL(LN2+1)
-}
-
-// |WKB| of WINDKB2A per integer squared radius LR = 0..(L/2)^2, FP64.
-void wkb_table(int nx, int N, std::vector<float>& t) {
-    const int cc = nx / 2;
-    const double alpha = 6.5, xnu = 1.5, aaaa = 0.9 * (4.0 / 2.0 / (double)N);
-    const double wkb0 = std::pow(alpha, xnu) / bessel_i15(alpha);
-    const double qrt = (2.0 * M_PI * aaaa) * (2.0 * M_PI * aaaa);
-    t.assign((size_t)cc * cc + 1, 1.0f);
-    for (int lr = 0; lr <= cc * cc; ++lr) {
-        const double sigma = qrt * lr - alpha * alpha;
-        double wkb;
-        if (std::fabs(sigma) < 1.0e-7) wkb = 1.0;
-        else if (sigma > 0.0) { const double art = std::sqrt(sigma); wkb = wkb0 * bessel_j15(art) / std::pow(art, xnu); }
-        else { const double art = std::sqrt(-sigma); wkb = wkb0 * bessel_i15(art) / std::pow(art, xnu); }
-        t[lr] = (float)std::fabs(wkb);
-    }
-}
-
-// CANG (cang.f:44-82) in FP64 from REAL doc registers, stored as REAL.  The
-// degree conversion is float32(pi)/180 evaluated in double (cang.f:50-53)
-// unless exact_pi.
-void cang(float phi, float theta, float psi, bool exact_pi, float* dm) {
-    const double dgr = exact_pi ? M_PI / 180.0 : (double)(float)3.14159265358979323846 / 180.0;
-    const double cphi = std::cos((double)phi * dgr), sphi = std::sin((double)phi * dgr);
-    const double cthe = std::cos((double)theta * dgr), sthe = std::sin((double)theta * dgr);
-    const double cpsi = std::cos((double)psi * dgr), spsi = std::sin((double)psi * dgr);
-    dm[0] = (float)(cphi * cthe * cpsi - sphi * spsi);
-    dm[1] = (float)(sphi * cthe * cpsi + cphi * spsi);
-    dm[2] = (float)(-sthe * cpsi);
-    dm[3] = (float)(-cphi * cthe * spsi - sphi * cpsi);
-    dm[4] = (float)(-sphi * cthe * spsi + cphi * cpsi);
-    dm[5] = (float)(sthe * spsi);
-    dm[6] = (float)(sthe * cphi);
-    dm[7] = (float)(sthe * sphi);
-    dm[8] = (float)(cthe);
-}
-
-inline unsigned nblocks(size_t n) { return (unsigned)((n + TPB - 1) / TPB); }
-
 }  // namespace
 
 // ------------------------------------------------------------------ context
@@ -444,9 +78,21 @@ struct spbp_ctx {
     // host staging for _host entry points
     void* stage = nullptr;
     size_t stage_cap = 0;
+    // brick strategy
+    int strategy = SPBP_STRAT_BRICK;
+    int C = 64;
+    int ncu = 256;
+    BrickGrid bg{};
+    int* bcnt = nullptr;    // [2 nb] hits per (brick, half)
+    int* boff = nullptr;    // [2 nb + 1] list offsets
+    int* bioff = nullptr;   // [2 nb + 1] work-item offsets
+    int* bcur = nullptr;    // [2 nb] fill cursors
+    int* bnext = nullptr;   // work-queue head
+    int* blist = nullptr;   // projection lists
+    size_t blist_cap = 0;
     // SPBP_TIMING: (start, stop, units) per recorded launch, per kernel kind
     struct Rec { hipEvent_t a, b; long units; };
-    std::vector<Rec> recs[4];
+    std::vector<Rec> recs[5];
     std::vector<hipEvent_t> evpool;
     std::string err;
 };
@@ -527,6 +173,12 @@ int ensure_chunk(spbp_ctx* c, int want_batch) {
     const int N = c->N, H = c->H, B = want_batch;
     HIPCHK(c, hipMalloc(&c->padbuf, (size_t)B * N * N * sizeof(float)));
     HIPCHK(c, hipMalloc(&c->spec, (size_t)B * N * H * sizeof(float2)));
+    if (c->strategy == SPBP_STRAT_BRICK) {
+        const size_t cap = (size_t)B * c->bg.nb;
+        if (c->blist) { (void)hipFree(c->blist); c->blist = nullptr; }
+        HIPCHK(c, hipMalloc(&c->blist, cap * sizeof(int)));
+        c->blist_cap = cap;
+    }
     int n[2] = {N, N};
     int inembed[2] = {N, N};
     int onembed[2] = {N, H};
@@ -604,7 +256,7 @@ int insert_impl(spbp_ctx* c, const void* imgs, int dtype, int nimg, const float*
                 return fail(c, SPBP_ERR_ANGLE, "MISSING ANGLE FOR IMAGE %d", p + 1);
             cang((float)a[2], (float)a[1], (float)a[0], exact, dm);   // CANG(PHI, THETA, PSI)
         }
-        for (int q = 0; q < 6; ++q) {
+        for (int q = 0; q < 9; ++q) {
             if (!std::isfinite(dm[q])) return fail(c, SPBP_ERR_ANGLE, "non-finite rotation for image %d", p + 1);
             g.r[q] = dm[q];
         }
@@ -614,7 +266,7 @@ int insert_impl(spbp_ctx* c, const void* imgs, int dtype, int nimg, const float*
             if (half[p] == 1 && c->nvol < 2) return fail(c, SPBP_ERR_ARG, "half 1 needs SPBP_TWO_HALVES");
             g.half = half[p];
         }
-        g.pad_ = 0;
+        g.pad_[0] = g.pad_[1] = 0;
     }
     HIPCHK(c, hipMemcpyAsync(c->geom_d, c->geom_h, (size_t)nimg * sizeof(Geom), hipMemcpyHostToDevice, st));
     HIPCHK(c, hipEventRecord(c->geom_evt, st));
@@ -652,11 +304,26 @@ int insert_impl(spbp_ctx* c, const void* imgs, int dtype, int nimg, const float*
             Timed t(c, SPBP_K_FFT2, b, st);
             FFTCHK(c, hipfftExecR2C(plan, (hipfftReal*)c->padbuf, (hipfftComplex*)c->spec));
         }
-        {
+        if (c->strategy == SPBP_STRAT_DIRECT) {
             Timed t(c, SPBP_K_INSERT, b, st);
             dim3 grid(nblocks((size_t)H * N), b);
             insert_direct_kernel<<<grid, TPB, 0, st>>>(c->spec, c->geom_d + p0, nx, N, c->tab, c->fltb,
                                                        c->X[0], c->W[0], c->X[1], c->W[1]);
+        } else {
+            const int nkeys = 2 * c->bg.nb;
+            const dim3 lgrid(c->bg.nb, (b + 1023) / 1024);
+            {
+                Timed t(c, SPBP_K_BIN, b, st);
+                fill_kernel<<<nblocks(nkeys), TPB, 0, st>>>((float*)c->bcnt, nkeys, 0.0f);   // zero bits
+                brick_count_kernel<<<lgrid, TPB, 0, st>>>(c->geom_d + p0, b, c->bg, nx, c->bcnt);
+                brick_scan_kernel<<<1, 1024, 0, st>>>(c->bcnt, nkeys, c->C, c->boff, c->bioff, c->bcur, c->bnext);
+                brick_fill_kernel<<<lgrid, TPB, 0, st>>>(c->geom_d + p0, b, c->bg, nx, c->boff, c->bcur, c->blist);
+            }
+            HIPCHK(c, hipGetLastError());
+            Timed t(c, SPBP_K_INSERT, b, st);
+            insert_brick_kernel<16, 16, 16><<<c->ncu * 2, TPB, 0, st>>>(
+                c->spec, c->geom_d + p0, nx, N, c->tab, c->fltb, c->bg, c->C, c->bcnt, c->boff, c->bioff,
+                c->blist, c->bnext, c->X[0], c->W[0], c->X[1], c->W[1]);
         }
         HIPCHK(c, hipGetLastError());
     }
@@ -731,6 +398,17 @@ int spbp_create(const spbp_opts* o, spbp_ctx** out) {
         const size_t per = (size_t)c->N * c->N * 4 + (size_t)c->N * c->H * 8;
         c->batch = (int)std::max<size_t>(16, std::min<size_t>(4096, ((size_t)512 << 20) / per));
     }
+    c->strategy = o->strategy == SPBP_STRAT_DIRECT ? SPBP_STRAT_DIRECT : SPBP_STRAT_BRICK;
+    if (o->strategy < 0 || o->strategy > SPBP_STRAT_BRICK) { delete c; return fail(nullptr, SPBP_ERR_ARG, "unknown strategy %d", o->strategy); }
+    c->C = o->brick_proj > 0 ? o->brick_proj : 64;
+    {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0) c->ncu = n;
+    }
+    c->bg.bx = c->bg.by = c->bg.bz = 16;
+    c->bg.nbx = (c->nx + 15) / 16;
+    c->bg.nby = c->bg.nbz = (2 * c->nx - 1 + 15) / 16;
+    c->bg.nb = c->bg.nbx * c->bg.nby * c->bg.nbz;
     int rc = SPBP_OK;
     auto bail = [&](int code) { spbp_destroy(c); return code; };
     for (int h = 0; h < c->nvol; ++h) {
@@ -745,6 +423,13 @@ int spbp_create(const spbp_opts* o, spbp_ctx** out) {
         hipMalloc(&c->flag, sizeof(int)) != hipSuccess ||
         hipEventCreateWithFlags(&c->geom_evt, hipEventDisableTiming) != hipSuccess)
         return bail(fail(nullptr, SPBP_ERR_DEVICE, "context set-up failed"));
+    {
+        const size_t nk = 2 * (size_t)c->bg.nb;
+        if (hipMalloc(&c->bcnt, nk * sizeof(int)) != hipSuccess || hipMalloc(&c->boff, (nk + 1) * sizeof(int)) != hipSuccess ||
+            hipMalloc(&c->bioff, (nk + 1) * sizeof(int)) != hipSuccess || hipMalloc(&c->bcur, nk * sizeof(int)) != hipSuccess ||
+            hipMalloc(&c->bnext, sizeof(int)) != hipSuccess)
+            return bail(fail(nullptr, SPBP_ERR_ALLOC, "brick lists"));
+    }
     rc = spbp_reset(c, nullptr);
     if (rc) return bail(rc);
     if (hipDeviceSynchronize() != hipSuccess) return bail(fail(nullptr, SPBP_ERR_DEVICE, "zeroing accumulators failed"));
@@ -760,7 +445,8 @@ int spbp_destroy(spbp_ctx* c) {
         if (c->X[h]) (void)hipFree(c->X[h]);
         if (c->W[h]) (void)hipFree(c->W[h]);
     }
-    void* dev_ptrs[] = {c->tab, c->padbuf, c->spec, c->geom_d, c->work, c->real, c->wkb, c->psum, c->pcnt, c->mean, c->flag};
+    void* dev_ptrs[] = {c->tab, c->padbuf, c->spec, c->geom_d, c->work, c->real, c->wkb, c->psum, c->pcnt, c->mean, c->flag,
+                        c->bcnt, c->boff, c->bioff, c->bcur, c->bnext, c->blist};
     for (void* p : dev_ptrs) if (p) (void)hipFree(p);
     if (c->geom_h) (void)hipHostFree(c->geom_h);
     if (c->stage) (void)hipHostFree(c->stage);
@@ -880,7 +566,7 @@ int spbp_export_raw(spbp_ctx* c, int half, float* X, float* W, void* stream) {
 
 int spbp_kernel_stats(spbp_ctx* c, int kernel, double* total_ms, long* launches, long* units) {
     if (!c) return fail(nullptr, SPBP_ERR_ARG, "NULL context");
-    if (kernel < 0 || kernel > 3) return fail(c, SPBP_ERR_ARG, "no kernel kind %d", kernel);
+    if (kernel < 0 || kernel > 4) return fail(c, SPBP_ERR_ARG, "no kernel kind %d", kernel);
     DeviceGuard dg(c->device);
     double ms = 0.0;
     long u = 0;

@@ -29,7 +29,7 @@ def test_create_rejects_bad_sizes_without_touching_a_device():
     L = _lib.lib()
     ctx = C.c_void_p()
     for nx in (0, 3, 2000):
-        rc = L.spbp_create(C.byref(_lib.Opts(nx, 0, -1, 0)), C.byref(ctx))
+        rc = L.spbp_create(C.byref(_lib.Opts(nx, 0, -1, 0, 0, 0)), C.byref(ctx))
         assert rc == _lib.ERR_SIZE and not ctx.value
     assert b"outside" in L.spbp_last_error(None)
     assert L.spbp_create(None, C.byref(ctx)) == _lib.ERR_ARG

@@ -54,13 +54,17 @@ def oracle_raw(imgs, ang, half=None):
     return grids
 
 
-@pytest.mark.parametrize("nx,m", [(16, 40), (15, 40), (32, 60)])
-def test_bp3f_parity_random(bpmod, nx, m):
+STRATEGIES = ["direct", "brick"]
+
+
+@pytest.mark.parametrize("strategy", STRATEGIES)
+@pytest.mark.parametrize("nx,m", [(16, 40), (15, 40), (32, 60), (37, 30)])
+def test_bp3f_parity_random(bpmod, nx, m, strategy):
     rng = np.random.default_rng(100 + nx)
     imgs = rng.standard_normal((m, nx, nx)).astype(np.float32)
     ang = synth.uniform_angles(m, rng)
-    print(f"case bp3f nx={nx} m={m}", flush=True)
-    with bpmod.Backprojector(nx) as b:
+    print(f"case bp3f nx={nx} m={m} {strategy}", flush=True)
+    with bpmod.Backprojector(nx, strategy=strategy) as b:
         b.insert(gpu(imgs), ang)
         vol = b.finalize()["full"].cpu().numpy()
         Xr, Wr = b.export_raw(0)
@@ -92,13 +96,14 @@ def test_testbp3f_known_answer_on_gpu(bpmod):
     assert rel(full, v) < 1e-6
 
 
-def test_bp32f_halves_parity(bpmod):
+@pytest.mark.parametrize("strategy", STRATEGIES)
+def test_bp32f_halves_parity(bpmod, strategy):
     rng = np.random.default_rng(7)
     nx, m = 24, 50
     imgs = rng.standard_normal((m, nx, nx)).astype(np.float32)
     ang = synth.uniform_angles(m, rng)
     half = bpmod.random_halves(m, seed=3)
-    full, h1, h2 = bpmod.bp32f(gpu(imgs), ang, half=half)
+    full, h1, h2 = bpmod.bp32f(gpu(imgs), ang, half=half, strategy=strategy)
     o = oracle.bp(imgs, ang, half=half, prec="f32")
     assert rel(full.cpu().numpy(), o["full"]) < 1e-5
     assert rel(h1.cpu().numpy(), o["h1"]) < 1e-5
@@ -219,3 +224,18 @@ def test_config2_shape_properties(bpmod):
     vs = bpmod.bp3f(imgs[sub].contiguous(), ang[sub]).cpu().numpy()
     o32 = oracle.bp(imgs[sub].cpu().numpy(), ang[sub], prec="f32")["full"]
     assert rel(vs, o32) < 1e-5
+
+
+def test_strategies_agree_config2_shape(bpmod):
+    """Direct scatter and brick-binned accumulate the same half grid (128^2
+    images, 256^3 grid, 400 projections over several chunks and work items)."""
+    ang, ph, imgs = synth.make_dataset(2, nproj=400, device="cuda")
+    out = {}
+    for strategy in STRATEGIES:
+        with bpmod.Backprojector(128, strategy=strategy, batch=150, brick_proj=16) as b:
+            b.insert(imgs, ang)
+            X, W = b.export_raw(0)
+            out[strategy] = (X.cpu().numpy(), W.cpu().numpy(), b.finalize()["full"].cpu().numpy())
+    (Xd, Wd, vd), (Xb, Wb, vb) = out["direct"], out["brick"]
+    assert np.array_equal(Wd > 0, Wb > 0)
+    assert rel(Wb, Wd) < 1e-6 and rel(Xb, Xd) < 1e-6 and rel(vb, vd) < 1e-6
