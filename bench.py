@@ -93,6 +93,8 @@ def main():
     ap.add_argument("--config", type=int, default=2, choices=[1, 2, 5])
     ap.add_argument("--nproj", type=int, default=None, help="override the projection count")
     ap.add_argument("--batch", type=int, default=0, help="projections per internal chunk (0 = auto)")
+    ap.add_argument("--strategy", default="auto", choices=["auto", "direct", "brick"])
+    ap.add_argument("--brick-proj", type=int, default=0, help="projections per brick work item (0 = auto)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--parity-proj", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
@@ -120,7 +122,7 @@ def main():
     ang, ph, imgs = synth.make_dataset(args.config, nproj=nproj, device="cuda")
     torch.cuda.synchronize()
 
-    b = bpmod.Backprojector(nx, timing=True, batch=args.batch)
+    b = bpmod.Backprojector(nx, timing=True, batch=args.batch, strategy=args.strategy, brick_proj=args.brick_proj)
 
     def step():
         b.reset()
@@ -145,6 +147,7 @@ def main():
     pad_ms, _, _ = b.kernel_stats(_lib.K_PAD)
     fft_ms, _, _ = b.kernel_stats(_lib.K_FFT2)
     fin_ms, nfin, _ = b.kernel_stats(_lib.K_FINALIZE)
+    bin_ms, _, _ = b.kernel_stats(_lib.K_BIN)
     b.close()
 
     U = distinct_voxels(bpmod, nx, synth.uniform_angles(16, np.random.default_rng(5)))
@@ -154,7 +157,7 @@ def main():
     achieved = b_alg * per_launch / avg_launch_s / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-            "kernel": "insert_direct_kernel", "bytes_per_unit": round(b_alg), "unit_of_work": "projection",
+            "kernel": "insert_direct_kernel" if args.strategy == "direct" else "insert_brick_kernel<16,16,8>", "bytes_per_unit": round(b_alg), "unit_of_work": "projection",
             "units_per_launch": round(per_launch, 1), "avg_launch_ms": round(avg_launch_s * 1e3, 4),
             "U_distinct_voxels": round(U, 1)}
 
@@ -166,7 +169,7 @@ def main():
                                   f"{2 * nx}^3 padded Fourier grid (config {args.config})",
                       "nx": nx, "nproj": nproj, "grid": 2 * nx, "images_dtype": "f32"},
            "breakdown_ms_per_step": {"pad": round(pad_ms / args.steps, 3), "fft2": round(fft_ms / args.steps, 3),
-                                     "insert": round(ins_ms / args.steps, 3),
+                                     "insert": round(ins_ms / args.steps, 3), "bin": round(bin_ms / args.steps, 3),
                                      "finalize": round(fin_ms / max(nfin, 1), 3)},
            "roofline": roof}
     if not args.no_parity:

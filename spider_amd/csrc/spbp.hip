@@ -45,6 +45,8 @@ using namespace spbp;
 
 namespace {
 thread_local std::string g_last_error;
+// brick of the LDS-binned strategy (voxels); 3 x 8 B per voxel in LDS
+constexpr int BRICK_X = 16, BRICK_Y = 16, BRICK_Z = 8;
 }  // namespace
 
 // ------------------------------------------------------------------ context
@@ -321,7 +323,7 @@ int insert_impl(spbp_ctx* c, const void* imgs, int dtype, int nimg, const float*
             }
             HIPCHK(c, hipGetLastError());
             Timed t(c, SPBP_K_INSERT, b, st);
-            insert_brick_kernel<16, 16, 16><<<c->ncu * 2, TPB, 0, st>>>(
+            insert_brick_kernel<BRICK_X, BRICK_Y, BRICK_Z><<<c->ncu * 3, TPB, 0, st>>>(
                 c->spec, c->geom_d + p0, nx, N, c->tab, c->fltb, c->bg, c->C, c->bcnt, c->boff, c->bioff,
                 c->blist, c->bnext, c->X[0], c->W[0], c->X[1], c->W[1]);
         }
@@ -405,9 +407,12 @@ int spbp_create(const spbp_opts* o, spbp_ctx** out) {
         int n = 0;
         if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0) c->ncu = n;
     }
-    c->bg.bx = c->bg.by = c->bg.bz = 16;
-    c->bg.nbx = (c->nx + 15) / 16;
-    c->bg.nby = c->bg.nbz = (2 * c->nx - 1 + 15) / 16;
+    c->bg.bx = BRICK_X;
+    c->bg.by = BRICK_Y;
+    c->bg.bz = BRICK_Z;
+    c->bg.nbx = (c->nx + BRICK_X - 1) / BRICK_X;
+    c->bg.nby = (2 * c->nx - 1 + BRICK_Y - 1) / BRICK_Y;
+    c->bg.nbz = (2 * c->nx - 1 + BRICK_Z - 1) / BRICK_Z;
     c->bg.nb = c->bg.nbx * c->bg.nby * c->bg.nbz;
     int rc = SPBP_OK;
     auto bail = [&](int code) { spbp_destroy(c); return code; };
