@@ -213,11 +213,17 @@ __global__ __launch_bounds__(TPB) void brick_fill_kernel(const Geom* __restrict_
     }
 }
 
+// item -> key map: item_key[i] = key for i in [item_off[k], item_off[k+1]).
+__global__ __launch_bounds__(TPB) void brick_itemkey_kernel(const int* __restrict__ item_off, int nkeys,
+                                                            int* __restrict__ item_key) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nkeys) return;
+    for (int i = item_off[k]; i < item_off[k + 1]; ++i) item_key[i] = k;
+}
+
 template <int BX, int BY, int BZ>
 struct BrickLds {
-    // the lattice lines crossing the expanded box, (max edge + 2 margins) * sqrt(3) + 2,
-    // must fit the 64-entry line table
-    static_assert(BX <= 31 && BY <= 31 && BZ <= 31, "brick edge too long for the line table");
+    static_assert(BX <= 31 && BY <= 31 && BZ <= 31, "brick edge too long for the 64-lane line table");
     // padded strides: neighbouring samples hit voxels one step apart in any axis,
     // which must fall in different LDS banks
     static constexpr int SY = BX + 1;
@@ -227,81 +233,28 @@ struct BrickLds {
     // ds_add_f32 (0.26 vs ~3 cycles per lane measured), and each per-tap term
     // stays the reference's fp32 product, so the sum is at least as accurate.
     double re[NS], im[NS], w[NS];
-    int line_lo[64];
-    int line_pref[65];
     int item;
 };
 
-// Taps of one effective sample into the LDS brick [x0,x0+BX) x [y0,..) x [z0,..).
-template <int BX, int BY, int BZ>
-__device__ inline void sample_taps_to_brick(BrickLds<BX, BY, BZ>& L, int x0, int y0, int z0,
-                                            float x, float y, float z, int ix, int iy, int iz, float2 v,
-                                            const float* __restrict__ tab, float fltb) {
-    float tx[5], ty[5], tz[5];
-#pragma unroll
-    for (int l = 0; l < 5; ++l) {
-        tx[l] = kb_weight(tab, __fsub_rn(x, (float)(ix + l - LN2)), fltb);
-        ty[l] = kb_weight(tab, __fsub_rn(y, (float)(iy + l - LN2)), fltb);
-        tz[l] = kb_weight(tab, __fsub_rn(z, (float)(iz + l - LN2)), fltb);
-    }
-    // direct taps p in B, p_x >= 0
-    {
-        const int zl = max(iz - LN2, z0), zh = min(iz + LN2, z0 + BZ - 1);
-        const int yl = max(iy - LN2, y0), yh = min(iy + LN2, y0 + BY - 1);
-        const int xl = max(max(ix - LN2, 0), x0), xh = min(ix + LN2, x0 + BX - 1);
-        for (int pz = zl; pz <= zh; ++pz) {
-            const float t_z = tz[pz - iz + LN2];
-            if (t_z == 0.0f) continue;
-            for (int py = yl; py <= yh; ++py) {
-                const float tyz = __fmul_rn(ty[py - iy + LN2], t_z);
-                if (tyz == 0.0f) continue;
-                const int base = (pz - z0) * L.SZ + (py - y0) * L.SY - x0;
-                for (int px = xl; px <= xh; ++px) {
-                    const float wg = __fmul_rn(tx[px - ix + LN2], tyz);
-                    if (wg == 0.0f) continue;
-                    const int o = base + px;
-                    atomicAdd(&L.re[o], (double)__fmul_rn(v.x, wg));
-                    atomicAdd(&L.im[o], (double)__fmul_rn(v.y, wg));
-                    atomicAdd(&L.w[o], (double)wg);
-                }
-            }
-        }
-    }
-    // mirrored taps: p_x < 0 written conjugated at -p, -p in B
-    if (ix < LN2 && x0 <= LN2) {
-        const int zl = max(iz - LN2, -(z0 + BZ - 1)), zh = min(iz + LN2, -z0);
-        const int yl = max(iy - LN2, -(y0 + BY - 1)), yh = min(iy + LN2, -y0);
-        const int xl = max(ix - LN2, -(x0 + BX - 1)), xh = min(-1, -x0);
-        for (int pz = zl; pz <= zh; ++pz) {
-            const float t_z = tz[pz - iz + LN2];
-            if (t_z == 0.0f) continue;
-            for (int py = yl; py <= yh; ++py) {
-                const float tyz = __fmul_rn(ty[py - iy + LN2], t_z);
-                if (tyz == 0.0f) continue;
-                const int base = (-pz - z0) * L.SZ + (-py - y0) * L.SY - x0;
-                for (int px = xl; px <= xh; ++px) {
-                    const float wg = __fmul_rn(tx[px - ix + LN2], tyz);
-                    if (wg == 0.0f) continue;
-                    const int o = base - px;
-                    atomicAdd(&L.re[o], (double)__fmul_rn(v.x, wg));
-                    atomicAdd(&L.im[o], (double)__fmul_rn(-v.y, wg));
-                    atomicAdd(&L.w[o], (double)wg);
-                }
-            }
-        }
-    }
-}
-
-// Enumerate the effective lattice samples of projection g whose E lies in
-// the box [lo, hi] (and, with `exclude`, not in [elo, ehi]); feed their taps
-// into the brick.  Lines are rows J of the lattice (b), runs are I (a), so
+// One wave feeds the taps of the effective lattice samples of one projection
+// whose E lies in the box [lo, hi] (and, with `exclude`, not in [elo, ehi])
+// into the LDS brick.  Lattice lines are rows J (b) of the sample lattice,
+// one per lane; candidates along a line are consecutive I (a), so
 // neighbouring lanes read neighbouring spectrum entries.
+struct Box {
+    float lo[3], hi[3];
+};
+
 template <int BX, int BY, int BZ>
-__device__ void brick_project(BrickLds<BX, BY, BZ>& L, const Geom& g, const float2* __restrict__ spec_p,
-                              int nx, int N, int x0, int y0, int z0,
-                              const float lo[3], const float hi[3], bool exclude, const float elo[3], const float ehi[3],
-                              const float* __restrict__ tab, float fltb) {
+__device__ __forceinline__ void wave_project(BrickLds<BX, BY, BZ>& L, const Geom& g, const float2* __restrict__ spec_p,
+                                             int nx, int N, int x0, int y0, int z0, const Box box, bool exclude,
+                                             const Box ex, const float* __restrict__ tab, float fltb) {
+    const float* lo = box.lo;
+    const float* hi = box.hi;
+    const float* elo = ex.lo;
+    const float* ehi = ex.hi;
     const int H = nx + 1;
+    const int lane = threadIdx.x & 63;
     // b range: extent of r2 . E over the box
     float bmn = 0.f, bmx = 0.f;
 #pragma unroll
@@ -311,52 +264,48 @@ __device__ void brick_project(BrickLds<BX, BY, BZ>& L, const Geom& g, const floa
         bmx += fmaxf(c * lo[k], c * hi[k]);
     }
     const int b0 = (int)ceilf(bmn - 0.01f), b1 = (int)floorf(bmx + 0.01f);
-    const int nl = min(b1 - b0 + 1, 64);
-    // per line b: a interval with lo <= a r1 + b r2 <= hi
-    if ((int)threadIdx.x < 64) {
-        int cnt = 0, alo = 0;
-        const int t = threadIdx.x;
-        if (t < nl) {
-            const float bb = (float)(b0 + t);
-            float amin = -1e30f, amax = 1e30f;
-            bool empty = false;
+    const int nl = b1 - b0 + 1;   // <= 64: edge + 2 margins, times sqrt(3), plus 2
+    // this lane's line b = b0 + lane: a interval with lo <= a r1 + b r2 <= hi
+    int cnt = 0, alo = 0;
+    if (lane < nl) {
+        const float bb = (float)(b0 + lane);
+        float amin = -1e30f, amax = 1e30f;
+        bool empty = false;
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                const float r1 = g.r[k], off = bb * g.r[3 + k];
-                if (fabsf(r1) > 1e-6f) {
-                    const float u = (lo[k] - off) / r1, w = (hi[k] - off) / r1;
-                    amin = fmaxf(amin, fminf(u, w));
-                    amax = fminf(amax, fmaxf(u, w));
-                } else if (off < lo[k] - 0.01f || off > hi[k] + 0.01f) {
-                    empty = true;
-                }
-            }
-            if (!empty) {
-                const int a0 = (int)ceilf(amin - 0.01f), a1 = (int)floorf(amax + 0.01f);
-                if (a1 >= a0) { alo = a0; cnt = a1 - a0 + 1; }
+        for (int k = 0; k < 3; ++k) {
+            const float r1 = g.r[k], off = bb * g.r[3 + k];
+            if (fabsf(r1) > 1e-6f) {
+                const float u = (lo[k] - off) / r1, w = (hi[k] - off) / r1;
+                amin = fmaxf(amin, fminf(u, w));
+                amax = fminf(amax, fmaxf(u, w));
+            } else if (off < lo[k] - 0.01f || off > hi[k] + 0.01f) {
+                empty = true;
             }
         }
-        // inclusive wave scan of cnt over lanes 0..63
-        int s = cnt;
-        for (int d = 1; d < 64; d <<= 1) {
-            const int u = __shfl_up(s, d, 64);
-            if (t >= d) s += u;
+        if (!empty) {
+            const int a0 = (int)ceilf(amin - 0.01f), a1 = (int)floorf(amax + 0.01f);
+            if (a1 >= a0) { alo = a0; cnt = a1 - a0 + 1; }
         }
-        L.line_lo[t] = alo;
-        L.line_pref[t + 1] = s;
-        if (t == 0) L.line_pref[0] = 0;
     }
-    __syncthreads();
-    const int total = L.line_pref[64];
-    for (int q = threadIdx.x; q < total; q += blockDim.x) {
-        // line of candidate q: largest l with line_pref[l] <= q
-        int lo_i = 0, hi_i = 63;
-        while (lo_i < hi_i) {
-            const int mid = (lo_i + hi_i + 1) >> 1;
-            if (L.line_pref[mid] <= q) lo_i = mid; else hi_i = mid - 1;
+    int pref = cnt;   // inclusive scan over the wave
+    for (int d = 1; d < 64; d <<= 1) {
+        const int u = __shfl_up(pref, d, 64);
+        if (lane >= d) pref += u;
+    }
+    const int total = __shfl(pref, 63, 64);
+    for (int q0 = 0; q0 < total; q0 += 64) {
+        const int q = q0 + lane;
+        // line of candidate q: smallest l with pref[l] > q (every lane searches)
+        int l = 0;
+#pragma unroll
+        for (int step = 32; step > 0; step >>= 1) {
+            const int t = __shfl(pref, l + step - 1, 64);
+            if (t <= q) l += step;
         }
-        const int bb = b0 + lo_i;
-        const int aa = L.line_lo[lo_i] + (q - L.line_pref[lo_i]);
+        const int lpref = __shfl(pref, l, 64), lcnt = __shfl(cnt, l, 64), lalo = __shfl(alo, l, 64);
+        if (q >= total) continue;
+        const int bb = b0 + l;
+        const int aa = lalo + (q - (lpref - lcnt));
         // canonical sample (I, J) of lattice point (aa, bb): the half plane I > 0 or I = 0, J >= 0
         const bool canon = aa > 0 || (aa == 0 && bb >= 0);
         const int I = canon ? aa : -aa, J = canon ? bb : -bb;
@@ -369,72 +318,118 @@ __device__ void brick_project(BrickLds<BX, BY, BZ>& L, const Geom& g, const floa
         if (x < lo[0] || x > hi[0] || y < lo[1] || y > hi[1] || z < lo[2] || z > hi[2]) continue;
         if (exclude && x >= elo[0] && x <= ehi[0] && y >= elo[1] && y <= ehi[1] && z >= elo[2] && z <= ehi[2]) continue;
         const int ix = nearest_voxel(x, N), iy = nearest_voxel(y, N), iz = nearest_voxel(z, N);
-        if (!in_bounds(ix, iy, iz, nx)) continue;
+        if (!in_bounds(ix, iy, iz, nx)) continue;                      // oneline.f:73-75
         float2 v = spectrum_value(spec_p, I, J, N, H);
         if (flip) v.y = -v.y;
-        sample_taps_to_brick<BX, BY, BZ>(L, x0, y0, z0, x, y, z, ix, iy, iz, v, tab, fltb);
+        // The nonzero taps of each axis are a run of at most 4 of the 5
+        // reference taps i-2..i+2 (the table vanishes beyond 1.8 voxels, and
+        // i-2 and i+2 are 4 apart): start at i-2 when its weight is nonzero.
+        float tx[4], ty[4], tz[4];
+        const float wx0 = kb_weight(tab, __fsub_rn(x, (float)(ix - 2)), fltb);
+        const float wy0 = kb_weight(tab, __fsub_rn(y, (float)(iy - 2)), fltb);
+        const float wz0 = kb_weight(tab, __fsub_rn(z, (float)(iz - 2)), fltb);
+        const int sx = wx0 != 0.0f ? ix - 2 : ix - 1;
+        const int sy = wy0 != 0.0f ? iy - 2 : iy - 1;
+        const int sz = wz0 != 0.0f ? iz - 2 : iz - 1;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            tx[t] = (t == 0 && sx == ix - 2) ? wx0 : kb_weight(tab, __fsub_rn(x, (float)(sx + t)), fltb);
+            ty[t] = (t == 0 && sy == iy - 2) ? wy0 : kb_weight(tab, __fsub_rn(y, (float)(sy + t)), fltb);
+            tz[t] = (t == 0 && sz == iz - 2) ? wz0 : kb_weight(tab, __fsub_rn(z, (float)(sz + t)), fltb);
+        }
+        const bool mirror_possible = sx < 0 && x0 <= LN2;
+#pragma unroll
+        for (int lz = 0; lz < 4; ++lz) {
+            const int pz = sz + lz;
+#pragma unroll
+            for (int ly = 0; ly < 4; ++ly) {
+                const int py = sy + ly;
+                const float tyz = __fmul_rn(ty[ly], tz[lz]);       // TY = TABI(Y) * TZ
+#pragma unroll
+                for (int lx = 0; lx < 4; ++lx) {
+                    const int px = sx + lx;
+                    const float wg = __fmul_rn(tx[lx], tyz);       // WG = TABI(X) * TY
+                    if (wg == 0.0f) continue;
+                    if (px >= 0) {                                  // direct tap
+                        if ((unsigned)(px - x0) < (unsigned)BX && (unsigned)(py - y0) < (unsigned)BY &&
+                            (unsigned)(pz - z0) < (unsigned)BZ) {
+                            const int o = (pz - z0) * L.SZ + (py - y0) * L.SY + (px - x0);
+                            atomicAdd(&L.re[o], (double)__fmul_rn(v.x, wg));
+                            atomicAdd(&L.im[o], (double)__fmul_rn(v.y, wg));
+                            atomicAdd(&L.w[o], (double)wg);
+                        }
+                    } else if (mirror_possible) {                   // reflected tap at -p, conjugated
+                        if ((unsigned)(-px - x0) < (unsigned)BX && (unsigned)(-py - y0) < (unsigned)BY &&
+                            (unsigned)(-pz - z0) < (unsigned)BZ) {
+                            const int o = (-pz - z0) * L.SZ + (-py - y0) * L.SY + (-px - x0);
+                            atomicAdd(&L.re[o], (double)__fmul_rn(v.x, wg));
+                            atomicAdd(&L.im[o], (double)__fmul_rn(-v.y, wg));
+                            atomicAdd(&L.w[o], (double)wg);
+                        }
+                    }
+                }
+            }
+        }
     }
-    __syncthreads();
 }
 
-// Persistent workgroups pull (key, projection slice) items from a queue.
+// Persistent workgroups pull (key, projection slice) items from a queue; the
+// four waves of a workgroup take the item's projections in turn and share the
+// brick through LDS atomics.
 template <int BX, int BY, int BZ>
 __global__ __launch_bounds__(TPB) void insert_brick_kernel(
     const float2* __restrict__ spec, const Geom* __restrict__ geom, int nx, int N,
     const float* __restrict__ tab, float fltb, BrickGrid bg, int C,
     const int* __restrict__ cnt, const int* __restrict__ off, const int* __restrict__ item_off,
-    const int* __restrict__ list, int* __restrict__ next_item,
+    const int* __restrict__ item_key, const int* __restrict__ list, int* __restrict__ next_item,
     float2* __restrict__ X0, float* __restrict__ W0, float2* __restrict__ X1, float* __restrict__ W1) {
     constexpr int NV = BX * BY * BZ;
     __shared__ BrickLds<BX, BY, BZ> L;
     const int nkeys = 2 * bg.nb;
     const int total = item_off[nkeys];
     const int H = nx + 1;
+    const int wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
     for (;;) {
         if (threadIdx.x == 0) L.item = atomicAdd(next_item, 1);
         __syncthreads();
         const int item = L.item;
         __syncthreads();
         if (item >= total) break;
-        // key: largest k with item_off[k] <= item
-        int lo_k = 0, hi_k = nkeys - 1;
-        while (lo_k < hi_k) {
-            const int mid = (lo_k + hi_k + 1) >> 1;
-            if (item_off[mid] <= item) lo_k = mid; else hi_k = mid - 1;
-        }
-        const int key = lo_k;
+        const int key = item_key[item];
         const int b = key % bg.nb, half = key / bg.nb;
         const int s0 = (item - item_off[key]) * C;
         const int ns = min(C, cnt[key] - s0);
+        const int* lst = list + off[key] + s0;
         int x0, y0, z0;
         brick_origin(bg, b, nx, x0, y0, z0);
         for (int i = threadIdx.x; i < L.NS; i += blockDim.x) { L.re[i] = 0.0; L.im[i] = 0.0; L.w[i] = 0.0; }
         __syncthreads();
-        const float lo[3] = {x0 - BRICK_MARGIN, y0 - BRICK_MARGIN, z0 - BRICK_MARGIN};
-        const float hi[3] = {x0 + BX - 1 + BRICK_MARGIN, y0 + BY - 1 + BRICK_MARGIN, z0 + BZ - 1 + BRICK_MARGIN};
-        const float mlo[3] = {-hi[0], -hi[1], -hi[2]};
-        const float mhi[3] = {-lo[0], -lo[1], -lo[2]};
-        for (int s = 0; s < ns; ++s) {
-            const int p = list[off[key] + s0 + s];
+        Box bx;
+        bx.lo[0] = x0 - BRICK_MARGIN; bx.lo[1] = y0 - BRICK_MARGIN; bx.lo[2] = z0 - BRICK_MARGIN;
+        bx.hi[0] = x0 + BX - 1 + BRICK_MARGIN; bx.hi[1] = y0 + BY - 1 + BRICK_MARGIN; bx.hi[2] = z0 + BZ - 1 + BRICK_MARGIN;
+        Box mb;
+        for (int k = 0; k < 3; ++k) { mb.lo[k] = -bx.hi[k]; mb.hi[k] = -bx.lo[k]; }
+        for (int s = wave; s < ns; s += nwave) {
+            const int p = lst[s];
             const Geom g = geom[p];
             const float2* spec_p = spec + (size_t)p * N * H;
-            brick_project<BX, BY, BZ>(L, g, spec_p, nx, N, x0, y0, z0, lo, hi, false, lo, hi, tab, fltb);
+            wave_project<BX, BY, BZ>(L, g, spec_p, nx, N, x0, y0, z0, bx, false, bx, tab, fltb);
             if (x0 <= LN2)
-                brick_project<BX, BY, BZ>(L, g, spec_p, nx, N, x0, y0, z0, mlo, mhi, true, lo, hi, tab, fltb);
+                wave_project<BX, BY, BZ>(L, g, spec_p, nx, N, x0, y0, z0, mb, true, bx, tab, fltb);
         }
+        __syncthreads();
         // add the brick to the half grid
         float2* __restrict__ X = half ? X1 : X0;
         float* __restrict__ W = half ? W1 : W0;
         for (int v = threadIdx.x; v < NV; v += blockDim.x) {
             const int lx = v % BX, ly = (v / BX) % BY, lz = v / (BX * BY);
             const int i = lz * L.SZ + ly * L.SY + lx;
-            const float w = (float)L.w[i];
             if (L.w[i] == 0.0) continue;
             const int kx = x0 + lx, ky = y0 + ly, kz = z0 + lz;
             const size_t o = ((size_t)(kz >= 0 ? kz : N + kz) * N + (ky >= 0 ? ky : N + ky)) * H + kx;
             unsafeAtomicAdd(&X[o].x, (float)L.re[i]);
             unsafeAtomicAdd(&X[o].y, (float)L.im[i]);
-            unsafeAtomicAdd(&W[o], w);
+            unsafeAtomicAdd(&W[o], (float)L.w[i]);
         }
         __syncthreads();
     }

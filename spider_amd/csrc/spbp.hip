@@ -91,6 +91,7 @@ struct spbp_ctx {
     int* bcur = nullptr;    // [2 nb] fill cursors
     int* bnext = nullptr;   // work-queue head
     int* blist = nullptr;   // projection lists
+    int* bikey = nullptr;   // work item -> key
     size_t blist_cap = 0;
     // SPBP_TIMING: (start, stop, units) per recorded launch, per kernel kind
     struct Rec { hipEvent_t a, b; long units; };
@@ -178,7 +179,10 @@ int ensure_chunk(spbp_ctx* c, int want_batch) {
     if (c->strategy == SPBP_STRAT_BRICK) {
         const size_t cap = (size_t)B * c->bg.nb;
         if (c->blist) { (void)hipFree(c->blist); c->blist = nullptr; }
+        if (c->bikey) { (void)hipFree(c->bikey); c->bikey = nullptr; }
         HIPCHK(c, hipMalloc(&c->blist, cap * sizeof(int)));
+        // items <= sum over keys of ceil(cnt / C) <= pairs + keys
+        HIPCHK(c, hipMalloc(&c->bikey, (cap + 2 * (size_t)c->bg.nb) * sizeof(int)));
         c->blist_cap = cap;
     }
     int n[2] = {N, N};
@@ -320,12 +324,13 @@ int insert_impl(spbp_ctx* c, const void* imgs, int dtype, int nimg, const float*
                 brick_count_kernel<<<lgrid, TPB, 0, st>>>(c->geom_d + p0, b, c->bg, nx, c->bcnt);
                 brick_scan_kernel<<<1, 1024, 0, st>>>(c->bcnt, nkeys, c->C, c->boff, c->bioff, c->bcur, c->bnext);
                 brick_fill_kernel<<<lgrid, TPB, 0, st>>>(c->geom_d + p0, b, c->bg, nx, c->boff, c->bcur, c->blist);
+                brick_itemkey_kernel<<<nblocks(nkeys), TPB, 0, st>>>(c->bioff, nkeys, c->bikey);
             }
             HIPCHK(c, hipGetLastError());
             Timed t(c, SPBP_K_INSERT, b, st);
             insert_brick_kernel<BRICK_X, BRICK_Y, BRICK_Z><<<c->ncu * 3, TPB, 0, st>>>(
                 c->spec, c->geom_d + p0, nx, N, c->tab, c->fltb, c->bg, c->C, c->bcnt, c->boff, c->bioff,
-                c->blist, c->bnext, c->X[0], c->W[0], c->X[1], c->W[1]);
+                c->bikey, c->blist, c->bnext, c->X[0], c->W[0], c->X[1], c->W[1]);
         }
         HIPCHK(c, hipGetLastError());
     }
@@ -451,7 +456,7 @@ int spbp_destroy(spbp_ctx* c) {
         if (c->W[h]) (void)hipFree(c->W[h]);
     }
     void* dev_ptrs[] = {c->tab, c->padbuf, c->spec, c->geom_d, c->work, c->real, c->wkb, c->psum, c->pcnt, c->mean, c->flag,
-                        c->bcnt, c->boff, c->bioff, c->bcur, c->bnext, c->blist};
+                        c->bcnt, c->boff, c->bioff, c->bcur, c->bnext, c->blist, c->bikey};
     for (void* p : dev_ptrs) if (p) (void)hipFree(p);
     if (c->geom_h) (void)hipHostFree(c->geom_h);
     if (c->stage) (void)hipHostFree(c->stage);
