@@ -337,35 +337,41 @@ __device__ __forceinline__ void wave_project(BrickLds<BX, BY, BZ>& L, const Geom
             ty[t] = (t == 0 && sy == iy - 2) ? wy0 : kb_weight(tab, __fsub_rn(y, (float)(sy + t)), fltb);
             tz[t] = (t == 0 && sz == iz - 2) ? wz0 : kb_weight(tab, __fsub_rn(z, (float)(sz + t)), fltb);
         }
-        const bool mirror_possible = sx < 0 && x0 <= LN2;
+        // Targets inside the brick, per axis: a tap p with p_x >= 0 is written at
+        // p, a tap with p_x < 0 at -p (conjugated).  Offsets are brick-local;
+        // out-of-brick positions get a flag instead of a branch.
+        int xo[4];
+        bool xok[4], xmir[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int px = sx + t;
+            xmir[t] = px < 0;
+            xo[t] = (xmir[t] ? -px : px) - x0;
+            xok[t] = (unsigned)xo[t] < (unsigned)BX;
+        }
+        const float vim = -v.y;
 #pragma unroll
         for (int lz = 0; lz < 4; ++lz) {
             const int pz = sz + lz;
+            const int zd = pz - z0, zm = -pz - z0;
+            const bool zdok = (unsigned)zd < (unsigned)BZ, zmok = (unsigned)zm < (unsigned)BZ;
 #pragma unroll
             for (int ly = 0; ly < 4; ++ly) {
                 const int py = sy + ly;
+                const int yd = py - y0, ym = -py - y0;
+                const bool dok = zdok && (unsigned)yd < (unsigned)BY;
+                const bool mok = zmok && (unsigned)ym < (unsigned)BY;
+                const int offd = zd * L.SZ + yd * L.SY, offm = zm * L.SZ + ym * L.SY;
                 const float tyz = __fmul_rn(ty[ly], tz[lz]);       // TY = TABI(Y) * TZ
 #pragma unroll
                 for (int lx = 0; lx < 4; ++lx) {
-                    const int px = sx + lx;
                     const float wg = __fmul_rn(tx[lx], tyz);       // WG = TABI(X) * TY
-                    if (wg == 0.0f) continue;
-                    if (px >= 0) {                                  // direct tap
-                        if ((unsigned)(px - x0) < (unsigned)BX && (unsigned)(py - y0) < (unsigned)BY &&
-                            (unsigned)(pz - z0) < (unsigned)BZ) {
-                            const int o = (pz - z0) * L.SZ + (py - y0) * L.SY + (px - x0);
-                            atomicAdd(&L.re[o], (double)__fmul_rn(v.x, wg));
-                            atomicAdd(&L.im[o], (double)__fmul_rn(v.y, wg));
-                            atomicAdd(&L.w[o], (double)wg);
-                        }
-                    } else if (mirror_possible) {                   // reflected tap at -p, conjugated
-                        if ((unsigned)(-px - x0) < (unsigned)BX && (unsigned)(-py - y0) < (unsigned)BY &&
-                            (unsigned)(-pz - z0) < (unsigned)BZ) {
-                            const int o = (-pz - z0) * L.SZ + (-py - y0) * L.SY + (-px - x0);
-                            atomicAdd(&L.re[o], (double)__fmul_rn(v.x, wg));
-                            atomicAdd(&L.im[o], (double)__fmul_rn(-v.y, wg));
-                            atomicAdd(&L.w[o], (double)wg);
-                        }
+                    const bool ok = xok[lx] && (xmir[lx] ? mok : dok) && wg != 0.0f;
+                    if (ok) {
+                        const int o = (xmir[lx] ? offm : offd) + xo[lx];
+                        atomicAdd(&L.re[o], (double)__fmul_rn(v.x, wg));
+                        atomicAdd(&L.im[o], (double)__fmul_rn(xmir[lx] ? vim : v.y, wg));
+                        atomicAdd(&L.w[o], (double)wg);
                     }
                 }
             }
