@@ -72,6 +72,22 @@ def cpu_baseline(imgs_host, ang, seconds):
                       f"{t:.1f} s on {threads} threads; oracle FFT is its own double-precision FFT"}
 
 
+def measured_traffic(kernel, nx, per_launch):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE measurement (profiles/traffic.json), scaled to this launch size;
+    None when no measurement matches."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as fh:
+            t = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    for rec in t.get("records", []):
+        if rec.get("kernel") == kernel and rec.get("nx") == nx:
+            return round(rec["bytes_per_unit"] * per_launch)
+    return None
+
+
 def parity_subset(bpmod, imgs, ang, m):
     import numpy as np
     import oracle
@@ -97,6 +113,8 @@ def main():
     ap.add_argument("--brick-proj", type=int, default=0, help="projections per brick work item (0 = auto)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--parity-proj", type=int, default=100)
+    ap.add_argument("--copy-calib", type=int, default=0,
+                    help="MiB: run one device-to-device copy of this size first (calibrates PMC byte counters)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
     args = ap.parse_args()
@@ -121,6 +139,14 @@ def main():
     log(f"bench: generating config {args.config}: {nproj} projections {nx}x{nx}")
     ang, ph, imgs = synth.make_dataset(args.config, nproj=nproj, device="cuda")
     torch.cuda.synchronize()
+
+    if args.copy_calib:
+        src = torch.empty(args.copy_calib << 20, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        log(f"bench: calibration copy of {args.copy_calib} MiB done")
 
     b = bpmod.Backprojector(nx, timing=True, batch=args.batch, strategy=args.strategy, brick_proj=args.brick_proj)
 
@@ -155,9 +181,10 @@ def main():
     avg_launch_s = ins_ms / 1e3 / launches
     per_launch = units / launches
     achieved = b_alg * per_launch / avg_launch_s / 1e9
+    kname = "insert_direct_kernel" if args.strategy == "direct" else "insert_brick_kernel<16,16,8>"
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-            "kernel": "insert_direct_kernel" if args.strategy == "direct" else "insert_brick_kernel<16,16,8>", "bytes_per_unit": round(b_alg), "unit_of_work": "projection",
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(kname, nx, per_launch),
+            "kernel": kname, "bytes_per_unit": round(b_alg), "unit_of_work": "projection",
             "units_per_launch": round(per_launch, 1), "avg_launch_ms": round(avg_launch_s * 1e3, 4),
             "U_distinct_voxels": round(U, 1)}
 
