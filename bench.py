@@ -207,7 +207,7 @@ def main():
         out["parity_subset_proj"] = min(args.parity_proj, nproj)
     if not args.no_cpu:
         log("bench: cpu baseline")
-        m = min(nproj, 4096)
+        m = min(nproj, 20000)
         out["cpu_baseline"] = cpu_baseline(imgs[:m].cpu().numpy(), ang[:m], args.cpu_seconds)
         out["gpu_over_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
     print(json.dumps(out), flush=True)

@@ -218,3 +218,70 @@ def bp32f(imgs, angles, half=None, seed=None, exact_pi: bool = False, **kw):
         bp.insert(imgs, angles, half=half)
         out = bp.finalize(full=True, halves=True)
     return out["full"], out["h1"], out["h2"]
+
+
+# ------------------------------------------------------------------ file-level operations
+def _load_series(template, numbers, angdoc, symdoc, ext, device):
+    from . import spiderio
+    torch = _torch()
+    if symdoc not in (None, "*", ""):
+        raise NotImplementedError("symmetry documents are not supported (answer '*')")
+    imgs = spiderio.read_images(template, numbers, ext)
+    if imgs.shape[1] != imgs.shape[2]:
+        raise SpbpError(_lib.ERR_SIZE, "ONLY WORKS ON SQUARE IMAGES")
+    try:
+        ang = spiderio.angles_for(spiderio.read_doc(angdoc), numbers)
+    except KeyError as e:
+        raise SpbpError(_lib.ERR_ANGLE, str(e)) from None
+    dev = torch.device("cuda", torch.cuda.current_device() if device is None else int(device))
+    return torch.from_numpy(imgs).to(dev), ang
+
+
+def bp3f_files(template, numbers, angdoc, symdoc, out, ext: str = "", device=None, **kw):
+    """`BP 3F`: template for 2D images, file numbers, angles doc, symmetries
+    doc ('*'), reconstructed volume (the answers of bp3f.f:127-283)."""
+    from . import spiderio
+    imgs, ang = _load_series(template, numbers, angdoc, symdoc, ext, device)
+    vol = bp3f(imgs, ang, **kw).cpu().numpy()
+    spiderio.write_image(out, vol)
+    return vol
+
+
+def bp32f_files(template, numbers, angdoc, symdoc, out_full, out_h1, out_h2, ext: str = "", seed=None,
+                half=None, device=None, **kw):
+    """`BP 32F`: as BP 3F plus the two sample volumes (bp32f.f:102-410)."""
+    from . import spiderio
+    imgs, ang = _load_series(template, numbers, angdoc, symdoc, ext, device)
+    full, h1, h2 = bp32f(imgs, ang, half=half, seed=seed, **kw)
+    vols = [v.cpu().numpy() for v in (full, h1, h2)]
+    for path, v in zip((out_full, out_h1, out_h2), vols):
+        spiderio.write_image(path, v)
+    return vols
+
+
+def bp3f_partial_files(template, numbers, angdoc, symdoc, out, ext: str = "", device=None, **kw):
+    """`BP 3F P`: write the post-SYMPLANE0 Fourier half grid and its weights
+    (bp3f.f:191-248) instead of a volume."""
+    from . import spiderio
+    imgs, ang = _load_series(template, numbers, angdoc, symdoc, ext, device)
+    with Backprojector(imgs.shape[-1], device=imgs.device.index, **kw) as b:
+        b.insert(imgs, ang)
+        X, W = b.export_partial()
+        return spiderio.write_partial(out, X.cpu().numpy(), W.cpu().numpy())
+
+
+def bp3f_merge_files(partials, out, device=None, **kw):
+    """`BP 3F M`: sum partial volumes and finalise (bp3f.f:531-739)."""
+    from . import spiderio
+    torch = _torch()
+    pairs = [spiderio.read_partial(p) for p in partials]
+    if not pairs:
+        raise SpbpError(_lib.ERR_ARG, "no partial volumes")
+    nx = pairs[0][1].shape[2] - 1
+    dev = torch.device("cuda", torch.cuda.current_device() if device is None else int(device))
+    with Backprojector(nx, device=dev.index, **kw) as b:
+        for X, W in pairs:
+            b.import_partial(torch.from_numpy(X).to(dev), torch.from_numpy(W).to(dev), accumulate=True)
+        vol = b.finalize()["full"].cpu().numpy()
+    spiderio.write_image(out, vol)
+    return vol

@@ -23,6 +23,24 @@ HIPCC_FLAGS = [
 ]
 
 
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+FORTRAN_SOURCES = [os.path.join(HERE, "fortran", "spbp_mod.f90"), os.path.join(HERE, "fortran", "bp_drv.f90")]
+DRIVER = os.path.join(HERE, "spbp_bp")
+
+
+def build_fortran(verbose: bool = False) -> str:
+    """The `BP 3F` / `BP 32F` Fortran driver (ISO_C_BINDING on libspbp)."""
+    mod_dir = os.path.join(HERE, "fortran", "_build")
+    os.makedirs(mod_dir, exist_ok=True)
+    cmd = [FLANG, "-O2", "-J", mod_dir, "-o", DRIVER + ".tmp", *FORTRAN_SOURCES,
+           "-L" + HERE, "-lspbp", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib"]
+    if verbose:
+        print(" ".join(cmd), file=sys.stderr)
+    subprocess.check_call(cmd)
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
+
+
 def build(verbose: bool = False) -> str:
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     cmd = [hipcc, *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), "-o", LIB + ".tmp", *SOURCES,
@@ -36,3 +54,4 @@ def build(verbose: bool = False) -> str:
 
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_fortran(verbose=True))
