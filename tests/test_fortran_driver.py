@@ -87,14 +87,14 @@ def test_python_file_operations(tmp_path):
     doc = spiderio.read_doc(os.path.join(d, "ang.dat"))
     ang_doc = spiderio.angles_for(doc, range(1, m + 1))   # registers as stored (1PGE12.5)
     v = bp.bp3f_files(os.path.join(d, "stk@***"), f"1-{m}", os.path.join(d, "ang.dat"), "*",
-                      os.path.join(d, "vol.dat"))
+                      os.path.join(d, "vol.dat"), ext=".dat")
     import oracle
     ref = oracle.bp(imgs, ang_doc, prec="f32")["full"]
     assert np.linalg.norm(v - ref) / np.linalg.norm(ref) < 1e-5
     np.testing.assert_array_equal(spiderio.read_image(os.path.join(d, "vol.dat")), v)
     p1 = os.path.join(d, "part1.dat")
     p2 = os.path.join(d, "part2.dat")
-    bp.bp3f_partial_files(os.path.join(d, "stk@***"), "1-17", os.path.join(d, "ang.dat"), "*", p1)
-    bp.bp3f_partial_files(os.path.join(d, "stk@***"), f"18-{m}", os.path.join(d, "ang.dat"), "*", p2)
+    bp.bp3f_partial_files(os.path.join(d, "stk@***"), "1-17", os.path.join(d, "ang.dat"), "*", p1, ext=".dat")
+    bp.bp3f_partial_files(os.path.join(d, "stk@***"), f"18-{m}", os.path.join(d, "ang.dat"), "*", p2, ext=".dat")
     vm = bp.bp3f_merge_files([p1, p2], os.path.join(d, "merged.dat"))
     assert np.linalg.norm(vm - v) / np.linalg.norm(v) < 1e-6
