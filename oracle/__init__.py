@@ -74,6 +74,7 @@ def lib():
         L.or_oneline_f32.argtypes = [C.c_int, C.c_int, _f32p, _f32p, _f32p, _f32p, _f32p, C.c_float]
         L.or_insert_f32.argtypes = [C.c_int, _f32p, _f32p, _f32p, _f32p, _f32p, C.c_float]
         L.or_insert_f64.argtypes = [C.c_int, _f64p, _f64p, _f64p, _f64p, _f64p, C.c_double]
+        L.or_insert_f32_acc64.argtypes = [C.c_int, _f64p, _f64p, _f32p, _f32p, _f32p, C.c_float]
         L.or_symplane0_f32.argtypes = [C.c_int, _f32p, _f32p]
         L.or_symplane0_f64.argtypes = [C.c_int, _f64p, _f64p]
         L.or_normalize_f32.argtypes = [C.c_int, _f32p, _f32p]
@@ -201,6 +202,13 @@ def insert(X, W, bi, dm, tab, fltb, prec="f32") -> None:
     dt = _dt(prec)
     getattr(lib(), "or_insert_" + prec)(nx, X.view(dt), W, bi.view(dt),
                                         np.ascontiguousarray(dm, dt), tab, dt(fltb))
+
+
+def insert_f32_acc64(X, W, bi, dm, tab, fltb) -> None:
+    """Reference geometry and fp32 per-tap terms accumulated in FP64 (X complex128, W float64)."""
+    nx = X.shape[2] - 1
+    lib().or_insert_f32_acc64(nx, X.view(np.float64), W, bi.view(np.float32),
+                              np.ascontiguousarray(dm, np.float32), tab, np.float32(fltb))
 
 
 def oneline_f32(J, X, W, bi, dm, tab, fltb) -> None:

@@ -368,8 +368,10 @@ void spectrum(const float* img, int nx, R* bi) {
 // oneline.f:38-158, argument names as in the caller: N = padded size (N2 = 2 NX),
 // nx = image size.  X(0:NX,N,N) complex, W(0:NX,N,N) real, BI(0:NX,N) complex,
 // dm = first six entries of the CANG matrix.
-template <typename R>
-void oneline(int J, int N, int nx, R* X, R* W, const R* BI, const R* dm, const R* tab, R fltb) {
+// A: accumulation type (R for the restatement; double with R = float gives
+// the reference's geometry and per-tap terms summed in FP64).
+template <typename R, typename A = R>
+void oneline(int J, int N, int nx, A* X, A* W, const R* BI, const R* dm, const R* tab, R fltb) {
     typedef Num<R> M;
     const int H = nx + 1;
     const int jp = (J >= 0) ? J : N + J;
@@ -401,9 +403,9 @@ void oneline(int J, int N, int nx, R* X, R* W, const R* BI, const R* dm, const R
                         const R wg = tab[M::nint(M::fabs_(x - R(ixp)) * fltb)] * ty;
                         if (wg != R(0)) {
                             const size_t o = ((size_t)iza * N + iya) * H + ixp;
-                            X[2 * o] = X[2 * o] + vr * wg;
-                            X[2 * o + 1] = X[2 * o + 1] + vi * wg;
-                            W[o] = W[o] + wg;
+                            X[2 * o] = X[2 * o] + (A)(vr * wg);
+                            X[2 * o + 1] = X[2 * o + 1] + (A)(vi * wg);
+                            W[o] = W[o] + (A)wg;
                         }
                     }
                 }
@@ -427,9 +429,9 @@ void oneline(int J, int N, int nx, R* X, R* W, const R* BI, const R* dm, const R
                         const R wg = tab[M::nint(M::fabs_(x - R(ixp)) * fltb)] * ty;
                         if (wg != R(0)) {
                             const size_t o = ((size_t)izt * N + iyt) * H + (-ixp);
-                            X[2 * o] = X[2 * o] + vr * wg;
-                            X[2 * o + 1] = X[2 * o + 1] + (-vi) * wg;
-                            W[o] = W[o] + wg;
+                            X[2 * o] = X[2 * o] + (A)(vr * wg);
+                            X[2 * o + 1] = X[2 * o + 1] + (A)((-vi) * wg);
+                            W[o] = W[o] + (A)wg;
                         }
                     }
                 }
@@ -667,6 +669,11 @@ void or_insert_f32(int nx, float* X, float* W, const float* bi, const float* dm,
 }
 void or_insert_f64(int nx, double* X, double* W, const double* bi, const double* dm, const double* tab, double fltb) {
     insert_projection<double>(nx, X, W, bi, dm, tab, fltb);
+}
+// reference geometry and fp32 per-tap terms, FP64 accumulation
+void or_insert_f32_acc64(int nx, double* X, double* W, const float* bi, const float* dm, const float* tab, float fltb) {
+    const int N = 2 * nx;
+    for (int J = -nx + 1; J <= nx; ++J) oneline<float, double>(J, N, nx, X, W, bi, dm, tab, fltb);
 }
 void or_symplane0_f32(int nx, float* X, float* W) { symplane0<float>(nx, X, W); }
 void or_symplane0_f64(int nx, double* X, double* W) { symplane0<double>(nx, X, W); }

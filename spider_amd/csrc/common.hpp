@@ -60,6 +60,21 @@ __global__ __launch_bounds__(TPB) void finite_kernel(const T* __restrict__ p, si
 
 inline unsigned nblocks(size_t n) { return (unsigned)((n + TPB - 1) / TPB); }
 
+// ------------------------------------------------------------------ compensated weights
+// W += x with the rounding error of that very addition kept in Wc: the
+// returning atomic gives the exact prior value, so the error of
+// s = fl(old + x) follows from Knuth's two-sum (needs round-to-nearest,
+// no contraction); `extra` is a further error term to carry (e.g. the
+// rounding of an FP64 partial sum to fp32).
+__device__ inline void twosum_atomic_add(float* __restrict__ W, float* __restrict__ Wc, size_t o, float x, float extra) {
+    const float old = atomicAdd(&W[o], x);
+    const float s = __fadd_rn(old, x);
+    const float bb = __fsub_rn(s, old);
+    const float e = __fadd_rn(__fsub_rn(old, __fsub_rn(s, bb)), __fsub_rn(x, bb));
+    const float c = __fadd_rn(e, extra);
+    if (c != 0.0f) unsafeAtomicAdd(&Wc[o], c);
+}
+
 // ------------------------------------------------------------------ ONELINE sample geometry
 // Shared by every insertion strategy so that all of them compute the sample
 // positions, nearest voxels and table indices with the same round-to-nearest

@@ -8,7 +8,8 @@ namespace spbp {
 struct Acc {
     const float2* X[2];
     const float* W[2];
-    int n;   // halves summed: 1 or 2
+    const float* Wc[2];   // compensation terms of the weights, or null
+    int n;                // halves summed: 1 or 2
 };
 
 // (j, k) of the kx = 0 plane is paired by SYMPLANE0 with (N-j, N-k) unless it
@@ -18,35 +19,45 @@ __device__ inline bool sym_pair(int j, int k, int nx) {
 }
 
 // Value of SYMPLANE0(half) at (i, j, k), halves added after (bp32f.f:925-930, 345-346).
-__device__ inline void load_sym(const Acc& a, int nx, int N, int i, int j, int k, float2& x, float& w) {
+__device__ inline void load_sym(const Acc& a, int nx, int N, int i, int j, int k, float2& x, float& w, double& wd) {
     const int H = nx + 1;
     const size_t o = ((size_t)k * N + j) * H + i;
     const bool pr = (i == 0) && sym_pair(j, k, nx);
     const size_t om = pr ? ((size_t)((N - k) % N) * N + (N - j) % N) * H : 0;
     x = make_float2(0.f, 0.f);
     w = 0.f;
+    wd = 0.0;   // FP64 weight incl. the compensation terms (when kept)
     for (int h = 0; h < a.n; ++h) {
         float2 xh = a.X[h][o];
         float wh = a.W[h][o];
+        double wdh = a.Wc[h] ? (double)wh + (double)a.Wc[h][o] : (double)wh;
         if (pr) {
             const float2 xm = a.X[h][om];
             xh.x = xh.x + xm.x;
             xh.y = xh.y + (-xm.y);
             wh = wh + a.W[h][om];
+            wdh += a.Wc[h] ? (double)a.W[h][om] + (double)a.Wc[h][om] : (double)a.W[h][om];
         }
         if (h == 0) { x = xh; w = wh; }
         else { x.x = x.x + xh.x; x.y = x.y + xh.y; w = w + wh; }
+        wd += wdh;
     }
 }
 
 __device__ inline float2 norm_val(const Acc& a, int nx, int N, int i, int j, int k) {
     float2 x;
     float w;
-    load_sym(a, nx, N, i, j, k, x, w);
-    if (w > 0.1f) {                                    // NRMW2 (bp3f.f:1359)
-        const float s = ((i + j + k) & 1) ? -1.0f : 1.0f;
-        return make_float2(__fdiv_rn(x.x * s, w), __fdiv_rn(x.y * s, w));
+    double wd;
+    load_sym(a, nx, N, i, j, k, x, w, wd);
+    const float s = ((i + j + k) & 1) ? -1.0f : 1.0f;
+    if (a.Wc[0]) {
+        // compensated weights: threshold and division in FP64
+        if (wd > (double)0.1f)
+            return make_float2((float)((double)(x.x * s) / wd), (float)((double)(x.y * s) / wd));
+        return make_float2(0.f, 0.f);
     }
+    if (w > 0.1f)                                      // NRMW2 (bp3f.f:1359)
+        return make_float2(__fdiv_rn(x.x * s, w), __fdiv_rn(x.y * s, w));
     return make_float2(0.f, 0.f);
 }
 
@@ -128,9 +139,10 @@ __global__ __launch_bounds__(TPB) void export_kernel(Acc a, int nx, int N, float
     const size_t r = idx / H;
     float2 x;
     float w;
-    load_sym(a, nx, N, i, (int)(r % N), (int)(r / N), x, w);
+    double wd;
+    load_sym(a, nx, N, i, (int)(r % N), (int)(r / N), x, w, wd);
     X[idx] = x;
-    W[idx] = w;
+    W[idx] = a.Wc[0] ? (float)wd : w;
 }
 
 // Adds (or sets) a post-SYMPLANE0 half-grid into raw accumulators.  Entries of

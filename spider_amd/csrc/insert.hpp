@@ -42,7 +42,8 @@ __global__ __launch_bounds__(TPB) void pad_kernel(const T* __restrict__ imgs, in
 __global__ __launch_bounds__(TPB) void insert_direct_kernel(
     const float2* __restrict__ spec, const Geom* __restrict__ geom, int nx, int N,
     const float* __restrict__ tab, float fltb,
-    float2* __restrict__ X0, float* __restrict__ W0, float2* __restrict__ X1, float* __restrict__ W1) {
+    float2* __restrict__ X0, float* __restrict__ W0, float2* __restrict__ X1, float* __restrict__ W1,
+    float* __restrict__ Wc0, float* __restrict__ Wc1) {
     const int H = nx + 1;
     const int p = blockIdx.y;
     const int lin = blockIdx.x * blockDim.x + threadIdx.x;
@@ -67,6 +68,7 @@ __global__ __launch_bounds__(TPB) void insert_direct_kernel(
     }
     float2* __restrict__ X = g.half ? X1 : X0;
     float* __restrict__ W = g.half ? W1 : W0;
+    float* __restrict__ Wc = g.half ? Wc1 : Wc0;   // null unless compensated weights
     const size_t plane = (size_t)N * H;
     for (int lz = 0; lz < 5; ++lz) {
         if (tz[lz] == 0.0f) continue;
@@ -88,12 +90,14 @@ __global__ __launch_bounds__(TPB) void insert_direct_kernel(
                     const size_t o = za + ya + px;
                     unsafeAtomicAdd(&X[o].x, __fmul_rn(v.x, wg));
                     unsafeAtomicAdd(&X[o].y, __fmul_rn(v.y, wg));
-                    unsafeAtomicAdd(&W[o], wg);
+                    if (Wc) twosum_atomic_add(W, Wc, o, wg, 0.0f);
+                    else unsafeAtomicAdd(&W[o], wg);
                 } else {                                         // reflected taps, conj (oneline.f:118-152)
                     const size_t o = zt + yt + (size_t)(-px);
                     unsafeAtomicAdd(&X[o].x, __fmul_rn(v.x, wg));
                     unsafeAtomicAdd(&X[o].y, __fmul_rn(-v.y, wg));
-                    unsafeAtomicAdd(&W[o], wg);
+                    if (Wc) twosum_atomic_add(W, Wc, o, wg, 0.0f);
+                    else unsafeAtomicAdd(&W[o], wg);
                 }
             }
         }
@@ -388,7 +392,8 @@ __global__ __launch_bounds__(TPB) void insert_brick_kernel(
     const float* __restrict__ tab, float fltb, BrickGrid bg, int C,
     const int* __restrict__ cnt, const int* __restrict__ off, const int* __restrict__ item_off,
     const int* __restrict__ item_key, const int* __restrict__ list, int* __restrict__ next_item,
-    float2* __restrict__ X0, float* __restrict__ W0, float2* __restrict__ X1, float* __restrict__ W1) {
+    float2* __restrict__ X0, float* __restrict__ W0, float2* __restrict__ X1, float* __restrict__ W1,
+    float* __restrict__ Wc0, float* __restrict__ Wc1) {
     constexpr int NV = BX * BY * BZ;
     __shared__ BrickLds<BX, BY, BZ> L;
     const int nkeys = 2 * bg.nb;
@@ -427,6 +432,7 @@ __global__ __launch_bounds__(TPB) void insert_brick_kernel(
         // add the brick to the half grid
         float2* __restrict__ X = half ? X1 : X0;
         float* __restrict__ W = half ? W1 : W0;
+        float* __restrict__ Wc = half ? Wc1 : Wc0;   // null unless compensated weights
         for (int v = threadIdx.x; v < NV; v += blockDim.x) {
             const int lx = v % BX, ly = (v / BX) % BY, lz = v / (BX * BY);
             const int i = lz * L.SZ + ly * L.SY + lx;
@@ -435,7 +441,9 @@ __global__ __launch_bounds__(TPB) void insert_brick_kernel(
             const size_t o = ((size_t)(kz >= 0 ? kz : N + kz) * N + (ky >= 0 ? ky : N + ky)) * H + kx;
             unsafeAtomicAdd(&X[o].x, (float)L.re[i]);
             unsafeAtomicAdd(&X[o].y, (float)L.im[i]);
-            unsafeAtomicAdd(&W[o], (float)L.w[i]);
+            const float wi = (float)L.w[i];
+            if (Wc) twosum_atomic_add(W, Wc, o, wi, (float)(L.w[i] - (double)wi));
+            else unsafeAtomicAdd(&W[o], wi);
         }
         __syncthreads();
     }

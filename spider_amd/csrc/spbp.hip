@@ -55,6 +55,7 @@ struct spbp_ctx {
     size_t nh = 0;
     float2* X[2] = {nullptr, nullptr};
     float* W[2] = {nullptr, nullptr};
+    float* Wc[2] = {nullptr, nullptr};   // SPBP_TWOSUM_W: compensation terms of W
     float* tab = nullptr;
     float fltb = 0.f;
     // insertion chunk
@@ -314,7 +315,7 @@ int insert_impl(spbp_ctx* c, const void* imgs, int dtype, int nimg, const float*
             Timed t(c, SPBP_K_INSERT, b, st);
             dim3 grid(nblocks((size_t)H * N), b);
             insert_direct_kernel<<<grid, TPB, 0, st>>>(c->spec, c->geom_d + p0, nx, N, c->tab, c->fltb,
-                                                       c->X[0], c->W[0], c->X[1], c->W[1]);
+                                                       c->X[0], c->W[0], c->X[1], c->W[1], c->Wc[0], c->Wc[1]);
         } else {
             const int nkeys = 2 * c->bg.nb;
             const dim3 lgrid(c->bg.nb, (b + 1023) / 1024);
@@ -330,7 +331,7 @@ int insert_impl(spbp_ctx* c, const void* imgs, int dtype, int nimg, const float*
             Timed t(c, SPBP_K_INSERT, b, st);
             insert_brick_kernel<BRICK_X, BRICK_Y, BRICK_Z><<<c->ncu * 3, TPB, 0, st>>>(
                 c->spec, c->geom_d + p0, nx, N, c->tab, c->fltb, c->bg, c->C, c->bcnt, c->boff, c->bioff,
-                c->bikey, c->blist, c->bnext, c->X[0], c->W[0], c->X[1], c->W[1]);
+                c->bikey, c->blist, c->bnext, c->X[0], c->W[0], c->X[1], c->W[1], c->Wc[0], c->Wc[1]);
         }
         HIPCHK(c, hipGetLastError());
     }
@@ -341,13 +342,15 @@ Acc make_acc(const spbp_ctx* c, int which) {
     Acc a;
     a.X[0] = a.X[1] = nullptr;
     a.W[0] = a.W[1] = nullptr;
+    a.Wc[0] = a.Wc[1] = nullptr;
     if (which < 0) {
         a.n = c->nvol;
-        for (int h = 0; h < c->nvol; ++h) { a.X[h] = c->X[h]; a.W[h] = c->W[h]; }
+        for (int h = 0; h < c->nvol; ++h) { a.X[h] = c->X[h]; a.W[h] = c->W[h]; a.Wc[h] = c->Wc[h]; }
     } else {
         a.n = 1;
         a.X[0] = c->X[which];
         a.W[0] = c->W[which];
+        a.Wc[0] = c->Wc[which];
     }
     return a;
 }
@@ -423,7 +426,8 @@ int spbp_create(const spbp_opts* o, spbp_ctx** out) {
     auto bail = [&](int code) { spbp_destroy(c); return code; };
     for (int h = 0; h < c->nvol; ++h) {
         if (hipMalloc(&c->X[h], c->nh * sizeof(float2)) != hipSuccess ||
-            hipMalloc(&c->W[h], c->nh * sizeof(float)) != hipSuccess)
+            hipMalloc(&c->W[h], c->nh * sizeof(float)) != hipSuccess ||
+            ((c->flags & SPBP_TWOSUM_W) && hipMalloc(&c->Wc[h], c->nh * sizeof(float)) != hipSuccess))
             return bail(fail(nullptr, SPBP_ERR_ALLOC, "BP 32F; XE, WE, WO, & XO: %zu bytes", c->nh * 12 * (size_t)c->nvol));
     }
     std::vector<float> tab;
@@ -454,6 +458,7 @@ int spbp_destroy(spbp_ctx* c) {
     for (int h = 0; h < 2; ++h) {
         if (c->X[h]) (void)hipFree(c->X[h]);
         if (c->W[h]) (void)hipFree(c->W[h]);
+        if (c->Wc[h]) (void)hipFree(c->Wc[h]);
     }
     void* dev_ptrs[] = {c->tab, c->padbuf, c->spec, c->geom_d, c->work, c->real, c->wkb, c->psum, c->pcnt, c->mean, c->flag,
                         c->bcnt, c->boff, c->bioff, c->bcur, c->bnext, c->blist, c->bikey};
@@ -477,6 +482,7 @@ int spbp_reset(spbp_ctx* c, void* stream) {
     for (int h = 0; h < c->nvol; ++h) {
         fill_kernel<<<nblocks(2 * c->nh), TPB, 0, st>>>((float*)c->X[h], 2 * c->nh, 0.0f);
         fill_kernel<<<nblocks(c->nh), TPB, 0, st>>>(c->W[h], c->nh, 0.0f);
+        if (c->Wc[h]) fill_kernel<<<nblocks(c->nh), TPB, 0, st>>>(c->Wc[h], c->nh, 0.0f);
     }
     HIPCHK(c, hipGetLastError());
     return SPBP_OK;
@@ -613,6 +619,7 @@ int spbp_import_partial(spbp_ctx* c, int half, const float* X, const float* W, i
     if (half < 0 || half >= c->nvol) return fail(c, SPBP_ERR_ARG, "no half %d", half);
     DeviceGuard dg(c->device);
     hipStream_t st = (hipStream_t)stream;
+    if (!accumulate && c->Wc[half]) fill_kernel<<<nblocks(c->nh), TPB, 0, st>>>(c->Wc[half], c->nh, 0.0f);
     import_kernel<<<nblocks(c->nh), TPB, 0, st>>>((const float2*)X, W, c->nx, c->N, c->X[half], c->W[half], accumulate);
     HIPCHK(c, hipGetLastError());
     return SPBP_OK;

@@ -240,3 +240,39 @@ def test_strategies_agree_config2_shape(bpmod):
     assert np.array_equal(Wd > 0, Wb > 0)
     # same terms, different fp32 summation order (sums of ~10^7 over 400 projections)
     assert rel(Wb, Wd) < 2e-6 and rel(Xb, Xd) < 5e-6 and rel(vb, vd) < 1e-5
+
+
+@pytest.mark.parametrize("strategy", STRATEGIES)
+def test_twosum_weights(bpmod, strategy):
+    """SPBP_TWOSUM_W: weights kept as fp32 value + compensation equal the exact
+    (FP64) sum of the reference's fp32 weight terms to within one fp32 rounding;
+    plain fp32 accumulation is no closer.  Volumes stay at parity."""
+    rng = np.random.default_rng(41)
+    nx, m = 32, 600
+    imgs = rng.standard_normal((m, nx, nx)).astype(np.float32)
+    ang = synth.uniform_angles(m, rng)
+    # the library evaluates the KB table in FP64 (rounded to fp32); the
+    # reference's single-precision table differs at the support edge, where
+    # sqrt(1 - q^2) cancels, so the exact sum is formed with the FP64 table
+    _, fltb = oracle.kb_table(2 * nx, "f32")
+    tab = oracle.kb_table(2 * nx, "f64")[0].astype(np.float32)
+    n = 2 * nx
+    X64 = np.zeros((n, n, nx + 1), np.complex128)
+    W64 = np.zeros((n, n, nx + 1), np.float64)
+    for p in range(m):
+        dm = oracle.cang(ang[p][2], ang[p][1], ang[p][0], prec="f32")
+        oracle.insert_f32_acc64(X64, W64, oracle.spectrum(imgs[p], "f32"), dm, tab, fltb)
+    oracle.symplane0(X64, W64, "f64")
+    err = {}
+    for twosum in (False, True):
+        with bpmod.Backprojector(nx, twosum=twosum, strategy=strategy) as b:
+            b.insert(gpu(imgs), ang)
+            _, W = b.export_partial()
+            vol = b.finalize()["full"].cpu().numpy()
+        W = W.cpu().numpy().astype(np.float64)
+        nz = W64 > 0
+        err[twosum] = np.max(np.abs(W[nz] - W64[nz]) / W64[nz])
+        o32 = oracle.bp(imgs, ang, prec="f32")["full"]
+        assert rel(vol, o32) < 1e-5
+    assert err[True] <= 2.0 ** -24 * 1.01          # correctly rounded fp32 of the exact sum
+    assert err[True] <= err[False]
