@@ -367,6 +367,32 @@ class RefLib:
                        C.byref(C.c_float(ri)), C.byref(aba), C.byref(dklp))
         return aba.value, dklp.value
 
+    # PR3DB(A,B,PR,AMP,CSUM1,LR,CSUM,CSUM2,AVSUM,LSD,NX,NY,NZ,DSCALE,NSCALE,
+    #       SCALE1,SSANG,INC,Y1,WI,SER): shell sums of the FSC (pr3db.f:34)
+    def pr3db(self, A, B, nx, wi=0.5, ser=b"C", ssang=90.0):
+        """A, B: (NZ, NY, LSD) float32 transforms in the SPIDER LSD layout."""
+        nz, ny, lsd = A.shape
+        A = np.ascontiguousarray(A, np.float32)
+        B = np.ascontiguousarray(B, np.float32)
+        inc = int(float(max(nx, ny, nz)) / wi) // 2 + 1
+        nscale = 20
+        pr = np.zeros((inc, nscale), np.float32)
+        amp = np.zeros((inc, nscale), np.float32)
+        avsum = np.zeros((inc, nscale), np.float32)
+        csum1 = np.zeros(inc, np.float32)
+        csum = np.zeros(inc, np.float32)
+        csum2 = np.zeros(inc, np.float32)
+        lr = np.zeros(inc, np.int32)
+        i32 = lambda v: C.byref(C.c_int32(v))
+        f32 = lambda v: C.byref(C.c_float(v))
+        vp = lambda a: a.ctypes.data_as(C.c_void_p)
+        scale1, scale2 = 0.2, 2.0
+        self.L.pr3db_(vp(A), vp(B), vp(pr), vp(amp), vp(csum1), vp(lr), vp(csum), vp(csum2), vp(avsum),
+                      i32(lsd), i32(nx), i32(ny), i32(nz), f32((scale2 - scale1) / (nscale - 1)), i32(nscale),
+                      f32(scale1), f32(ssang), i32(inc), f32(float(max(nx, ny, nz))), f32(wi),
+                      C.c_char_p(ser), C.c_size_t(1))
+        return {"csum": csum, "csum1": csum1, "csum2": csum2, "lr": lr, "inc": inc}
+
     def besi1(self, x: float) -> float:
         return self.L.besi1_(C.byref(C.c_float(x)))
 

@@ -80,6 +80,25 @@ def main():
         cases[f"c{ci}_X"] = X
         cases[f"c{ci}_W"] = W
     np.savez_compressed(os.path.join(OUT, "oneline.npz"), **cases)
+
+    # FSC shell sums (PR3DB) of two 24^3 volumes: transforms in SPIDER's LSD
+    # layout (conjugated numpy rfftn, the SPIDER sign)
+    n = 24
+    v1 = rng.standard_normal((n, n, n)).astype(np.float32)
+    v2 = (0.6 * v1 + 0.8 * rng.standard_normal((n, n, n))).astype(np.float32)
+    fsc_out = {"v1": v1, "v2": v2}
+    for tag, wi in (("w05", 0.5), ("w10", 1.0)):
+        lsd = []
+        for v in (v1, v2):
+            F = np.conj(np.fft.rfftn(v.astype(np.float64)))
+            a = np.zeros((n, n, n + 2), np.float32)
+            a[:, :, 0::2] = F.real
+            a[:, :, 1::2] = F.imag
+            lsd.append(a)
+        r = R.pr3db(lsd[0], lsd[1], n, wi=wi)
+        for k in ("csum", "csum1", "csum2", "lr"):
+            fsc_out[f"{tag}_{k}"] = r[k]
+    np.savez_compressed(os.path.join(OUT, "fsc.npz"), **fsc_out)
     print("golden fixtures written to", OUT)
 
 
