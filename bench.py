@@ -101,12 +101,202 @@ def parity_subset(bpmod, imgs, ang, m):
     return r64, r32
 
 
+def run_batched(args):
+    """Twelve BP 32F reconstructions with box sizes from {64, 96, 128, 192} and
+    U[500, 30000] projections each (seeded blob phantoms, SNR 0.1).  Plain
+    form: one context after another, fp32 images, default plan, insert +
+    finalise.  Batched form: every context tuned on its own images, BF16
+    image storage, all insertions captured once in one HIP graph (one
+    persistent scatter launch per round of chunks) and replayed, then the
+    same finalisations.  Both forms are timed over the same work; the FSC
+    between their volumes and against the phantoms shows they agree."""
+    import numpy as np
+    import torch
+    from spider_amd import bp as bpmod
+    from spider_amd import fsc as F
+    from spider_amd import synth
+
+    rng = np.random.Generator(np.random.PCG64(synth.config_seed(3)))
+    nvol = args.volumes
+    sizes = rng.choice([64, 96, 128, 192], nvol)
+    counts = rng.integers(500, 30001, nvol)
+    if args.nproj:
+        counts = np.minimum(counts, args.nproj)
+    vols = []
+    for v in range(nvol):
+        nx, m = int(sizes[v]), int(counts[v])
+        r = np.random.Generator(np.random.PCG64(synth.config_seed(3, v + 1)))
+        ph = synth.blob_phantom(nx, max(10, nx * 60 // 128), nx / 2 - 6, r)
+        ang = synth.uniform_angles(m, r)
+        imgs = synth.project(ph, synth.rotations(ang), device="cuda")
+        synth.add_noise(imgs, 0.1, synth.config_seed(3, v + 1) + 7)
+        vols.append(dict(nx=nx, m=m, ph=ph, ang=ang, imgs=imgs, imgs16=imgs.to(torch.bfloat16),
+                         half=bpmod.random_halves(m, seed=v)))
+        log(f"batched: volume {v}: {nx}^3 from {m} projections")
+    total = int(sum(counts))
+    torch.cuda.synchronize()
+
+    def finalize_all(ctxs):
+        return [c.finalize(full=True, halves=True) for c in ctxs]
+
+    # plain: one context after another
+    plain = [bpmod.Backprojector(d["nx"], two_halves=True) for d in vols]
+
+    def run_plain():
+        for c, d in zip(plain, vols):
+            c.reset()
+            c.insert(d["imgs"], d["ang"], d["half"])
+        return finalize_all(plain)
+
+    run_plain()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out_plain = run_plain()
+    torch.cuda.synchronize()
+    t_plain = (time.perf_counter() - t0) / args.steps
+    log(f"batched: plain form {t_plain:.3f} s per pass")
+
+    # batched: tuned plans, BF16 storage, one captured graph
+    fast = [bpmod.Backprojector(d["nx"], two_halves=True) for d in vols]
+    t0 = time.perf_counter()
+    plans = [c.tune(d["imgs16"], d["ang"]) for c, d in zip(fast, vols)]
+    torch.cuda.synchronize()
+    t_tune = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    g = bpmod.BatchGraph([(c, d["imgs16"], d["ang"], d["half"]) for c, d in zip(fast, vols)], reset_first=True)
+    torch.cuda.synchronize()
+    t_capture = time.perf_counter() - t0
+
+    def run_fast():
+        g.launch()
+        return finalize_all(fast)
+
+    run_fast()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out_fast = run_fast()
+    torch.cuda.synchronize()
+    t_fast = (time.perf_counter() - t0) / args.steps
+    log(f"batched: batched form {t_fast:.3f} s per pass")
+
+    per = []
+    for d, a, b in zip(vols, out_plain, out_fast):
+        truth = torch.tensor(d["ph"].volume(), dtype=torch.float32, device="cuda")
+        per.append({"nx": d["nx"], "nproj": d["m"],
+                    "fsc05_plain_vs_phantom": round(F.resolution(F.fsc(a["full"], truth)), 4),
+                    "fsc05_batched_vs_phantom": round(F.resolution(F.fsc(b["full"], truth)), 4),
+                    "min_fsc_plain_vs_batched": round(float(np.min(F.fsc(a["full"], b["full"])["fsc"][1:])), 6)})
+    g.close()
+    for c in plain + fast:
+        c.close()
+    out = {"metric": "projections/s, 12 uneven BP 32F volumes: batched/BF16/graph vs plain",
+           "value": round(total / t_fast, 1), "unit": "projections/s", "higher_is_better": True,
+           "n_gpus": 1, "steps": args.steps, "dtype": "bf16 images, f32 accumulation",
+           "data": "synthetic (seeded blob phantoms, analytic projections, SNR 0.1)",
+           "config": {"workload": f"{nvol} BP 32F volumes, boxes {sorted(set(int(x) for x in sizes))}, "
+                                  f"{total} projections in all (config 3)",
+                      "sizes": [int(x) for x in sizes], "counts": [int(x) for x in counts]},
+           "plain_proj_per_s": round(total / t_plain, 1), "speedup_vs_plain": round(t_plain / t_fast, 3),
+           "plain_seconds": round(t_plain, 4), "batched_seconds": round(t_fast, 4),
+           "tune_seconds": round(t_tune, 3), "capture_seconds": round(t_capture, 3),
+           "plans": [{"strategy": p["strategy"], "brick": p["brick"], "brick_proj": p["brick_proj"]} for p in plans],
+           "volumes": per}
+    print(json.dumps(out), flush=True)
+
+
+def run_recon(args):
+    """128^3 from 20,000 noisy projections: plain BP 32F against CGLS with the
+    KB-gather projector (A) and its adjoint (the insertion, A^T), both scored
+    by FSC against the phantom; also one autograd step (loss on a batch of
+    256 projections + backward) and the gather kernel's own rate."""
+    import numpy as np
+    import torch
+    from spider_amd import _lib, synth
+    from spider_amd import bp as bpmod
+    from spider_amd import fsc as F
+    from spider_amd import project
+
+    rec = synth.CONFIGS[4]
+    nx = rec["nx"]
+    nproj = args.nproj or rec["nproj"]
+    log(f"recon: generating {nproj} projections {nx}x{nx}, SNR {rec['snr']}")
+    ang, ph, imgs = synth.make_dataset(4, nproj=nproj, device="cuda")
+    truth = torch.tensor(ph.volume(), dtype=torch.float32, device="cuda")
+    half = bpmod.random_halves(nproj, seed=synth.config_seed(4))
+    torch.cuda.synchronize()
+
+    def bp32f():
+        with bpmod.Backprojector(nx, two_halves=True) as b:
+            b.insert(imgs, ang, half)
+            v = b.finalize(full=True, halves=True)
+        torch.cuda.synchronize()
+        return v
+
+    bp32f()   # warm-up (plans, allocations)
+    t0 = time.perf_counter()
+    vols = bp32f()
+    t_bp = time.perf_counter() - t0
+    c_bp = F.fsc(vols["full"], truth)
+    res_bp = F.resolution(c_bp)
+    res_halves = F.resolution(F.fsc(vols["h1"], vols["h2"]), 0.143)
+    log(f"recon: BP 32F {t_bp:.3f} s, FSC(0.5) vs phantom at {res_bp:.4f}")
+
+    P = project.Projector(nx, angles=ang, batch=256, timing=True)
+    curve = []
+
+    def cb(it, x):
+        curve.append(F.resolution(F.fsc(x, truth)))
+        log(f"recon: CG iteration {it}: FSC(0.5) vs phantom at {curve[-1]:.4f}")
+
+    x, hist = project.cgls(P, imgs, iters=args.cg_iters, callback=cb)
+    g_ms, g_launch, g_units = P.ctx.kernel_stats(_lib.K_GATHER)
+    s_ms, s_launch, s_units = P.ctx.kernel_stats(_lib.K_INSERT)
+    it_s = [h["seconds"] for h in hist]
+    eq = next((h["iter"] for h, r in zip(hist, curve) if r >= res_bp), None)
+
+    # one autograd step on a batch of 256 projections
+    sel = slice(0, min(256, nproj))
+    v = x.clone().requires_grad_(True)
+    for rep in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        loss = 0.5 * torch.sum((P(v, sel) - imgs[sel]) ** 2)
+        loss.backward()
+        torch.cuda.synchronize()
+        t_ad = time.perf_counter() - t0
+        v.grad = None
+    P.close()
+    out = {"metric": "CG reconstruction with the KB-gather projector vs plain BP 32F (FSC vs phantom)",
+           "value": round(float(np.median(it_s)), 4), "unit": "s per CG iteration",
+           "higher_is_better": False, "n_gpus": 1, "dtype": "f32",
+           "data": "synthetic (seeded blob phantom, analytic projections, white noise)",
+           "config": {"workload": f"{nx}^3 from {nproj} projections {nx}x{nx} SNR {rec['snr']}, "
+                                  f"CGLS batches of 256 (config 4)", "nx": nx, "nproj": nproj},
+           "bp32f_seconds": round(t_bp, 4), "bp32f_fsc05_vs_phantom": round(res_bp, 4),
+           "bp32f_halves_fsc0143": round(res_halves, 4),
+           "cg_iters": len(hist), "cg_seconds_total": round(sum(it_s), 3),
+           "cg_fsc05_vs_phantom": [round(r, 4) for r in curve],
+           "cg_iter_reaching_bp32f_fsc": eq,
+           "cg_residual": [round(h["residual"], 3) for h in hist],
+           "gather_proj_per_s": round(g_units / (g_ms / 1e3), 1) if g_ms > 0 else None,
+           "scatter_proj_per_s": round(s_units / (s_ms / 1e3), 1) if s_ms > 0 else None,
+           "autograd_step_256_ms": round(t_ad * 1e3, 3)}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=2, choices=[1, 2, 5])
+    ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4, 5],
+                    help="2: plain BP 3F throughput (default); 3: 12 uneven BP 32F volumes, "
+                         "plain one-by-one vs batched (tuned plans, BF16 images, one captured graph); "
+                         "4: CG reconstruction with the gather projector beside BP 32F")
+    ap.add_argument("--volumes", type=int, default=12, help="config 3: number of volumes")
+    ap.add_argument("--cg-iters", type=int, default=30)
     ap.add_argument("--nproj", type=int, default=None, help="override the projection count")
     ap.add_argument("--batch", type=int, default=0, help="projections per internal chunk (0 = auto)")
     ap.add_argument("--strategy", default="auto", choices=["auto", "direct", "brick"])
@@ -133,6 +323,12 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
+    if args.config == 4:
+        run_recon(args)
+        return
+    if args.config == 3:
+        run_batched(args)
+        return
     rec = synth.CONFIGS[args.config]
     nx = rec["nx"]
     nproj = args.nproj or rec["nproj"]

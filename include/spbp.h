@@ -59,6 +59,7 @@ extern "C" {
 #define SPBP_K_FFT2     2   /* batched 2D forward FFT (units: projections) */
 #define SPBP_K_FINALIZE 3   /* one whole finalisation (units: volumes)    */
 #define SPBP_K_BIN      4   /* brick work-list build (units: projections) */
+#define SPBP_K_GATHER   5   /* central-slice KB gather (units: projections) */
 
 /* insertion strategies */
 #define SPBP_STRAT_AUTO   0   /* the library's choice                                     */
@@ -85,6 +86,27 @@ const char* spbp_last_error(const spbp_ctx* ctx);
 /* Zero the accumulators. */
 int spbp_reset(spbp_ctx* ctx, void* stream);
 
+/* Central-slice Kaiser-Bessel gather: the exact adjoint (transpose, complex
+ * numbers taken as pairs of reals) of the ONELINE scatter (oneline.f:38-158).
+ * For each projection i and lattice sample (I, J), J = -NX+1..NX:
+ *   out_bi[i][J mod N][I] = c( sum_direct w X[t] + conj(sum_reflected w X[-t]) )
+ * over the taps t, weights w and reflections ONELINE would use for that
+ * sample (c = conj for a Friedel-flipped sample); samples ONELINE skips are 0.
+ * X: caller-owned complex64 half grid, (NX+1) N N (device);
+ * out_bi: nimg x N x (NX+1) complex64 (device), ONELINE's BI layout (SPIDER
+ * sign and checkerboard applied).  Rotations from angles_deg (host nimg x 3,
+ * as spbp_insert) or, when dm is not NULL, dm (host nimg x 9 fp32).
+ * No counterpart in the reference: SPIDER's own projector PJ 3G
+ * (pj3g.f:125) uses a different kernel; this is the projector whose adjoint
+ * is BP 3F's insertion, for gradient and CG reconstruction. */
+int spbp_gather(spbp_ctx* ctx, const float* X, int nimg, const double* angles_deg, const float* dm,
+                float* out_bi, void* stream);
+/* ONELINE scatter of BI-layout spectra (the adjoint of spbp_gather), added
+ * into a caller-owned half grid X and, when W is not NULL, weights W.  Same
+ * insertion kernels as spbp_insert (BP32FQ's projection loop,
+ * bp32f.f:802-913) minus PADD2 and the forward FFT. */
+int spbp_scatter(spbp_ctx* ctx, const float* bi, int nimg, const double* angles_deg, const float* dm,
+                 float* X, float* W, void* stream);
 /* Insert projections: PADD2 + forward FFT + checkerboard + ONELINE over all
  * rows, per image (the BP32FQ / BP3FQ projection loop, bp32f.f:802-913,
  * bp3f.f:403-493).  half[i] = 0 -> first sample volume (XE), 1 -> second
@@ -129,6 +151,59 @@ int spbp_reset_stats(spbp_ctx* ctx);
 /* Sizes (elements) of one half-grid: complex X count = (NX+1) N N. */
 size_t spbp_grid_elems(const spbp_ctx* ctx);
 
+/* Symmetries (BP 3F / BP 32F "SYMMETRIES DOC", bp32f.f:133-172): nsym doc
+ * rows (PSI, THETA, PHI) -> matrices SM by CANG (BUILDS, builds.f:35-63).
+ * With nsym > 1 every image is inserted once per DMS = MATMUL(SM_k, DM)
+ * (bp32f.f:866-872) and every finalised volume goes through SYMVOL
+ * (symvol.f:39, bp32f.f:278-303).  nsym <= 1 switches symmetries off, as a
+ * one-row or absent document does in the reference.  Applies to inserts made
+ * after the call; the gather / scatter entry points ignore it. */
+int spbp_set_symmetries(spbp_ctx* ctx, int nsym, const double* angles_deg);
+/* ---- batching, tuning and graphs (no counterpart in the reference: its
+ * projection loop is one OpenMP loop per operation, bp32f.f:802-913) ---- */
+/* Plan = insertion strategy (SPBP_STRAT_DIRECT / _BRICK), brick shape
+ * (0: 16x16x8, 1: 16x8x8, 2: 8x8x8 voxels = the LDS bin) and projections per
+ * brick work item.  spbp_tune times every plan on a sample of up to 1024 of
+ * the given images (into scratch grids: the accumulators are untouched) and
+ * keeps the fastest; spbp_get_plan reports the plan and, when ms_per_proj is
+ * not NULL, the 13 measured ms per projection ([shape][C = 16, 32, 64, 128],
+ * then direct). */
+int spbp_tune(spbp_ctx* ctx, const void* imgs, int dtype, int nimg, const double* angles_deg, const float* dm,
+              void* stream);
+int spbp_get_plan(const spbp_ctx* ctx, int* strategy, int* shape, int* brick_proj, float* ms_per_proj);
+int spbp_set_plan(spbp_ctx* ctx, int strategy, int shape, int brick_proj);
+/* One insertion job: spbp_insert's arguments for one context. */
+typedef struct {
+    spbp_ctx* ctx;
+    const void* imgs;          /* device, nimg x NX x NX of ctx's NX */
+    int dtype;                 /* SPBP_F32 / SPBP_BF16 */
+    int nimg;
+    const double* angles_deg;  /* host nimg x 3, or NULL when dm is given */
+    const float* dm;           /* host nimg x 9, or NULL */
+    const int8_t* half;        /* host, may be NULL */
+} spbp_job;
+/* Insert several jobs (distinct contexts, any box sizes, at most 64) as
+ * spbp_insert would, with the scatter work of all of them in one persistent
+ * kernel launch per round of chunks.  The contexts share one brick shape:
+ * the best total when all are tuned, else the first job's. */
+int spbp_insert_batch(const spbp_job* jobs, int njobs, void* stream);
+/* A batch captured as a HIP graph (pad, FFT, list build, persistent kernel;
+ * with reset_first, zeroing of the accumulators first).  The graph keeps its
+ * own copy of the geometry; images are read from the jobs' device pointers at
+ * every launch.  The contexts must stay alive and keep their plan and chunk
+ * size while the graph exists; other calls on them in between are allowed.
+ * spbp_graph_launch replays it on `stream`. */
+typedef struct spbp_graph spbp_graph;
+int spbp_graph_create(const spbp_job* jobs, int njobs, int reset_first, spbp_graph** out);
+int spbp_graph_launch(spbp_graph* g, void* stream);
+int spbp_graph_destroy(spbp_graph* g);
+/* Host-side tables, exactly as the kernels use them (no GPU needed):
+ * the KB interpolation table TABI(0:4999) and FLTB for padded size n2
+ * (bp32f.f:193-214, fillbessi0.f:88-117), evaluated in FP64 and stored fp32;
+ * the CANG matrices (cang.f:44-82) of n doc-angle triples (PSI, THETA, PHI),
+ * n x 9 fp32, with flags & SPBP_EXACT_PI selecting exact pi. */
+int spbp_kb_table(int n2, float* tab, float* fltb);
+int spbp_rotations(int n, const double* angles_deg, int flags, float* dm);
 /* Library version string. */
 const char* spbp_version(void);
 

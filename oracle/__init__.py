@@ -75,6 +75,14 @@ def lib():
         L.or_insert_f32.argtypes = [C.c_int, _f32p, _f32p, _f32p, _f32p, _f32p, C.c_float]
         L.or_insert_f64.argtypes = [C.c_int, _f64p, _f64p, _f64p, _f64p, _f64p, C.c_double]
         L.or_insert_f32_acc64.argtypes = [C.c_int, _f64p, _f64p, _f32p, _f32p, _f32p, C.c_float]
+        L.or_gather.argtypes = [C.c_int, _f64p, C.c_int, _f32p, _f32p, C.c_float, _f64p]
+        L.or_bp_sym_f32.argtypes = [C.c_int, C.c_int, _f32p, _f64p, vp, vp, vp, vp, C.c_int, _f64p]
+        L.or_bp_sym_f32.restype = C.c_int
+        L.or_sym_matmul_f32.argtypes = [_f32p, _f32p, _f32p]
+        L.or_triquad.argtypes = [C.c_double, C.c_double, C.c_double, _f32p]
+        L.or_triquad.restype = C.c_float
+        L.or_symvol.argtypes = [C.c_int, _f32p, _f32p, _f32p, C.c_int]
+        L.or_scatter.argtypes = [C.c_int, _f64p, C.c_int, _f32p, _f32p, C.c_float, _f64p, _f64p]
         L.or_symplane0_f32.argtypes = [C.c_int, _f32p, _f32p]
         L.or_symplane0_f64.argtypes = [C.c_int, _f64p, _f64p]
         L.or_normalize_f32.argtypes = [C.c_int, _f32p, _f32p]
@@ -211,6 +219,44 @@ def insert_f32_acc64(X, W, bi, dm, tab, fltb) -> None:
                               np.ascontiguousarray(dm, np.float32), tab, np.float32(fltb))
 
 
+def lib_table(n2: int):
+    """TABI evaluated in FP64 and rounded to fp32, FLTB = 4999/3 in fp32: the
+    table libspbp builds (its kernels' weights), for checks of functions that
+    have no reference counterpart (the projector)."""
+    t, _ = kb_table(n2, "f64")
+    return t.astype(np.float32), np.float32(np.float32(4999.0) / np.float32(3.0))
+
+
+def gather(X, dm, table=None):
+    """Central-slice KB gather (transpose of ONELINE) of the FP64 half grid
+    X (complex128, N x N x (NX+1)) for rotations dm (nproj x 9 fp32):
+    nproj x N x (NX+1) complex128 spectra in the BI layout.  table = (tab,
+    fltb), default lib_table(N)."""
+    X = np.ascontiguousarray(X, np.complex128)
+    nx = X.shape[2] - 1
+    N = 2 * nx
+    tab, fltb = table if table is not None else lib_table(N)
+    dm = np.ascontiguousarray(np.reshape(dm, (-1, 9)), np.float32)
+    out = np.zeros((dm.shape[0], N, nx + 1), np.complex128)
+    lib().or_gather(nx, X.view(np.float64), dm.shape[0], dm, np.ascontiguousarray(tab, np.float32), float(fltb),
+                    out.view(np.float64))
+    return out
+
+
+def scatter(bi, dm, nx, table=None):
+    """ONELINE scatter of FP64 BI spectra (nproj x N x (NX+1) complex128) with
+    fp32 weights, FP64 terms: returns (X, W).  table as for gather()."""
+    bi = np.ascontiguousarray(bi, np.complex128)
+    dm = np.ascontiguousarray(np.reshape(dm, (-1, 9)), np.float32)
+    N = 2 * nx
+    tab, fltb = table if table is not None else lib_table(N)
+    X = np.zeros((N, N, nx + 1), np.complex128)
+    W = np.zeros((N, N, nx + 1), np.float64)
+    lib().or_scatter(nx, bi.view(np.float64), dm.shape[0], dm, np.ascontiguousarray(tab, np.float32), float(fltb),
+                     X.view(np.float64), W)
+    return X, W
+
+
 def oneline_f32(J, X, W, bi, dm, tab, fltb) -> None:
     nx = X.shape[2] - 1
     lib().or_oneline_f32(J, nx, X.view(np.float32), W, bi.view(np.float32),
@@ -252,6 +298,56 @@ def finalize(X, W, prec="f32") -> np.ndarray:
     Xc = np.array(X, copy=True)
     out = np.zeros((nx, nx, nx), dt)
     getattr(lib(), "or_finalize_" + prec)(nx, Xc.view(dt), np.ascontiguousarray(W, dt), out)
+    return out
+
+
+def sym_matrices(sym_angles):
+    """BUILDS (builds.f:35-63): fp32 CANG matrices of the symmetry doc rows
+    (PSI, THETA, PHI), (nsym, 9)."""
+    a = np.asarray(sym_angles, np.float64).reshape(-1, 3)
+    return np.stack([cang(r[2], r[1], r[0], prec="f32") for r in a]).astype(np.float32).reshape(-1, 9)
+
+
+def sym_matmul(sm, dm):
+    """DMS = MATMUL(SM, DM) in fp32 (Fortran (3,3) storage, 9 floats each)."""
+    out = np.zeros(9, np.float32)
+    lib().or_sym_matmul_f32(np.ascontiguousarray(sm, np.float32).ravel(),
+                            np.ascontiguousarray(dm, np.float32).ravel(), out)
+    return out
+
+
+def triquad(r, s, t, f):
+    return np.float32(lib().or_triquad(float(r), float(s), float(t), np.ascontiguousarray(f, np.float32).ravel()))
+
+
+def symvol(q1, sm):
+    """SYMVOL of an NX^3 fp32 volume ([z][y][x]) with matrices sm (nsym, 9)."""
+    q1 = np.ascontiguousarray(q1, np.float32)
+    sm = np.ascontiguousarray(sm, np.float32).reshape(-1, 9)
+    q2 = np.zeros_like(q1)
+    lib().or_symvol(q1.shape[0], q1, q2, sm.ravel(), sm.shape[0])
+    return q2
+
+
+def bp_sym(imgs, angles, sym_angles, half=None):
+    """FP32-faithful BP 3F / BP 32F with a symmetry document (sym_angles:
+    (nsym, 3) PSI, THETA, PHI rows): every projection inserted once per
+    DMS = SM DM, and SYMVOL applied to every output volume."""
+    imgs = np.ascontiguousarray(imgs, np.float32)
+    nimg, nx, _ = imgs.shape
+    ang = np.ascontiguousarray(np.asarray(angles, np.float64).reshape(nimg, 3))
+    sym = np.ascontiguousarray(np.asarray(sym_angles, np.float64).reshape(-1, 3))
+    out = {"full": np.zeros((nx, nx, nx), np.float32)}
+    hp = None
+    if half is not None:
+        hp = np.ascontiguousarray(np.asarray(half, np.int8).reshape(nimg))
+        out["h1"] = np.zeros((nx, nx, nx), np.float32)
+        out["h2"] = np.zeros((nx, nx, nx), np.float32)
+    ptr = lambda a: None if a is None else a.ctypes.data
+    rc = lib().or_bp_sym_f32(nx, nimg, imgs, ang, ptr(hp), ptr(out["full"]), ptr(out.get("h1")),
+                             ptr(out.get("h2")), sym.shape[0], sym)
+    if rc != 0:
+        raise RuntimeError(f"oracle bp failed: {rc}")
     return out
 
 
@@ -332,6 +428,26 @@ class RefLib:
         self.L.cang_(C.byref(C.c_float(phi)), C.byref(C.c_float(theta)), C.byref(C.c_float(psi)),
                      C.byref(C.c_int32(0)), ss.ctypes.data_as(C.c_void_p), dm.ctypes.data_as(C.c_void_p))
         return dm
+
+    # TRIQUAD(R,S,T,F): REAL function, DOUBLE PRECISION offsets
+    def triquad(self, r, s, t, f) -> np.float32:
+        self.L.triquad_.restype = C.c_float
+        f = np.ascontiguousarray(f, np.float32).ravel()
+        return np.float32(self.L.triquad_(C.byref(C.c_double(r)), C.byref(C.c_double(s)), C.byref(C.c_double(t)),
+                                          f.ctypes.data_as(C.c_void_p)))
+
+    # SYMVOL(Q1,Q2,KLX,KNX,KLY,KNY,KLZ,KNZ,RM,NSYM)
+    def symvol(self, q1, sm) -> np.ndarray:
+        q1 = np.ascontiguousarray(q1, np.float32)
+        nx = q1.shape[0]
+        sm = np.ascontiguousarray(sm, np.float32).reshape(-1, 9)
+        q2 = np.zeros_like(q1)
+        klx = -(nx // 2)
+        knx = nx // 2 - 1 if nx % 2 == 0 else nx // 2
+        i = lambda v: C.byref(C.c_int32(v))
+        self.L.symvol_(q1.ctypes.data_as(C.c_void_p), q2.ctypes.data_as(C.c_void_p), i(klx), i(knx), i(klx), i(knx),
+                       i(klx), i(knx), sm.ctypes.data_as(C.c_void_p), i(sm.shape[0]))
+        return q2
 
     # FILLBESSIL(N2,LN2,LTAB,FLTB,TABI,ALPHA,AAAA,NNN) -> into COMMON /TABS/
     def fillbessil(self, n2: int):

@@ -370,8 +370,10 @@ void spectrum(const float* img, int nx, R* bi) {
 // dm = first six entries of the CANG matrix.
 // A: accumulation type (R for the restatement; double with R = float gives
 // the reference's geometry and per-tap terms summed in FP64).
-template <typename R, typename A = R>
-void oneline(int J, int N, int nx, A* X, A* W, const R* BI, const R* dm, const R* tab, R fltb) {
+// V: type of the spectrum values (R unless given; V = double with R = float
+// forms every term w * v in FP64 from the fp32 weight).
+template <typename R, typename A = R, typename V = R>
+void oneline(int J, int N, int nx, A* X, A* W, const V* BI, const R* dm, const R* tab, R fltb) {
     typedef Num<R> M;
     const int H = nx + 1;
     const int jp = (J >= 0) ? J : N + J;
@@ -380,7 +382,7 @@ void oneline(int J, int N, int nx, A* X, A* W, const R* BI, const R* dm, const R
         R x = R(I) * dm[0] + R(J) * dm[3];
         R y = R(I) * dm[1] + R(J) * dm[4];
         R z = R(I) * dm[2] + R(J) * dm[5];
-        R vr = BI[2 * ((size_t)jp * H + I)], vi = BI[2 * ((size_t)jp * H + I) + 1];
+        V vr = BI[2 * ((size_t)jp * H + I)], vi = BI[2 * ((size_t)jp * H + I) + 1];
         if (x < R(0)) { x = -x; y = -y; z = -z; vi = -vi; }
         const int ix = (int)(x + R(0.5) + R(N)) - N;
         const int iy = (int)(y + R(0.5) + R(N)) - N;
@@ -443,6 +445,85 @@ template <typename R>
 void insert_projection(int nx, R* X, R* W, const R* bi, const R* dm, const R* tab, R fltb) {
     const int N = 2 * nx;
     for (int J = -nx + 1; J <= nx; ++J) oneline<R>(J, N, nx, X, W, bi, dm, tab, fltb);
+}
+
+// ---------------------------------------------------------------- gather (adjoint of ONELINE)
+// Transpose of oneline<float, double, double> (complex numbers as pairs of
+// reals): ONELINE adds w v at each direct tap t and w conj(v) at -t for each
+// reflected tap, v = BI(I, J) conjugated for a Friedel-flipped sample, so
+//   out(I, J) = c( sum_direct w X[t] + conj(sum_reflected w X[-t]) ),
+// c = conj for a flipped sample.  Same taps, bounds and fp32 weights as
+// oneline.f:53-152 (restated in oneline() above); sums in FP64.
+void gather_line(int J, int N, int nx, const double* X, double* out, const float* dm, const float* tab, float fltb) {
+    typedef Num<float> M;
+    const int H = nx + 1;
+    const int jp = (J >= 0) ? J : N + J;
+    for (int I = 0; I <= nx; ++I) {
+        double* o = out + 2 * ((size_t)jp * H + I);
+        o[0] = o[1] = 0.0;
+        if (!(((I * I + J * J) < (N * N / 4)) && !(I == 0 && J < 0))) continue;
+        float x = float(I) * dm[0] + float(J) * dm[3];
+        float y = float(I) * dm[1] + float(J) * dm[4];
+        float z = float(I) * dm[2] + float(J) * dm[5];
+        const bool flip = x < 0.0f;
+        if (flip) { x = -x; y = -y; z = -z; }
+        const int ix = (int)(x + 0.5f + float(N)) - N;
+        const int iy = (int)(y + 0.5f + float(N)) - N;
+        const int iz = (int)(z + 0.5f + float(N)) - N;
+        if (!(ix <= nx - LN2 - 1 && iy >= -nx + 2 + LN2 && iy <= nx - LN2 - 1 &&
+              iz >= -nx + 2 + LN2 && iz <= nx - LN2 - 1)) continue;
+        double dre = 0, dim = 0, mre = 0, mim = 0;
+        if (ix >= 0) {
+            const int lb = -std::min(ix, LN2);
+            for (int lz = -LN2; lz <= LN2; ++lz) {
+                const int izp = iz + lz;
+                const int iza = izp >= 0 ? izp : N + izp;
+                const float tz = tab[M::nint(M::fabs_(z - float(izp)) * fltb)];
+                if (tz == 0.0f) continue;
+                for (int ly = -LN2; ly <= LN2; ++ly) {
+                    const int iyp = iy + ly;
+                    const int iya = iyp >= 0 ? iyp : N + iyp;
+                    const float ty = tab[M::nint(M::fabs_(y - float(iyp)) * fltb)] * tz;
+                    if (ty == 0.0f) continue;
+                    for (int ixp = lb + ix; ixp <= LN2 + ix; ++ixp) {
+                        const float wg = tab[M::nint(M::fabs_(x - float(ixp)) * fltb)] * ty;
+                        if (wg != 0.0f) {
+                            const size_t t = ((size_t)iza * N + iya) * H + ixp;
+                            dre += (double)wg * X[2 * t];
+                            dim += (double)wg * X[2 * t + 1];
+                        }
+                    }
+                }
+            }
+        }
+        if (ix < LN2) {
+            for (int lz = -LN2; lz <= LN2; ++lz) {
+                const int izp = iz + lz;
+                int izt = -izp;
+                if (izp > 0) izt = N + izt;
+                const float tz = tab[M::nint(M::fabs_(z - float(izp)) * fltb)];
+                if (tz == 0.0f) continue;
+                for (int ly = -LN2; ly <= LN2; ++ly) {
+                    const int iyp = iy + ly;
+                    int iyt = -iyp;
+                    if (iyp > 0) iyt = iyt + N;
+                    const float ty = tab[M::nint(M::fabs_(y - float(iyp)) * fltb)] * tz;
+                    if (ty == 0.0f) continue;
+                    for (int ixp = ix - LN2; ixp <= -1; ++ixp) {
+                        const float wg = tab[M::nint(M::fabs_(x - float(ixp)) * fltb)] * ty;
+                        if (wg != 0.0f) {
+                            const size_t t = ((size_t)izt * N + iyt) * H + (-ixp);
+                            mre += (double)wg * X[2 * t];
+                            mim += (double)wg * X[2 * t + 1];
+                        }
+                    }
+                }
+            }
+        }
+        const double im = dim - mim;
+        o[0] = dre + mre;
+        o[1] = flip ? -im : im;
+    }
 }
 
 // ---------------------------------------------------------------- SYMPLANE0
@@ -580,6 +661,106 @@ void finalize_volume(int nx, R* X, const R* W, R* out) {
     window<R>(nx, real.data(), out);
 }
 
+// ---------------------------------------------------------------- symmetries
+// DMS = MATMUL(SM, DM) (bp32f.f:866-872): 3x3 matrices stored as Fortran
+// DM(3,3), i.e. element (i, j) at [i + 3 j]; products and sums in R, k = 1..3.
+template <typename R>
+void sym_matmul(const R* sm, const R* dm, R* out) {
+    for (int j = 0; j < 3; ++j)
+        for (int i = 0; i < 3; ++i) {
+            R acc = sm[i] * dm[3 * j];
+            acc = acc + sm[i + 3] * dm[1 + 3 * j];
+            acc = acc + sm[i + 6] * dm[2 + 3 * j];
+            out[i + 3 * j] = acc;
+        }
+}
+
+// TRIQUAD (triquad.f): 27-point triquadratic interpolation at offsets
+// (R, S, T) in [0, 1) from the point F(14).  The reference's implicit typing
+// makes the products RS, ST, RT, RST and the factors 1-R^2, 1-R, 1+R, ...
+// REAL (rounded from DOUBLE), while a term that multiplies by R, S or T
+// itself is evaluated in DOUBLE; the sum runs left to right, in REAL until
+// the first DOUBLE term, and the result is rounded to REAL.
+float triquad_ref(double R, double S, double T, const float* F) {
+    const float C2 = 1.0f / 2.0f, C4 = 1.0f / 4.0f, C8 = 1.0f / 8.0f;
+    const float RS = (float)(R * S), ST = (float)(S * T), RT = (float)(R * T);
+    const float RST = (float)(R * (double)ST);
+    const float RSQ = (float)(1.0 - R * R), SSQ = (float)(1.0 - S * S), TSQ = (float)(1.0 - T * T);
+    const float RM1 = (float)(1.0 - R), SM1 = (float)(1.0 - S), TM1 = (float)(1.0 - T);
+    const float RP1 = (float)(1.0 + R), SP1 = (float)(1.0 + S), TP1 = (float)(1.0 + T);
+    // terms 1-4 are REAL; term 5 is the first DOUBLE one
+    float a = (-C8) * RST * RM1 * SM1 * TM1 * F[0];
+    a = a + (C4) * ST * RSQ * SM1 * TM1 * F[1];
+    a = a + (C8) * RST * RP1 * SM1 * TM1 * F[2];
+    a = a + (C4) * RT * RM1 * SSQ * TM1 * F[3];
+    double d = (double)a + (double)(-C2) * T * RSQ * SSQ * TM1 * F[4];
+    d = d + (double)((-C4) * RT * RP1 * SSQ * TM1 * F[5]);
+    d = d + (double)((C8) * RST * RM1 * SP1 * TM1 * F[6]);
+    d = d + (double)((-C4) * ST * RSQ * SP1 * TM1 * F[7]);
+    d = d + (double)((-C8) * RST * RP1 * SP1 * TM1 * F[8]);
+    d = d + (double)((C4) * RS * RM1 * SM1 * TSQ * F[9]);
+    d = d + (double)(-C2) * S * RSQ * SM1 * TSQ * F[10];
+    d = d + (double)((-C4) * RS * RP1 * SM1 * TSQ * F[11]);
+    d = d + (double)(-C2) * R * RM1 * SSQ * TSQ * F[12];
+    d = d + (double)(RSQ * SSQ * TSQ * F[13]);
+    d = d + (double)(C2) * R * RP1 * SSQ * TSQ * F[14];
+    d = d + (double)((-C4) * RS * RM1 * SP1 * TSQ * F[15]);
+    d = d + (double)(C2) * S * RSQ * SP1 * TSQ * F[16];
+    d = d + (double)((C4) * RS * RP1 * SP1 * TSQ * F[17]);
+    d = d + (double)((C8) * RST * RM1 * SM1 * TP1 * F[18]);
+    d = d + (double)((-C4) * ST * RSQ * SM1 * TP1 * F[19]);
+    d = d + (double)((-C8) * RST * RP1 * SM1 * TP1 * F[20]);
+    d = d + (double)((-C4) * RT * RM1 * SSQ * TP1 * F[21]);
+    d = d + (double)(C2) * T * RSQ * SSQ * TP1 * F[22];
+    d = d + (double)((C4) * RT * RP1 * SSQ * TP1 * F[23]);
+    d = d + (double)((-C8) * RST * RM1 * SP1 * TP1 * F[24]);
+    d = d + (double)((C4) * ST * RSQ * SP1 * TP1 * F[25]);
+    d = d + (double)((C8) * RST * RP1 * SP1 * TP1 * F[26]);
+    return (float)d;
+}
+
+// SYMVOL (symvol.f:39-219, the non-SP_MP branch): Q2 = (1/NSYM) sum over
+// symmetry matrices RM of Q1 sampled at RM^T r by TRIQUAD, for voxels
+// r = (IX, IY, IZ) in [KLX, KNX]^3 (KLX = -NX/2, KNX = NX/2 - 1 for even NX,
+// NX/2 for odd) with |r|^2 <= IRADI = (min |bound| - 1)^2; other voxels and
+// identity matrices (RM(1,1) = RM(2,2) = RM(3,3) = 1) take Q1(r).  Volumes
+// x fastest; sm: nsym x 9 Fortran-ordered matrices.
+void symvol_ref(int nx, const float* q1, float* q2, const float* sm, int nsym) {
+    const int klx = -nx / 2, knx = (nx % 2 == 0) ? nx / 2 - 1 : nx / 2;
+    const int mn = std::min(std::abs(klx), std::abs(knx));
+    const int iradi = (mn - 1) * (mn - 1);
+    auto at = [&](int ix, int iy, int iz) { return ((size_t)(iz - klx) * nx + (iy - klx)) * nx + (ix - klx); };
+    const size_t nv = (size_t)nx * nx * nx;
+    std::fill(q2, q2 + nv, 0.0f);
+    for (int s = 0; s < nsym; ++s) {
+        const float* rm = sm + 9 * s;
+        const bool ident = rm[0] == 1.0f && rm[4] == 1.0f && rm[8] == 1.0f;
+        for (int iz = klx; iz <= knx; ++iz)
+            for (int iy = klx; iy <= knx; ++iy) {
+                double qrt[3];
+                for (int k = 0; k < 3; ++k) qrt[k] = (double)(rm[3 * k + 1] * (float)iy + rm[3 * k + 2] * (float)iz);
+                for (int ix = klx; ix <= knx; ++ix) {
+                    float v;
+                    const int icrd = ix * ix + iy * iy + iz * iz;
+                    if (ident || icrd > iradi) {
+                        v = q1[at(ix, iy, iz)];
+                    } else {
+                        double qr[3];
+                        for (int k = 0; k < 3; ++k) qr[k] = qrt[k] + (double)(rm[3 * k] * (float)ix);
+                        const int iox = (int)std::floor(qr[0]), ioy = (int)std::floor(qr[1]), ioz = (int)std::floor(qr[2]);
+                        float F[27];
+                        for (int l = 0; l < 27; ++l)
+                            F[l] = q1[at(iox + (l % 3) - 1, ioy + (l / 3) % 3 - 1, ioz + l / 9 - 1)];
+                        v = triquad_ref(qr[0] - iox, qr[1] - ioy, qr[2] - ioz, F);
+                    }
+                    q2[at(ix, iy, iz)] = q2[at(ix, iy, iz)] + v;
+                }
+            }
+    }
+    const float fn = (float)nsym;
+    for (size_t q = 0; q < nv; ++q) q2[q] = q2[q] / fn;
+}
+
 template <typename R> struct Rot;
 template <> struct Rot<float> {
     static void make(const double* ang, float* dm, int) { cang_ref((float)ang[2], (float)ang[1], (float)ang[0], dm); }
@@ -594,9 +775,13 @@ template <> struct Rot<double> {
 // angles: nimg x 3 doc-file registers (PSI, THETA, PHI) in degrees; the
 // rotation is CANG(PHI, THETA, PSI) (bp3f.f:1290-1291).
 // exact_pi (f64 only): 0 -> the reference's degree conversion (default), 1 -> exact pi.
+// sym: nsym x 3 doc registers of the symmetry operations (BUILDS,
+// builds.f:35-63: CANG(PHI, THETA, PSI) of each row); used when nsym > 1:
+// each projection is inserted once per DMS = SM DM (bp32f.f:866-872) and every
+// output volume is passed through SYMVOL after WINDKB2A (bp32f.f:278-303).
 template <typename R>
 int bp_run(int nx, int nimg, const float* imgs, const double* angles, const int8_t* half,
-           R* out_full, R* out_h1, R* out_h2, R* xw_out, int exact_pi) {
+           R* out_full, R* out_h1, R* out_h2, R* xw_out, int exact_pi, int nsym = 1, const double* sym = nullptr) {
     if (nx < 2) return 101;
     const int N = 2 * nx, H = nx + 1;
     const size_t nh = (size_t)H * N * N;
@@ -607,11 +792,19 @@ int bp_run(int nx, int nimg, const float* imgs, const double* angles, const int8
     kb_table<R>(N, tab.data(), &fltb);
     std::vector<R> bi(2 * (size_t)H * N);
     R dm[9];
+    if (nsym < 1 || (nsym > 1 && !sym)) return 102;
+    std::vector<R> sm(9 * (size_t)nsym);
+    for (int k = 0; k < nsym && nsym > 1; ++k) Rot<R>::make(sym + 3 * k, sm.data() + 9 * k, exact_pi);
     for (int p = 0; p < nimg; ++p) {
         Rot<R>::make(angles + 3 * p, dm, exact_pi);
         spectrum<R>(imgs + (size_t)p * nx * nx, nx, bi.data());
         const int v = half ? (half[p] ? 1 : 0) : 0;
-        insert_projection<R>(nx, X.data() + 2 * nh * v, W.data() + nh * v, bi.data(), dm, tab.data(), fltb);
+        for (int k = 0; k < nsym; ++k) {
+            R dms[9];
+            if (nsym > 1) sym_matmul<R>(sm.data() + 9 * k, dm, dms);
+            else std::copy(dm, dm + 9, dms);
+            insert_projection<R>(nx, X.data() + 2 * nh * v, W.data() + nh * v, bi.data(), dms, tab.data(), fltb);
+        }
     }
     for (int v = 0; v < nvol; ++v) symplane0<R>(nx, X.data() + 2 * nh * v, W.data() + nh * v);
     if (nvol == 2) {
@@ -626,6 +819,17 @@ int bp_run(int nx, int nimg, const float* imgs, const double* angles, const int8
     } else {
         if (xw_out) { std::copy(X.begin(), X.end(), xw_out); std::copy(W.begin(), W.end(), xw_out + 2 * nh); }
         if (out_full) finalize_volume<R>(nx, X.data(), W.data(), out_full);
+    }
+    if (nsym > 1) {
+        // SYMVOL in single precision, as the reference's REAL volumes
+        const size_t nv = (size_t)nx * nx * nx;
+        std::vector<float> q1(nv), q2(nv), smf(sm.begin(), sm.end());
+        for (R* out : {out_full, out_h1, out_h2}) {
+            if (!out) continue;
+            for (size_t q = 0; q < nv; ++q) q1[q] = (float)out[q];
+            symvol_ref(nx, q1.data(), q2.data(), smf.data(), nsym);
+            for (size_t q = 0; q < nv; ++q) out[q] = (R)q2[q];
+        }
     }
     return 0;
 }
@@ -675,6 +879,23 @@ void or_insert_f32_acc64(int nx, double* X, double* W, const float* bi, const fl
     const int N = 2 * nx;
     for (int J = -nx + 1; J <= nx; ++J) oneline<float, double>(J, N, nx, X, W, bi, dm, tab, fltb);
 }
+// Gather of nproj central slices from the FP64 half grid X (2 (NX+1) N N
+// doubles) into out (nproj x N x (NX+1) complex, BI layout); fp32 table tab
+// (5000 entries, step fltb) and rotations dm (nproj x 9), FP64 sums.
+void or_gather(int nx, const double* X, int nproj, const float* dm, const float* tab, float fltb, double* out) {
+    const int N = 2 * nx, H = nx + 1;
+    for (int p = 0; p < nproj; ++p)
+        for (int J = -nx + 1; J <= nx; ++J)
+            gather_line(J, N, nx, X, out + 2 * (size_t)p * N * H, dm + 9 * (size_t)p, tab, fltb);
+}
+// ONELINE scatter of FP64 BI spectra with an fp32 table and rotations, FP64
+// terms and sums: the transpose of or_gather with the same table.
+void or_scatter(int nx, const double* bi, int nproj, const float* dm, const float* tab, float fltb, double* X, double* W) {
+    const int N = 2 * nx, H = nx + 1;
+    for (int p = 0; p < nproj; ++p)
+        for (int J = -nx + 1; J <= nx; ++J)
+            oneline<float, double, double>(J, N, nx, X, W, bi + 2 * (size_t)p * N * H, dm + 9 * (size_t)p, tab, fltb);
+}
 void or_symplane0_f32(int nx, float* X, float* W) { symplane0<float>(nx, X, W); }
 void or_symplane0_f64(int nx, double* X, double* W) { symplane0<double>(nx, X, W); }
 void or_normalize_f32(int nx, float* X, const float* W) { normalize<float>(nx, X, W); }
@@ -694,6 +915,13 @@ int or_bp_f64(int nx, int nimg, const float* imgs, const double* angles, const i
               double* out_full, double* out_h1, double* out_h2, double* xw_out, int exact_pi) {
     return bp_run<double>(nx, nimg, imgs, angles, half, out_full, out_h1, out_h2, xw_out, exact_pi);
 }
+int or_bp_sym_f32(int nx, int nimg, const float* imgs, const double* angles, const int8_t* half,
+                  float* out_full, float* out_h1, float* out_h2, int nsym, const double* sym) {
+    return bp_run<float>(nx, nimg, imgs, angles, half, out_full, out_h1, out_h2, nullptr, 0, nsym, sym);
+}
+void or_sym_matmul_f32(const float* sm, const float* dm, float* out) { sym_matmul<float>(sm, dm, out); }
+float or_triquad(double r, double s, double t, const float* f) { return triquad_ref(r, s, t, f); }
+void or_symvol(int nx, const float* q1, float* q2, const float* sm, int nsym) { symvol_ref(nx, q1, q2, sm, nsym); }
 
 // Number of distinct half-grid voxels that receive a nonzero KB weight from one
 // projection (Friedel-reflected targets counted once): the unit U(NX) of the

@@ -36,6 +36,7 @@ K_PAD = 1
 K_FFT2 = 2
 K_FINALIZE = 3
 K_BIN = 4
+K_GATHER = 5
 
 STRAT_AUTO = 0
 STRAT_DIRECT = 1
@@ -46,6 +47,15 @@ class SpbpError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"spbp error {code}: {msg}")
         self.code = code
+
+
+class Job(C.Structure):
+    _fields_ = [("ctx", C.c_void_p), ("imgs", C.c_void_p), ("dtype", C.c_int), ("nimg", C.c_int),
+                ("angles_deg", C.c_void_p), ("dm", C.c_void_p), ("half", C.c_void_p)]
+
+
+SHAPES = [(16, 16, 8), (16, 8, 8), (8, 8, 8)]
+TUNE_C = [16, 32, 64, 128]
 
 
 class Opts(C.Structure):
@@ -82,9 +92,24 @@ def lib():
         L.spbp_export_raw.argtypes = [vp, C.c_int, vp, vp, vp]
         L.spbp_kernel_stats.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_long), C.POINTER(C.c_long)]
         L.spbp_reset_stats.argtypes = [vp]
+        L.spbp_gather.argtypes = [vp, vp, C.c_int, vp, vp, vp, vp]
+        L.spbp_scatter.argtypes = [vp, vp, C.c_int, vp, vp, vp, vp, vp]
+        L.spbp_kb_table.argtypes = [C.c_int, vp, C.POINTER(C.c_float)]
+        L.spbp_tune.argtypes = [vp, vp, C.c_int, C.c_int, vp, vp, vp]
+        L.spbp_set_symmetries.argtypes = [vp, C.c_int, vp]
+        L.spbp_get_plan.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), vp]
+        L.spbp_set_plan.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+        L.spbp_insert_batch.argtypes = [vp, C.c_int, vp]
+        L.spbp_graph_create.argtypes = [vp, C.c_int, C.c_int, C.POINTER(vp)]
+        L.spbp_graph_launch.argtypes = [vp, vp]
+        L.spbp_graph_destroy.argtypes = [vp]
+        L.spbp_rotations.argtypes = [C.c_int, vp, C.c_int, vp]
         for name in ("spbp_create", "spbp_destroy", "spbp_reset", "spbp_insert", "spbp_insert_dm",
                      "spbp_insert_host", "spbp_finalize", "spbp_finalize_host", "spbp_export_partial",
-                     "spbp_import_partial", "spbp_export_raw", "spbp_kernel_stats", "spbp_reset_stats"):
+                     "spbp_import_partial", "spbp_export_raw", "spbp_kernel_stats", "spbp_reset_stats",
+                     "spbp_gather", "spbp_scatter", "spbp_kb_table", "spbp_rotations", "spbp_tune",
+                     "spbp_get_plan", "spbp_set_plan", "spbp_insert_batch", "spbp_graph_create",
+                     "spbp_graph_launch", "spbp_graph_destroy", "spbp_set_symmetries"):
             getattr(L, name).restype = C.c_int
         _LIB = L
     return _LIB
@@ -102,3 +127,22 @@ def check(rc: int, ctx=None) -> None:
     if rc != OK:
         msg = lib().spbp_last_error(ctx)
         raise SpbpError(rc, msg.decode() if msg else "")
+
+
+def kb_table(n2: int):
+    """(TABI(0:4999) float32, FLTB) for padded size n2, from the library."""
+    import numpy as np
+    tab = np.zeros(5000, np.float32)
+    f = C.c_float()
+    check(lib().spbp_kb_table(int(n2), tab.ctypes.data_as(C.c_void_p), C.byref(f)))
+    return tab, np.float32(f.value)
+
+
+def rotations(angles, exact_pi: bool = False):
+    """CANG matrices (n, 9) float32 of doc registers (PSI, THETA, PHI), from the library."""
+    import numpy as np
+    a = np.ascontiguousarray(np.asarray(angles, np.float64).reshape(-1, 3))
+    dm = np.zeros((a.shape[0], 9), np.float32)
+    check(lib().spbp_rotations(a.shape[0], a.ctypes.data_as(C.c_void_p), EXACT_PI if exact_pi else 0,
+                               dm.ctypes.data_as(C.c_void_p)))
+    return dm

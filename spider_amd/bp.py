@@ -43,7 +43,7 @@ class Backprojector:
 
     def __init__(self, nx: int, two_halves: bool = False, exact_pi: bool = False,
                  twosum: bool = False, check_finite: bool = False, device=None, batch: int = 0,
-                 timing: bool = False, strategy: str = "auto", brick_proj: int = 0):
+                 timing: bool = False, strategy: str = "auto", brick_proj: int = 0, symmetries=None):
         torch = _torch()
         if not torch.cuda.is_available():
             raise RuntimeError("spider_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
@@ -59,6 +59,8 @@ class Backprojector:
         self._ctx = ctx
         self.nx = int(nx)
         self.two_halves = bool(two_halves)
+        if symmetries is not None:
+            self.set_symmetries(symmetries)
 
     # -------------------------------------------------------------- lifecycle
     def close(self) -> None:
@@ -88,6 +90,15 @@ class Backprojector:
 
     def reset(self) -> None:
         self._check(_lib.lib().spbp_reset(self._ctx, _stream(self.device)))
+
+    def set_symmetries(self, angles) -> None:
+        """Symmetry operations as doc rows (PSI, THETA, PHI), (nsym, 3); one
+        row or None switches symmetrisation off (bp32f.f:133-172)."""
+        if angles is None:
+            self._check(_lib.lib().spbp_set_symmetries(self._ctx, 0, None))
+            return
+        a = np.ascontiguousarray(np.asarray(angles, np.float64).reshape(-1, 3))
+        self._check(_lib.lib().spbp_set_symmetries(self._ctx, a.shape[0], a.ctypes.data_as(C.c_void_p)))
 
     # -------------------------------------------------------------- insertion
     def _images(self, imgs):
@@ -141,6 +152,111 @@ class Backprojector:
                                               None if hp is None else hp.ctypes.data_as(C.c_void_p),
                                               _stream(self.device)))
         self._keep = (imgs, dm, hp)
+
+    # -------------------------------------------------------------- gather / scatter of spectra
+    def _rotation_args(self, n, angles, dm):
+        if dm is not None:
+            dm = np.ascontiguousarray(np.asarray(dm, np.float32).reshape(-1, 9))
+            if dm.shape[0] != n:
+                raise SpbpError(_lib.ERR_ANGLE, f"{dm.shape[0]} rotations for {n} projections")
+            return None, dm
+        if angles is None:
+            raise SpbpError(_lib.ERR_ANGLE, "MISSING ANGLE FOR IMAGE")
+        ang = np.ascontiguousarray(np.asarray(angles, np.float64).reshape(-1, 3))
+        if ang.shape[0] != n:
+            raise SpbpError(_lib.ERR_ANGLE, f"MISSING ANGLE FOR IMAGE: {ang.shape[0]} angles for {n} projections")
+        return ang, None
+
+    def _grid_tensor(self, X, name):
+        torch = _torch()
+        if not isinstance(X, torch.Tensor) or X.dtype != torch.complex64 or tuple(X.shape) != self.grid_shape:
+            raise ValueError(f"{name} must be a complex64 tensor of shape {self.grid_shape}")
+        if X.device != self.device or not X.is_contiguous():
+            raise ValueError(f"{name} must be contiguous on {self.device}")
+        if X.is_conj() or X.is_neg():
+            raise ValueError(f"{name} is a lazily conjugated / negated view; resolve it first")
+        return X
+
+    def gather(self, X, angles=None, dm=None):
+        """Central-slice KB gather (the adjoint of the insertion) of the half
+        grid X (N, N, NX+1) complex64 for n projections (angles (n, 3) or dm
+        (n, 9)): (n, N, NX+1) complex64 spectra in the BI layout."""
+        torch = _torch()
+        if isinstance(X, torch.Tensor):
+            X = X.resolve_conj().resolve_neg().contiguous()
+        X = self._grid_tensor(X, "X")
+        n = (np.asarray(dm).reshape(-1, 9) if dm is not None else np.asarray(angles).reshape(-1, 3)).shape[0]
+        ang, dmc = self._rotation_args(n, angles, dm)
+        N, _, H = self.grid_shape
+        out = torch.empty((n, N, H), dtype=torch.complex64, device=self.device)
+        self._check(_lib.lib().spbp_gather(self._ctx, C.c_void_p(X.data_ptr()), n,
+                                           None if ang is None else ang.ctypes.data_as(C.c_void_p),
+                                           None if dmc is None else dmc.ctypes.data_as(C.c_void_p),
+                                           C.c_void_p(out.data_ptr()), _stream(self.device)))
+        return out
+
+    def scatter(self, B, angles=None, dm=None, X=None, W=None):
+        """ONELINE scatter of BI-layout spectra B (n, N, NX+1) complex64, added
+        into X (a new zero grid when None) and W when given; returns X."""
+        torch = _torch()
+        N, _, H = self.grid_shape
+        if not isinstance(B, torch.Tensor) or B.dtype != torch.complex64 or B.dim() != 3 or tuple(B.shape[1:]) != (N, H):
+            raise ValueError(f"B must be complex64 (n, {N}, {H})")
+        if B.device != self.device:
+            raise ValueError(f"B on {B.device}, context on {self.device}")
+        B = B.resolve_conj().resolve_neg().contiguous()
+        n = B.shape[0]
+        ang, dmc = self._rotation_args(n, angles, dm)
+        if X is None:
+            X = torch.zeros(self.grid_shape, dtype=torch.complex64, device=self.device)
+        X = self._grid_tensor(X, "X")
+        wp = None
+        if W is not None:
+            if W.dtype != torch.float32 or tuple(W.shape) != self.grid_shape or not W.is_contiguous() or W.device != self.device:
+                raise ValueError(f"W must be contiguous float32 {self.grid_shape} on {self.device}")
+            wp = C.c_void_p(W.data_ptr())
+        self._check(_lib.lib().spbp_scatter(self._ctx, C.c_void_p(B.data_ptr()), n,
+                                            None if ang is None else ang.ctypes.data_as(C.c_void_p),
+                                            None if dmc is None else dmc.ctypes.data_as(C.c_void_p),
+                                            C.c_void_p(X.data_ptr()), wp, _stream(self.device)))
+        self._keep = (B,)
+        return X
+
+    # -------------------------------------------------------------- plans
+    def tune(self, imgs, angles=None, dm=None) -> dict:
+        """Time every insertion plan on a sample of these images (up to 1024;
+        the accumulators are untouched) and keep the fastest.  Returns plan()."""
+        imgs, dt = self._images(imgs)
+        n = imgs.shape[0]
+        ang, dmc = self._rotation_args(n, angles, dm)
+        self._check(_lib.lib().spbp_tune(self._ctx, C.c_void_p(imgs.data_ptr()), dt, n,
+                                         None if ang is None else ang.ctypes.data_as(C.c_void_p),
+                                         None if dmc is None else dmc.ctypes.data_as(C.c_void_p),
+                                         _stream(self.device)))
+        return self.plan()
+
+    def plan(self) -> dict:
+        st, sh, cp = C.c_int(), C.c_int(), C.c_int()
+        ms = (C.c_float * (len(_lib.SHAPES) * len(_lib.TUNE_C) + 1))()
+        self._check(_lib.lib().spbp_get_plan(self._ctx, C.byref(st), C.byref(sh), C.byref(cp), ms))
+        table = np.array(ms[:-1], np.float64).reshape(len(_lib.SHAPES), len(_lib.TUNE_C))
+        return {"strategy": {_lib.STRAT_DIRECT: "direct", _lib.STRAT_BRICK: "brick"}[st.value],
+                "brick": _lib.SHAPES[sh.value], "shape": sh.value, "brick_proj": cp.value,
+                "ms_per_proj": table, "direct_ms_per_proj": float(ms[-1])}
+
+    def set_plan(self, strategy="brick", shape: int = 0, brick_proj: int = 0) -> None:
+        strat = {"auto": _lib.STRAT_AUTO, "direct": _lib.STRAT_DIRECT, "brick": _lib.STRAT_BRICK}[strategy]
+        self._check(_lib.lib().spbp_set_plan(self._ctx, strat, int(shape), int(brick_proj)))
+
+    def _job(self, imgs, angles=None, dm=None, half=None):
+        imgs, dt = self._images(imgs)
+        n = imgs.shape[0]
+        ang, dmc = self._rotation_args(n, angles, dm)
+        hp = None if half is None else np.ascontiguousarray(np.asarray(half).astype(np.int8).reshape(n))
+        job = _lib.Job(self._ctx.value, imgs.data_ptr(), dt, n,
+                       None if ang is None else ang.ctypes.data, None if dmc is None else dmc.ctypes.data,
+                       None if hp is None else hp.ctypes.data)
+        return job, (imgs, ang, dmc, hp)
 
     # -------------------------------------------------------------- finalisation
     def finalize(self, full=True, halves=False) -> dict:
@@ -203,7 +319,8 @@ class Backprojector:
 
 
 def bp3f(imgs, angles, exact_pi: bool = False, **kw):
-    """`BP 3F`: one volume from all projections."""
+    """`BP 3F`: one volume from all projections (symmetries=(nsym, 3) doc rows
+    optional)."""
     with Backprojector(imgs.shape[-1], exact_pi=exact_pi, device=imgs.device.index, **kw) as bp:
         bp.insert(imgs, angles)
         return bp.finalize()["full"]
@@ -221,11 +338,27 @@ def bp32f(imgs, angles, half=None, seed=None, exact_pi: bool = False, **kw):
 
 
 # ------------------------------------------------------------------ file-level operations
+def read_symmetries(symdoc, ext: str = ""):
+    """Symmetry doc rows (PSI, THETA, PHI) for keys 1..max key, or None for
+    '*' (GETDOCDAT + BUILDS, bp32f.f:133-172, builds.f:44-52)."""
+    from . import spiderio
+    if symdoc in (None, "*", ""):
+        return None
+    path = symdoc if (not ext or symdoc.endswith(ext)) else symdoc + ext
+    doc = spiderio.read_doc(path)
+    if not doc:
+        return None
+    rows = []
+    for k in range(1, max(doc) + 1):
+        if k not in doc or len(doc[k]) < 3:
+            raise SpbpError(_lib.ERR_ANGLE, f"MISSING SYMMETRY {k}")
+        rows.append(doc[k][:3])
+    return np.asarray(rows, np.float64)
+
+
 def _load_series(template, numbers, angdoc, symdoc, ext, device):
     from . import spiderio
     torch = _torch()
-    if symdoc not in (None, "*", ""):
-        raise NotImplementedError("symmetry documents are not supported (answer '*')")
     imgs = spiderio.read_images(template, numbers, ext)
     if imgs.shape[1] != imgs.shape[2]:
         raise SpbpError(_lib.ERR_SIZE, "ONLY WORKS ON SQUARE IMAGES")
@@ -242,7 +375,7 @@ def bp3f_files(template, numbers, angdoc, symdoc, out, ext: str = "", device=Non
     doc ('*'), reconstructed volume (the answers of bp3f.f:127-283)."""
     from . import spiderio
     imgs, ang = _load_series(template, numbers, angdoc, symdoc, ext, device)
-    vol = bp3f(imgs, ang, **kw).cpu().numpy()
+    vol = bp3f(imgs, ang, symmetries=read_symmetries(symdoc, ext), **kw).cpu().numpy()
     spiderio.write_image(out, vol)
     return vol
 
@@ -252,7 +385,7 @@ def bp32f_files(template, numbers, angdoc, symdoc, out_full, out_h1, out_h2, ext
     """`BP 32F`: as BP 3F plus the two sample volumes (bp32f.f:102-410)."""
     from . import spiderio
     imgs, ang = _load_series(template, numbers, angdoc, symdoc, ext, device)
-    full, h1, h2 = bp32f(imgs, ang, half=half, seed=seed, **kw)
+    full, h1, h2 = bp32f(imgs, ang, half=half, seed=seed, symmetries=read_symmetries(symdoc, ext), **kw)
     vols = [v.cpu().numpy() for v in (full, h1, h2)]
     for path, v in zip((out_full, out_h1, out_h2), vols):
         spiderio.write_image(path, v)
@@ -264,14 +397,15 @@ def bp3f_partial_files(template, numbers, angdoc, symdoc, out, ext: str = "", de
     (bp3f.f:191-248) instead of a volume."""
     from . import spiderio
     imgs, ang = _load_series(template, numbers, angdoc, symdoc, ext, device)
-    with Backprojector(imgs.shape[-1], device=imgs.device.index, **kw) as b:
+    with Backprojector(imgs.shape[-1], device=imgs.device.index, symmetries=read_symmetries(symdoc, ext), **kw) as b:
         b.insert(imgs, ang)
         X, W = b.export_partial()
         return spiderio.write_partial(out, X.cpu().numpy(), W.cpu().numpy())
 
 
-def bp3f_merge_files(partials, out, device=None, **kw):
-    """`BP 3F M`: sum partial volumes and finalise (bp3f.f:531-739)."""
+def bp3f_merge_files(partials, out, device=None, symdoc=None, ext: str = "", **kw):
+    """`BP 3F M`: sum partial volumes and finalise (bp3f.f:531-739); with a
+    symmetries doc the merged volume is symmetrised by SYMVOL (bp3f.f:700-722)."""
     from . import spiderio
     torch = _torch()
     pairs = [spiderio.read_partial(p) for p in partials]
@@ -279,9 +413,61 @@ def bp3f_merge_files(partials, out, device=None, **kw):
         raise SpbpError(_lib.ERR_ARG, "no partial volumes")
     nx = pairs[0][1].shape[2] - 1
     dev = torch.device("cuda", torch.cuda.current_device() if device is None else int(device))
-    with Backprojector(nx, device=dev.index, **kw) as b:
+    with Backprojector(nx, device=dev.index, symmetries=read_symmetries(symdoc, ext), **kw) as b:
         for X, W in pairs:
             b.import_partial(torch.from_numpy(X).to(dev), torch.from_numpy(W).to(dev), accumulate=True)
         vol = b.finalize()["full"].cpu().numpy()
     spiderio.write_image(out, vol)
     return vol
+
+
+def _jobs(items):
+    """items: (Backprojector, imgs, angles) or (Backprojector, imgs, angles, half)
+    tuples, or dicts with keys ctx, imgs, angles / dm, half."""
+    jobs, keep = [], []
+    for it in items:
+        if isinstance(it, dict):
+            j, k = it["ctx"]._job(it["imgs"], it.get("angles"), it.get("dm"), it.get("half"))
+        else:
+            ctx, imgs, angles = it[0], it[1], it[2]
+            j, k = ctx._job(imgs, angles, None, it[3] if len(it) > 3 else None)
+        jobs.append(j)
+        keep.append(k)
+    arr = (_lib.Job * len(jobs))(*jobs)
+    return arr, keep
+
+
+def insert_batch(items, device=None) -> None:
+    """Insert several reconstructions (any box sizes) with one persistent
+    scatter launch per round of chunks (spbp_insert_batch)."""
+    arr, keep = _jobs(items)
+    _lib.check(_lib.lib().spbp_insert_batch(arr, len(arr), _stream(device)))
+    return keep
+
+
+class BatchGraph:
+    """A batch of insertions captured once as a HIP graph and replayed:
+    `launch()` re-runs the whole batch (zeroing the accumulators first when
+    reset_first) on torch's current stream.  Images are re-read from the same
+    device tensors at every launch; the geometry is fixed at capture."""
+
+    def __init__(self, items, reset_first: bool = True):
+        arr, self._keep = _jobs(items)
+        g = C.c_void_p()
+        _lib.check(_lib.lib().spbp_graph_create(arr, len(arr), 1 if reset_first else 0, C.byref(g)))
+        self._g = g
+        self.device = _torch().device("cuda", _torch().cuda.current_device())
+
+    def launch(self) -> None:
+        _lib.check(_lib.lib().spbp_graph_launch(self._g, _stream(self.device)))
+
+    def close(self) -> None:
+        if getattr(self, "_g", None) is not None and self._g.value:
+            _lib.lib().spbp_graph_destroy(self._g)
+            self._g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -13,10 +13,11 @@ constexpr int LTAB = 4999;   // KB table length - 1 (oneline.f:44)
 constexpr int TPB = 256;
 
 
-struct Geom {        // per projection: CANG's matrix DM(1..9) + half flag
+struct Geom {        // per inserted section: CANG's matrix DM(1..9) (times a symmetry), half flag
     float r[9];
     int half;
-    int pad_[2];
+    int img;         // image (spectrum) index, counted from the call's first image
+    int pad_;
 };
 
 

@@ -165,4 +165,90 @@ __global__ __launch_bounds__(TPB) void import_kernel(const float2* __restrict__ 
     W[idx] = w;
 }
 
+// ------------------------------------------------------------------ SYMVOL
+// TRIQUAD (triquad.f): 27-point triquadratic interpolation at offsets
+// (R, S, T) from the centre point F[13].  Implicit typing in the reference
+// makes RS, ST, RT, RST and the factors 1-R^2, 1-R, 1+R, ... REAL (rounded
+// from DOUBLE); a term multiplied by R, S or T itself is evaluated in DOUBLE;
+// the sum runs left to right, in REAL until the first DOUBLE term; the result
+// is REAL.  Reproduced operation by operation (no contraction).
+__device__ inline float triquad(double R, double S, double T, const float* F) {
+    const float C2 = 0.5f, C4 = 0.25f, C8 = 0.125f;
+    const float RS = (float)(R * S), ST = (float)(S * T), RT = (float)(R * T);
+    const float RST = (float)(R * (double)ST);
+    const float RSQ = (float)(1.0 - R * R), SSQ = (float)(1.0 - S * S), TSQ = (float)(1.0 - T * T);
+    const float RM1 = (float)(1.0 - R), SM1 = (float)(1.0 - S), TM1 = (float)(1.0 - T);
+    const float RP1 = (float)(1.0 + R), SP1 = (float)(1.0 + S), TP1 = (float)(1.0 + T);
+    float a = (-C8) * RST * RM1 * SM1 * TM1 * F[0];
+    a = a + (C4) * ST * RSQ * SM1 * TM1 * F[1];
+    a = a + (C8) * RST * RP1 * SM1 * TM1 * F[2];
+    a = a + (C4) * RT * RM1 * SSQ * TM1 * F[3];
+    double d = (double)a + (double)(-C2) * T * RSQ * SSQ * TM1 * F[4];
+    d = d + (double)((-C4) * RT * RP1 * SSQ * TM1 * F[5]);
+    d = d + (double)((C8) * RST * RM1 * SP1 * TM1 * F[6]);
+    d = d + (double)((-C4) * ST * RSQ * SP1 * TM1 * F[7]);
+    d = d + (double)((-C8) * RST * RP1 * SP1 * TM1 * F[8]);
+    d = d + (double)((C4) * RS * RM1 * SM1 * TSQ * F[9]);
+    d = d + (double)(-C2) * S * RSQ * SM1 * TSQ * F[10];
+    d = d + (double)((-C4) * RS * RP1 * SM1 * TSQ * F[11]);
+    d = d + (double)(-C2) * R * RM1 * SSQ * TSQ * F[12];
+    d = d + (double)(RSQ * SSQ * TSQ * F[13]);
+    d = d + (double)(C2) * R * RP1 * SSQ * TSQ * F[14];
+    d = d + (double)((-C4) * RS * RM1 * SP1 * TSQ * F[15]);
+    d = d + (double)(C2) * S * RSQ * SP1 * TSQ * F[16];
+    d = d + (double)((C4) * RS * RP1 * SP1 * TSQ * F[17]);
+    d = d + (double)((C8) * RST * RM1 * SM1 * TP1 * F[18]);
+    d = d + (double)((-C4) * ST * RSQ * SM1 * TP1 * F[19]);
+    d = d + (double)((-C8) * RST * RP1 * SM1 * TP1 * F[20]);
+    d = d + (double)((-C4) * RT * RM1 * SSQ * TP1 * F[21]);
+    d = d + (double)(C2) * T * RSQ * SSQ * TP1 * F[22];
+    d = d + (double)((C4) * RT * RP1 * SSQ * TP1 * F[23]);
+    d = d + (double)((-C8) * RST * RM1 * SP1 * TP1 * F[24]);
+    d = d + (double)((C4) * ST * RSQ * SP1 * TP1 * F[25]);
+    d = d + (double)((C8) * RST * RP1 * SP1 * TP1 * F[26]);
+    return (float)d;
+}
+
+// SYMVOL (symvol.f:39-219): q2(r) = (1/NSYM) sum_k v_k(r) for r in
+// [KLX, KNX]^3, KLX = -NX/2, KNX = NX/2 - 1 (even NX) or NX/2 (odd);
+// v_k = q1 sampled at RM_k^T r by TRIQUAD when |r|^2 <= (min|bound| - 1)^2 and
+// RM_k is not the identity (diagonal all 1.0), else q1(r).  Summed in the
+// reference's order (k = 1..NSYM) in fp32.  One thread per voxel.
+__global__ __launch_bounds__(TPB) void symvol_kernel(const float* __restrict__ q1, float* __restrict__ q2, int nx,
+                                                     const float* __restrict__ sm, int nsym) {
+    const size_t nv = (size_t)nx * nx * nx;
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nv) return;
+    const int klx = -nx / 2, knx = (nx % 2 == 0) ? nx / 2 - 1 : nx / 2;
+    const int mn = min(abs(klx), abs(knx));
+    const int iradi = (mn - 1) * (mn - 1);
+    const int ix = (int)(idx % nx) + klx, iy = (int)((idx / nx) % nx) + klx, iz = (int)(idx / ((size_t)nx * nx)) + klx;
+    const int icrd = ix * ix + iy * iy + iz * iz;
+    float acc = 0.0f;
+    for (int k = 0; k < nsym; ++k) {
+        const float* rm = sm + 9 * k;
+        const bool ident = rm[0] == 1.0f && rm[4] == 1.0f && rm[8] == 1.0f;
+        float v;
+        if (ident || icrd > iradi) {
+            v = q1[idx];
+        } else {
+            double qr[3];
+            for (int c = 0; c < 3; ++c) {
+                const double qrt = (double)__fadd_rn(__fmul_rn(rm[3 * c + 1], (float)iy), __fmul_rn(rm[3 * c + 2], (float)iz));
+                qr[c] = qrt + (double)__fmul_rn(rm[3 * c], (float)ix);
+            }
+            const int iox = (int)floor(qr[0]), ioy = (int)floor(qr[1]), ioz = (int)floor(qr[2]);
+            float F[27];
+#pragma unroll
+            for (int l = 0; l < 27; ++l) {
+                const int x = iox + (l % 3) - 1 - klx, y = ioy + (l / 3) % 3 - 1 - klx, z = ioz + l / 9 - 1 - klx;
+                F[l] = q1[((size_t)z * nx + y) * nx + x];
+            }
+            v = triquad(qr[0] - iox, qr[1] - ioy, qr[2] - ioz, F);
+        }
+        acc = __fadd_rn(acc, v);
+    }
+    q2[idx] = acc / (float)nsym;
+}
+
 }  // namespace spbp

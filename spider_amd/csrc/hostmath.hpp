@@ -73,5 +73,16 @@ void cang(float phi, float theta, float psi, bool exact_pi, float* dm) {
 }
 
 
+// DMS = MATMUL(SM, DM) (bp32f.f:866-872) for 3x3 matrices in Fortran (3,3)
+// storage (element (i, j) at [i + 3 j]); fp32 products and sums, k = 1..3.
+void sym_matmul(const float* sm, const float* dm, float* out) {
+    for (int j = 0; j < 3; ++j)
+        for (int i = 0; i < 3; ++i) {
+            float acc = sm[i] * dm[3 * j];
+            acc = acc + sm[i + 3] * dm[1 + 3 * j];
+            acc = acc + sm[i + 6] * dm[2 + 3 * j];
+            out[i + 3 * j] = acc;
+        }
+}
 
 }  // namespace spbp

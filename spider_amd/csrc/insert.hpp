@@ -40,10 +40,11 @@ __global__ __launch_bounds__(TPB) void pad_kernel(const T* __restrict__ imgs, in
 // One thread per sample point (I, J) of one projection, FP32 atomics straight
 // into the half grid (the reference's ONELINE loop, oneline.f:53-156).
 __global__ __launch_bounds__(TPB) void insert_direct_kernel(
-    const float2* __restrict__ spec, const Geom* __restrict__ geom, int nx, int N,
+    const float2* __restrict__ spec, int img0, const Geom* __restrict__ geom, int nx, int N,
     const float* __restrict__ tab, float fltb,
     float2* __restrict__ X0, float* __restrict__ W0, float2* __restrict__ X1, float* __restrict__ W1,
     float* __restrict__ Wc0, float* __restrict__ Wc1) {
+    // W0/W1 may be null (no weights wanted: scatter of BI spectra)
     const int H = nx + 1;
     const int p = blockIdx.y;
     const int lin = blockIdx.x * blockDim.x + threadIdx.x;
@@ -55,7 +56,7 @@ __global__ __launch_bounds__(TPB) void insert_direct_kernel(
     const Geom g = geom[p];
     float x, y, z;
     sample_pos(g, I, J, x, y, z);
-    float2 v = spectrum_value(spec + (size_t)p * N * H, I, J, N, H);
+    float2 v = spectrum_value(spec + (size_t)(g.img - img0) * N * H, I, J, N, H);
     if (x < 0.0f) { x = -x; y = -y; z = -z; v.y = -v.y; }          // oneline.f:60-67
     const int ix = nearest_voxel(x, N), iy = nearest_voxel(y, N), iz = nearest_voxel(z, N);
     if (!in_bounds(ix, iy, iz, nx)) return;
@@ -91,13 +92,13 @@ __global__ __launch_bounds__(TPB) void insert_direct_kernel(
                     unsafeAtomicAdd(&X[o].x, __fmul_rn(v.x, wg));
                     unsafeAtomicAdd(&X[o].y, __fmul_rn(v.y, wg));
                     if (Wc) twosum_atomic_add(W, Wc, o, wg, 0.0f);
-                    else unsafeAtomicAdd(&W[o], wg);
+                    else if (W) unsafeAtomicAdd(&W[o], wg);
                 } else {                                         // reflected taps, conj (oneline.f:118-152)
                     const size_t o = zt + yt + (size_t)(-px);
                     unsafeAtomicAdd(&X[o].x, __fmul_rn(v.x, wg));
                     unsafeAtomicAdd(&X[o].y, __fmul_rn(-v.y, wg));
                     if (Wc) twosum_atomic_add(W, Wc, o, wg, 0.0f);
-                    else unsafeAtomicAdd(&W[o], wg);
+                    else if (W) unsafeAtomicAdd(&W[o], wg);
                 }
             }
         }
@@ -383,34 +384,68 @@ __device__ __forceinline__ void wave_project(BrickLds<BX, BY, BZ>& L, const Geom
     }
 }
 
-// Persistent workgroups pull (key, projection slice) items from a queue; the
-// four waves of a workgroup take the item's projections in turn and share the
-// brick through LDS atomics.
+// One insertion job for the brick kernel: a context's chunk of projections
+// (spectra, geometry), its table, brick grid and lists, and its half grids.
+struct BrickJob {
+    const float2* spec;   // spectra of images img0, img0 + 1, ...
+    int img0;
+    const Geom* geom;
+    const float* tab;
+    float fltb;
+    int nx, N, C;
+    BrickGrid bg;
+    const int* cnt;
+    const int* off;
+    const int* item_off;
+    const int* item_key;
+    const int* list;
+    float2* X0;
+    float* W0;
+    float2* X1;
+    float* W1;
+    float* Wc0;
+    float* Wc1;
+};
+constexpr int MAX_JOBS = 64;
+
+// Persistent workgroups pull (job, key, projection slice) items from one
+// queue spanning every job of the launch (job-major order); the four waves of
+// a workgroup take the item's projections in turn and share the brick
+// through LDS atomics.  The queue head must be zero at launch.
 template <int BX, int BY, int BZ>
-__global__ __launch_bounds__(TPB) void insert_brick_kernel(
-    const float2* __restrict__ spec, const Geom* __restrict__ geom, int nx, int N,
-    const float* __restrict__ tab, float fltb, BrickGrid bg, int C,
-    const int* __restrict__ cnt, const int* __restrict__ off, const int* __restrict__ item_off,
-    const int* __restrict__ item_key, const int* __restrict__ list, int* __restrict__ next_item,
-    float2* __restrict__ X0, float* __restrict__ W0, float2* __restrict__ X1, float* __restrict__ W1,
-    float* __restrict__ Wc0, float* __restrict__ Wc1) {
+__global__ __launch_bounds__(TPB) void insert_brick_kernel(const BrickJob* __restrict__ jobs, int njobs,
+                                                           int* __restrict__ next_item) {
     constexpr int NV = BX * BY * BZ;
     __shared__ BrickLds<BX, BY, BZ> L;
-    const int nkeys = 2 * bg.nb;
-    const int total = item_off[nkeys];
-    const int H = nx + 1;
+    __shared__ int jbase[MAX_JOBS + 1];
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int j = 0; j < njobs; ++j) {
+            jbase[j] = acc;
+            acc += jobs[j].item_off[2 * jobs[j].bg.nb];
+        }
+        jbase[njobs] = acc;
+    }
+    __syncthreads();
+    const int total = jbase[njobs];
     const int wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
     for (;;) {
         if (threadIdx.x == 0) L.item = atomicAdd(next_item, 1);
         __syncthreads();
-        const int item = L.item;
+        const int gitem = L.item;
         __syncthreads();
-        if (item >= total) break;
-        const int key = item_key[item];
+        if (gitem >= total) break;
+        int j = 0;
+        while (gitem >= jbase[j + 1]) ++j;
+        const BrickJob& J = jobs[j];
+        const int nx = J.nx, N = J.N, H = nx + 1;
+        const BrickGrid bg = J.bg;
+        const int item = gitem - jbase[j];
+        const int key = J.item_key[item];
         const int b = key % bg.nb, half = key / bg.nb;
-        const int s0 = (item - item_off[key]) * C;
-        const int ns = min(C, cnt[key] - s0);
-        const int* lst = list + off[key] + s0;
+        const int s0 = (item - J.item_off[key]) * J.C;
+        const int ns = min(J.C, J.cnt[key] - s0);
+        const int* lst = J.list + J.off[key] + s0;
         int x0, y0, z0;
         brick_origin(bg, b, nx, x0, y0, z0);
         for (int i = threadIdx.x; i < L.NS; i += blockDim.x) { L.re[i] = 0.0; L.im[i] = 0.0; L.w[i] = 0.0; }
@@ -420,30 +455,33 @@ __global__ __launch_bounds__(TPB) void insert_brick_kernel(
         bx.hi[0] = x0 + BX - 1 + BRICK_MARGIN; bx.hi[1] = y0 + BY - 1 + BRICK_MARGIN; bx.hi[2] = z0 + BZ - 1 + BRICK_MARGIN;
         Box mb;
         for (int k = 0; k < 3; ++k) { mb.lo[k] = -bx.hi[k]; mb.hi[k] = -bx.lo[k]; }
+        const float* __restrict__ tab = J.tab;
+        const float fltb = J.fltb;
         for (int s = wave; s < ns; s += nwave) {
             const int p = lst[s];
-            const Geom g = geom[p];
-            const float2* spec_p = spec + (size_t)p * N * H;
+            const Geom g = J.geom[p];
+            const float2* spec_p = J.spec + (size_t)(g.img - J.img0) * N * H;
             wave_project<BX, BY, BZ>(L, g, spec_p, nx, N, x0, y0, z0, bx, false, bx, tab, fltb);
             if (x0 <= LN2)
                 wave_project<BX, BY, BZ>(L, g, spec_p, nx, N, x0, y0, z0, mb, true, bx, tab, fltb);
         }
         __syncthreads();
         // add the brick to the half grid
-        float2* __restrict__ X = half ? X1 : X0;
-        float* __restrict__ W = half ? W1 : W0;
-        float* __restrict__ Wc = half ? Wc1 : Wc0;   // null unless compensated weights
+        float2* __restrict__ X = half ? J.X1 : J.X0;
+        float* __restrict__ W = half ? J.W1 : J.W0;
+        float* __restrict__ Wc = half ? J.Wc1 : J.Wc0;   // null unless compensated weights
         for (int v = threadIdx.x; v < NV; v += blockDim.x) {
             const int lx = v % BX, ly = (v / BX) % BY, lz = v / (BX * BY);
             const int i = lz * L.SZ + ly * L.SY + lx;
             if (L.w[i] == 0.0) continue;
             const int kx = x0 + lx, ky = y0 + ly, kz = z0 + lz;
+            if (kx > nx || ky > nx || kz > nx) continue;   // brick overhangs the grid (partial last brick)
             const size_t o = ((size_t)(kz >= 0 ? kz : N + kz) * N + (ky >= 0 ? ky : N + ky)) * H + kx;
             unsafeAtomicAdd(&X[o].x, (float)L.re[i]);
             unsafeAtomicAdd(&X[o].y, (float)L.im[i]);
             const float wi = (float)L.w[i];
             if (Wc) twosum_atomic_add(W, Wc, o, wi, (float)(L.w[i] - (double)wi));
-            else unsafeAtomicAdd(&W[o], wi);
+            else if (W) unsafeAtomicAdd(&W[o], wi);
         }
         __syncthreads();
     }

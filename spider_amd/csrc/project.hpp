@@ -1,0 +1,94 @@
+// project.hpp -- central-slice KB gather: the exact adjoint of the ONELINE
+// scatter (oneline.f:38-158), and the BI -> raw spectrum conversion that lets
+// BI-layout spectra go through the insertion kernels.
+#pragma once
+#include "common.hpp"
+
+namespace spbp {
+
+// For lattice sample (I, J) of projection p, ONELINE adds w * v to X at each
+// direct tap t (t_x >= 0) and w * conj(v) at -t for each reflected tap
+// (t_x < 0), where v is BI(I, J), conjugated when the sample is Friedel-
+// flipped (x < 0).  Viewing complex numbers as pairs of reals, the transpose
+// of that map is
+//     out(I, J) = c( sum_direct w X[t] + conj(sum_reflected w X[-t]) ),
+// c = conj when flipped, identity otherwise; samples ONELINE skips give 0.
+// The taps, the table lookups and the fp32 weights w are computed exactly as
+// the insertion kernels compute them, and the sum is accumulated in FP64.
+// One thread per output entry; out is nproj x N x H in the BI layout (row
+// jp = J mod N, I fastest).
+__global__ __launch_bounds__(TPB) void gather_kernel(const float2* __restrict__ X, const Geom* __restrict__ geom,
+                                                     int nx, int N, const float* __restrict__ tab, float fltb,
+                                                     float2* __restrict__ out) {
+    const int H = nx + 1;
+    const int p = blockIdx.y;
+    const int lin = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lin >= H * N) return;
+    const int jp = lin / H;
+    const int I = lin - jp * H;
+    const int J = jp <= nx ? jp : jp - N;
+    float2* o = out + (size_t)p * N * H + lin;
+    if (!(I * I + J * J < (N * N) / 4) || (I == 0 && J < 0)) { *o = make_float2(0.f, 0.f); return; }
+    const Geom g = geom[p];
+    float x, y, z;
+    sample_pos(g, I, J, x, y, z);
+    const bool flip = x < 0.0f;
+    if (flip) { x = -x; y = -y; z = -z; }
+    const int ix = nearest_voxel(x, N), iy = nearest_voxel(y, N), iz = nearest_voxel(z, N);
+    if (!in_bounds(ix, iy, iz, nx)) { *o = make_float2(0.f, 0.f); return; }
+    float tx[5], ty[5], tz[5];
+#pragma unroll
+    for (int l = 0; l < 5; ++l) {
+        tx[l] = kb_weight(tab, __fsub_rn(x, (float)(ix + l - LN2)), fltb);
+        ty[l] = kb_weight(tab, __fsub_rn(y, (float)(iy + l - LN2)), fltb);
+        tz[l] = kb_weight(tab, __fsub_rn(z, (float)(iz + l - LN2)), fltb);
+    }
+    const size_t plane = (size_t)N * H;
+    double dre = 0.0, dim = 0.0, mre = 0.0, mim = 0.0;
+    for (int lz = 0; lz < 5; ++lz) {
+        if (tz[lz] == 0.0f) continue;
+        const int izp = iz + lz - LN2;
+        const size_t za = (size_t)(izp >= 0 ? izp : N + izp) * plane;
+        const size_t zt = (size_t)(izp <= 0 ? -izp : N - izp) * plane;   // wrap(-izp)
+        for (int ly = 0; ly < 5; ++ly) {
+            const float tyz = __fmul_rn(ty[ly], tz[lz]);
+            if (tyz == 0.0f) continue;
+            const int iyp = iy + ly - LN2;
+            const size_t ya = (size_t)(iyp >= 0 ? iyp : N + iyp) * H;
+            const size_t yt = (size_t)(iyp <= 0 ? -iyp : N - iyp) * H;
+#pragma unroll
+            for (int lx = 0; lx < 5; ++lx) {
+                const float wg = __fmul_rn(tx[lx], tyz);
+                if (wg == 0.0f) continue;
+                const int px = ix + lx - LN2;
+                const double w = (double)wg;
+                if (px >= 0) {
+                    const float2 v = X[za + ya + px];
+                    dre += w * (double)v.x;
+                    dim += w * (double)v.y;
+                } else {
+                    const float2 v = X[zt + yt + (size_t)(-px)];
+                    mre += w * (double)v.x;
+                    mim += w * (double)v.y;
+                }
+            }
+        }
+    }
+    const double re = dre + mre, im = dim - mim;
+    *o = make_float2((float)re, (float)(flip ? -im : im));
+}
+
+// BI(I, J) -> the raw R2C value s that spectrum_value() maps to it:
+// BI = c conj(s), c = (-1)^(I + jp), so s = c conj(BI).
+__global__ __launch_bounds__(TPB) void bi_to_raw_kernel(const float2* __restrict__ bi, int N, int H, size_t n,
+                                                        float2* __restrict__ raw) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    const size_t r = idx % ((size_t)N * H);
+    const int jp = (int)(r / H), I = (int)(r - (size_t)jp * H);
+    const float sg = ((I + jp) & 1) ? -1.0f : 1.0f;
+    const float2 v = bi[idx];
+    raw[idx] = make_float2(v.x * sg, -v.y * sg);
+}
+
+}  // namespace spbp

@@ -40,13 +40,17 @@
 #include "finalize.hpp"
 #include "hostmath.hpp"
 #include "insert.hpp"
+#include "project.hpp"
 
 using namespace spbp;
 
 namespace {
 thread_local std::string g_last_error;
-// brick of the LDS-binned strategy (voxels); 3 x 8 B per voxel in LDS
-constexpr int BRICK_X = 16, BRICK_Y = 16, BRICK_Z = 8;
+// brick shapes of the LDS-binned strategy (voxels; 3 x 8 B per voxel in LDS)
+constexpr int NSHAPES = 3;
+constexpr int SHAPES[NSHAPES][3] = {{16, 16, 8}, {16, 8, 8}, {8, 8, 8}};
+constexpr int NCVALS = 4;
+constexpr int CVALS[NCVALS] = {16, 32, 64, 128};   // projections per work item tried by the tuner
 }  // namespace
 
 // ------------------------------------------------------------------ context
@@ -83,7 +87,21 @@ struct spbp_ctx {
     size_t stage_cap = 0;
     // brick strategy
     int strategy = SPBP_STRAT_BRICK;
+    int shape = 0;   // index into SHAPES
     int C = 64;
+    bool tuned = false;
+    bool capturing = false;   // a batch graph is being captured: no host syncs, no timing events
+    // symmetries (BUILDS): each image is inserted once per matrix, SYMVOL after the window
+    int nsym = 1;
+    std::vector<float> sm;    // nsym x 9, Fortran (3,3) order
+    float* sm_d = nullptr;
+    float* symtmp = nullptr;  // NX^3
+    float tune_ms[NSHAPES][NCVALS] = {};   // spbp_tune: ms per projection of each brick plan
+    float tune_direct_ms = 0.f;
+    BrickJob* jobs_d = nullptr;   // brick-kernel job table
+    BrickJob* jobs_h = nullptr;   // pinned staging
+    int jobs_cap = 0;
+    hipEvent_t jobs_evt = nullptr;
     int ncu = 256;
     BrickGrid bg{};
     int* bcnt = nullptr;    // [2 nb] hits per (brick, half)
@@ -96,7 +114,7 @@ struct spbp_ctx {
     size_t blist_cap = 0;
     // SPBP_TIMING: (start, stop, units) per recorded launch, per kernel kind
     struct Rec { hipEvent_t a, b; long units; };
-    std::vector<Rec> recs[5];
+    std::vector<Rec> recs[6];
     std::vector<hipEvent_t> evpool;
     std::string err;
 };
@@ -154,7 +172,7 @@ struct Timed {
     hipStream_t st;
     hipEvent_t a = nullptr;
     Timed(spbp_ctx* c_, int kind_, long units_, hipStream_t st_) : c(c_), kind(kind_), units(units_), st(st_) {
-        if (c->flags & SPBP_TIMING) {
+        if ((c->flags & SPBP_TIMING) && !c->capturing) {
             a = take_event(c);
             if (a) (void)hipEventRecord(a, st);
         }
@@ -168,8 +186,49 @@ struct Timed {
     }
 };
 
+// Brick grid of shape `shape` and its per-key arrays.
+int configure_bricks(spbp_ctx* c, int shape) {
+    const int bx = SHAPES[shape][0], by = SHAPES[shape][1], bz = SHAPES[shape][2];
+    BrickGrid g;
+    g.bx = bx;
+    g.by = by;
+    g.bz = bz;
+    g.nbx = (c->nx + bx - 1) / bx;
+    g.nby = (2 * c->nx - 1 + by - 1) / by;
+    g.nbz = (2 * c->nx - 1 + bz - 1) / bz;
+    g.nb = g.nbx * g.nby * g.nbz;
+    if (c->bcnt && g.nb == c->bg.nb && shape == c->shape) return SPBP_OK;
+    int* bufs[4] = {c->bcnt, c->boff, c->bioff, c->bcur};
+    for (int* q : bufs) if (q) (void)hipFree(q);
+    c->bcnt = c->boff = c->bioff = c->bcur = nullptr;
+    if (c->blist) { (void)hipFree(c->blist); c->blist = nullptr; }
+    if (c->bikey) { (void)hipFree(c->bikey); c->bikey = nullptr; }
+    c->blist_cap = 0;
+    c->bg = g;
+    c->shape = shape;
+    const size_t nk = 2 * (size_t)g.nb;
+    HIPCHK(c, hipMalloc(&c->bcnt, nk * sizeof(int)));
+    HIPCHK(c, hipMalloc(&c->boff, (nk + 1) * sizeof(int)));
+    HIPCHK(c, hipMalloc(&c->bioff, (nk + 1) * sizeof(int)));
+    HIPCHK(c, hipMalloc(&c->bcur, nk * sizeof(int)));
+    return SPBP_OK;
+}
+
+// Projection lists for chunks of up to B projections.
+int ensure_lists(spbp_ctx* c, int B) {
+    const size_t cap = (size_t)B * c->nsym * c->bg.nb;   // B images, nsym sections each
+    if (c->blist && c->blist_cap >= cap) return SPBP_OK;
+    if (c->blist) { (void)hipFree(c->blist); c->blist = nullptr; }
+    if (c->bikey) { (void)hipFree(c->bikey); c->bikey = nullptr; }
+    HIPCHK(c, hipMalloc(&c->blist, cap * sizeof(int)));
+    // items <= sum over keys of ceil(cnt / C) <= pairs + keys
+    HIPCHK(c, hipMalloc(&c->bikey, (cap + 2 * (size_t)c->bg.nb) * sizeof(int)));
+    c->blist_cap = cap;
+    return SPBP_OK;
+}
+
 int ensure_chunk(spbp_ctx* c, int want_batch) {
-    if (c->padbuf && c->fwd_batch >= want_batch) return SPBP_OK;
+    if (c->padbuf && c->fwd_batch >= want_batch) return ensure_lists(c, c->fwd_batch);
     if (c->padbuf) { (void)hipFree(c->padbuf); c->padbuf = nullptr; }
     if (c->spec) { (void)hipFree(c->spec); c->spec = nullptr; }
     if (c->fwd) { hipfftDestroy(c->fwd); c->fwd = 0; }
@@ -177,15 +236,8 @@ int ensure_chunk(spbp_ctx* c, int want_batch) {
     const int N = c->N, H = c->H, B = want_batch;
     HIPCHK(c, hipMalloc(&c->padbuf, (size_t)B * N * N * sizeof(float)));
     HIPCHK(c, hipMalloc(&c->spec, (size_t)B * N * H * sizeof(float2)));
-    if (c->strategy == SPBP_STRAT_BRICK) {
-        const size_t cap = (size_t)B * c->bg.nb;
-        if (c->blist) { (void)hipFree(c->blist); c->blist = nullptr; }
-        if (c->bikey) { (void)hipFree(c->bikey); c->bikey = nullptr; }
-        HIPCHK(c, hipMalloc(&c->blist, cap * sizeof(int)));
-        // items <= sum over keys of ceil(cnt / C) <= pairs + keys
-        HIPCHK(c, hipMalloc(&c->bikey, (cap + 2 * (size_t)c->bg.nb) * sizeof(int)));
-        c->blist_cap = cap;
-    }
+    int rc = ensure_lists(c, B);
+    if (rc) return rc;
     int n[2] = {N, N};
     int inembed[2] = {N, N};
     int onembed[2] = {N, H};
@@ -240,20 +292,18 @@ int ensure_final(spbp_ctx* c) {
     return SPBP_OK;
 }
 
-int insert_impl(spbp_ctx* c, const void* imgs, int dtype, int nimg, const float* dm_in,
-                const double* ang, const int8_t* half, hipStream_t st) {
-    if (!c) return fail(nullptr, SPBP_ERR_ARG, "NULL context");
-    if (nimg < 0) return fail(c, SPBP_ERR_ARG, "negative image count %d", nimg);
-    if (nimg == 0) return SPBP_OK;
-    if (!imgs) return fail(c, SPBP_ERR_ARG, "NULL image pointer");
-    if (dtype != SPBP_F32 && dtype != SPBP_BF16) return fail(c, SPBP_ERR_ARG, "unknown image type %d", dtype);
+// Rotations (and half flags) of nimg projections into c->geom_d, queued on st.
+// From dm (nimg x 9 fp32) when given, else from the doc angles (PSI, THETA, PHI).
+// With use_sym and symmetries set, image p gets entries p*nsym + k with
+// DMS = MATMUL(SM_k, DM) (bp32f.f:866-872), all reading spectrum p.
+int stage_geom(spbp_ctx* c, int nimg, const float* dm_in, const double* ang, const int8_t* half, hipStream_t st,
+               bool use_sym = true) {
     if (!dm_in && !ang) return fail(c, SPBP_ERR_ANGLE, "MISSING ANGLE FOR IMAGE (no angles given)");
-    DeviceGuard dg(c->device);
-    int rc = ensure_geom(c, (size_t)nimg);
+    const int ns = use_sym ? c->nsym : 1;
+    int rc = ensure_geom(c, (size_t)nimg * ns);
     if (rc) return rc;
     const bool exact = (c->flags & SPBP_EXACT_PI) != 0;
     for (int p = 0; p < nimg; ++p) {
-        Geom& g = c->geom_h[p];
         float dm[9];
         if (dm_in) {
             std::memcpy(dm, dm_in + 9 * (size_t)p, sizeof dm);
@@ -263,79 +313,223 @@ int insert_impl(spbp_ctx* c, const void* imgs, int dtype, int nimg, const float*
                 return fail(c, SPBP_ERR_ANGLE, "MISSING ANGLE FOR IMAGE %d", p + 1);
             cang((float)a[2], (float)a[1], (float)a[0], exact, dm);   // CANG(PHI, THETA, PSI)
         }
-        for (int q = 0; q < 9; ++q) {
+        for (int q = 0; q < 9; ++q)
             if (!std::isfinite(dm[q])) return fail(c, SPBP_ERR_ANGLE, "non-finite rotation for image %d", p + 1);
-            g.r[q] = dm[q];
-        }
-        g.half = 0;
+        int h = 0;
         if (half) {
             if (half[p] != 0 && half[p] != 1) return fail(c, SPBP_ERR_ARG, "half flag %d of image %d is not 0/1", half[p], p + 1);
             if (half[p] == 1 && c->nvol < 2) return fail(c, SPBP_ERR_ARG, "half 1 needs SPBP_TWO_HALVES");
-            g.half = half[p];
+            h = half[p];
         }
-        g.pad_[0] = g.pad_[1] = 0;
+        for (int k = 0; k < ns; ++k) {
+            Geom& g = c->geom_h[(size_t)p * ns + k];
+            if (ns > 1) sym_matmul(c->sm.data() + 9 * k, dm, g.r);
+            else std::memcpy(g.r, dm, sizeof dm);
+            g.half = h;
+            g.img = p;
+            g.pad_ = 0;
+        }
     }
-    HIPCHK(c, hipMemcpyAsync(c->geom_d, c->geom_h, (size_t)nimg * sizeof(Geom), hipMemcpyHostToDevice, st));
+    HIPCHK(c, hipMemcpyAsync(c->geom_d, c->geom_h, (size_t)nimg * ns * sizeof(Geom), hipMemcpyHostToDevice, st));
     HIPCHK(c, hipEventRecord(c->geom_evt, st));
-    const int nx = c->nx, N = c->N, H = c->H;
-    const size_t esz = dtype == SPBP_BF16 ? 2 : 4;
-    if (c->flags & SPBP_CHECK_FINITE) {
-        const size_t n = (size_t)nimg * nx * nx;
-        fill_kernel<<<1, TPB, 0, st>>>((float*)c->flag, 1, 0.0f);   // zero bits
-        if (dtype == SPBP_BF16)
-            finite_kernel<uint16_t><<<nblocks(n), TPB, 0, st>>>((const uint16_t*)imgs, n, c->flag);
-        else
-            finite_kernel<float><<<nblocks(n), TPB, 0, st>>>((const float*)imgs, n, c->flag);
-        int f = 0;
-        HIPCHK(c, hipMemcpyAsync(&f, c->flag, sizeof(int), hipMemcpyDeviceToHost, st));
-        HIPCHK(c, hipStreamSynchronize(st));
-        if (f) return fail(c, SPBP_ERR_NONFINITE, "NaN or Inf in the images");
+    return SPBP_OK;
+}
+
+// Insertion targets: half 0 -> (X0, W0, Wc0), half 1 -> (X1, W1, Wc1); W, Wc may be null.
+struct Targets {
+    float2* X0; float* W0; float2* X1; float* W1; float* Wc0; float* Wc1;
+};
+
+BrickJob brick_job(const spbp_ctx* c, int img0, const Geom* geom_d, const Targets& t) {
+    BrickJob j;
+    j.spec = c->spec;
+    j.img0 = img0;
+    j.geom = geom_d;
+    j.tab = c->tab;
+    j.fltb = c->fltb;
+    j.nx = c->nx;
+    j.N = c->N;
+    j.C = c->C;
+    j.bg = c->bg;
+    j.cnt = c->bcnt;
+    j.off = c->boff;
+    j.item_off = c->bioff;
+    j.item_key = c->bikey;
+    j.list = c->blist;
+    j.X0 = t.X0; j.W0 = t.W0; j.X1 = t.X1; j.W1 = t.W1; j.Wc0 = t.Wc0; j.Wc1 = t.Wc1;
+    return j;
+}
+
+unsigned brick_grid(const spbp_ctx* c) {
+    // workgroups resident per CU: LDS-bound (52.7 KB) for the largest brick,
+    // VGPR-bound (4 waves of 256-thread groups per SIMD) for the others
+    return (unsigned)c->ncu * (c->shape == 0 ? 3u : 4u);
+}
+
+// One persistent brick-kernel launch over the device job table table_d[0..n),
+// all jobs with c's brick shape; `counter` (the queue head) must be zero on st.
+int launch_bricks_dev(spbp_ctx* c, const BrickJob* table_d, int n, int* counter, hipStream_t st) {
+    const unsigned grid = brick_grid(c);
+    switch (c->shape) {
+        case 0: insert_brick_kernel<16, 16, 8><<<grid, TPB, 0, st>>>(table_d, n, counter); break;
+        case 1: insert_brick_kernel<16, 8, 8><<<grid, TPB, 0, st>>>(table_d, n, counter); break;
+        default: insert_brick_kernel<8, 8, 8><<<grid, TPB, 0, st>>>(table_d, n, counter); break;
     }
-    rc = ensure_chunk(c, std::min(c->batch, nimg));
+    HIPCHK(c, hipGetLastError());
+    return SPBP_OK;
+}
+
+// Same with a host job table, staged through c's pinned buffer.
+int launch_bricks(spbp_ctx* c, const BrickJob* jh, int n, int* counter, hipStream_t st) {
+    if (n < 1 || n > MAX_JOBS) return fail(c, SPBP_ERR_ARG, "%d jobs in one launch (1..%d)", n, MAX_JOBS);
+    if (c->jobs_cap < n) {
+        if (c->jobs_h) { (void)hipEventSynchronize(c->jobs_evt); (void)hipHostFree(c->jobs_h); c->jobs_h = nullptr; }
+        if (c->jobs_d) { (void)hipFree(c->jobs_d); c->jobs_d = nullptr; }
+        HIPCHK(c, hipHostMalloc((void**)&c->jobs_h, MAX_JOBS * sizeof(BrickJob), hipHostMallocDefault));
+        HIPCHK(c, hipMalloc(&c->jobs_d, MAX_JOBS * sizeof(BrickJob)));
+        c->jobs_cap = MAX_JOBS;
+    } else {
+        HIPCHK(c, hipEventSynchronize(c->jobs_evt));   // the previous upload has left the staging buffer
+    }
+    std::memcpy(c->jobs_h, jh, n * sizeof(BrickJob));
+    HIPCHK(c, hipMemcpyAsync(c->jobs_d, c->jobs_h, n * sizeof(BrickJob), hipMemcpyHostToDevice, st));
+    HIPCHK(c, hipEventRecord(c->jobs_evt, st));
+    return launch_bricks_dev(c, c->jobs_d, n, counter, st);
+}
+
+// Per-(brick, half) projection lists of b projections (geometry at geom_d);
+// also zeroes c's queue head.
+int build_lists(spbp_ctx* c, const Geom* geom_d, int b, hipStream_t st) {
+    const int nkeys = 2 * c->bg.nb;
+    const dim3 lgrid(c->bg.nb, (b + 1023) / 1024);
+    Timed tm(c, SPBP_K_BIN, b, st);
+    fill_kernel<<<nblocks(nkeys), TPB, 0, st>>>((float*)c->bcnt, nkeys, 0.0f);   // zero bits
+    brick_count_kernel<<<lgrid, TPB, 0, st>>>(geom_d, b, c->bg, c->nx, c->bcnt);
+    brick_scan_kernel<<<1, 1024, 0, st>>>(c->bcnt, nkeys, c->C, c->boff, c->bioff, c->bcur, c->bnext);
+    brick_fill_kernel<<<lgrid, TPB, 0, st>>>(geom_d, b, c->bg, c->nx, c->boff, c->bcur, c->blist);
+    brick_itemkey_kernel<<<nblocks(nkeys), TPB, 0, st>>>(c->bioff, nkeys, c->bikey);
+    HIPCHK(c, hipGetLastError());
+    return SPBP_OK;
+}
+
+// ng sections (geometry at geom_d) of images img0.. whose spectra are in c->spec.
+int launch_direct(spbp_ctx* c, int img0, const Geom* geom_d, int ng, const Targets& t, hipStream_t st, long units) {
+    Timed tm(c, SPBP_K_INSERT, units, st);
+    for (int g0 = 0; g0 < ng; g0 += 65535) {   // grid.y limit
+        dim3 grid(nblocks((size_t)c->H * c->N), std::min(65535, ng - g0));
+        insert_direct_kernel<<<grid, TPB, 0, st>>>(c->spec, img0, geom_d + g0, c->nx, c->N, c->tab, c->fltb,
+                                                   t.X0, t.W0, t.X1, t.W1, t.Wc0, t.Wc1);
+    }
+    HIPCHK(c, hipGetLastError());
+    return SPBP_OK;
+}
+
+// ONELINE over ng sections (geometry at geom_d) of images img0.. whose raw
+// R2C spectra are in c->spec, with the context's strategy; `units` = images.
+int run_insert(spbp_ctx* c, int img0, const Geom* geom_d, int ng, const Targets& t, hipStream_t st, long units) {
+    if (c->strategy == SPBP_STRAT_DIRECT) return launch_direct(c, img0, geom_d, ng, t, st, units);
+    int rc = build_lists(c, geom_d, ng, st);
     if (rc) return rc;
+    const BrickJob job = brick_job(c, img0, geom_d, t);
+    Timed tm(c, SPBP_K_INSERT, units, st);
+    return launch_bricks(c, &job, 1, c->bnext, st);
+}
+
+// PADD2 + batched forward FFT of b images into c->spec.
+int pad_fft(spbp_ctx* c, const void* src, int dtype, int b, hipStream_t st) {
+    {
+        Timed tm(c, SPBP_K_PAD, b, st);
+        if (dtype == SPBP_BF16)
+            pad_kernel<uint16_t><<<b, TPB, 0, st>>>((const uint16_t*)src, c->nx, c->N, c->padbuf);
+        else
+            pad_kernel<float><<<b, TPB, 0, st>>>((const float*)src, c->nx, c->N, c->padbuf);
+    }
+    HIPCHK(c, hipGetLastError());
+    int rc = SPBP_OK;
+    hipfftHandle plan = plan_for(c, b, &rc);
+    if (rc) return rc;
+    FFTCHK(c, hipfftSetStream(plan, st));
+    Timed tm(c, SPBP_K_FFT2, b, st);
+    FFTCHK(c, hipfftExecR2C(plan, (hipfftReal*)c->padbuf, (hipfftComplex*)c->spec));
+    return SPBP_OK;
+}
+
+int check_images(spbp_ctx* c, const void* imgs, int dtype, int nimg, hipStream_t st) {
+    if (dtype != SPBP_F32 && dtype != SPBP_BF16) return fail(c, SPBP_ERR_ARG, "unknown image type %d", dtype);
+    if (!(c->flags & SPBP_CHECK_FINITE)) return SPBP_OK;
+    if (c->capturing) return fail(c, SPBP_ERR_ARG, "SPBP_CHECK_FINITE cannot be used in a captured batch");
+    const size_t n = (size_t)nimg * c->nx * c->nx;
+    fill_kernel<<<1, TPB, 0, st>>>((float*)c->flag, 1, 0.0f);   // zero bits
+    if (dtype == SPBP_BF16)
+        finite_kernel<uint16_t><<<nblocks(n), TPB, 0, st>>>((const uint16_t*)imgs, n, c->flag);
+    else
+        finite_kernel<float><<<nblocks(n), TPB, 0, st>>>((const float*)imgs, n, c->flag);
+    int f = 0;
+    HIPCHK(c, hipMemcpyAsync(&f, c->flag, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    if (f) return fail(c, SPBP_ERR_NONFINITE, "NaN or Inf in the images");
+    return SPBP_OK;
+}
+
+int insert_impl(spbp_ctx* c, const void* imgs, int dtype, int nimg, const float* dm_in,
+                const double* ang, const int8_t* half, hipStream_t st) {
+    if (!c) return fail(nullptr, SPBP_ERR_ARG, "NULL context");
+    if (nimg < 0) return fail(c, SPBP_ERR_ARG, "negative image count %d", nimg);
+    if (nimg == 0) return SPBP_OK;
+    if (!imgs) return fail(c, SPBP_ERR_ARG, "NULL image pointer");
+    DeviceGuard dg(c->device);
+    int rc = check_images(c, imgs, dtype, nimg, st);
+    if (rc) return rc;
+    if ((rc = stage_geom(c, nimg, dm_in, ang, half, st))) return rc;
+    if ((rc = ensure_chunk(c, std::min(c->batch, nimg)))) return rc;
+    const size_t esz = dtype == SPBP_BF16 ? 2 : 4;
+    const Targets t{c->X[0], c->W[0], c->X[1], c->W[1], c->Wc[0], c->Wc[1]};
+    const int ns = c->nsym;
     for (int p0 = 0; p0 < nimg; p0 += c->fwd_batch) {
         const int b = std::min(c->fwd_batch, nimg - p0);
-        const char* src = (const char*)imgs + (size_t)p0 * nx * nx * esz;
-        {
-            Timed t(c, SPBP_K_PAD, b, st);
-            if (dtype == SPBP_BF16)
-                pad_kernel<uint16_t><<<b, TPB, 0, st>>>((const uint16_t*)src, nx, N, c->padbuf);
-            else
-                pad_kernel<float><<<b, TPB, 0, st>>>((const float*)src, nx, N, c->padbuf);
-        }
-        HIPCHK(c, hipGetLastError());
-        hipfftHandle plan = plan_for(c, b, &rc);
-        if (rc) return rc;
-        FFTCHK(c, hipfftSetStream(plan, st));
-        {
-            Timed t(c, SPBP_K_FFT2, b, st);
-            FFTCHK(c, hipfftExecR2C(plan, (hipfftReal*)c->padbuf, (hipfftComplex*)c->spec));
-        }
-        if (c->strategy == SPBP_STRAT_DIRECT) {
-            Timed t(c, SPBP_K_INSERT, b, st);
-            dim3 grid(nblocks((size_t)H * N), b);
-            insert_direct_kernel<<<grid, TPB, 0, st>>>(c->spec, c->geom_d + p0, nx, N, c->tab, c->fltb,
-                                                       c->X[0], c->W[0], c->X[1], c->W[1], c->Wc[0], c->Wc[1]);
-        } else {
-            const int nkeys = 2 * c->bg.nb;
-            const dim3 lgrid(c->bg.nb, (b + 1023) / 1024);
-            {
-                Timed t(c, SPBP_K_BIN, b, st);
-                fill_kernel<<<nblocks(nkeys), TPB, 0, st>>>((float*)c->bcnt, nkeys, 0.0f);   // zero bits
-                brick_count_kernel<<<lgrid, TPB, 0, st>>>(c->geom_d + p0, b, c->bg, nx, c->bcnt);
-                brick_scan_kernel<<<1, 1024, 0, st>>>(c->bcnt, nkeys, c->C, c->boff, c->bioff, c->bcur, c->bnext);
-                brick_fill_kernel<<<lgrid, TPB, 0, st>>>(c->geom_d + p0, b, c->bg, nx, c->boff, c->bcur, c->blist);
-                brick_itemkey_kernel<<<nblocks(nkeys), TPB, 0, st>>>(c->bioff, nkeys, c->bikey);
-            }
-            HIPCHK(c, hipGetLastError());
-            Timed t(c, SPBP_K_INSERT, b, st);
-            insert_brick_kernel<BRICK_X, BRICK_Y, BRICK_Z><<<c->ncu * 3, TPB, 0, st>>>(
-                c->spec, c->geom_d + p0, nx, N, c->tab, c->fltb, c->bg, c->C, c->bcnt, c->boff, c->bioff,
-                c->bikey, c->blist, c->bnext, c->X[0], c->W[0], c->X[1], c->W[1], c->Wc[0], c->Wc[1]);
-        }
-        HIPCHK(c, hipGetLastError());
+        if ((rc = pad_fft(c, (const char*)imgs + (size_t)p0 * c->nx * c->nx * esz, dtype, b, st))) return rc;
+        if ((rc = run_insert(c, p0, c->geom_d + (size_t)p0 * ns, b * ns, t, st, b))) return rc;
     }
     return SPBP_OK;
+}
+
+// Choose brick shape s for c (and its best C for that shape when tuned).
+int apply_shape(spbp_ctx* c, int s) {
+    int rc = configure_bricks(c, s);
+    if (rc) return rc;
+    if (c->tuned) {
+        int best = 0;
+        for (int k = 1; k < NCVALS; ++k) if (c->tune_ms[s][k] < c->tune_ms[s][best]) best = k;
+        c->C = CVALS[best];
+    }
+    return c->fwd_batch > 0 ? ensure_lists(c, c->fwd_batch) : SPBP_OK;
+}
+
+// Time (ms, best of `reps`) one insertion plan over the b projections in
+// c->spec into the scratch targets t.
+int time_plan(spbp_ctx* c, const Geom* geom_d, int b, const Targets& t, hipStream_t st, int reps, float* ms) {
+    const int ng = b * c->nsym;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    const int saved = c->flags;
+    c->flags &= ~SPBP_TIMING;
+    int rc = run_insert(c, 0, geom_d, ng, t, st, b);   // warm-up
+    float best = 3.0e38f;
+    for (int r = 0; r < reps && rc == SPBP_OK; ++r) {
+        (void)hipEventRecord(e0, st);
+        rc = run_insert(c, 0, geom_d, ng, t, st, b);
+        (void)hipEventRecord(e1, st);
+        if (hipEventSynchronize(e1) != hipSuccess) { rc = fail(c, SPBP_ERR_DEVICE, "tuning run failed"); break; }
+        float e = 0.f;
+        (void)hipEventElapsedTime(&e, e0, e1);
+        best = std::min(best, e);
+    }
+    c->flags = saved;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = best;
+    return rc;
 }
 
 Acc make_acc(const spbp_ctx* c, int which) {
@@ -369,6 +563,12 @@ int finalize_one(spbp_ctx* c, int which, float* out, hipStream_t st) {
     shell_mean_kernel<<<1, TPB, 0, st>>>(c->psum, c->pcnt, (int)nb, c->mean);
     subtract_kernel<<<nb, TPB, 0, st>>>(out, c->mean, nx);
     HIPCHK(c, hipGetLastError());
+    if (c->nsym > 1) {   // real-space symmetrisation after WINDKB2A (bp32f.f:278-303)
+        if (!c->symtmp) HIPCHK(c, hipMalloc(&c->symtmp, nv * sizeof(float)));
+        HIPCHK(c, hipMemcpyAsync(c->symtmp, out, nv * sizeof(float), hipMemcpyDeviceToDevice, st));
+        symvol_kernel<<<nb, TPB, 0, st>>>(c->symtmp, out, nx, c->sm_d, c->nsym);
+        HIPCHK(c, hipGetLastError());
+    }
     return SPBP_OK;
 }
 
@@ -415,13 +615,6 @@ int spbp_create(const spbp_opts* o, spbp_ctx** out) {
         int n = 0;
         if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0) c->ncu = n;
     }
-    c->bg.bx = BRICK_X;
-    c->bg.by = BRICK_Y;
-    c->bg.bz = BRICK_Z;
-    c->bg.nbx = (c->nx + BRICK_X - 1) / BRICK_X;
-    c->bg.nby = (2 * c->nx - 1 + BRICK_Y - 1) / BRICK_Y;
-    c->bg.nbz = (2 * c->nx - 1 + BRICK_Z - 1) / BRICK_Z;
-    c->bg.nb = c->bg.nbx * c->bg.nby * c->bg.nbz;
     int rc = SPBP_OK;
     auto bail = [&](int code) { spbp_destroy(c); return code; };
     for (int h = 0; h < c->nvol; ++h) {
@@ -435,15 +628,11 @@ int spbp_create(const spbp_opts* o, spbp_ctx** out) {
     if (hipMalloc(&c->tab, tab.size() * sizeof(float)) != hipSuccess ||
         hipMemcpy(c->tab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
         hipMalloc(&c->flag, sizeof(int)) != hipSuccess ||
-        hipEventCreateWithFlags(&c->geom_evt, hipEventDisableTiming) != hipSuccess)
+        hipEventCreateWithFlags(&c->geom_evt, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->jobs_evt, hipEventDisableTiming) != hipSuccess)
         return bail(fail(nullptr, SPBP_ERR_DEVICE, "context set-up failed"));
-    {
-        const size_t nk = 2 * (size_t)c->bg.nb;
-        if (hipMalloc(&c->bcnt, nk * sizeof(int)) != hipSuccess || hipMalloc(&c->boff, (nk + 1) * sizeof(int)) != hipSuccess ||
-            hipMalloc(&c->bioff, (nk + 1) * sizeof(int)) != hipSuccess || hipMalloc(&c->bcur, nk * sizeof(int)) != hipSuccess ||
-            hipMalloc(&c->bnext, sizeof(int)) != hipSuccess)
-            return bail(fail(nullptr, SPBP_ERR_ALLOC, "brick lists"));
-    }
+    if (hipMalloc(&c->bnext, sizeof(int)) != hipSuccess) return bail(fail(nullptr, SPBP_ERR_ALLOC, "brick queue"));
+    if ((rc = configure_bricks(c, 0))) { g_last_error = c->err; return bail(rc); }
     rc = spbp_reset(c, nullptr);
     if (rc) return bail(rc);
     if (hipDeviceSynchronize() != hipSuccess) return bail(fail(nullptr, SPBP_ERR_DEVICE, "zeroing accumulators failed"));
@@ -455,20 +644,24 @@ int spbp_destroy(spbp_ctx* c) {
     if (!c) return SPBP_OK;
     DeviceGuard dg(c->device);
     if (c->geom_evt) (void)hipEventSynchronize(c->geom_evt);
+    if (c->jobs_evt) (void)hipEventSynchronize(c->jobs_evt);
     for (int h = 0; h < 2; ++h) {
         if (c->X[h]) (void)hipFree(c->X[h]);
         if (c->W[h]) (void)hipFree(c->W[h]);
         if (c->Wc[h]) (void)hipFree(c->Wc[h]);
     }
     void* dev_ptrs[] = {c->tab, c->padbuf, c->spec, c->geom_d, c->work, c->real, c->wkb, c->psum, c->pcnt, c->mean, c->flag,
-                        c->bcnt, c->boff, c->bioff, c->bcur, c->bnext, c->blist, c->bikey};
+                        c->bcnt, c->boff, c->bioff, c->bcur, c->bnext, c->blist, c->bikey, c->jobs_d,
+                        c->sm_d, c->symtmp};
     for (void* p : dev_ptrs) if (p) (void)hipFree(p);
     if (c->geom_h) (void)hipHostFree(c->geom_h);
+    if (c->jobs_h) (void)hipHostFree(c->jobs_h);
     if (c->stage) (void)hipHostFree(c->stage);
     if (c->fwd) hipfftDestroy(c->fwd);
     if (c->fwd_tail) hipfftDestroy(c->fwd_tail);
     if (c->inv) hipfftDestroy(c->inv);
     if (c->geom_evt) (void)hipEventDestroy(c->geom_evt);
+    if (c->jobs_evt) (void)hipEventDestroy(c->jobs_evt);
     spbp_reset_stats(c);
     for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
     delete c;
@@ -497,6 +690,378 @@ int spbp_insert_dm(spbp_ctx* c, const void* imgs, int dtype, int nimg, const flo
                    const int8_t* half, void* stream) {
     if (!dm) return fail(c, SPBP_ERR_ANGLE, "MISSING ANGLE FOR IMAGE (no rotations given)");
     return insert_impl(c, imgs, dtype, nimg, dm, nullptr, half, (hipStream_t)stream);
+}
+
+int spbp_gather(spbp_ctx* c, const float* X, int nimg, const double* angles_deg, const float* dm,
+                float* out_bi, void* stream) {
+    if (!c) return fail(nullptr, SPBP_ERR_ARG, "NULL context");
+    if (nimg < 0) return fail(c, SPBP_ERR_ARG, "negative projection count %d", nimg);
+    if (nimg == 0) return SPBP_OK;
+    if (!X || !out_bi) return fail(c, SPBP_ERR_ARG, "NULL grid or output pointer");
+    DeviceGuard dg(c->device);
+    hipStream_t st = (hipStream_t)stream;
+    int rc = stage_geom(c, nimg, dm, angles_deg, nullptr, st, false);
+    if (rc) return rc;
+    const int N = c->N, H = c->H;
+    const unsigned gx = nblocks((size_t)H * N);
+    for (int p0 = 0; p0 < nimg; p0 += 65535) {
+        const int b = std::min(65535, nimg - p0);
+        Timed tm(c, SPBP_K_GATHER, b, st);
+        gather_kernel<<<dim3(gx, b), TPB, 0, st>>>((const float2*)X, c->geom_d + p0, c->nx, N, c->tab, c->fltb,
+                                                   (float2*)out_bi + (size_t)p0 * N * H);
+    }
+    HIPCHK(c, hipGetLastError());
+    return SPBP_OK;
+}
+
+int spbp_scatter(spbp_ctx* c, const float* bi, int nimg, const double* angles_deg, const float* dm,
+                 float* X, float* W, void* stream) {
+    if (!c) return fail(nullptr, SPBP_ERR_ARG, "NULL context");
+    if (nimg < 0) return fail(c, SPBP_ERR_ARG, "negative projection count %d", nimg);
+    if (nimg == 0) return SPBP_OK;
+    if (!bi || !X) return fail(c, SPBP_ERR_ARG, "NULL spectrum or grid pointer");
+    DeviceGuard dg(c->device);
+    hipStream_t st = (hipStream_t)stream;
+    int rc = stage_geom(c, nimg, dm, angles_deg, nullptr, st, false);
+    if (rc) return rc;
+    rc = ensure_chunk(c, std::min(c->batch, nimg));
+    if (rc) return rc;
+    const int N = c->N, H = c->H;
+    const Targets t{(float2*)X, W, nullptr, nullptr, nullptr, nullptr};
+    for (int p0 = 0; p0 < nimg; p0 += c->fwd_batch) {
+        const int b = std::min(c->fwd_batch, nimg - p0);
+        const size_t n = (size_t)b * N * H;
+        bi_to_raw_kernel<<<nblocks(n), TPB, 0, st>>>((const float2*)bi + (size_t)p0 * N * H, N, H, n, c->spec);
+        HIPCHK(c, hipGetLastError());
+        if ((rc = run_insert(c, p0, c->geom_d + p0, b, t, st, b))) return rc;
+    }
+    return SPBP_OK;
+}
+
+int spbp_tune(spbp_ctx* c, const void* imgs, int dtype, int nimg, const double* angles_deg, const float* dm,
+              void* stream) {
+    if (!c) return fail(nullptr, SPBP_ERR_ARG, "NULL context");
+    if (nimg <= 0 || !imgs) return fail(c, SPBP_ERR_ARG, "tuning needs a sample of images");
+    if (dtype != SPBP_F32 && dtype != SPBP_BF16) return fail(c, SPBP_ERR_ARG, "unknown image type %d", dtype);
+    DeviceGuard dg(c->device);
+    hipStream_t st = (hipStream_t)stream;
+    const int m = std::min(nimg, std::min(c->batch, 1024));
+    int rc = stage_geom(c, m, dm, angles_deg, nullptr, st);
+    if (rc) return rc;
+    if ((rc = ensure_chunk(c, std::min(c->batch, m)))) return rc;
+    const int b = std::min(c->fwd_batch, m);
+    if ((rc = pad_fft(c, imgs, dtype, b, st))) return rc;
+    float2* Xs = nullptr;
+    float* Ws = nullptr;
+    HIPCHK(c, hipMalloc(&Xs, c->nh * sizeof(float2)));
+    if (hipMalloc(&Ws, c->nh * sizeof(float)) != hipSuccess) { (void)hipFree(Xs); return fail(c, SPBP_ERR_ALLOC, "tuning scratch"); }
+    const Targets t{Xs, Ws, nullptr, nullptr, nullptr, nullptr};
+    const int s_strat = c->strategy, s_shape = c->shape, s_C = c->C;
+    float ms = 0.f;
+    c->strategy = SPBP_STRAT_DIRECT;
+    rc = time_plan(c, c->geom_d, b, t, st, 1, &ms);
+    c->tune_direct_ms = ms / b;
+    c->strategy = SPBP_STRAT_BRICK;
+    for (int sh = 0; sh < NSHAPES && rc == SPBP_OK; ++sh) {
+        if ((rc = configure_bricks(c, sh)) || (rc = ensure_lists(c, c->fwd_batch))) break;
+        for (int k = 0; k < NCVALS && rc == SPBP_OK; ++k) {
+            c->C = CVALS[k];
+            rc = time_plan(c, c->geom_d, b, t, st, 2, &ms);
+            c->tune_ms[sh][k] = ms / b;
+        }
+    }
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(Xs);
+    (void)hipFree(Ws);
+    if (rc) {
+        c->strategy = s_strat;
+        c->C = s_C;
+        (void)configure_bricks(c, s_shape);
+        (void)ensure_lists(c, c->fwd_batch);
+        return rc;
+    }
+    int bs = 0, bk = 0;
+    for (int sh = 0; sh < NSHAPES; ++sh)
+        for (int k = 0; k < NCVALS; ++k)
+            if (c->tune_ms[sh][k] < c->tune_ms[bs][bk]) { bs = sh; bk = k; }
+    c->tuned = true;
+    c->strategy = c->tune_direct_ms < c->tune_ms[bs][bk] ? SPBP_STRAT_DIRECT : SPBP_STRAT_BRICK;
+    c->C = CVALS[bk];
+    if ((rc = configure_bricks(c, bs))) return rc;
+    return ensure_lists(c, c->fwd_batch);
+}
+
+int spbp_get_plan(const spbp_ctx* c, int* strategy, int* shape, int* brick_proj, float* ms_per_proj) {
+    if (!c) return fail(nullptr, SPBP_ERR_ARG, "NULL context");
+    if (strategy) *strategy = c->strategy;
+    if (shape) *shape = c->shape;
+    if (brick_proj) *brick_proj = c->C;
+    if (ms_per_proj) {
+        // row-major [NSHAPES][NCVALS] brick times, then the direct time; 0 until tuned
+        for (int sh = 0; sh < NSHAPES; ++sh)
+            for (int k = 0; k < NCVALS; ++k) ms_per_proj[sh * NCVALS + k] = c->tuned ? c->tune_ms[sh][k] : 0.f;
+        ms_per_proj[NSHAPES * NCVALS] = c->tuned ? c->tune_direct_ms : 0.f;
+    }
+    return SPBP_OK;
+}
+
+int spbp_set_plan(spbp_ctx* c, int strategy, int shape, int brick_proj) {
+    if (!c) return fail(nullptr, SPBP_ERR_ARG, "NULL context");
+    if (strategy < SPBP_STRAT_AUTO || strategy > SPBP_STRAT_BRICK) return fail(c, SPBP_ERR_ARG, "unknown strategy %d", strategy);
+    if (shape < 0 || shape >= NSHAPES) return fail(c, SPBP_ERR_ARG, "brick shape %d outside 0..%d", shape, NSHAPES - 1);
+    if (brick_proj < 0) return fail(c, SPBP_ERR_ARG, "negative projections per work item");
+    DeviceGuard dg(c->device);
+    c->strategy = strategy == SPBP_STRAT_DIRECT ? SPBP_STRAT_DIRECT : SPBP_STRAT_BRICK;
+    if (brick_proj > 0) c->C = brick_proj;
+    int rc = configure_bricks(c, shape);
+    if (rc) return rc;
+    return c->fwd_batch > 0 ? ensure_lists(c, c->fwd_batch) : SPBP_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Checks a batch and gives every context the shared brick shape: the one
+// with the least total tuned time (each job at its own best C), or the first
+// context's shape when any is untuned.
+int prepare_batch(const spbp_job* jobs, int njobs) {
+    if (!jobs || njobs < 1 || njobs > MAX_JOBS) return fail(nullptr, SPBP_ERR_ARG, "batch of %d jobs (1..%d)", njobs, MAX_JOBS);
+    for (int j = 0; j < njobs; ++j) {
+        spbp_ctx* c = jobs[j].ctx;
+        if (!c) return fail(nullptr, SPBP_ERR_ARG, "job %d has no context", j);
+        if (c->device != jobs[0].ctx->device) return fail(c, SPBP_ERR_ARG, "job %d is on another device", j);
+        for (int k = 0; k < j; ++k)
+            if (jobs[k].ctx == c) return fail(c, SPBP_ERR_ARG, "jobs %d and %d share a context", k, j);
+        if (jobs[j].nimg < 0) return fail(c, SPBP_ERR_ARG, "job %d: negative image count", j);
+        if (jobs[j].nimg > 0 && !jobs[j].imgs) return fail(c, SPBP_ERR_ARG, "job %d: NULL image pointer", j);
+        if (jobs[j].dtype != SPBP_F32 && jobs[j].dtype != SPBP_BF16) return fail(c, SPBP_ERR_ARG, "job %d: unknown image type", j);
+    }
+    bool all_tuned = true;
+    for (int j = 0; j < njobs; ++j) all_tuned = all_tuned && jobs[j].ctx->tuned;
+    int shape = jobs[0].ctx->shape;
+    if (all_tuned) {
+        double best = 1e300;
+        for (int sh = 0; sh < NSHAPES; ++sh) {
+            double tot = 0.0;
+            for (int j = 0; j < njobs; ++j) {
+                const spbp_ctx* c = jobs[j].ctx;
+                float m = c->tune_ms[sh][0];
+                for (int k = 1; k < NCVALS; ++k) m = std::min(m, c->tune_ms[sh][k]);
+                if (c->strategy == SPBP_STRAT_DIRECT) m = c->tune_direct_ms;
+                tot += (double)m * jobs[j].nimg;
+            }
+            if (tot < best) { best = tot; shape = sh; }
+        }
+    }
+    for (int j = 0; j < njobs; ++j) {
+        int rc = apply_shape(jobs[j].ctx, shape);
+        if (rc) return rc;
+    }
+    return SPBP_OK;
+}
+
+// Host-side job table of round r: the brick jobs of the contexts that still
+// have a chunk in that round (direct-strategy contexts run their own kernel).
+// Depends only on device addresses, so it can be built before any work runs.
+int round_table(const spbp_job* jobs, int njobs, Geom* const* geom, int r, BrickJob* table, spbp_ctx** lead) {
+    int n = 0;
+    *lead = nullptr;
+    for (int j = 0; j < njobs; ++j) {
+        spbp_ctx* c = jobs[j].ctx;
+        const int p0 = r * c->fwd_batch;
+        if (p0 >= jobs[j].nimg || c->strategy == SPBP_STRAT_DIRECT) continue;
+        const Targets t{c->X[0], c->W[0], c->X[1], c->W[1], c->Wc[0], c->Wc[1]};
+        table[n++] = brick_job(c, p0, geom[j] + (size_t)p0 * c->nsym, t);
+        if (!*lead) *lead = c;   // its list build zeroes its queue head
+    }
+    return n;
+}
+
+// Enqueue the batch on st: per round, every job with projections left pads and
+// transforms its next chunk and builds its lists (or runs the direct kernel);
+// then one brick-kernel launch serves all jobs of the round.  geom[j] is job
+// j's device geometry.  With tables (graph capture), tables[r] must already
+// hold round r's job table in device memory: nothing is copied from the host.
+int enqueue_batch(const spbp_job* jobs, int njobs, Geom* const* geom, BrickJob* const* tables, hipStream_t st) {
+    int rounds = 0;
+    for (int j = 0; j < njobs; ++j) {
+        spbp_ctx* c = jobs[j].ctx;
+        if (jobs[j].nimg > 0) rounds = std::max(rounds, (jobs[j].nimg + c->fwd_batch - 1) / c->fwd_batch);
+    }
+    for (int r = 0; r < rounds; ++r) {
+        long units = 0;
+        for (int j = 0; j < njobs; ++j) {
+            spbp_ctx* c = jobs[j].ctx;
+            const int p0 = r * c->fwd_batch;
+            if (p0 >= jobs[j].nimg) continue;
+            const int b = std::min(c->fwd_batch, jobs[j].nimg - p0);
+            const size_t esz = jobs[j].dtype == SPBP_BF16 ? 2 : 4;
+            int rc = pad_fft(c, (const char*)jobs[j].imgs + (size_t)p0 * c->nx * c->nx * esz, jobs[j].dtype, b, st);
+            if (rc) return rc;
+            const Targets t{c->X[0], c->W[0], c->X[1], c->W[1], c->Wc[0], c->Wc[1]};
+            const Geom* gj = geom[j] + (size_t)p0 * c->nsym;
+            if (c->strategy == SPBP_STRAT_DIRECT) {
+                if ((rc = launch_direct(c, p0, gj, b * c->nsym, t, st, b))) return rc;
+                continue;
+            }
+            if ((rc = build_lists(c, gj, b * c->nsym, st))) return rc;
+            units += b;
+        }
+        BrickJob table[MAX_JOBS];
+        spbp_ctx* lead = nullptr;
+        const int n = round_table(jobs, njobs, geom, r, table, &lead);
+        if (!n) continue;
+        Timed tm(lead, SPBP_K_INSERT, units, st);
+        const int rc = tables ? launch_bricks_dev(lead, tables[r], n, lead->bnext, st)
+                              : launch_bricks(lead, table, n, lead->bnext, st);
+        if (rc) return rc;
+    }
+    return SPBP_OK;
+}
+
+int batch_rounds(const spbp_job* jobs, int njobs) {
+    int rounds = 0;
+    for (int j = 0; j < njobs; ++j)
+        if (jobs[j].nimg > 0) rounds = std::max(rounds, (jobs[j].nimg + jobs[j].ctx->fwd_batch - 1) / jobs[j].ctx->fwd_batch);
+    return rounds;
+}
+
+int stage_batch(const spbp_job* jobs, int njobs, hipStream_t st) {
+    for (int j = 0; j < njobs; ++j) {
+        spbp_ctx* c = jobs[j].ctx;
+        if (jobs[j].nimg == 0) continue;
+        int rc = check_images(c, jobs[j].imgs, jobs[j].dtype, jobs[j].nimg, st);
+        if (rc) return rc;
+        if ((rc = stage_geom(c, jobs[j].nimg, jobs[j].dm, jobs[j].angles_deg, jobs[j].half, st))) return rc;
+        if ((rc = ensure_chunk(c, std::min(c->batch, jobs[j].nimg)))) return rc;
+    }
+    return SPBP_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ batch graph
+struct spbp_graph {
+    int device = 0;
+    std::vector<spbp_job> jobs;
+    std::vector<Geom*> geom;          // per job, device (owned)
+    std::vector<BrickJob*> tables;    // per round, device (owned)
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+extern "C" {
+
+int spbp_insert_batch(const spbp_job* jobs, int njobs, void* stream) {
+    int rc = prepare_batch(jobs, njobs);
+    if (rc) return rc;
+    DeviceGuard dg(jobs[0].ctx->device);
+    hipStream_t st = (hipStream_t)stream;
+    if ((rc = stage_batch(jobs, njobs, st))) return rc;
+    std::vector<Geom*> geom(njobs);
+    for (int j = 0; j < njobs; ++j) geom[j] = jobs[j].ctx->geom_d;
+    return enqueue_batch(jobs, njobs, geom.data(), nullptr, st);
+}
+
+int spbp_graph_destroy(spbp_graph* g) {
+    if (!g) return SPBP_OK;
+    DeviceGuard dg(g->device);
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    for (Geom* p : g->geom) if (p) (void)hipFree(p);
+    for (BrickJob* p : g->tables) if (p) (void)hipFree(p);
+    if (g->stream) (void)hipStreamDestroy(g->stream);
+    delete g;
+    return SPBP_OK;
+}
+
+int spbp_graph_create(const spbp_job* jobs, int njobs, int reset_first, spbp_graph** out) {
+    if (!out) return fail(nullptr, SPBP_ERR_ARG, "NULL output pointer");
+    *out = nullptr;
+    int rc = prepare_batch(jobs, njobs);
+    if (rc) return rc;
+    for (int j = 0; j < njobs; ++j)
+        if (jobs[j].ctx->flags & SPBP_CHECK_FINITE)
+            return fail(jobs[j].ctx, SPBP_ERR_ARG, "SPBP_CHECK_FINITE cannot be used in a captured batch");
+    spbp_graph* g = new spbp_graph;
+    g->device = jobs[0].ctx->device;
+    g->jobs.assign(jobs, jobs + njobs);
+    DeviceGuard dg(g->device);
+    auto bail = [&](int code) { spbp_graph_destroy(g); return code; };
+    if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(nullptr, SPBP_ERR_DEVICE, "stream creation failed"));
+    // everything that allocates or synchronises happens before the capture:
+    // geometry into graph-owned buffers, chunk buffers, FFT plans (one dry
+    // pass over the chunk sizes), job tables
+    if ((rc = stage_batch(jobs, njobs, g->stream))) return bail(rc);
+    g->geom.assign(njobs, nullptr);
+    for (int j = 0; j < njobs; ++j) {
+        spbp_ctx* c = jobs[j].ctx;
+        if (jobs[j].nimg == 0) continue;
+        const size_t ng = (size_t)jobs[j].nimg * c->nsym;
+        if (hipMalloc(&g->geom[j], ng * sizeof(Geom)) != hipSuccess) return bail(fail(c, SPBP_ERR_ALLOC, "graph geometry"));
+        if (hipMemcpyAsync(g->geom[j], c->geom_d, ng * sizeof(Geom), hipMemcpyDeviceToDevice, g->stream) != hipSuccess)
+            return bail(fail(c, SPBP_ERR_DEVICE, "graph geometry copy"));
+        // run each FFT plan the batch uses once, outside the capture (lazy
+        // set-up inside the FFT library then happens here); the chunk buffers
+        // are scratch, so this changes no result
+        const int tail = jobs[j].nimg % c->fwd_batch;
+        const size_t esz = jobs[j].dtype == SPBP_BF16 ? 2 : 4;
+        if ((rc = pad_fft(c, jobs[j].imgs, jobs[j].dtype, std::min(c->fwd_batch, jobs[j].nimg), g->stream))) return bail(rc);
+        if (tail && jobs[j].nimg > c->fwd_batch &&
+            (rc = pad_fft(c, (const char*)jobs[j].imgs + (size_t)(jobs[j].nimg - tail) * c->nx * c->nx * esz,
+                          jobs[j].dtype, tail, g->stream)))
+            return bail(rc);
+    }
+    if (hipStreamSynchronize(g->stream) != hipSuccess) return bail(fail(nullptr, SPBP_ERR_DEVICE, "graph set-up"));
+    // every round's job table is uploaded now, synchronously: the captured
+    // work only reads these device copies
+    const int rounds = batch_rounds(jobs, njobs);
+    g->tables.assign(rounds, nullptr);
+    for (int r = 0; r < rounds; ++r) {
+        BrickJob table[MAX_JOBS];
+        spbp_ctx* lead = nullptr;
+        const int n = round_table(jobs, njobs, g->geom.data(), r, table, &lead);
+        if (!n) continue;
+        if (hipMalloc(&g->tables[r], MAX_JOBS * sizeof(BrickJob)) != hipSuccess ||
+            hipMemcpy(g->tables[r], table, n * sizeof(BrickJob), hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(nullptr, SPBP_ERR_ALLOC, "graph job tables"));
+    }
+    for (int j = 0; j < njobs; ++j) jobs[j].ctx->capturing = true;
+    rc = SPBP_OK;
+    if (hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        rc = fail(nullptr, SPBP_ERR_DEVICE, "hipStreamBeginCapture failed");
+    } else {
+        if (reset_first)
+            for (int j = 0; j < njobs && rc == SPBP_OK; ++j) {
+                spbp_ctx* c = jobs[j].ctx;
+                for (int h = 0; h < c->nvol; ++h) {
+                    fill_kernel<<<nblocks(2 * c->nh), TPB, 0, g->stream>>>((float*)c->X[h], 2 * c->nh, 0.0f);
+                    fill_kernel<<<nblocks(c->nh), TPB, 0, g->stream>>>(c->W[h], c->nh, 0.0f);
+                    if (c->Wc[h]) fill_kernel<<<nblocks(c->nh), TPB, 0, g->stream>>>(c->Wc[h], c->nh, 0.0f);
+                }
+            }
+        if (rc == SPBP_OK) rc = enqueue_batch(jobs, njobs, g->geom.data(), g->tables.data(), g->stream);
+        const hipError_t e = hipStreamEndCapture(g->stream, &g->graph);
+        if (rc == SPBP_OK && e != hipSuccess) rc = fail(nullptr, SPBP_ERR_DEVICE, "stream capture failed: %s", hipGetErrorString(e));
+    }
+    for (int j = 0; j < njobs; ++j) jobs[j].ctx->capturing = false;
+    if (rc) return bail(rc);
+    if (hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess)
+        return bail(fail(nullptr, SPBP_ERR_DEVICE, "hipGraphInstantiate failed"));
+    *out = g;
+    return SPBP_OK;
+}
+
+int spbp_graph_launch(spbp_graph* g, void* stream) {
+    if (!g || !g->exec) return fail(nullptr, SPBP_ERR_ARG, "NULL graph");
+    DeviceGuard dg(g->device);
+    const hipError_t e = hipGraphLaunch(g->exec, (hipStream_t)stream);
+    if (e != hipSuccess) return fail(nullptr, SPBP_ERR_DEVICE, "hipGraphLaunch: %s", hipGetErrorString(e));
+    return SPBP_OK;
 }
 
 int spbp_insert_host(spbp_ctx* c, const void* imgs_host, int dtype, int nimg, const double* angles_deg,
@@ -580,9 +1145,55 @@ int spbp_export_raw(spbp_ctx* c, int half, float* X, float* W, void* stream) {
     return SPBP_OK;
 }
 
+int spbp_set_symmetries(spbp_ctx* c, int nsym, const double* angles_deg) {
+    if (!c) return fail(nullptr, SPBP_ERR_ARG, "NULL context");
+    if (nsym < 0 || (nsym > 1 && !angles_deg)) return fail(c, SPBP_ERR_ARG, "bad symmetry arguments");
+    DeviceGuard dg(c->device);
+    if (nsym <= 1) {   // one operation (the identity) or none: no symmetrisation (bp32f.f:169)
+        c->nsym = 1;
+        return SPBP_OK;
+    }
+    std::vector<float> sm(9 * (size_t)nsym);
+    for (int k = 0; k < nsym; ++k) {
+        const double* a = angles_deg + 3 * (size_t)k;
+        if (!std::isfinite(a[0]) || !std::isfinite(a[1]) || !std::isfinite(a[2]))
+            return fail(c, SPBP_ERR_ANGLE, "MISSING SYMMETRY %d", k + 1);   // BUILDS (builds.f:47-52)
+        cang((float)a[2], (float)a[1], (float)a[0], (c->flags & SPBP_EXACT_PI) != 0, sm.data() + 9 * k);
+    }
+    HIPCHK(c, hipDeviceSynchronize());   // no queued work may still read the old tables
+    if (c->sm_d) { (void)hipFree(c->sm_d); c->sm_d = nullptr; }
+    HIPCHK(c, hipMalloc(&c->sm_d, sm.size() * sizeof(float)));
+    HIPCHK(c, hipMemcpy(c->sm_d, sm.data(), sm.size() * sizeof(float), hipMemcpyHostToDevice));
+    c->sm = sm;
+    c->nsym = nsym;
+    c->tuned = false;
+    return c->fwd_batch > 0 ? ensure_lists(c, c->fwd_batch) : SPBP_OK;
+}
+
+int spbp_kb_table(int n2, float* tab, float* fltb) {
+    if (n2 < 2 || !tab || !fltb) return fail(nullptr, SPBP_ERR_ARG, "bad table arguments");
+    std::vector<float> t;
+    float f = 0.f;
+    kb_table(n2, t, f);
+    std::memcpy(tab, t.data(), t.size() * sizeof(float));
+    *fltb = f;
+    return SPBP_OK;
+}
+
+int spbp_rotations(int n, const double* ang, int flags, float* dm) {
+    if (n < 0 || (n > 0 && (!ang || !dm))) return fail(nullptr, SPBP_ERR_ARG, "bad rotation arguments");
+    for (int p = 0; p < n; ++p) {
+        const double* a = ang + 3 * (size_t)p;
+        if (!std::isfinite(a[0]) || !std::isfinite(a[1]) || !std::isfinite(a[2]))
+            return fail(nullptr, SPBP_ERR_ANGLE, "MISSING ANGLE FOR IMAGE %d", p + 1);
+        cang((float)a[2], (float)a[1], (float)a[0], (flags & SPBP_EXACT_PI) != 0, dm + 9 * (size_t)p);
+    }
+    return SPBP_OK;
+}
+
 int spbp_kernel_stats(spbp_ctx* c, int kernel, double* total_ms, long* launches, long* units) {
     if (!c) return fail(nullptr, SPBP_ERR_ARG, "NULL context");
-    if (kernel < 0 || kernel > 4) return fail(c, SPBP_ERR_ARG, "no kernel kind %d", kernel);
+    if (kernel < 0 || kernel > SPBP_K_GATHER) return fail(c, SPBP_ERR_ARG, "no kernel kind %d", kernel);
     DeviceGuard dg(c->device);
     double ms = 0.0;
     long u = 0;
