@@ -184,10 +184,11 @@ def run_batched(args):
     per = []
     for d, a, b in zip(vols, out_plain, out_fast):
         truth = torch.tensor(d["ph"].volume(), dtype=torch.float32, device="cuda")
+        fsc_ab = F.fsc(a["full"], b["full"])
         per.append({"nx": d["nx"], "nproj": d["m"],
                     "fsc05_plain_vs_phantom": round(F.resolution(F.fsc(a["full"], truth)), 4),
                     "fsc05_batched_vs_phantom": round(F.resolution(F.fsc(b["full"], truth)), 4),
-                    "min_fsc_plain_vs_batched": round(float(np.min(F.fsc(a["full"], b["full"])["fsc"][1:])), 6)})
+                    "min_fsc_plain_vs_batched": round(float(np.min(fsc_ab["fsc"][fsc_ab["voxels"] > 0])), 6)})
     g.close()
     for c in plain + fast:
         c.close()
