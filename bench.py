@@ -88,12 +88,12 @@ def measured_traffic(kernel, nx, per_launch):
     return None
 
 
-def parity_subset(bpmod, imgs, ang, m):
+def parity_subset(bpmod, imgs, ang, m, **kw):
     import numpy as np
     import oracle
     sub = imgs[:m].contiguous()
-    vol = bpmod.bp3f(sub, ang[:m]).cpu().numpy().astype(np.float64)
-    host = sub.cpu().numpy()
+    vol = bpmod.bp3f(sub, ang[:m], **kw).cpu().numpy().astype(np.float64)
+    host = sub.float().cpu().numpy()   # BF16 images widen exactly
     o64 = oracle.bp(host, ang[:m], prec="f64")["full"]
     o32 = oracle.bp(host, ang[:m], prec="f32")["full"].astype(np.float64)
     r64 = float(np.linalg.norm(vol - o64) / np.linalg.norm(o64))
@@ -306,6 +306,8 @@ def main():
     ap.add_argument("--parity-proj", type=int, default=100)
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="MiB: run one device-to-device copy of this size first (calibrates PMC byte counters)")
+    ap.add_argument("--bf16", action="store_true", help="store the images as BF16 (default for config 5)")
+    ap.add_argument("--twosum", action="store_true", help="compensated weights (default for config 5)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
     args = ap.parse_args()
@@ -335,6 +337,10 @@ def main():
     nproj = args.nproj or rec["nproj"]
     log(f"bench: generating config {args.config}: {nproj} projections {nx}x{nx}")
     ang, ph, imgs = synth.make_dataset(args.config, nproj=nproj, device="cuda")
+    bf16 = args.bf16 or args.config == 5
+    twosum = args.twosum or args.config == 5
+    if bf16:
+        imgs = imgs.to(torch.bfloat16)
     torch.cuda.synchronize()
 
     if args.copy_calib:
@@ -345,7 +351,8 @@ def main():
         del src, dst
         log(f"bench: calibration copy of {args.copy_calib} MiB done")
 
-    b = bpmod.Backprojector(nx, timing=True, batch=args.batch, strategy=args.strategy, brick_proj=args.brick_proj)
+    b = bpmod.Backprojector(nx, timing=True, batch=args.batch, strategy=args.strategy, brick_proj=args.brick_proj,
+                            twosum=twosum)
 
     def step():
         b.reset()
@@ -374,7 +381,9 @@ def main():
     b.close()
 
     U = distinct_voxels(bpmod, nx, synth.uniform_angles(16, np.random.default_rng(5)))
-    b_alg = 4.0 * nx * nx + 24.0 * U                 # bytes per projection
+    # bytes per projection: the image once, and the read + write of each
+    # distinct voxel's complex64 value and fp32 weight (+ fp32 compensation)
+    b_alg = (2.0 if bf16 else 4.0) * nx * nx + (32.0 if twosum else 24.0) * U
     avg_launch_s = ins_ms / 1e3 / launches
     per_launch = units / launches
     achieved = b_alg * per_launch / avg_launch_s / 1e9
@@ -387,25 +396,30 @@ def main():
 
     out = {"metric": METRIC, "value": round(value, 2), "unit": "projections/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 3),
-           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "bf16 images, f32 accumulation" if bf16 else "f32",
            "data": "synthetic (seeded blob phantom, analytic projections, white noise)",
-           "config": {"workload": f"BP 3F plain, {nx}x{nx} fp32 images, {nproj} projections, "
-                                  f"{2 * nx}^3 padded Fourier grid (config {args.config})",
-                      "nx": nx, "nproj": nproj, "grid": 2 * nx, "images_dtype": "f32"},
+           "config": {"workload": f"BP 3F plain, {nx}x{nx} {'bf16' if bf16 else 'fp32'} images, {nproj} projections, "
+                                  f"{2 * nx}^3 padded Fourier grid{', compensated weights' if twosum else ''} "
+                                  f"(config {args.config})",
+                      "nx": nx, "nproj": nproj, "grid": 2 * nx, "images_dtype": "bf16" if bf16 else "f32",
+                      "twosum_weights": twosum},
            "breakdown_ms_per_step": {"pad": round(pad_ms / args.steps, 3), "fft2": round(fft_ms / args.steps, 3),
                                      "insert": round(ins_ms / args.steps, 3), "bin": round(bin_ms / args.steps, 3),
                                      "finalize": round(fin_ms / max(nfin, 1), 3)},
            "roofline": roof}
     if not args.no_parity:
         log("bench: parity subset vs oracle")
-        r64, r32 = parity_subset(bpmod, imgs, ang, min(args.parity_proj, nproj))
+        r64, r32 = parity_subset(bpmod, imgs, ang, min(args.parity_proj, nproj), twosum=twosum)
         out["rel_l2_vs_fp64"] = r64
         out["rel_l2_vs_ref_f32"] = r32
+        g64, _ = parity_subset(bpmod, imgs, ang, min(args.parity_proj, nproj), twosum=twosum, geom_f64=True)
+        out["rel_l2_vs_fp64_with_geom_f64"] = g64
         out["parity_subset_proj"] = min(args.parity_proj, nproj)
     if not args.no_cpu:
         log("bench: cpu baseline")
-        m = min(nproj, 20000)
-        out["cpu_baseline"] = cpu_baseline(imgs[:m].cpu().numpy(), ang[:m], args.cpu_seconds)
+        m = min(nproj, 20000 if nx <= 128 else 2000)
+        out["cpu_baseline"] = cpu_baseline(imgs[:m].float().cpu().numpy(), ang[:m], args.cpu_seconds)
         out["gpu_over_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
     print(json.dumps(out), flush=True)
 

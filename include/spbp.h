@@ -52,6 +52,9 @@ extern "C" {
 #define SPBP_TWOSUM_W     0x04  /* compensated (two-sum) accumulation of the weights           */
 #define SPBP_CHECK_FINITE 0x08  /* reject images holding NaN / Inf                             */
 #define SPBP_TIMING       0x10  /* time each kernel launch with HIP events (spbp_kernel_stats) */
+#define SPBP_GEOM_F64     0x20  /* FP64 rotations (from FP64 angles), sample positions, nearest voxels
+                                   and table indices: departs from the reference's fp32 geometry
+                                   (oneline.f:56-105) to agree with an FP64 computation          */
 
 /* kernels reported by spbp_kernel_stats */
 #define SPBP_K_INSERT   0   /* ONELINE scatter (units: projections)       */

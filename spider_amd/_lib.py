@@ -30,6 +30,7 @@ EXACT_PI = 0x02
 TWOSUM_W = 0x04
 CHECK_FINITE = 0x08
 TIMING = 0x10
+GEOM_F64 = 0x20
 
 K_INSERT = 0
 K_PAD = 1

@@ -18,6 +18,7 @@ struct Geom {        // per inserted section: CANG's matrix DM(1..9) (times a sy
     int half;
     int img;         // image (spectrum) index, counted from the call's first image
     int pad_;
+    double d[6];     // DM(1..6) in FP64 (used by the FP64-geometry instantiations only)
 };
 
 
@@ -104,6 +105,29 @@ __device__ inline bool in_bounds(int ix, int iy, int iz, int nx) {
 __device__ inline float kb_weight(const float* __restrict__ tab, float d, float fltb) {
     return tab[(int)roundf(__fmul_rn(fabsf(d), fltb))];
 }
+
+// FP64-geometry variants (SPBP_GEOM_F64): the same steps on FP64 positions
+// from the FP64 rotation, FLTB = 4999/3 in FP64.  They depart from the
+// reference's fp32 geometry on purpose, to agree with an FP64 computation.
+__device__ inline void sample_pos(const Geom& g, int I, int J, double& x, double& y, double& z) {
+    const double fI = (double)I, fJ = (double)J;
+    x = __dadd_rn(__dmul_rn(fI, g.d[0]), __dmul_rn(fJ, g.d[3]));
+    y = __dadd_rn(__dmul_rn(fI, g.d[1]), __dmul_rn(fJ, g.d[4]));
+    z = __dadd_rn(__dmul_rn(fI, g.d[2]), __dmul_rn(fJ, g.d[5]));
+}
+__device__ inline int nearest_voxel(double v, int N) {
+    return (int)__dadd_rn(__dadd_rn(v, 0.5), (double)N) - N;
+}
+__device__ inline float kb_weight(const float* __restrict__ tab, double d, double fltb) {
+    return tab[(int)round(__dmul_rn(fabs(d), fltb))];
+}
+// position minus an integer voxel coordinate, in the position's precision
+__device__ inline float dsub(float a, int b) { return __fsub_rn(a, (float)b); }
+__device__ inline double dsub(double a, int b) { return __dsub_rn(a, (double)b); }
+// FLTB in the position's precision
+template <typename P> __device__ inline P fltb_of(float f);
+template <> __device__ inline float fltb_of<float>(float f) { return f; }
+template <> __device__ inline double fltb_of<double>(float) { return 4999.0 / 3.0; }
 
 // BI(I, JP) of the raw R2C output with the SPIDER sign (conjugate,
 // fftw3.f:411-419) and the checkerboard (-1)^(I+J+1), J 1-based (bp32f.f:859-864).

@@ -73,6 +73,34 @@ void cang(float phi, float theta, float psi, bool exact_pi, float* dm) {
 }
 
 
+// CANG in FP64 throughout, from FP64 angles (SPBP_GEOM_F64: no fp32 rounding
+// of the angles or of the matrix).
+void cang_d(double phi, double theta, double psi, bool exact_pi, double* dm) {
+    const double dgr = exact_pi ? M_PI / 180.0 : (double)(float)3.14159265358979323846 / 180.0;
+    const double cphi = std::cos(phi * dgr), sphi = std::sin(phi * dgr);
+    const double cthe = std::cos(theta * dgr), sthe = std::sin(theta * dgr);
+    const double cpsi = std::cos(psi * dgr), spsi = std::sin(psi * dgr);
+    dm[0] = cphi * cthe * cpsi - sphi * spsi;
+    dm[1] = sphi * cthe * cpsi + cphi * spsi;
+    dm[2] = -sthe * cpsi;
+    dm[3] = -cphi * cthe * spsi - sphi * cpsi;
+    dm[4] = -sphi * cthe * spsi + cphi * cpsi;
+    dm[5] = sthe * spsi;
+    dm[6] = sthe * cphi;
+    dm[7] = sthe * sphi;
+    dm[8] = cthe;
+}
+
+void sym_matmul_d(const double* sm, const double* dm, double* out) {
+    for (int j = 0; j < 3; ++j)
+        for (int i = 0; i < 3; ++i) {
+            double acc = sm[i] * dm[3 * j];
+            acc = acc + sm[i + 3] * dm[1 + 3 * j];
+            acc = acc + sm[i + 6] * dm[2 + 3 * j];
+            out[i + 3 * j] = acc;
+        }
+}
+
 // DMS = MATMUL(SM, DM) (bp32f.f:866-872) for 3x3 matrices in Fortran (3,3)
 // storage (element (i, j) at [i + 3 j]); fp32 products and sums, k = 1..3.
 void sym_matmul(const float* sm, const float* dm, float* out) {

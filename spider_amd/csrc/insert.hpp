@@ -39,9 +39,10 @@ __global__ __launch_bounds__(TPB) void pad_kernel(const T* __restrict__ imgs, in
 // ------------------------------------------------------------------ strategy 1: direct scatter
 // One thread per sample point (I, J) of one projection, FP32 atomics straight
 // into the half grid (the reference's ONELINE loop, oneline.f:53-156).
+template <typename P>   // position precision: float (the reference's) or double (SPBP_GEOM_F64)
 __global__ __launch_bounds__(TPB) void insert_direct_kernel(
     const float2* __restrict__ spec, int img0, const Geom* __restrict__ geom, int nx, int N,
-    const float* __restrict__ tab, float fltb,
+    const float* __restrict__ tab, float fltb_f,
     float2* __restrict__ X0, float* __restrict__ W0, float2* __restrict__ X1, float* __restrict__ W1,
     float* __restrict__ Wc0, float* __restrict__ Wc1) {
     // W0/W1 may be null (no weights wanted: scatter of BI spectra)
@@ -54,18 +55,19 @@ __global__ __launch_bounds__(TPB) void insert_direct_kernel(
     const int J = jrow - nx + 1;                                   // J = -NX+1 .. NX
     if (!(I * I + J * J < (N * N) / 4) || (I == 0 && J < 0)) return;   // oneline.f:53-55
     const Geom g = geom[p];
-    float x, y, z;
+    const P fltb = fltb_of<P>(fltb_f);
+    P x, y, z;
     sample_pos(g, I, J, x, y, z);
     float2 v = spectrum_value(spec + (size_t)(g.img - img0) * N * H, I, J, N, H);
-    if (x < 0.0f) { x = -x; y = -y; z = -z; v.y = -v.y; }          // oneline.f:60-67
+    if (x < (P)0) { x = -x; y = -y; z = -z; v.y = -v.y; }          // oneline.f:60-67
     const int ix = nearest_voxel(x, N), iy = nearest_voxel(y, N), iz = nearest_voxel(z, N);
     if (!in_bounds(ix, iy, iz, nx)) return;
     float tx[5], ty[5], tz[5];
 #pragma unroll
     for (int l = 0; l < 5; ++l) {
-        tx[l] = kb_weight(tab, __fsub_rn(x, (float)(ix + l - LN2)), fltb);
-        ty[l] = kb_weight(tab, __fsub_rn(y, (float)(iy + l - LN2)), fltb);
-        tz[l] = kb_weight(tab, __fsub_rn(z, (float)(iz + l - LN2)), fltb);
+        tx[l] = kb_weight(tab, dsub(x, ix + l - LN2), fltb);
+        ty[l] = kb_weight(tab, dsub(y, iy + l - LN2), fltb);
+        tz[l] = kb_weight(tab, dsub(z, iz + l - LN2), fltb);
     }
     float2* __restrict__ X = g.half ? X1 : X0;
     float* __restrict__ W = g.half ? W1 : W0;
@@ -250,10 +252,10 @@ struct Box {
     float lo[3], hi[3];
 };
 
-template <int BX, int BY, int BZ>
+template <int BX, int BY, int BZ, typename P>
 __device__ __forceinline__ void wave_project(BrickLds<BX, BY, BZ>& L, const Geom& g, const float2* __restrict__ spec_p,
                                              int nx, int N, int x0, int y0, int z0, const Box box, bool exclude,
-                                             const Box ex, const float* __restrict__ tab, float fltb) {
+                                             const Box ex, const float* __restrict__ tab, P fltb) {
     const float* lo = box.lo;
     const float* hi = box.hi;
     const float* elo = ex.lo;
@@ -315,9 +317,9 @@ __device__ __forceinline__ void wave_project(BrickLds<BX, BY, BZ>& L, const Geom
         const bool canon = aa > 0 || (aa == 0 && bb >= 0);
         const int I = canon ? aa : -aa, J = canon ? bb : -bb;
         if (I > nx || J < -nx + 1 || J > nx || !(I * I + J * J < nx * nx)) continue;   // oneline.f:53-55
-        float x, y, z;
+        P x, y, z;
         sample_pos(g, I, J, x, y, z);
-        const bool flip = x < 0.0f;                                    // oneline.f:60-67
+        const bool flip = x < (P)0;                                    // oneline.f:60-67
         if (flip == canon) continue;   // (aa, bb) must be the sample's effective lattice point
         if (flip) { x = -x; y = -y; z = -z; }
         if (x < lo[0] || x > hi[0] || y < lo[1] || y > hi[1] || z < lo[2] || z > hi[2]) continue;
@@ -330,17 +332,17 @@ __device__ __forceinline__ void wave_project(BrickLds<BX, BY, BZ>& L, const Geom
         // reference taps i-2..i+2 (the table vanishes beyond 1.8 voxels, and
         // i-2 and i+2 are 4 apart): start at i-2 when its weight is nonzero.
         float tx[4], ty[4], tz[4];
-        const float wx0 = kb_weight(tab, __fsub_rn(x, (float)(ix - 2)), fltb);
-        const float wy0 = kb_weight(tab, __fsub_rn(y, (float)(iy - 2)), fltb);
-        const float wz0 = kb_weight(tab, __fsub_rn(z, (float)(iz - 2)), fltb);
+        const float wx0 = kb_weight(tab, dsub(x, ix - 2), fltb);
+        const float wy0 = kb_weight(tab, dsub(y, iy - 2), fltb);
+        const float wz0 = kb_weight(tab, dsub(z, iz - 2), fltb);
         const int sx = wx0 != 0.0f ? ix - 2 : ix - 1;
         const int sy = wy0 != 0.0f ? iy - 2 : iy - 1;
         const int sz = wz0 != 0.0f ? iz - 2 : iz - 1;
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            tx[t] = (t == 0 && sx == ix - 2) ? wx0 : kb_weight(tab, __fsub_rn(x, (float)(sx + t)), fltb);
-            ty[t] = (t == 0 && sy == iy - 2) ? wy0 : kb_weight(tab, __fsub_rn(y, (float)(sy + t)), fltb);
-            tz[t] = (t == 0 && sz == iz - 2) ? wz0 : kb_weight(tab, __fsub_rn(z, (float)(sz + t)), fltb);
+            tx[t] = (t == 0 && sx == ix - 2) ? wx0 : kb_weight(tab, dsub(x, sx + t), fltb);
+            ty[t] = (t == 0 && sy == iy - 2) ? wy0 : kb_weight(tab, dsub(y, sy + t), fltb);
+            tz[t] = (t == 0 && sz == iz - 2) ? wz0 : kb_weight(tab, dsub(z, sz + t), fltb);
         }
         // Targets inside the brick, per axis: a tap p with p_x >= 0 is written at
         // p, a tap with p_x < 0 at -p (conjugated).  Offsets are brick-local;
@@ -412,7 +414,7 @@ constexpr int MAX_JOBS = 64;
 // queue spanning every job of the launch (job-major order); the four waves of
 // a workgroup take the item's projections in turn and share the brick
 // through LDS atomics.  The queue head must be zero at launch.
-template <int BX, int BY, int BZ>
+template <int BX, int BY, int BZ, typename P>
 __global__ __launch_bounds__(TPB) void insert_brick_kernel(const BrickJob* __restrict__ jobs, int njobs,
                                                            int* __restrict__ next_item) {
     constexpr int NV = BX * BY * BZ;
@@ -456,14 +458,14 @@ __global__ __launch_bounds__(TPB) void insert_brick_kernel(const BrickJob* __res
         Box mb;
         for (int k = 0; k < 3; ++k) { mb.lo[k] = -bx.hi[k]; mb.hi[k] = -bx.lo[k]; }
         const float* __restrict__ tab = J.tab;
-        const float fltb = J.fltb;
+        const P fltb = fltb_of<P>(J.fltb);
         for (int s = wave; s < ns; s += nwave) {
             const int p = lst[s];
             const Geom g = J.geom[p];
             const float2* spec_p = J.spec + (size_t)(g.img - J.img0) * N * H;
-            wave_project<BX, BY, BZ>(L, g, spec_p, nx, N, x0, y0, z0, bx, false, bx, tab, fltb);
+            wave_project<BX, BY, BZ, P>(L, g, spec_p, nx, N, x0, y0, z0, bx, false, bx, tab, fltb);
             if (x0 <= LN2)
-                wave_project<BX, BY, BZ>(L, g, spec_p, nx, N, x0, y0, z0, mb, true, bx, tab, fltb);
+                wave_project<BX, BY, BZ, P>(L, g, spec_p, nx, N, x0, y0, z0, mb, true, bx, tab, fltb);
         }
         __syncthreads();
         // add the brick to the half grid

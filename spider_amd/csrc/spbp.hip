@@ -94,6 +94,7 @@ struct spbp_ctx {
     // symmetries (BUILDS): each image is inserted once per matrix, SYMVOL after the window
     int nsym = 1;
     std::vector<float> sm;    // nsym x 9, Fortran (3,3) order
+    std::vector<double> smd;  // the same in FP64 (SPBP_GEOM_F64)
     float* sm_d = nullptr;
     float* symtmp = nullptr;  // NX^3
     float tune_ms[NSHAPES][NCVALS] = {};   // spbp_tune: ms per projection of each brick plan
@@ -303,15 +304,19 @@ int stage_geom(spbp_ctx* c, int nimg, const float* dm_in, const double* ang, con
     int rc = ensure_geom(c, (size_t)nimg * ns);
     if (rc) return rc;
     const bool exact = (c->flags & SPBP_EXACT_PI) != 0;
+    const bool f64 = (c->flags & SPBP_GEOM_F64) != 0;
     for (int p = 0; p < nimg; ++p) {
         float dm[9];
+        double dd[9];
         if (dm_in) {
             std::memcpy(dm, dm_in + 9 * (size_t)p, sizeof dm);
+            for (int q = 0; q < 9; ++q) dd[q] = dm[q];
         } else {
             const double* a = ang + 3 * (size_t)p;
             if (!std::isfinite(a[0]) || !std::isfinite(a[1]) || !std::isfinite(a[2]))
                 return fail(c, SPBP_ERR_ANGLE, "MISSING ANGLE FOR IMAGE %d", p + 1);
             cang((float)a[2], (float)a[1], (float)a[0], exact, dm);   // CANG(PHI, THETA, PSI)
+            if (f64) cang_d(a[2], a[1], a[0], exact, dd);
         }
         for (int q = 0; q < 9; ++q)
             if (!std::isfinite(dm[q])) return fail(c, SPBP_ERR_ANGLE, "non-finite rotation for image %d", p + 1);
@@ -328,6 +333,15 @@ int stage_geom(spbp_ctx* c, int nimg, const float* dm_in, const double* ang, con
             g.half = h;
             g.img = p;
             g.pad_ = 0;
+            if (f64) {
+                double dms[9];
+                if (ns > 1) sym_matmul_d(c->smd.data() + 9 * k, dd, dms);
+                else std::memcpy(dms, dd, sizeof dms);
+                for (int q = 0; q < 6; ++q) g.d[q] = dms[q];
+                for (int q = 0; q < 9; ++q) g.r[q] = (float)dms[q];   // for the brick tests
+            } else {
+                for (int q = 0; q < 6; ++q) g.d[q] = g.r[q];
+            }
         }
     }
     HIPCHK(c, hipMemcpyAsync(c->geom_d, c->geom_h, (size_t)nimg * ns * sizeof(Geom), hipMemcpyHostToDevice, st));
@@ -363,17 +377,22 @@ BrickJob brick_job(const spbp_ctx* c, int img0, const Geom* geom_d, const Target
 unsigned brick_grid(const spbp_ctx* c) {
     // workgroups resident per CU: LDS-bound (52.7 KB) for the largest brick,
     // VGPR-bound (4 waves of 256-thread groups per SIMD) for the others
-    return (unsigned)c->ncu * (c->shape == 0 ? 3u : 4u);
+    // (the FP64-geometry instantiations need 156 VGPRs: 3 per CU for every shape)
+    return (unsigned)c->ncu * ((c->shape == 0 || (c->flags & SPBP_GEOM_F64)) ? 3u : 4u);
 }
 
 // One persistent brick-kernel launch over the device job table table_d[0..n),
 // all jobs with c's brick shape; `counter` (the queue head) must be zero on st.
 int launch_bricks_dev(spbp_ctx* c, const BrickJob* table_d, int n, int* counter, hipStream_t st) {
     const unsigned grid = brick_grid(c);
-    switch (c->shape) {
-        case 0: insert_brick_kernel<16, 16, 8><<<grid, TPB, 0, st>>>(table_d, n, counter); break;
-        case 1: insert_brick_kernel<16, 8, 8><<<grid, TPB, 0, st>>>(table_d, n, counter); break;
-        default: insert_brick_kernel<8, 8, 8><<<grid, TPB, 0, st>>>(table_d, n, counter); break;
+    const bool f64 = (c->flags & SPBP_GEOM_F64) != 0;
+    switch (c->shape * 2 + (f64 ? 1 : 0)) {
+        case 0: insert_brick_kernel<16, 16, 8, float><<<grid, TPB, 0, st>>>(table_d, n, counter); break;
+        case 1: insert_brick_kernel<16, 16, 8, double><<<grid, TPB, 0, st>>>(table_d, n, counter); break;
+        case 2: insert_brick_kernel<16, 8, 8, float><<<grid, TPB, 0, st>>>(table_d, n, counter); break;
+        case 3: insert_brick_kernel<16, 8, 8, double><<<grid, TPB, 0, st>>>(table_d, n, counter); break;
+        case 4: insert_brick_kernel<8, 8, 8, float><<<grid, TPB, 0, st>>>(table_d, n, counter); break;
+        default: insert_brick_kernel<8, 8, 8, double><<<grid, TPB, 0, st>>>(table_d, n, counter); break;
     }
     HIPCHK(c, hipGetLastError());
     return SPBP_OK;
@@ -417,8 +436,12 @@ int launch_direct(spbp_ctx* c, int img0, const Geom* geom_d, int ng, const Targe
     Timed tm(c, SPBP_K_INSERT, units, st);
     for (int g0 = 0; g0 < ng; g0 += 65535) {   // grid.y limit
         dim3 grid(nblocks((size_t)c->H * c->N), std::min(65535, ng - g0));
-        insert_direct_kernel<<<grid, TPB, 0, st>>>(c->spec, img0, geom_d + g0, c->nx, c->N, c->tab, c->fltb,
-                                                   t.X0, t.W0, t.X1, t.W1, t.Wc0, t.Wc1);
+        if (c->flags & SPBP_GEOM_F64)
+            insert_direct_kernel<double><<<grid, TPB, 0, st>>>(c->spec, img0, geom_d + g0, c->nx, c->N, c->tab, c->fltb,
+                                                               t.X0, t.W0, t.X1, t.W1, t.Wc0, t.Wc1);
+        else
+            insert_direct_kernel<float><<<grid, TPB, 0, st>>>(c->spec, img0, geom_d + g0, c->nx, c->N, c->tab, c->fltb,
+                                                              t.X0, t.W0, t.X1, t.W1, t.Wc0, t.Wc1);
     }
     HIPCHK(c, hipGetLastError());
     return SPBP_OK;
@@ -836,6 +859,8 @@ int prepare_batch(const spbp_job* jobs, int njobs) {
         if (jobs[j].nimg < 0) return fail(c, SPBP_ERR_ARG, "job %d: negative image count", j);
         if (jobs[j].nimg > 0 && !jobs[j].imgs) return fail(c, SPBP_ERR_ARG, "job %d: NULL image pointer", j);
         if (jobs[j].dtype != SPBP_F32 && jobs[j].dtype != SPBP_BF16) return fail(c, SPBP_ERR_ARG, "job %d: unknown image type", j);
+        if ((c->flags ^ jobs[0].ctx->flags) & SPBP_GEOM_F64)
+            return fail(c, SPBP_ERR_ARG, "job %d: SPBP_GEOM_F64 differs from job 0 (one kernel serves the batch)", j);
     }
     bool all_tuned = true;
     for (int j = 0; j < njobs; ++j) all_tuned = all_tuned && jobs[j].ctx->tuned;
@@ -1154,17 +1179,20 @@ int spbp_set_symmetries(spbp_ctx* c, int nsym, const double* angles_deg) {
         return SPBP_OK;
     }
     std::vector<float> sm(9 * (size_t)nsym);
+    std::vector<double> smd(9 * (size_t)nsym);
     for (int k = 0; k < nsym; ++k) {
         const double* a = angles_deg + 3 * (size_t)k;
         if (!std::isfinite(a[0]) || !std::isfinite(a[1]) || !std::isfinite(a[2]))
             return fail(c, SPBP_ERR_ANGLE, "MISSING SYMMETRY %d", k + 1);   // BUILDS (builds.f:47-52)
         cang((float)a[2], (float)a[1], (float)a[0], (c->flags & SPBP_EXACT_PI) != 0, sm.data() + 9 * k);
+        cang_d(a[2], a[1], a[0], (c->flags & SPBP_EXACT_PI) != 0, smd.data() + 9 * k);
     }
     HIPCHK(c, hipDeviceSynchronize());   // no queued work may still read the old tables
     if (c->sm_d) { (void)hipFree(c->sm_d); c->sm_d = nullptr; }
     HIPCHK(c, hipMalloc(&c->sm_d, sm.size() * sizeof(float)));
     HIPCHK(c, hipMemcpy(c->sm_d, sm.data(), sm.size() * sizeof(float), hipMemcpyHostToDevice));
     c->sm = sm;
+    c->smd = smd;
     c->nsym = nsym;
     c->tuned = false;
     return c->fwd_batch > 0 ? ensure_lists(c, c->fwd_batch) : SPBP_OK;

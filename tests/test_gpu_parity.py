@@ -294,3 +294,22 @@ def test_symmetries_match_oracle(bpmod, strategy):
         out = {k: v.cpu().numpy() for k, v in b.finalize(full=True, halves=True).items()}
     for k in ("full", "h1", "h2"):
         assert rel(out[k], ref[k]) <= 1e-5, (k, rel(out[k], ref[k]))
+
+
+@pytest.mark.parametrize("strategy", STRATEGIES)
+def test_geom_f64_matches_fp64_oracle(bpmod, strategy):
+    """SPBP_GEOM_F64: FP64 rotations, positions and table indices.  Against
+    the FP64 oracle (same geometry in double) the gap is the fp32 table values,
+    products and sums only: <= 1e-5, far inside the <= 1e-4 bar; the default
+    (reference) geometry stays at its own distance from FP64."""
+    rng = np.random.default_rng(77)
+    nx, m = 48, 80
+    imgs = rng.standard_normal((m, nx, nx)).astype(np.float32)
+    ang = synth.uniform_angles(m, rng)
+    half = bpmod.random_halves(m, seed=2)
+    ref = oracle.bp(imgs, ang, half=half, prec="f64")
+    with bpmod.Backprojector(nx, two_halves=True, strategy=strategy, geom_f64=True) as b:
+        b.insert(gpu(imgs), ang, half)
+        out = {k: v.cpu().numpy() for k, v in b.finalize(full=True, halves=True).items()}
+    for k in ("full", "h1", "h2"):
+        assert rel(out[k], ref[k]) <= 1e-5, (k, rel(out[k], ref[k]))
