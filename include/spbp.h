@@ -154,6 +154,15 @@ int spbp_reset_stats(spbp_ctx* ctx);
 /* Sizes (elements) of one half-grid: complex X count = (NX+1) N N. */
 size_t spbp_grid_elems(const spbp_ctx* ctx);
 
+/* RB 3F / RB 32F (bp32f.f:805-837, CALLRTSQ): each image is first rotated by
+ * its in-plane angle and shifted (RTSQ with QUADRI interpolation, rtsq.f:56,
+ * quadri.f:83) and then inserted as spbp_insert does.  inplane: host
+ * nimg x 3 = doc registers 6, 7, 8 (angle in degrees, x shift, y shift). */
+int spbp_insert_rb(spbp_ctx* ctx, const void* imgs, int dtype, int nimg, const double* angles_deg,
+                   const float* inplane, const int8_t* half, void* stream);
+/* RTSQ alone (scale 1): nimg NX x NX device images (fp32 / bf16) -> out
+ * (device fp32), params host nimg x 3 as above.  Synchronises the stream. */
+int spbp_rtsq(const void* imgs, int dtype, int nimg, int nx, const float* params, float* out, void* stream);
 /* Symmetries (BP 3F / BP 32F "SYMMETRIES DOC", bp32f.f:133-172): nsym doc
  * rows (PSI, THETA, PHI) -> matrices SM by CANG (BUILDS, builds.f:35-63).
  * With nsym > 1 every image is inserted once per DMS = MATMUL(SM_k, DM)

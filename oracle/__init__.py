@@ -82,6 +82,9 @@ def lib():
         L.or_triquad.argtypes = [C.c_double, C.c_double, C.c_double, _f32p]
         L.or_triquad.restype = C.c_float
         L.or_symvol.argtypes = [C.c_int, _f32p, _f32p, _f32p, C.c_int]
+        L.or_quadri.argtypes = [C.c_float, C.c_float, C.c_int, C.c_int, _f32p]
+        L.or_quadri.restype = C.c_float
+        L.or_rtsq.argtypes = [_f32p, _f32p, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float]
         L.or_scatter.argtypes = [C.c_int, _f64p, C.c_int, _f32p, _f32p, C.c_float, _f64p, _f64p]
         L.or_symplane0_f32.argtypes = [C.c_int, _f32p, _f32p]
         L.or_symplane0_f64.argtypes = [C.c_int, _f64p, _f64p]
@@ -329,6 +332,19 @@ def symvol(q1, sm):
     return q2
 
 
+def quadri(x, y, img):
+    img = np.ascontiguousarray(img, np.float32)
+    return np.float32(lib().or_quadri(float(x), float(y), img.shape[1], img.shape[0], img))
+
+
+def rtsq(img, theta, shx, shy, scale=1.0):
+    """RTSQ of one image ([y][x] fp32): rotate, scale and shift with QUADRI."""
+    img = np.ascontiguousarray(img, np.float32)
+    out = np.zeros_like(img)
+    lib().or_rtsq(img, out, img.shape[1], img.shape[0], float(theta), float(scale), float(shx), float(shy))
+    return out
+
+
 def bp_sym(imgs, angles, sym_angles, half=None):
     """FP32-faithful BP 3F / BP 32F with a symmetry document (sym_angles:
     (nsym, 3) PSI, THETA, PHI rows): every projection inserted once per
@@ -435,6 +451,26 @@ class RefLib:
         f = np.ascontiguousarray(f, np.float32).ravel()
         return np.float32(self.L.triquad_(C.byref(C.c_double(r)), C.byref(C.c_double(s)), C.byref(C.c_double(t)),
                                           f.ctypes.data_as(C.c_void_p)))
+
+    # QUADRI(XX,YY,NX,NY,FDATA): REAL function
+    def quadri(self, x, y, img) -> np.float32:
+        self.L.quadri_.restype = C.c_float
+        img = np.ascontiguousarray(img, np.float32)
+        return np.float32(self.L.quadri_(C.byref(C.c_float(x)), C.byref(C.c_float(y)),
+                                         C.byref(C.c_int32(img.shape[1])), C.byref(C.c_int32(img.shape[0])),
+                                         img.ctypes.data_as(C.c_void_p)))
+
+    # RTSQ(XIMG,BUFOUT,NX,NY,NXP,NYP,THETA,SCLI,SHXI,SHYI,IRTFLG)
+    def rtsq(self, img, theta, shx, shy, scale=1.0) -> np.ndarray:
+        img = np.ascontiguousarray(img, np.float32)
+        ny, nx = img.shape
+        out = np.zeros_like(img)
+        irt = C.c_int32()
+        i = lambda v: C.byref(C.c_int32(v))
+        f = lambda v: C.byref(C.c_float(v))
+        self.L.rtsq_(img.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p), i(nx), i(ny), i(nx), i(ny),
+                     f(theta), f(scale), f(shx), f(shy), C.byref(irt))
+        return out
 
     # SYMVOL(Q1,Q2,KLX,KNX,KLY,KNY,KLZ,KNZ,RM,NSYM)
     def symvol(self, q1, sm) -> np.ndarray:

@@ -761,6 +761,76 @@ void symvol_ref(int nx, const float* q1, float* q2, const float* sm, int nsym) {
     for (size_t q = 0; q < nv; ++q) q2[q] = q2[q] / fn;
 }
 
+// ---------------------------------------------------------------- RB 3F pre-rotation
+// QUADRI (quadri.f:83-150): quadratic interpolation of FDATA(NX, NY) (x fastest,
+// 1-based coordinates) at (XX, YY) with wrap-around, in REAL arithmetic and
+// the reference's order of operations.
+float quadri_ref(float xx, float yy, int nx, int ny, const float* f) {
+    float x = xx, y = yy;
+    if (x < 1.0f) x = x + (float)((1 - (int)x / nx) * nx);
+    if (x > (float)nx + 0.5f) x = std::fmod(x - 1.0f, (float)nx) + 1.0f;
+    if (y < 1.0f) y = y + (float)((1 - (int)y / ny) * ny);
+    if (y > (float)ny + 0.5f) y = std::fmod(y - 1.0f, (float)ny) + 1.0f;
+    const int i = (int)x, j = (int)y;
+    const float dx0 = x - (float)i, dy0 = y - (float)j;
+    int ip1 = i + 1, im1 = i - 1, jp1 = j + 1, jm1 = j - 1;
+    if (ip1 > nx) ip1 -= nx;
+    if (im1 < 1) im1 += nx;
+    if (jp1 > ny) jp1 -= ny;
+    if (jm1 < 1) jm1 += ny;
+    auto F = [&](int a, int b) { return f[(size_t)(b - 1) * nx + (a - 1)]; };
+    const float f0 = F(i, j);
+    const float c1 = F(ip1, j) - f0;
+    const float c2 = (c1 - f0 + F(im1, j)) * 0.5f;
+    const float c3 = F(i, jp1) - f0;
+    const float c4 = (c3 - f0 + F(i, jm1)) * 0.5f;
+    const float dxb = dx0 - 1.0f, dyb = dy0 - 1.0f;
+    const float hxc = std::signbit(dx0) ? -1.0f : 1.0f;   // INT(SIGN(1.0, DX0)) stored as REAL
+    const float hyc = std::signbit(dy0) ? -1.0f : 1.0f;
+    int ic = (int)((float)i + hxc), jc = (int)((float)j + hyc);
+    if (ic > nx) ic -= nx;
+    else if (ic < 1) ic += nx;
+    if (jc > ny) jc -= ny;
+    else if (jc < 1) jc += ny;
+    const float c5 = ((((((F(ic, jc) - f0) - hxc * c1) - (hxc * (hxc - 1.0f)) * c2) - hyc * c3) -
+                       (hyc * (hyc - 1.0f)) * c4) * (hxc * hyc));
+    return (f0 + dx0 * ((c1 + dxb * c2) + dy0 * c5)) + dy0 * (c3 + dyb * c4);
+}
+
+// RTSQ (rtsq.f:56-150) with NXP = NX, NYP = NY: rotate by THETA degrees, scale
+// by SCLI, shift by (SHXI, SHYI), sampling with QUADRI; REAL arithmetic.  The
+// angle's cosine and sine are single-precision library values.
+void rtsq_ref(const float* img, float* out, int nx, int ny, float theta, float scli, float shxi, float shyi) {
+    const float dgr_to_rad = (float)3.14159265358979323846 / 180.0f;
+    const float shx = std::fmod(shxi, (float)nx), shy = std::fmod(shyi, (float)ny);
+    const int iycen = ny / 2 + 1, ixcen = nx / 2 + 1;
+    const float rx1 = (float)(-(nx / 2)), ry1 = (float)(-(ny / 2));
+    float rx2 = (float)(nx / 2), ry2 = (float)(ny / 2), rxe1, rxe2, rye1, rye2;
+    if (nx % 2 == 0) { rx2 = rx2 - 1.0f; rxe1 = (float)(-nx); rxe2 = (float)nx; }
+    else { rxe1 = (float)(-nx - 1); rxe2 = (float)(nx + 1); }
+    if (ny % 2 == 0) { ry2 = ry2 - 1.0f; rye1 = (float)(-ny); rye2 = (float)ny; }
+    else { rye1 = (float)(-ny - 1); rye2 = (float)(ny + 1); }
+    const float ang = theta * dgr_to_rad;
+    const float cod = std::cos(ang), sid = std::sin(ang);
+    const float coddscli = cod / scli, siddscli = sid / scli;
+    const float fixcenmshx = (float)(-ixcen) - shx, fiycenmshy = (float)(-iycen) - shy;
+    for (int iy = 1; iy <= ny; ++iy) {
+        float yi = (float)iy + fiycenmshy;
+        if (yi < ry1) yi = std::min(yi + rye2, ry2);
+        if (yi > ry2) yi = std::max(yi + rye1, ry1);
+        const float ycod = yi * coddscli + (float)iycen;
+        const float ysid = (-yi) * siddscli + (float)ixcen;
+        for (int ix = 1; ix <= nx; ++ix) {
+            float xi = (float)ix + fixcenmshx;
+            if (xi < rx1) xi = std::min(xi + rxe2, rx2);
+            if (xi > rx2) xi = std::max(xi + rxe1, rx1);
+            const float yold = xi * siddscli + ycod;
+            const float xold = xi * coddscli + ysid;
+            out[(size_t)(iy - 1) * nx + (ix - 1)] = quadri_ref(xold, yold, nx, ny, img);
+        }
+    }
+}
+
 template <typename R> struct Rot;
 template <> struct Rot<float> {
     static void make(const double* ang, float* dm, int) { cang_ref((float)ang[2], (float)ang[1], (float)ang[0], dm); }
@@ -922,6 +992,10 @@ int or_bp_sym_f32(int nx, int nimg, const float* imgs, const double* angles, con
 void or_sym_matmul_f32(const float* sm, const float* dm, float* out) { sym_matmul<float>(sm, dm, out); }
 float or_triquad(double r, double s, double t, const float* f) { return triquad_ref(r, s, t, f); }
 void or_symvol(int nx, const float* q1, float* q2, const float* sm, int nsym) { symvol_ref(nx, q1, q2, sm, nsym); }
+float or_quadri(float x, float y, int nx, int ny, const float* f) { return quadri_ref(x, y, nx, ny, f); }
+void or_rtsq(const float* img, float* out, int nx, int ny, float theta, float scli, float shx, float shy) {
+    rtsq_ref(img, out, nx, ny, theta, scli, shx, shy);
+}
 
 // Number of distinct half-grid voxels that receive a nonzero KB weight from one
 // projection (Friedel-reflected targets counted once): the unit U(NX) of the

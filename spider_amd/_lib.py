@@ -98,6 +98,8 @@ def lib():
         L.spbp_kb_table.argtypes = [C.c_int, vp, C.POINTER(C.c_float)]
         L.spbp_tune.argtypes = [vp, vp, C.c_int, C.c_int, vp, vp, vp]
         L.spbp_set_symmetries.argtypes = [vp, C.c_int, vp]
+        L.spbp_insert_rb.argtypes = [vp, vp, C.c_int, C.c_int, vp, vp, vp, vp]
+        L.spbp_rtsq.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp]
         L.spbp_get_plan.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), vp]
         L.spbp_set_plan.argtypes = [vp, C.c_int, C.c_int, C.c_int]
         L.spbp_insert_batch.argtypes = [vp, C.c_int, vp]
@@ -110,7 +112,8 @@ def lib():
                      "spbp_import_partial", "spbp_export_raw", "spbp_kernel_stats", "spbp_reset_stats",
                      "spbp_gather", "spbp_scatter", "spbp_kb_table", "spbp_rotations", "spbp_tune",
                      "spbp_get_plan", "spbp_set_plan", "spbp_insert_batch", "spbp_graph_create",
-                     "spbp_graph_launch", "spbp_graph_destroy", "spbp_set_symmetries"):
+                     "spbp_graph_launch", "spbp_graph_destroy", "spbp_set_symmetries", "spbp_insert_rb",
+                     "spbp_rtsq"):
             getattr(L, name).restype = C.c_int
         _LIB = L
     return _LIB

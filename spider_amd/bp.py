@@ -124,8 +124,10 @@ class Backprojector:
             raise TypeError(f"unsupported image dtype {imgs.dtype}")
         return imgs.contiguous(), dt
 
-    def insert(self, imgs, angles, half=None) -> None:
-        """Insert projections with doc-file angles (n, 3): PSI, THETA, PHI."""
+    def insert(self, imgs, angles, half=None, inplane=None) -> None:
+        """Insert projections with doc-file angles (n, 3): PSI, THETA, PHI.
+        inplane (n, 3) = (angle, x shift, y shift), doc registers 6-8: `RB 3F`
+        pre-rotation of each image by RTSQ before the insertion."""
         imgs, dt = self._images(imgs)
         n = imgs.shape[0]
         ang = np.ascontiguousarray(np.asarray(angles, np.float64).reshape(-1, 3))
@@ -134,11 +136,19 @@ class Backprojector:
         hp = None
         if half is not None:
             hp = np.ascontiguousarray(np.asarray(half).astype(np.int8).reshape(n))
-        self._check(_lib.lib().spbp_insert(self._ctx, C.c_void_p(imgs.data_ptr()), dt, n,
-                                           ang.ctypes.data_as(C.c_void_p),
-                                           None if hp is None else hp.ctypes.data_as(C.c_void_p),
-                                           _stream(self.device)))
-        self._keep = (imgs, ang, hp)   # alive until the next call orders after this one
+        hptr = None if hp is None else hp.ctypes.data_as(C.c_void_p)
+        if inplane is None:
+            self._check(_lib.lib().spbp_insert(self._ctx, C.c_void_p(imgs.data_ptr()), dt, n,
+                                               ang.ctypes.data_as(C.c_void_p), hptr, _stream(self.device)))
+            self._keep = (imgs, ang, hp)   # alive until the next call orders after this one
+            return
+        ip = np.ascontiguousarray(np.asarray(inplane, np.float32).reshape(-1, 3))
+        if ip.shape[0] != n:
+            raise SpbpError(_lib.ERR_ANGLE, f"MISSING ANGLE FOR IMAGE: {ip.shape[0]} in-plane entries for {n} images")
+        self._check(_lib.lib().spbp_insert_rb(self._ctx, C.c_void_p(imgs.data_ptr()), dt, n,
+                                              ang.ctypes.data_as(C.c_void_p), ip.ctypes.data_as(C.c_void_p),
+                                              hptr, _stream(self.device)))
+        self._keep = (imgs, ang, hp, ip)
 
     def insert_rotations(self, imgs, dm, half=None) -> None:
         """Insert projections with precomputed CANG matrices (n, 9) fp32."""
@@ -319,21 +329,38 @@ class Backprojector:
         self._keep = (X, W)
 
 
-def bp3f(imgs, angles, exact_pi: bool = False, **kw):
-    """`BP 3F`: one volume from all projections (symmetries=(nsym, 3) doc rows
-    optional)."""
+def rtsq(imgs, params):
+    """RTSQ (rtsq.f:56, scale 1) of device images (n, NX, NX) fp32/bf16 by
+    params (n, 3) = (angle in degrees, x shift, y shift): fp32 (n, NX, NX)."""
+    torch = _torch()
+    if imgs.dim() != 3 or imgs.shape[1] != imgs.shape[2]:
+        raise SpbpError(_lib.ERR_SIZE, "ONLY WORKS ON SQUARE IMAGES")
+    imgs = imgs.contiguous()
+    dt = _lib.BF16 if imgs.dtype == torch.bfloat16 else _lib.F32
+    if dt == _lib.F32 and imgs.dtype != torch.float32:
+        raise TypeError(f"unsupported image dtype {imgs.dtype}")
+    p = np.ascontiguousarray(np.asarray(params, np.float32).reshape(-1, 3))
+    out = torch.empty(imgs.shape, dtype=torch.float32, device=imgs.device)
+    _lib.check(_lib.lib().spbp_rtsq(C.c_void_p(imgs.data_ptr()), dt, imgs.shape[0], imgs.shape[1],
+                                    p.ctypes.data_as(C.c_void_p), C.c_void_p(out.data_ptr()), _stream(imgs.device)))
+    return out
+
+
+def bp3f(imgs, angles, exact_pi: bool = False, inplane=None, **kw):
+    """`BP 3F` (`RB 3F` with inplane): one volume from all projections
+    (symmetries=(nsym, 3) doc rows optional)."""
     with Backprojector(imgs.shape[-1], exact_pi=exact_pi, device=imgs.device.index, **kw) as bp:
-        bp.insert(imgs, angles)
+        bp.insert(imgs, angles, inplane=inplane)
         return bp.finalize()["full"]
 
 
-def bp32f(imgs, angles, half=None, seed=None, exact_pi: bool = False, **kw):
+def bp32f(imgs, angles, half=None, seed=None, exact_pi: bool = False, inplane=None, **kw):
     """`BP 32F`: overall volume plus the two sample volumes.  `half` gives the
     split (0 = first sample volume); default is a RANDLIST-style random split."""
     if half is None:
         half = random_halves(imgs.shape[0], seed)
     with Backprojector(imgs.shape[-1], two_halves=True, exact_pi=exact_pi, device=imgs.device.index, **kw) as bp:
-        bp.insert(imgs, angles, half=half)
+        bp.insert(imgs, angles, half=half, inplane=inplane)
         out = bp.finalize(full=True, halves=True)
     return out["full"], out["h1"], out["h2"]
 
@@ -357,17 +384,21 @@ def read_symmetries(symdoc, ext: str = ""):
     return np.asarray(rows, np.float64)
 
 
-def _load_series(template, numbers, angdoc, symdoc, ext, device):
+def _load_series(template, numbers, angdoc, symdoc, ext, device, inplane=False):
     from . import spiderio
     torch = _torch()
     imgs = spiderio.read_images(template, numbers, ext)
     if imgs.shape[1] != imgs.shape[2]:
         raise SpbpError(_lib.ERR_SIZE, "ONLY WORKS ON SQUARE IMAGES")
     try:
-        ang = spiderio.angles_for(spiderio.read_doc(angdoc), numbers)
+        doc = spiderio.read_doc(angdoc)
+        ang = spiderio.angles_for(doc, numbers)
+        rot = spiderio.inplane_for(doc, numbers) if inplane else None
     except KeyError as e:
         raise SpbpError(_lib.ERR_ANGLE, str(e)) from None
     dev = torch.device("cuda", torch.cuda.current_device() if device is None else int(device))
+    if inplane:
+        return torch.from_numpy(imgs).to(dev), ang, rot
     return torch.from_numpy(imgs).to(dev), ang
 
 
@@ -387,6 +418,28 @@ def bp32f_files(template, numbers, angdoc, symdoc, out_full, out_h1, out_h2, ext
     from . import spiderio
     imgs, ang = _load_series(template, numbers, angdoc, symdoc, ext, device)
     full, h1, h2 = bp32f(imgs, ang, half=half, seed=seed, symmetries=read_symmetries(symdoc, ext), **kw)
+    vols = [v.cpu().numpy() for v in (full, h1, h2)]
+    for path, v in zip((out_full, out_h1, out_h2), vols):
+        spiderio.write_image(path, v)
+    return vols
+
+
+def rb3f_files(template, numbers, angdoc, symdoc, out, ext: str = "", device=None, **kw):
+    """`RB 3F`: as `BP 3F`, each image first rotated by its in-plane angle and
+    shifted (doc registers 6, 7, 8; RTSQ, bp32f.f:805-837)."""
+    from . import spiderio
+    imgs, ang, rot = _load_series(template, numbers, angdoc, symdoc, ext, device, inplane=True)
+    vol = bp3f(imgs, ang, inplane=rot, symmetries=read_symmetries(symdoc, ext), **kw).cpu().numpy()
+    spiderio.write_image(out, vol)
+    return vol
+
+
+def rb32f_files(template, numbers, angdoc, symdoc, out_full, out_h1, out_h2, ext: str = "", seed=None,
+                half=None, device=None, **kw):
+    """`RB 32F`: `BP 32F` with the RTSQ pre-rotation of `RB 3F`."""
+    from . import spiderio
+    imgs, ang, rot = _load_series(template, numbers, angdoc, symdoc, ext, device, inplane=True)
+    full, h1, h2 = bp32f(imgs, ang, half=half, seed=seed, inplane=rot, symmetries=read_symmetries(symdoc, ext), **kw)
     vols = [v.cpu().numpy() for v in (full, h1, h2)]
     for path, v in zip((out_full, out_h1, out_h2), vols):
         spiderio.write_image(path, v)

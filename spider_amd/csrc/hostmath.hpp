@@ -73,6 +73,21 @@ void cang(float phi, float theta, float psi, bool exact_pi, float* dm) {
 }
 
 
+// RTSQ set-up (rtsq.f:93-126) for a square NX image, scale 1: cos/sin of the
+// REAL angle in single precision, shifts reduced by AMOD, -(centre + shift).
+// Outputs coddscli, siddscli, fixcenmshx, fiycenmshy.
+void rtsq_param(int nx, float theta, float shxi, float shyi, float* out4) {
+    const float dgr_to_rad = (float)3.14159265358979323846 / 180.0f;
+    const float shx = std::fmod(shxi, (float)nx), shy = std::fmod(shyi, (float)nx);
+    const int cen = nx / 2 + 1;
+    const float ang = theta * dgr_to_rad;
+    const float cod = std::cos(ang), sid = std::sin(ang);
+    out4[0] = cod / 1.0f;
+    out4[1] = sid / 1.0f;
+    out4[2] = (float)(-cen) - shx;
+    out4[3] = (float)(-cen) - shy;
+}
+
 // CANG in FP64 throughout, from FP64 angles (SPBP_GEOM_F64: no fp32 rounding
 // of the angles or of the matrix).
 void cang_d(double phi, double theta, double psi, bool exact_pi, double* dm) {

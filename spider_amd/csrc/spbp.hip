@@ -41,6 +41,7 @@
 #include "hostmath.hpp"
 #include "insert.hpp"
 #include "project.hpp"
+#include "rotate.hpp"
 
 using namespace spbp;
 
@@ -97,6 +98,13 @@ struct spbp_ctx {
     std::vector<double> smd;  // the same in FP64 (SPBP_GEOM_F64)
     float* sm_d = nullptr;
     float* symtmp = nullptr;  // NX^3
+    // RB pre-rotation: rotated chunk and per-image RTSQ parameters
+    float* rotbuf = nullptr;
+    int rot_cap = 0;
+    RtsqParam* rprm_d = nullptr;
+    RtsqParam* rprm_h = nullptr;   // pinned
+    int rprm_cap = 0;
+    hipEvent_t rprm_evt = nullptr;
     float tune_ms[NSHAPES][NCVALS] = {};   // spbp_tune: ms per projection of each brick plan
     float tune_direct_ms = 0.f;
     BrickJob* jobs_d = nullptr;   // brick-kernel job table
@@ -494,8 +502,35 @@ int check_images(spbp_ctx* c, const void* imgs, int dtype, int nimg, hipStream_t
     return SPBP_OK;
 }
 
+// RTSQ parameters of nimg images (in-plane angle, x shift, y shift per image)
+// into c->rprm_d, queued on st.
+int stage_rtsq(spbp_ctx* c, int nimg, const float* inplane, hipStream_t st) {
+    if (c->rprm_cap < nimg) {
+        if (c->rprm_h) { (void)hipEventSynchronize(c->rprm_evt); (void)hipHostFree(c->rprm_h); c->rprm_h = nullptr; }
+        if (c->rprm_d) { (void)hipFree(c->rprm_d); c->rprm_d = nullptr; }
+        const int cap = std::max(nimg, 1024);
+        HIPCHK(c, hipHostMalloc((void**)&c->rprm_h, cap * sizeof(RtsqParam), hipHostMallocDefault));
+        HIPCHK(c, hipMalloc(&c->rprm_d, cap * sizeof(RtsqParam)));
+        if (!c->rprm_evt) HIPCHK(c, hipEventCreateWithFlags(&c->rprm_evt, hipEventDisableTiming));
+        c->rprm_cap = cap;
+    } else {
+        HIPCHK(c, hipEventSynchronize(c->rprm_evt));
+    }
+    for (int p = 0; p < nimg; ++p) {
+        const float* q = inplane + 3 * (size_t)p;
+        if (!std::isfinite(q[0]) || !std::isfinite(q[1]) || !std::isfinite(q[2]))
+            return fail(c, SPBP_ERR_ANGLE, "MISSING ANGLE FOR IMAGE %d (in-plane angle or shift)", p + 1);
+        float v[4];
+        rtsq_param(c->nx, q[0], q[1], q[2], v);
+        c->rprm_h[p] = RtsqParam{v[0], v[1], v[2], v[3]};
+    }
+    HIPCHK(c, hipMemcpyAsync(c->rprm_d, c->rprm_h, nimg * sizeof(RtsqParam), hipMemcpyHostToDevice, st));
+    HIPCHK(c, hipEventRecord(c->rprm_evt, st));
+    return SPBP_OK;
+}
+
 int insert_impl(spbp_ctx* c, const void* imgs, int dtype, int nimg, const float* dm_in,
-                const double* ang, const int8_t* half, hipStream_t st) {
+                const double* ang, const int8_t* half, hipStream_t st, const float* inplane = nullptr) {
     if (!c) return fail(nullptr, SPBP_ERR_ARG, "NULL context");
     if (nimg < 0) return fail(c, SPBP_ERR_ARG, "negative image count %d", nimg);
     if (nimg == 0) return SPBP_OK;
@@ -504,13 +539,31 @@ int insert_impl(spbp_ctx* c, const void* imgs, int dtype, int nimg, const float*
     int rc = check_images(c, imgs, dtype, nimg, st);
     if (rc) return rc;
     if ((rc = stage_geom(c, nimg, dm_in, ang, half, st))) return rc;
+    if (inplane && (rc = stage_rtsq(c, nimg, inplane, st))) return rc;
     if ((rc = ensure_chunk(c, std::min(c->batch, nimg)))) return rc;
     const size_t esz = dtype == SPBP_BF16 ? 2 : 4;
+    const size_t npix = (size_t)c->nx * c->nx;
+    if (inplane && c->rot_cap < c->fwd_batch) {
+        if (c->rotbuf) { (void)hipFree(c->rotbuf); c->rotbuf = nullptr; }
+        HIPCHK(c, hipMalloc(&c->rotbuf, (size_t)c->fwd_batch * npix * sizeof(float)));
+        c->rot_cap = c->fwd_batch;
+    }
     const Targets t{c->X[0], c->W[0], c->X[1], c->W[1], c->Wc[0], c->Wc[1]};
     const int ns = c->nsym;
     for (int p0 = 0; p0 < nimg; p0 += c->fwd_batch) {
         const int b = std::min(c->fwd_batch, nimg - p0);
-        if ((rc = pad_fft(c, (const char*)imgs + (size_t)p0 * c->nx * c->nx * esz, dtype, b, st))) return rc;
+        const char* src = (const char*)imgs + (size_t)p0 * npix * esz;
+        if (inplane) {   // RTSQ before PADD2 (bp32f.f:805-837)
+            const dim3 grid(nblocks(npix), b);
+            if (dtype == SPBP_BF16)
+                rtsq_kernel<uint16_t><<<grid, TPB, 0, st>>>((const uint16_t*)src, c->rprm_d + p0, c->nx, c->rotbuf);
+            else
+                rtsq_kernel<float><<<grid, TPB, 0, st>>>((const float*)src, c->rprm_d + p0, c->nx, c->rotbuf);
+            HIPCHK(c, hipGetLastError());
+            if ((rc = pad_fft(c, c->rotbuf, SPBP_F32, b, st))) return rc;
+        } else if ((rc = pad_fft(c, src, dtype, b, st))) {
+            return rc;
+        }
         if ((rc = run_insert(c, p0, c->geom_d + (size_t)p0 * ns, b * ns, t, st, b))) return rc;
     }
     return SPBP_OK;
@@ -675,10 +728,13 @@ int spbp_destroy(spbp_ctx* c) {
     }
     void* dev_ptrs[] = {c->tab, c->padbuf, c->spec, c->geom_d, c->work, c->real, c->wkb, c->psum, c->pcnt, c->mean, c->flag,
                         c->bcnt, c->boff, c->bioff, c->bcur, c->bnext, c->blist, c->bikey, c->jobs_d,
-                        c->sm_d, c->symtmp};
+                        c->sm_d, c->symtmp, c->rotbuf, c->rprm_d};
     for (void* p : dev_ptrs) if (p) (void)hipFree(p);
     if (c->geom_h) (void)hipHostFree(c->geom_h);
     if (c->jobs_h) (void)hipHostFree(c->jobs_h);
+    if (c->rprm_evt) (void)hipEventSynchronize(c->rprm_evt);
+    if (c->rprm_h) (void)hipHostFree(c->rprm_h);
+    if (c->rprm_evt) (void)hipEventDestroy(c->rprm_evt);
     if (c->stage) (void)hipHostFree(c->stage);
     if (c->fwd) hipfftDestroy(c->fwd);
     if (c->fwd_tail) hipfftDestroy(c->fwd_tail);
@@ -1086,6 +1142,47 @@ int spbp_graph_launch(spbp_graph* g, void* stream) {
     DeviceGuard dg(g->device);
     const hipError_t e = hipGraphLaunch(g->exec, (hipStream_t)stream);
     if (e != hipSuccess) return fail(nullptr, SPBP_ERR_DEVICE, "hipGraphLaunch: %s", hipGetErrorString(e));
+    return SPBP_OK;
+}
+
+int spbp_insert_rb(spbp_ctx* c, const void* imgs, int dtype, int nimg, const double* angles_deg,
+                   const float* inplane, const int8_t* half, void* stream) {
+    if (!c) return fail(nullptr, SPBP_ERR_ARG, "NULL context");
+    if (nimg > 0 && !inplane) return fail(c, SPBP_ERR_ANGLE, "MISSING ANGLE FOR IMAGE (no in-plane angles / shifts)");
+    return insert_impl(c, imgs, dtype, nimg, nullptr, angles_deg, half, (hipStream_t)stream, inplane);
+}
+
+int spbp_rtsq(const void* imgs, int dtype, int nimg, int nx, const float* params, float* out, void* stream) {
+    if (nimg < 0 || nx < 2) return fail(nullptr, SPBP_ERR_ARG, "bad image count or size");
+    if (nimg == 0) return SPBP_OK;
+    if (!imgs || !params || !out) return fail(nullptr, SPBP_ERR_ARG, "NULL pointer");
+    if (dtype != SPBP_F32 && dtype != SPBP_BF16) return fail(nullptr, SPBP_ERR_ARG, "unknown image type %d", dtype);
+    std::vector<RtsqParam> h(nimg);
+    for (int p = 0; p < nimg; ++p) {
+        const float* q = params + 3 * (size_t)p;
+        if (!std::isfinite(q[0]) || !std::isfinite(q[1]) || !std::isfinite(q[2]))
+            return fail(nullptr, SPBP_ERR_ANGLE, "non-finite rotation or shift for image %d", p + 1);
+        float v[4];
+        rtsq_param(nx, q[0], q[1], q[2], v);
+        h[p] = RtsqParam{v[0], v[1], v[2], v[3]};
+    }
+    hipStream_t st = (hipStream_t)stream;
+    RtsqParam* d = nullptr;
+    HIPCHK(nullptr, hipMalloc(&d, nimg * sizeof(RtsqParam)));
+    hipError_t e = hipMemcpyAsync(d, h.data(), nimg * sizeof(RtsqParam), hipMemcpyHostToDevice, st);
+    const size_t npix = (size_t)nx * nx;
+    for (int p0 = 0; p0 < nimg && e == hipSuccess; p0 += 65535) {
+        const int b = std::min(65535, nimg - p0);
+        const dim3 grid(nblocks(npix), b);
+        if (dtype == SPBP_BF16)
+            rtsq_kernel<uint16_t><<<grid, TPB, 0, st>>>((const uint16_t*)imgs + p0 * npix, d + p0, nx, out + p0 * npix);
+        else
+            rtsq_kernel<float><<<grid, TPB, 0, st>>>((const float*)imgs + p0 * npix, d + p0, nx, out + p0 * npix);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(nullptr, SPBP_ERR_DEVICE, "rtsq: %s", hipGetErrorString(e));
     return SPBP_OK;
 }
 

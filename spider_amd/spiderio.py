@@ -219,6 +219,17 @@ def angles_for(doc: dict, numbers) -> np.ndarray:
     return np.asarray(out, np.float64)
 
 
+def inplane_for(doc: dict, numbers) -> np.ndarray:
+    """Registers 6, 7, 8 (in-plane angle, x shift, y shift) per image number,
+    as RB 3F / RB 32F read them (bp32f.f:122-127, 805-837)."""
+    out = []
+    for k in _expand_numbers(numbers):
+        if k not in doc or len(doc[k]) < 8:
+            raise KeyError(f"MISSING ANGLE FOR IMAGE {k} (registers 6-8)")
+        out.append(doc[k][5:8])
+    return np.asarray(out, np.float64)
+
+
 # ------------------------------------------------------------------ BP 3F P partial volumes
 def write_partial(path: str, X, W) -> str:
     """`BP 3F P` output: the Fourier half grid X (N, N, NX+1) complex as an
