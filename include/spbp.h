@@ -160,6 +160,8 @@ size_t spbp_grid_elems(const spbp_ctx* ctx);
  * nimg x 3 = doc registers 6, 7, 8 (angle in degrees, x shift, y shift). */
 int spbp_insert_rb(spbp_ctx* ctx, const void* imgs, int dtype, int nimg, const double* angles_deg,
                    const float* inplane, const int8_t* half, void* stream);
+int spbp_insert_rb_host(spbp_ctx* ctx, const void* imgs_host, int dtype, int nimg, const double* angles_deg,
+                        const float* inplane, const int8_t* half, void* stream);
 /* RTSQ alone (scale 1): nimg NX x NX device images (fp32 / bf16) -> out
  * (device fp32), params host nimg x 3 as above.  Synchronises the stream. */
 int spbp_rtsq(const void* imgs, int dtype, int nimg, int nx, const float* params, float* out, void* stream);

@@ -100,6 +100,7 @@ def lib():
         L.spbp_set_symmetries.argtypes = [vp, C.c_int, vp]
         L.spbp_insert_rb.argtypes = [vp, vp, C.c_int, C.c_int, vp, vp, vp, vp]
         L.spbp_rtsq.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp]
+        L.spbp_insert_rb_host.argtypes = [vp, vp, C.c_int, C.c_int, vp, vp, vp, vp]
         L.spbp_get_plan.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), vp]
         L.spbp_set_plan.argtypes = [vp, C.c_int, C.c_int, C.c_int]
         L.spbp_insert_batch.argtypes = [vp, C.c_int, vp]
@@ -113,7 +114,7 @@ def lib():
                      "spbp_gather", "spbp_scatter", "spbp_kb_table", "spbp_rotations", "spbp_tune",
                      "spbp_get_plan", "spbp_set_plan", "spbp_insert_batch", "spbp_graph_create",
                      "spbp_graph_launch", "spbp_graph_destroy", "spbp_set_symmetries", "spbp_insert_rb",
-                     "spbp_rtsq"):
+                     "spbp_rtsq", "spbp_insert_rb_host"):
             getattr(L, name).restype = C.c_int
         _LIB = L
     return _LIB

@@ -1186,13 +1186,17 @@ int spbp_rtsq(const void* imgs, int dtype, int nimg, int nx, const float* params
     return SPBP_OK;
 }
 
-int spbp_insert_host(spbp_ctx* c, const void* imgs_host, int dtype, int nimg, const double* angles_deg,
-                     const int8_t* half, void* stream) {
+}  // extern "C"
+
+namespace {
+// Host images staged chunk by chunk through one device buffer, each chunk
+// inserted as spbp_insert (or spbp_insert_rb with inplane) would.
+int insert_host_impl(spbp_ctx* c, const void* imgs_host, int dtype, int nimg, const double* angles_deg,
+                     const float* inplane, const int8_t* half, hipStream_t st) {
     if (!c) return fail(nullptr, SPBP_ERR_ARG, "NULL context");
     if (nimg <= 0) return nimg == 0 ? SPBP_OK : fail(c, SPBP_ERR_ARG, "negative image count %d", nimg);
     if (!imgs_host) return fail(c, SPBP_ERR_ARG, "NULL image pointer");
     DeviceGuard dg(c->device);
-    hipStream_t st = (hipStream_t)stream;
     const size_t esz = dtype == SPBP_BF16 ? 2 : 4;
     const size_t per = (size_t)c->nx * c->nx * esz;
     const int chunk = std::max(1, std::min(nimg, c->batch));
@@ -1203,12 +1207,27 @@ int spbp_insert_host(spbp_ctx* c, const void* imgs_host, int dtype, int nimg, co
         const int b = std::min(chunk, nimg - p0);
         hipError_t e = hipMemcpyAsync(dimg, (const char*)imgs_host + per * p0, per * b, hipMemcpyHostToDevice, st);
         if (e != hipSuccess) { rc = fail(c, SPBP_ERR_DEVICE, "image upload: %s", hipGetErrorString(e)); break; }
-        rc = insert_impl(c, dimg, dtype, b, nullptr, angles_deg + 3 * (size_t)p0, half ? half + p0 : nullptr, st);
+        rc = insert_impl(c, dimg, dtype, b, nullptr, angles_deg ? angles_deg + 3 * (size_t)p0 : nullptr,
+                         half ? half + p0 : nullptr, st, inplane ? inplane + 3 * (size_t)p0 : nullptr);
         e = hipStreamSynchronize(st);
         if (rc == SPBP_OK && e != hipSuccess) rc = fail(c, SPBP_ERR_DEVICE, "insert: %s", hipGetErrorString(e));
     }
     (void)hipFree(dimg);
     return rc;
+}
+}  // namespace
+
+extern "C" {
+
+int spbp_insert_host(spbp_ctx* c, const void* imgs_host, int dtype, int nimg, const double* angles_deg,
+                     const int8_t* half, void* stream) {
+    return insert_host_impl(c, imgs_host, dtype, nimg, angles_deg, nullptr, half, (hipStream_t)stream);
+}
+
+int spbp_insert_rb_host(spbp_ctx* c, const void* imgs_host, int dtype, int nimg, const double* angles_deg,
+                        const float* inplane, const int8_t* half, void* stream) {
+    if (c && nimg > 0 && !inplane) return fail(c, SPBP_ERR_ANGLE, "MISSING ANGLE FOR IMAGE (no in-plane angles / shifts)");
+    return insert_host_impl(c, imgs_host, dtype, nimg, angles_deg, inplane, half, (hipStream_t)stream);
 }
 
 int spbp_finalize(spbp_ctx* c, float* out_full, float* out_h1, float* out_h2, void* stream) {

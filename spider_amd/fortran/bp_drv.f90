@@ -1,14 +1,18 @@
-! bp_drv.f90 -- `BP 3F` / `BP 32F` operation driver on libspbp.
+! bp_drv.f90 -- `BP 3F` / `BP 32F` / `RB 3F` / `RB 32F` operation driver on libspbp.
 !
 ! Keeps the reference operation's signature and answer sequence
 ! (bp3f.f:127-283, bp32f.f:102-410): the answers are read one per line from
 ! standard input, as a SPIDER procedure supplies them:
 !
-!   BP 3F :  template, file numbers, angles doc, symmetries doc ('*'), volume
-!   BP 32F:  template, file numbers, angles doc, symmetries doc ('*'),
+!   BP 3F :  template, file numbers, angles doc, symmetries doc ('*' = none), volume
+!   BP 32F:  template, file numbers, angles doc, symmetries doc,
 !            reconstructed volume, first sample volume, second sample volume
+!   RB 3F / RB 32F: as BP 3F / BP 32F, then the template for the transformed
+!            images ('*': not written -- the only answer accepted here);
+!            registers 6, 7, 8 of the angles doc give the in-plane angle and
+!            shifts applied by RTSQ before the insertion (bp32f.f:805-837)
 !
-! Usage:  spbp_bp 3F|32F [data-extension] < answers
+! Usage:  spbp_bp 3F|32F|RB3F|RB32F [data-extension] < answers
 ! Text after ';' on an answer line is a comment.  Images are SPIDER 2D files
 ! (`name***` templates or `stack@***`), the angles doc holds PSI, THETA, PHI
 ! in registers 1-3 per image number, volumes are written as SPIDER files.
@@ -22,20 +26,24 @@ program spbp_bp
    real(c_float), allocatable, target :: imgs(:, :, :), vol(:, :, :, :)
    real(c_double), allocatable :: angles(:, :)
    integer(c_int8_t), allocatable, target :: half(:)
-   integer, allocatable :: dockeys(:)
+   integer, allocatable :: dockeys(:), docn(:)
    real, allocatable :: docregs(:, :)
    real, allocatable :: img(:, :)
    type(spbp_opts) :: opts
    type(c_ptr) :: ctx
    integer :: nang, nx, ny, k, ierr, nout, kidx, i, j, iord
    integer(c_int) :: rc
-   logical :: is32
+   logical :: is32, isrb
+   real(c_float), allocatable :: inplane(:, :)
+   real(c_double), allocatable :: symang(:, :)
+   integer :: nsym
    real :: x
    logical :: itmp
 
    call get_command_argument(1, op)
    ext = ' '
    if (command_argument_count() >= 2) call get_command_argument(2, ext)
+   isrb = .false.
    select case (trim(op))
    case ('3F', '3f', 'BP 3F')
       is32 = .false.
@@ -43,8 +51,16 @@ program spbp_bp
    case ('32F', '32f', 'BP 32F')
       is32 = .true.
       nout = 3
+   case ('RB3F', 'rb3f', 'RB 3F')
+      is32 = .false.
+      isrb = .true.
+      nout = 1
+   case ('RB32F', 'rb32f', 'RB 32F')
+      is32 = .true.
+      isrb = .true.
+      nout = 3
    case default
-      write (*, '(A)') ' usage: spbp_bp 3F|32F [data-extension] < answers'
+      write (*, '(A)') ' usage: spbp_bp 3F|32F|RB3F|RB32F [data-extension] < answers'
       stop 2
    end select
 
@@ -55,7 +71,10 @@ program spbp_bp
    do k = 1, nout
       call answer('OUTPUT VOLUME', outs(k))
    end do
-   if (trim(symdoc) /= '*') call errt(203, 'SYMMETRIES DOC: ONLY * (NO SYMMETRY) IS SUPPORTED')
+   if (isrb) then
+      call answer('TRANSFORMED OUTPUT IMAGES TEMPLATE', fname)
+      if (trim(fname) /= '*') call errt(203, 'TRANSFORMED OUTPUT IMAGES: ONLY * (NOT WRITTEN) IS SUPPORTED')
+   end if
 
    call expand_numbers(numstr, nums)
    nang = size(nums)
@@ -78,14 +97,34 @@ program spbp_bp
       imgs(:, :, k) = img
    end do
 
-   ! angles (BUILDM1: registers PSI, THETA, PHI of each image's key)
-   call read_doc(angdoc, dockeys, docregs)
+   ! angles (BUILDM1: registers PSI, THETA, PHI of each image's key); RB:
+   ! registers 6-8 (in-plane angle, x shift, y shift)
+   call read_doc(angdoc, dockeys, docregs, docn)
    allocate (angles(3, nang))
+   if (isrb) allocate (inplane(3, nang))
    do k = 1, nang
       j = findloc(dockeys, nums(k), dim=1)
       if (j == 0) call errt(102, 'MISSING ANGLE FOR IMAGE')
       angles(:, k) = real(docregs(1:3, j), c_double)
+      if (isrb) then
+         if (docn(j) < 8) call errt(102, 'MISSING ANGLE FOR IMAGE (REGISTERS 6-8)')
+         inplane(:, k) = real(docregs(6:8, j), c_float)
+      end if
    end do
+
+   ! symmetries (BUILDS: keys 1..max, each row PSI, THETA, PHI)
+   nsym = 0
+   if (trim(symdoc) /= '*') then
+      deallocate (dockeys, docregs, docn)
+      call read_doc(symdoc, dockeys, docregs, docn)
+      nsym = maxval(dockeys)
+      allocate (symang(3, nsym))
+      do k = 1, nsym
+         j = findloc(dockeys, k, dim=1)
+         if (j == 0) call errt(102, 'MISSING SYMMETRY')
+         symang(:, k) = real(docregs(1:3, j), c_double)
+      end do
+   end if
 
    ! half-set split for BP 32F as RANDLIST (bp32f.f:544-553): first half
    ! true, then NANG random swaps; true -> first sample volume
@@ -108,7 +147,15 @@ program spbp_bp
    opts%device = -1
    rc = spbp_create(opts, ctx)
    if (rc /= SPBP_OK) call errt(int(rc), trim(spbp_errmsg(c_null_ptr)))
-   if (is32) then
+   if (nsym > 1) then
+      rc = spbp_set_symmetries(ctx, int(nsym, c_int), symang)
+      if (rc /= SPBP_OK) call errt(int(rc), trim(spbp_errmsg(ctx)))
+   end if
+   if (isrb .and. is32) then
+      rc = spbp_insert_rb_host(ctx, imgs, SPBP_F32, int(nang, c_int), angles, inplane, c_loc(half), c_null_ptr)
+   else if (isrb) then
+      rc = spbp_insert_rb_host(ctx, imgs, SPBP_F32, int(nang, c_int), angles, inplane, c_null_ptr, c_null_ptr)
+   else if (is32) then
       rc = spbp_insert_host(ctx, imgs, SPBP_F32, int(nang, c_int), angles, c_loc(half), c_null_ptr)
    else
       rc = spbp_insert_host(ctx, imgs, SPBP_F32, int(nang, c_int), angles, c_null_ptr, c_null_ptr)
@@ -313,18 +360,20 @@ contains
    end subroutine write_spider_vol
 
    ! SPIDER document file: key, register count, registers; ';' lines are comments.
-   subroutine read_doc(fname, keys, regs)
+   ! keys, their first 8 registers (zero beyond the line's count) and counts
+   subroutine read_doc(fname, keys, regs, nregs)
       character(len=*), intent(in) :: fname
       integer, allocatable, intent(out) :: keys(:)
       real, allocatable, intent(out) :: regs(:, :)
+      integer, allocatable, intent(out) :: nregs(:)
       character(len=2048) :: line, tline
       integer :: u, ios, key, n, m, i
       real :: vals(64)
-      integer, allocatable :: tk(:)
+      integer, allocatable :: tk(:), tn(:)
       real, allocatable :: tr(:, :)
       open (newunit=u, file=with_ext(fname, ext), status='old', action='read', iostat=ios)
       if (ios /= 0) call errt(18, 'CANNOT OPEN DOC FILE: '//trim(fname))
-      allocate (keys(0), regs(3, 0))
+      allocate (keys(0), regs(8, 0), nregs(0))
       m = 0
       do
          read (u, '(A)', iostat=ios) line
@@ -341,13 +390,16 @@ contains
          end if
          if (n < 3) cycle
          m = m + 1
-         allocate (tk(m), tr(3, m))
+         allocate (tk(m), tr(8, m), tn(m))
          tk(1:m - 1) = keys
          tr(:, 1:m - 1) = regs
+         tn(1:m - 1) = nregs
          tk(m) = key
-         tr(:, m) = vals(1:3)
+         tr(:, m) = vals(1:8)
+         tn(m) = n
          call move_alloc(tk, keys)
          call move_alloc(tr, regs)
+         call move_alloc(tn, nregs)
       end do
       close (u)
    end subroutine read_doc

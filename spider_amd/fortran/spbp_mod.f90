@@ -14,7 +14,7 @@ module spbp
       SPBP_ERR_NONFINITE = 205
    integer(c_int), parameter, public :: SPBP_F32 = 0, SPBP_BF16 = 1
    integer(c_int), parameter, public :: SPBP_TWO_HALVES = 1, SPBP_EXACT_PI = 2, SPBP_TWOSUM_W = 4, &
-      SPBP_CHECK_FINITE = 8, SPBP_TIMING = 16
+      SPBP_CHECK_FINITE = 8, SPBP_TIMING = 16, SPBP_GEOM_F64 = 32
    integer(c_int), parameter, public :: SPBP_STRAT_AUTO = 0, SPBP_STRAT_DIRECT = 1, SPBP_STRAT_BRICK = 2
 
    type, bind(c), public :: spbp_opts
@@ -26,7 +26,8 @@ module spbp
       integer(c_int) :: brick_proj = 0
    end type spbp_opts
 
-   public :: spbp_create, spbp_destroy, spbp_insert_host, spbp_finalize_host, spbp_reset, spbp_errmsg
+   public :: spbp_create, spbp_destroy, spbp_insert_host, spbp_finalize_host, spbp_reset, spbp_errmsg, &
+      spbp_set_symmetries, spbp_insert_rb_host
 
    interface
       integer(c_int) function spbp_create(opts, ctx) bind(c, name='spbp_create')
@@ -57,6 +58,27 @@ module spbp
          real(c_double), intent(in) :: angles(*)
          type(c_ptr), value :: half, stream
       end function spbp_insert_host
+
+      ! RB 3F / RB 32F: as spbp_insert_host, each image first rotated and
+      ! shifted by RTSQ; inplane: 3 x nimg REAL (doc registers 6, 7, 8)
+      integer(c_int) function spbp_insert_rb_host(ctx, imgs, dtype, nimg, angles, inplane, half, stream) &
+            bind(c, name='spbp_insert_rb_host')
+         import :: c_int, c_ptr, c_float, c_double
+         type(c_ptr), value :: ctx
+         real(c_float), intent(in) :: imgs(*)
+         integer(c_int), value :: dtype, nimg
+         real(c_double), intent(in) :: angles(*)
+         real(c_float), intent(in) :: inplane(*)
+         type(c_ptr), value :: half, stream
+      end function spbp_insert_rb_host
+
+      ! symmetries doc rows: 3 x nsym DOUBLE PRECISION (PSI, THETA, PHI)
+      integer(c_int) function spbp_set_symmetries(ctx, nsym, angles) bind(c, name='spbp_set_symmetries')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: nsym
+         real(c_double), intent(in) :: angles(*)
+      end function spbp_set_symmetries
 
       ! NX x NX x NX REAL volumes, c_loc(...) or c_null_ptr each
       integer(c_int) function spbp_finalize_host(ctx, out_full, out_h1, out_h2, stream) &

@@ -49,6 +49,11 @@ def test_driver_rejects_non_square_and_missing_angles(tmp_path):
     rc, out = run("3F", ["jnk***", "1,21", "savangbprp", "*", "out"], str(tmp_path))
     assert rc != 0 and "ERROR (102)" in out and "MISSING ANGLE" in out
     rc, out = run("3F", ["jnk***", "1-2", "savangbprp", "c1sym", "out"], str(tmp_path))
+    assert rc != 0 and "ERROR (18)" in out and "c1sym" in out          # symmetries doc not found
+    # RB: the angles doc has no registers 6-8; a transformed-images template is not written
+    rc, out = run("RB3F", ["jnk***", "1-2", "savangbprp", "*", "out", "*"], str(tmp_path))
+    assert rc != 0 and "ERROR (102)" in out and "6-8" in out
+    rc, out = run("RB3F", ["jnk***", "1-2", "savangbprp", "*", "out", "rot***"], str(tmp_path))
     assert rc != 0 and "ERROR (203)" in out
 
 
@@ -98,3 +103,35 @@ def test_python_file_operations(tmp_path):
     bp.bp3f_partial_files(os.path.join(d, "stk@***"), f"18-{m}", os.path.join(d, "ang.dat"), "*", p2, ext=".dat")
     vm = bp.bp3f_merge_files([p1, p2], os.path.join(d, "merged.dat"))
     assert np.linalg.norm(vm - v) / np.linalg.norm(v) < 1e-6
+
+
+@pytest.mark.gpu
+def test_driver_rb32f_and_symmetries(tmp_path):
+    """RB 32F with registers 6-8 and a C2 symmetries doc through the Fortran
+    driver, against the Python operations on the same files and the oracle."""
+    from spider_amd import bp
+    import oracle
+    d = str(tmp_path)
+    rng = np.random.default_rng(41)
+    nx, m = 24, 30
+    ang = synth.uniform_angles(m, rng)
+    rot = np.stack([rng.uniform(0, 360, m), rng.uniform(-2, 2, m), rng.uniform(-2, 2, m)], axis=1)
+    imgs = rng.standard_normal((m, nx, nx)).astype(np.float32)
+    for k in range(m):
+        spiderio.write_image(os.path.join(d, f"img{k + 1:03d}.dat"), imgs[k])
+    spiderio.write_doc(os.path.join(d, "ang.dat"),
+                       {k + 1: list(ang[k]) + [k + 1, k + 1] + list(rot[k]) for k in range(m)})
+    spiderio.write_doc(os.path.join(d, "sym.dat"), {1: [0.0, 0.0, 0.0], 2: [0.0, 0.0, 180.0]})
+    rc, out = run("RB32F", ["img***", f"1-{m}", "ang", "sym", "b", "e", "o", "*"], d)
+    assert rc == 0, out
+    vd = spiderio.read_image(os.path.join(d, "b.dat"))
+    vp, _, _ = bp.rb32f_files(os.path.join(d, "img***"), f"1-{m}", os.path.join(d, "ang.dat"),
+                              os.path.join(d, "sym.dat"), os.path.join(d, "pb.dat"), os.path.join(d, "pe.dat"),
+                              os.path.join(d, "po.dat"), ext=".dat")
+    assert np.linalg.norm(vd - vp) / np.linalg.norm(vp) < 1e-6
+    doc = spiderio.read_doc(os.path.join(d, "ang.dat"))
+    a = spiderio.angles_for(doc, range(1, m + 1))
+    r = spiderio.inplane_for(doc, range(1, m + 1)).astype(np.float32)
+    rimg = np.stack([oracle.rtsq(imgs[k], *r[k]) for k in range(m)])
+    ref = oracle.bp_sym(rimg, a, np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 180.0]]))["full"]
+    assert np.linalg.norm(vp - ref) / np.linalg.norm(ref) < 1e-5
