@@ -302,6 +302,8 @@ def main():
     ap.add_argument("--batch", type=int, default=0, help="projections per internal chunk (0 = auto)")
     ap.add_argument("--strategy", default="auto", choices=["auto", "direct", "brick"])
     ap.add_argument("--brick-proj", type=int, default=0, help="projections per brick work item (0 = auto)")
+    ap.add_argument("--shape", type=int, default=None, help="brick shape 0: 16x16x8, 1: 16x8x8, 2: 8x8x8")
+    ap.add_argument("--tune", action="store_true", help="run the plan tuner on 1024 of the images first")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--parity-proj", type=int, default=100)
     ap.add_argument("--copy-calib", type=int, default=0,
@@ -353,6 +355,16 @@ def main():
 
     b = bpmod.Backprojector(nx, timing=True, batch=args.batch, strategy=args.strategy, brick_proj=args.brick_proj,
                             twosum=twosum)
+    t_tune = None
+    if args.shape is not None:
+        b.set_plan("direct" if args.strategy == "direct" else "brick", args.shape, args.brick_proj)
+    if args.tune:
+        t0 = time.perf_counter()
+        b.tune(imgs[:1024], ang[:1024])
+        torch.cuda.synchronize()
+        t_tune = time.perf_counter() - t0
+    plan = b.plan()
+    log(f"bench: plan {plan['strategy']} brick {plan['brick']} C={plan['brick_proj']}")
 
     def step():
         b.reset()
@@ -387,7 +399,8 @@ def main():
     avg_launch_s = ins_ms / 1e3 / launches
     per_launch = units / launches
     achieved = b_alg * per_launch / avg_launch_s / 1e9
-    kname = "insert_direct_kernel" if args.strategy == "direct" else "insert_brick_kernel<16,16,8>"
+    kname = ("insert_direct_kernel" if plan["strategy"] == "direct"
+             else "insert_brick_kernel<%d,%d,%d>" % tuple(plan["brick"]))
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(kname, nx, per_launch),
             "kernel": kname, "bytes_per_unit": round(b_alg), "unit_of_work": "projection",
@@ -407,7 +420,9 @@ def main():
            "breakdown_ms_per_step": {"pad": round(pad_ms / args.steps, 3), "fft2": round(fft_ms / args.steps, 3),
                                      "insert": round(ins_ms / args.steps, 3), "bin": round(bin_ms / args.steps, 3),
                                      "finalize": round(fin_ms / max(nfin, 1), 3)},
-           "roofline": roof}
+           "roofline": roof,
+           "plan": {"strategy": plan["strategy"], "brick": list(plan["brick"]), "brick_proj": plan["brick_proj"],
+                    "tune_seconds": None if t_tune is None else round(t_tune, 3)}}
     if not args.no_parity:
         log("bench: parity subset vs oracle")
         r64, r32 = parity_subset(bpmod, imgs, ang, min(args.parity_proj, nproj), twosum=twosum)

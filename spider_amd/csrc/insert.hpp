@@ -236,12 +236,49 @@ struct BrickLds {
     static constexpr int SY = BX + 1;
     static constexpr int SZ = BY * SY + 1;
     static constexpr int NS = BZ * SZ;
-    // FP64 partial sums: gfx950's LDS ds_add_f64 runs ~11x the lane rate of
-    // ds_add_f32 (0.26 vs ~3 cycles per lane measured), and each per-tap term
-    // stays the reference's fp32 product, so the sum is at least as accurate.
-    double re[NS], im[NS], w[NS];
+    // gfx950's LDS runs ds_add_u64 at 0.14 cycles per lane, ds_add_f64 at
+    // 0.26 and ds_add_f32 at ~3 (measured).  X partial sums are exact 64-bit
+    // fixed point: each per-tap term stays the reference's fp32 product, is
+    // scaled by a power of two (exact) and rounded to an integer unit of
+    // 2^-48 of the work item's largest spectrum magnitude, so the brick sum is
+    // exact and only the final conversion rounds.  Weights span many orders
+    // of magnitude (products of three table values near the support edge), so
+    // they keep FP64 partial sums, which preserve their relative accuracy.
+    unsigned long long re[NS], im[NS];
+    double w[NS];
     int item;
+    float vmax[TPB / 64];
 };
+
+constexpr int VBITS = 48;                   // |term| < 2^48 units; item sums < 2^62
+
+// x (|x| < 2^51) rounded to the nearest integer, as a two's-complement
+// 64-bit value: add 1.5 * 2^52 and read the low mantissa bits.
+__device__ inline unsigned long long to_fixed(double x) {
+    const double t = __dadd_rn(x, 6755399441055744.0);
+    return (unsigned long long)(__double_as_longlong(t) - 0x4338000000000000LL);
+}
+__device__ inline double from_fixed(unsigned long long a) { return (double)(long long)a; }
+
+// Per image of a chunk: the largest |Re| or |Im| of its raw R2C spectrum
+// (the magnitudes the insertion's terms are formed from).  Block per image.
+__global__ __launch_bounds__(TPB) void spec_absmax_kernel(const float2* __restrict__ spec, int nper,
+                                                          float* __restrict__ out) {
+    const float2* s = spec + (size_t)blockIdx.x * nper;
+    float m = 0.0f;
+    for (int i = threadIdx.x; i < nper; i += blockDim.x) {
+        const float2 v = s[i];
+        m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+    }
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_down(m, off, 64));
+    __shared__ float sm[TPB / 64];
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < (int)(blockDim.x >> 6); ++k) m = fmaxf(m, sm[k]);
+        out[blockIdx.x] = m;
+    }
+}
 
 // One wave feeds the taps of the effective lattice samples of one projection
 // whose E lies in the box [lo, hi] (and, with `exclude`, not in [elo, ehi])
@@ -255,7 +292,7 @@ struct Box {
 template <int BX, int BY, int BZ, typename P>
 __device__ __forceinline__ void wave_project(BrickLds<BX, BY, BZ>& L, const Geom& g, const float2* __restrict__ spec_p,
                                              int nx, int N, int x0, int y0, int z0, const Box box, bool exclude,
-                                             const Box ex, const float* __restrict__ tab, P fltb) {
+                                             const Box ex, const float* __restrict__ tab, P fltb, double vscale) {
     const float* lo = box.lo;
     const float* hi = box.hi;
     const float* elo = ex.lo;
@@ -376,8 +413,8 @@ __device__ __forceinline__ void wave_project(BrickLds<BX, BY, BZ>& L, const Geom
                     const bool ok = xok[lx] && (xmir[lx] ? mok : dok) && wg != 0.0f;
                     if (ok) {
                         const int o = (xmir[lx] ? offm : offd) + xo[lx];
-                        atomicAdd(&L.re[o], (double)__fmul_rn(v.x, wg));
-                        atomicAdd(&L.im[o], (double)__fmul_rn(xmir[lx] ? vim : v.y, wg));
+                        atomicAdd(&L.re[o], to_fixed(__dmul_rn((double)__fmul_rn(v.x, wg), vscale)));
+                        atomicAdd(&L.im[o], to_fixed(__dmul_rn((double)__fmul_rn(xmir[lx] ? vim : v.y, wg), vscale)));
                         atomicAdd(&L.w[o], (double)wg);
                     }
                 }
@@ -390,6 +427,7 @@ __device__ __forceinline__ void wave_project(BrickLds<BX, BY, BZ>& L, const Geom
 // (spectra, geometry), its table, brick grid and lists, and its half grids.
 struct BrickJob {
     const float2* spec;   // spectra of images img0, img0 + 1, ...
+    const float* vmax;    // largest |Re|, |Im| of each of those spectra
     int img0;
     const Geom* geom;
     const float* tab;
@@ -450,8 +488,20 @@ __global__ __launch_bounds__(TPB) void insert_brick_kernel(const BrickJob* __res
         const int* lst = J.list + J.off[key] + s0;
         int x0, y0, z0;
         brick_origin(bg, b, nx, x0, y0, z0);
-        for (int i = threadIdx.x; i < L.NS; i += blockDim.x) { L.re[i] = 0.0; L.im[i] = 0.0; L.w[i] = 0.0; }
+        for (int i = threadIdx.x; i < L.NS; i += blockDim.x) { L.re[i] = 0ull; L.im[i] = 0ull; L.w[i] = 0.0; }
+        // fixed-point unit of the item: 2^-VBITS of its largest spectrum
+        // magnitude (rounded up to a power of two), so every term is below
+        // 2^VBITS units and a voxel's sum over the item below 2^62
+        float m = 0.0f;
+        for (int s = threadIdx.x; s < ns; s += blockDim.x) m = fmaxf(m, J.vmax[J.geom[lst[s]].img - J.img0]);
+        for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_down(m, off, 64));
+        if ((threadIdx.x & 63) == 0) L.vmax[wave] = m;
         __syncthreads();
+        m = L.vmax[0];
+        for (int k = 1; k < nwave; ++k) m = fmaxf(m, L.vmax[k]);
+        int e = 0;
+        if (m > 0.0f && isfinite(m)) (void)frexpf(m, &e);   // m < 2^e
+        const double vscale = ldexp(1.0, VBITS - e), vinv = ldexp(1.0, e - VBITS);
         Box bx;
         bx.lo[0] = x0 - BRICK_MARGIN; bx.lo[1] = y0 - BRICK_MARGIN; bx.lo[2] = z0 - BRICK_MARGIN;
         bx.hi[0] = x0 + BX - 1 + BRICK_MARGIN; bx.hi[1] = y0 + BY - 1 + BRICK_MARGIN; bx.hi[2] = z0 + BZ - 1 + BRICK_MARGIN;
@@ -463,9 +513,9 @@ __global__ __launch_bounds__(TPB) void insert_brick_kernel(const BrickJob* __res
             const int p = lst[s];
             const Geom g = J.geom[p];
             const float2* spec_p = J.spec + (size_t)(g.img - J.img0) * N * H;
-            wave_project<BX, BY, BZ, P>(L, g, spec_p, nx, N, x0, y0, z0, bx, false, bx, tab, fltb);
+            wave_project<BX, BY, BZ, P>(L, g, spec_p, nx, N, x0, y0, z0, bx, false, bx, tab, fltb, vscale);
             if (x0 <= LN2)
-                wave_project<BX, BY, BZ, P>(L, g, spec_p, nx, N, x0, y0, z0, mb, true, bx, tab, fltb);
+                wave_project<BX, BY, BZ, P>(L, g, spec_p, nx, N, x0, y0, z0, mb, true, bx, tab, fltb, vscale);
         }
         __syncthreads();
         // add the brick to the half grid
@@ -479,10 +529,11 @@ __global__ __launch_bounds__(TPB) void insert_brick_kernel(const BrickJob* __res
             const int kx = x0 + lx, ky = y0 + ly, kz = z0 + lz;
             if (kx > nx || ky > nx || kz > nx) continue;   // brick overhangs the grid (partial last brick)
             const size_t o = ((size_t)(kz >= 0 ? kz : N + kz) * N + (ky >= 0 ? ky : N + ky)) * H + kx;
-            unsafeAtomicAdd(&X[o].x, (float)L.re[i]);
-            unsafeAtomicAdd(&X[o].y, (float)L.im[i]);
-            const float wi = (float)L.w[i];
-            if (Wc) twosum_atomic_add(W, Wc, o, wi, (float)(L.w[i] - (double)wi));
+            unsafeAtomicAdd(&X[o].x, (float)__dmul_rn(from_fixed(L.re[i]), vinv));
+            unsafeAtomicAdd(&X[o].y, (float)__dmul_rn(from_fixed(L.im[i]), vinv));
+            const double wd = L.w[i];
+            const float wi = (float)wd;
+            if (Wc) twosum_atomic_add(W, Wc, o, wi, (float)(wd - (double)wi));
             else if (W) unsafeAtomicAdd(&W[o], wi);
         }
         __syncthreads();

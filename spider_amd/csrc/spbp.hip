@@ -66,6 +66,7 @@ struct spbp_ctx {
     // insertion chunk
     float* padbuf = nullptr;
     float2* spec = nullptr;
+    float* vmax = nullptr;    // per chunk image: largest |Re|, |Im| of its spectrum
     Geom* geom_d = nullptr;
     Geom* geom_h = nullptr;   // pinned
     size_t geom_cap = 0;
@@ -88,8 +89,8 @@ struct spbp_ctx {
     size_t stage_cap = 0;
     // brick strategy
     int strategy = SPBP_STRAT_BRICK;
-    int shape = 0;   // index into SHAPES
-    int C = 64;
+    int shape = 1;   // index into SHAPES
+    int C = 32;
     bool tuned = false;
     bool capturing = false;   // a batch graph is being captured: no host syncs, no timing events
     // symmetries (BUILDS): each image is inserted once per matrix, SYMVOL after the window
@@ -245,6 +246,8 @@ int ensure_chunk(spbp_ctx* c, int want_batch) {
     const int N = c->N, H = c->H, B = want_batch;
     HIPCHK(c, hipMalloc(&c->padbuf, (size_t)B * N * N * sizeof(float)));
     HIPCHK(c, hipMalloc(&c->spec, (size_t)B * N * H * sizeof(float2)));
+    if (c->vmax) { (void)hipFree(c->vmax); c->vmax = nullptr; }
+    HIPCHK(c, hipMalloc(&c->vmax, (size_t)B * sizeof(float)));
     int rc = ensure_lists(c, B);
     if (rc) return rc;
     int n[2] = {N, N};
@@ -365,6 +368,7 @@ struct Targets {
 BrickJob brick_job(const spbp_ctx* c, int img0, const Geom* geom_d, const Targets& t) {
     BrickJob j;
     j.spec = c->spec;
+    j.vmax = c->vmax;
     j.img0 = img0;
     j.geom = geom_d;
     j.tab = c->tab;
@@ -480,8 +484,12 @@ int pad_fft(spbp_ctx* c, const void* src, int dtype, int b, hipStream_t st) {
     hipfftHandle plan = plan_for(c, b, &rc);
     if (rc) return rc;
     FFTCHK(c, hipfftSetStream(plan, st));
-    Timed tm(c, SPBP_K_FFT2, b, st);
-    FFTCHK(c, hipfftExecR2C(plan, (hipfftReal*)c->padbuf, (hipfftComplex*)c->spec));
+    {
+        Timed tm(c, SPBP_K_FFT2, b, st);
+        FFTCHK(c, hipfftExecR2C(plan, (hipfftReal*)c->padbuf, (hipfftComplex*)c->spec));
+    }
+    spec_absmax_kernel<<<b, TPB, 0, st>>>(c->spec, c->N * c->H, c->vmax);
+    HIPCHK(c, hipGetLastError());
     return SPBP_OK;
 }
 
@@ -686,7 +694,8 @@ int spbp_create(const spbp_opts* o, spbp_ctx** out) {
     }
     c->strategy = o->strategy == SPBP_STRAT_DIRECT ? SPBP_STRAT_DIRECT : SPBP_STRAT_BRICK;
     if (o->strategy < 0 || o->strategy > SPBP_STRAT_BRICK) { delete c; return fail(nullptr, SPBP_ERR_ARG, "unknown strategy %d", o->strategy); }
-    c->C = o->brick_proj > 0 ? o->brick_proj : 64;
+    if (o->brick_proj > 128) { delete c; return fail(nullptr, SPBP_ERR_ARG, "brick_proj %d above 128", o->brick_proj); }
+    c->C = o->brick_proj > 0 ? o->brick_proj : 32;
     {
         int n = 0;
         if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0) c->ncu = n;
@@ -708,7 +717,9 @@ int spbp_create(const spbp_opts* o, spbp_ctx** out) {
         hipEventCreateWithFlags(&c->jobs_evt, hipEventDisableTiming) != hipSuccess)
         return bail(fail(nullptr, SPBP_ERR_DEVICE, "context set-up failed"));
     if (hipMalloc(&c->bnext, sizeof(int)) != hipSuccess) return bail(fail(nullptr, SPBP_ERR_ALLOC, "brick queue"));
-    if ((rc = configure_bricks(c, 0))) { g_last_error = c->err; return bail(rc); }
+    // default plan: 16x8x8 bricks, 32 projections per item (fastest measured at
+    // 128^2 into 256^3; spbp_tune picks per workload)
+    if ((rc = configure_bricks(c, 1))) { g_last_error = c->err; return bail(rc); }
     rc = spbp_reset(c, nullptr);
     if (rc) return bail(rc);
     if (hipDeviceSynchronize() != hipSuccess) return bail(fail(nullptr, SPBP_ERR_DEVICE, "zeroing accumulators failed"));
@@ -726,7 +737,7 @@ int spbp_destroy(spbp_ctx* c) {
         if (c->W[h]) (void)hipFree(c->W[h]);
         if (c->Wc[h]) (void)hipFree(c->Wc[h]);
     }
-    void* dev_ptrs[] = {c->tab, c->padbuf, c->spec, c->geom_d, c->work, c->real, c->wkb, c->psum, c->pcnt, c->mean, c->flag,
+    void* dev_ptrs[] = {c->tab, c->padbuf, c->spec, c->vmax, c->geom_d, c->work, c->real, c->wkb, c->psum, c->pcnt, c->mean, c->flag,
                         c->bcnt, c->boff, c->bioff, c->bcur, c->bnext, c->blist, c->bikey, c->jobs_d,
                         c->sm_d, c->symtmp, c->rotbuf, c->rprm_d};
     for (void* p : dev_ptrs) if (p) (void)hipFree(p);
@@ -811,6 +822,7 @@ int spbp_scatter(spbp_ctx* c, const float* bi, int nimg, const double* angles_de
         const int b = std::min(c->fwd_batch, nimg - p0);
         const size_t n = (size_t)b * N * H;
         bi_to_raw_kernel<<<nblocks(n), TPB, 0, st>>>((const float2*)bi + (size_t)p0 * N * H, N, H, n, c->spec);
+        spec_absmax_kernel<<<b, TPB, 0, st>>>(c->spec, N * H, c->vmax);
         HIPCHK(c, hipGetLastError());
         if ((rc = run_insert(c, p0, c->geom_d + p0, b, t, st, b))) return rc;
     }
@@ -888,7 +900,7 @@ int spbp_set_plan(spbp_ctx* c, int strategy, int shape, int brick_proj) {
     if (!c) return fail(nullptr, SPBP_ERR_ARG, "NULL context");
     if (strategy < SPBP_STRAT_AUTO || strategy > SPBP_STRAT_BRICK) return fail(c, SPBP_ERR_ARG, "unknown strategy %d", strategy);
     if (shape < 0 || shape >= NSHAPES) return fail(c, SPBP_ERR_ARG, "brick shape %d outside 0..%d", shape, NSHAPES - 1);
-    if (brick_proj < 0) return fail(c, SPBP_ERR_ARG, "negative projections per work item");
+    if (brick_proj < 0 || brick_proj > 128) return fail(c, SPBP_ERR_ARG, "projections per work item %d outside 0..128", brick_proj);
     DeviceGuard dg(c->device);
     c->strategy = strategy == SPBP_STRAT_DIRECT ? SPBP_STRAT_DIRECT : SPBP_STRAT_BRICK;
     if (brick_proj > 0) c->C = brick_proj;
