@@ -238,10 +238,11 @@ struct BrickLds {
     static constexpr int NS = BZ * SZ;
     // gfx950's LDS runs ds_add_u64 at 0.14 cycles per lane, ds_add_f64 at
     // 0.26 and ds_add_f32 at ~3 (measured).  X partial sums are exact 64-bit
-    // fixed point: each per-tap term stays the reference's fp32 product, is
-    // scaled by a power of two (exact) and rounded to an integer unit of
-    // 2^-48 of the work item's largest spectrum magnitude, so the brick sum is
-    // exact and only the final conversion rounds.  Weights span many orders
+    // fixed point: each per-tap term is the exact product of the fp32
+    // spectrum value and the fp32 weight (two 24-bit mantissas fit FP64),
+    // scaled by a power of two and rounded to an integer unit of 2^-48 of the
+    // work item's largest spectrum magnitude; the brick sum is exact and only
+    // that rounding and the final conversion round.  Weights span many orders
     // of magnitude (products of three table values near the support edge), so
     // they keep FP64 partial sums, which preserve their relative accuracy.
     unsigned long long re[NS], im[NS];
@@ -253,10 +254,12 @@ struct BrickLds {
 constexpr int VBITS = 48;                   // |term| < 2^48 units; item sums < 2^62
 
 // x (|x| < 2^51) rounded to the nearest integer, as a two's-complement
-// 64-bit value: add 1.5 * 2^52 and read the low mantissa bits.
+// 64-bit value: add 1.5 * 2^52 and read the mantissa bits.  The constant's
+// low word is zero, so only the high word needs its offset removed.
 __device__ inline unsigned long long to_fixed(double x) {
-    const double t = __dadd_rn(x, 6755399441055744.0);
-    return (unsigned long long)(__double_as_longlong(t) - 0x4338000000000000LL);
+    const unsigned long long b = (unsigned long long)__double_as_longlong(__dadd_rn(x, 6755399441055744.0));
+    const unsigned hi = (unsigned)(b >> 32) - 0x43380000u;
+    return ((unsigned long long)hi << 32) | (unsigned)b;
 }
 __device__ inline double from_fixed(unsigned long long a) { return (double)(long long)a; }
 
@@ -381,19 +384,23 @@ __device__ __forceinline__ void wave_project(BrickLds<BX, BY, BZ>& L, const Geom
             ty[t] = (t == 0 && sy == iy - 2) ? wy0 : kb_weight(tab, dsub(y, sy + t), fltb);
             tz[t] = (t == 0 && sz == iz - 2) ? wz0 : kb_weight(tab, dsub(z, sz + t), fltb);
         }
+        // the value in fixed-point units (exact: a power-of-two scale); the
+        // imaginary part per x tap: conjugated at mirrored targets
+        const double vxs = __dmul_rn((double)v.x, vscale), vys = __dmul_rn((double)v.y, vscale);
         // Targets inside the brick, per axis: a tap p with p_x >= 0 is written at
         // p, a tap with p_x < 0 at -p (conjugated).  Offsets are brick-local;
         // out-of-brick positions get a flag instead of a branch.
         int xo[4];
         bool xok[4], xmir[4];
+        double vim[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             const int px = sx + t;
             xmir[t] = px < 0;
             xo[t] = (xmir[t] ? -px : px) - x0;
             xok[t] = (unsigned)xo[t] < (unsigned)BX;
+            vim[t] = xmir[t] ? -vys : vys;
         }
-        const float vim = -v.y;
 #pragma unroll
         for (int lz = 0; lz < 4; ++lz) {
             const int pz = sz + lz;
@@ -413,9 +420,10 @@ __device__ __forceinline__ void wave_project(BrickLds<BX, BY, BZ>& L, const Geom
                     const bool ok = xok[lx] && (xmir[lx] ? mok : dok) && wg != 0.0f;
                     if (ok) {
                         const int o = (xmir[lx] ? offm : offd) + xo[lx];
-                        atomicAdd(&L.re[o], to_fixed(__dmul_rn((double)__fmul_rn(v.x, wg), vscale)));
-                        atomicAdd(&L.im[o], to_fixed(__dmul_rn((double)__fmul_rn(xmir[lx] ? vim : v.y, wg), vscale)));
-                        atomicAdd(&L.w[o], (double)wg);
+                        const double wd = (double)wg;
+                        atomicAdd(&L.re[o], to_fixed(__dmul_rn(vxs, wd)));
+                        atomicAdd(&L.im[o], to_fixed(__dmul_rn(vim[lx], wd)));
+                        atomicAdd(&L.w[o], wd);
                     }
                 }
             }
@@ -452,9 +460,11 @@ constexpr int MAX_JOBS = 64;
 // queue spanning every job of the launch (job-major order); the four waves of
 // a workgroup take the item's projections in turn and share the brick
 // through LDS atomics.  The queue head must be zero at launch.
+// The fp32-geometry instantiations are held to 128 VGPRs (4 waves per SIMD);
+// the FP64-geometry ones need more and run 3.
 template <int BX, int BY, int BZ, typename P>
-__global__ __launch_bounds__(TPB) void insert_brick_kernel(const BrickJob* __restrict__ jobs, int njobs,
-                                                           int* __restrict__ next_item) {
+__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(sizeof(P) == 4 ? 4 : 3)))
+void insert_brick_kernel(const BrickJob* __restrict__ jobs, int njobs, int* __restrict__ next_item) {
     constexpr int NV = BX * BY * BZ;
     __shared__ BrickLds<BX, BY, BZ> L;
     __shared__ int jbase[MAX_JOBS + 1];
