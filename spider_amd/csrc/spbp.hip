@@ -389,7 +389,7 @@ BrickJob brick_job(const spbp_ctx* c, int img0, const Geom* geom_d, const Target
 unsigned brick_grid(const spbp_ctx* c) {
     // workgroups resident per CU: LDS-bound (52.7 KB) for the largest brick,
     // VGPR-bound (4 waves of 256-thread groups per SIMD) for the others
-    // (the FP64-geometry instantiations need 156 VGPRs: 3 per CU for every shape)
+    // (the FP64-geometry instantiations need ~165 VGPRs: 3 per CU for every shape)
     return (unsigned)c->ncu * ((c->shape == 0 || (c->flags & SPBP_GEOM_F64)) ? 3u : 4u);
 }
 
