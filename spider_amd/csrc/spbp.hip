@@ -48,8 +48,8 @@ using namespace spbp;
 namespace {
 thread_local std::string g_last_error;
 // brick shapes of the LDS-binned strategy (voxels; 3 x 8 B per voxel in LDS)
-constexpr int NSHAPES = 3;
-constexpr int SHAPES[NSHAPES][3] = {{16, 16, 8}, {16, 8, 8}, {8, 8, 8}};
+constexpr int NSHAPES = 4;
+constexpr int SHAPES[NSHAPES][3] = {{16, 16, 8}, {16, 8, 8}, {8, 8, 8}, {16, 12, 8}};
 constexpr int NCVALS = 4;
 constexpr int CVALS[NCVALS] = {16, 32, 64, 128};   // projections per work item tried by the tuner
 }  // namespace
@@ -89,7 +89,7 @@ struct spbp_ctx {
     size_t stage_cap = 0;
     // brick strategy
     int strategy = SPBP_STRAT_BRICK;
-    int shape = 1;   // index into SHAPES
+    int shape = 3;   // index into SHAPES
     int C = 32;
     bool tuned = false;
     bool capturing = false;   // a batch graph is being captured: no host syncs, no timing events
@@ -390,7 +390,7 @@ unsigned brick_grid(const spbp_ctx* c) {
     // workgroups resident per CU: LDS-bound (52.7 KB) for the largest brick,
     // VGPR-bound (4 waves of 256-thread groups per SIMD) for the others
     // (the FP64-geometry instantiations need ~165 VGPRs: 3 per CU for every shape)
-    return (unsigned)c->ncu * ((c->shape == 0 || (c->flags & SPBP_GEOM_F64)) ? 3u : 4u);
+    return (unsigned)c->ncu * ((c->shape == 0 || (c->flags & SPBP_GEOM_F64)) ? 3u : 4u);   // 16x12x8: 39.4 KB, 4 fit
 }
 
 // One persistent brick-kernel launch over the device job table table_d[0..n),
@@ -404,7 +404,9 @@ int launch_bricks_dev(spbp_ctx* c, const BrickJob* table_d, int n, int* counter,
         case 2: insert_brick_kernel<16, 8, 8, float><<<grid, TPB, 0, st>>>(table_d, n, counter); break;
         case 3: insert_brick_kernel<16, 8, 8, double><<<grid, TPB, 0, st>>>(table_d, n, counter); break;
         case 4: insert_brick_kernel<8, 8, 8, float><<<grid, TPB, 0, st>>>(table_d, n, counter); break;
-        default: insert_brick_kernel<8, 8, 8, double><<<grid, TPB, 0, st>>>(table_d, n, counter); break;
+        case 5: insert_brick_kernel<8, 8, 8, double><<<grid, TPB, 0, st>>>(table_d, n, counter); break;
+        case 6: insert_brick_kernel<16, 12, 8, float><<<grid, TPB, 0, st>>>(table_d, n, counter); break;
+        default: insert_brick_kernel<16, 12, 8, double><<<grid, TPB, 0, st>>>(table_d, n, counter); break;
     }
     HIPCHK(c, hipGetLastError());
     return SPBP_OK;
@@ -717,9 +719,9 @@ int spbp_create(const spbp_opts* o, spbp_ctx** out) {
         hipEventCreateWithFlags(&c->jobs_evt, hipEventDisableTiming) != hipSuccess)
         return bail(fail(nullptr, SPBP_ERR_DEVICE, "context set-up failed"));
     if (hipMalloc(&c->bnext, sizeof(int)) != hipSuccess) return bail(fail(nullptr, SPBP_ERR_ALLOC, "brick queue"));
-    // default plan: 16x8x8 bricks, 32 projections per item (fastest measured at
+    // default plan: 16x12x8 bricks, 32 projections per item (fastest measured at
     // 128^2 into 256^3; spbp_tune picks per workload)
-    if ((rc = configure_bricks(c, 1))) { g_last_error = c->err; return bail(rc); }
+    if ((rc = configure_bricks(c, 3))) { g_last_error = c->err; return bail(rc); }
     rc = spbp_reset(c, nullptr);
     if (rc) return bail(rc);
     if (hipDeviceSynchronize() != hipSuccess) return bail(fail(nullptr, SPBP_ERR_DEVICE, "zeroing accumulators failed"));
