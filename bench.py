@@ -72,10 +72,11 @@ def cpu_baseline(imgs_host, ang, seconds):
                       f"{t:.1f} s on {threads} threads; oracle FFT is its own double-precision FFT"}
 
 
-def measured_traffic(kernel, nx, per_launch):
+def measured_traffic(kernel, nx, per_launch, images_dtype="f32", twosum=False):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE measurement (profiles/traffic.json), scaled to this launch size;
-    None when no measurement matches."""
+    WRITE_SIZE measurement (profiles/traffic.json) for the same NX, image
+    storage and weight compensation, scaled to this launch size; None when no
+    measurement matches."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as fh:
@@ -83,7 +84,8 @@ def measured_traffic(kernel, nx, per_launch):
     except (OSError, ValueError):
         return None
     for rec in t.get("records", []):
-        if rec.get("kernel") == kernel and rec.get("nx") == nx:
+        if (rec.get("kernel") == kernel and rec.get("nx") == nx
+                and rec.get("images_dtype", "f32") == images_dtype and bool(rec.get("twosum", False)) == twosum):
             return round(rec["bytes_per_unit"] * per_launch)
     return None
 
@@ -402,7 +404,7 @@ def main():
     kname = ("insert_direct_kernel" if plan["strategy"] == "direct"
              else "insert_brick_kernel<%d,%d,%d>" % tuple(plan["brick"]))
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(kname, nx, per_launch),
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(kname, nx, per_launch, "bf16" if bf16 else "f32", twosum),
             "kernel": kname, "bytes_per_unit": round(b_alg), "unit_of_work": "projection",
             "units_per_launch": round(per_launch, 1), "avg_launch_ms": round(avg_launch_s * 1e3, 4),
             "U_distinct_voxels": round(U, 1)}

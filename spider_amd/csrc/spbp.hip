@@ -692,7 +692,9 @@ int spbp_create(const spbp_opts* o, spbp_ctx** out) {
         c->batch = o->batch;
     } else {
         const size_t per = (size_t)c->N * c->N * 4 + (size_t)c->N * c->H * 8;
-        c->batch = (int)std::max<size_t>(16, std::min<size_t>(4096, ((size_t)512 << 20) / per));
+        // 2 GiB of padded images + spectra per chunk (of 288 GB): larger chunks
+        // mean fewer list builds and fewer persistent-kernel tails
+        c->batch = (int)std::max<size_t>(16, std::min<size_t>(4096, ((size_t)2048 << 20) / per));
     }
     c->strategy = o->strategy == SPBP_STRAT_DIRECT ? SPBP_STRAT_DIRECT : SPBP_STRAT_BRICK;
     if (o->strategy < 0 || o->strategy > SPBP_STRAT_BRICK) { delete c; return fail(nullptr, SPBP_ERR_ARG, "unknown strategy %d", o->strategy); }
