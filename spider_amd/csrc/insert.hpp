@@ -253,13 +253,16 @@ struct BrickLds {
 
 constexpr int VBITS = 48;                   // |term| < 2^48 units; item sums < 2^62
 
-// x (|x| < 2^51) rounded to the nearest integer, as a two's-complement
-// 64-bit value: add 1.5 * 2^52 and read the mantissa bits.  The constant's
-// low word is zero, so only the high word needs its offset removed.
-__device__ inline unsigned long long to_fixed(double x) {
-    const unsigned long long b = (unsigned long long)__double_as_longlong(__dadd_rn(x, 6755399441055744.0));
-    const unsigned hi = (unsigned)(b >> 32) - 0x43380000u;
-    return ((unsigned long long)hi << 32) | (unsigned)b;
+// a * b (|a b| < 2^51) rounded to the nearest integer, as a two's-complement
+// 64-bit value: add 1.5 * 2^52 and read the mantissa bits.  The constant's low
+// word is zero, so only the high word needs its offset removed.  The callers'
+// products are exact in FP64 (two fp32 mantissas, one scaled by a power of
+// two), so the fused multiply-add rounds once, exactly as the rounded product
+// plus the offset would.
+__device__ inline unsigned long long to_fixed_prod(double a, double b) {
+    const unsigned long long r = (unsigned long long)__double_as_longlong(__fma_rn(a, b, 6755399441055744.0));
+    const unsigned hi = (unsigned)(r >> 32) - 0x43380000u;
+    return ((unsigned long long)hi << 32) | (unsigned)r;
 }
 __device__ inline double from_fixed(unsigned long long a) { return (double)(long long)a; }
 
@@ -292,7 +295,10 @@ struct Box {
     float lo[3], hi[3];
 };
 
-template <int BX, int BY, int BZ, typename P>
+// MIR: the brick starts at kx <= LN2, so taps with p_x < 0 can land in it
+// mirrored (at -p_x <= LN2); elsewhere they never do and the tap loop drops
+// the per-tap mirror selects.
+template <int BX, int BY, int BZ, bool MIR, typename P>
 __device__ __forceinline__ void wave_project(BrickLds<BX, BY, BZ>& L, const Geom& g, const float2* __restrict__ spec_p,
                                              int nx, int N, int x0, int y0, int z0, const Box box, bool exclude,
                                              const Box ex, const float* __restrict__ tab, P fltb, double vscale) {
@@ -384,46 +390,78 @@ __device__ __forceinline__ void wave_project(BrickLds<BX, BY, BZ>& L, const Geom
             ty[t] = (t == 0 && sy == iy - 2) ? wy0 : kb_weight(tab, dsub(y, sy + t), fltb);
             tz[t] = (t == 0 && sz == iz - 2) ? wz0 : kb_weight(tab, dsub(z, sz + t), fltb);
         }
-        // the value in fixed-point units (exact: a power-of-two scale); the
-        // imaginary part per x tap: conjugated at mirrored targets
+        // the value in fixed-point units (exact: a power-of-two scale)
         const double vxs = __dmul_rn((double)v.x, vscale), vys = __dmul_rn((double)v.y, vscale);
-        // Targets inside the brick, per axis: a tap p with p_x >= 0 is written at
-        // p, a tap with p_x < 0 at -p (conjugated).  Offsets are brick-local;
-        // out-of-brick positions get a flag instead of a branch.
-        int xo[4];
-        bool xok[4], xmir[4];
-        double vim[4];
+        if constexpr (MIR) {
+            // Targets inside the brick, per axis: a tap p with p_x >= 0 is written
+            // at p, a tap with p_x < 0 at -p (conjugated).  Offsets are
+            // brick-local; out-of-brick positions get a flag instead of a branch.
+            int xo[4];
+            bool xok[4], xmir[4];
+            double vim[4];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const int px = sx + t;
-            xmir[t] = px < 0;
-            xo[t] = (xmir[t] ? -px : px) - x0;
-            xok[t] = (unsigned)xo[t] < (unsigned)BX;
-            vim[t] = xmir[t] ? -vys : vys;
-        }
+            for (int t = 0; t < 4; ++t) {
+                const int px = sx + t;
+                xmir[t] = px < 0;
+                xo[t] = (xmir[t] ? -px : px) - x0;
+                xok[t] = (unsigned)xo[t] < (unsigned)BX;
+                vim[t] = xmir[t] ? -vys : vys;
+            }
 #pragma unroll
-        for (int lz = 0; lz < 4; ++lz) {
-            const int pz = sz + lz;
-            const int zd = pz - z0, zm = -pz - z0;
-            const bool zdok = (unsigned)zd < (unsigned)BZ, zmok = (unsigned)zm < (unsigned)BZ;
+            for (int lz = 0; lz < 4; ++lz) {
+                const int pz = sz + lz;
+                const int zd = pz - z0, zm = -pz - z0;
+                const bool zdok = (unsigned)zd < (unsigned)BZ, zmok = (unsigned)zm < (unsigned)BZ;
 #pragma unroll
-            for (int ly = 0; ly < 4; ++ly) {
-                const int py = sy + ly;
-                const int yd = py - y0, ym = -py - y0;
-                const bool dok = zdok && (unsigned)yd < (unsigned)BY;
-                const bool mok = zmok && (unsigned)ym < (unsigned)BY;
-                const int offd = zd * L.SZ + yd * L.SY, offm = zm * L.SZ + ym * L.SY;
-                const float tyz = __fmul_rn(ty[ly], tz[lz]);       // TY = TABI(Y) * TZ
+                for (int ly = 0; ly < 4; ++ly) {
+                    const int py = sy + ly;
+                    const int yd = py - y0, ym = -py - y0;
+                    const bool dok = zdok && (unsigned)yd < (unsigned)BY;
+                    const bool mok = zmok && (unsigned)ym < (unsigned)BY;
+                    const int offd = zd * L.SZ + yd * L.SY, offm = zm * L.SZ + ym * L.SY;
+                    const float tyz = __fmul_rn(ty[ly], tz[lz]);       // TY = TABI(Y) * TZ
 #pragma unroll
-                for (int lx = 0; lx < 4; ++lx) {
-                    const float wg = __fmul_rn(tx[lx], tyz);       // WG = TABI(X) * TY
-                    const bool ok = xok[lx] && (xmir[lx] ? mok : dok) && wg != 0.0f;
-                    if (ok) {
-                        const int o = (xmir[lx] ? offm : offd) + xo[lx];
-                        const double wd = (double)wg;
-                        atomicAdd(&L.re[o], to_fixed(__dmul_rn(vxs, wd)));
-                        atomicAdd(&L.im[o], to_fixed(__dmul_rn(vim[lx], wd)));
-                        atomicAdd(&L.w[o], wd);
+                    for (int lx = 0; lx < 4; ++lx) {
+                        const float wg = __fmul_rn(tx[lx], tyz);       // WG = TABI(X) * TY
+                        const bool ok = xok[lx] && (xmir[lx] ? mok : dok) && wg != 0.0f;
+                        if (ok) {
+                            const int o = (xmir[lx] ? offm : offd) + xo[lx];
+                            const double wd = (double)wg;
+                            atomicAdd(&L.re[o], to_fixed_prod(vxs, wd));
+                            atomicAdd(&L.im[o], to_fixed_prod(vim[lx], wd));
+                            atomicAdd(&L.w[o], wd);
+                        }
+                    }
+                }
+            }
+        } else {
+            int xo[4];
+            bool xok[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                xo[t] = sx + t - x0;
+                xok[t] = (unsigned)xo[t] < (unsigned)BX;
+            }
+#pragma unroll
+            for (int lz = 0; lz < 4; ++lz) {
+                const int zd = sz + lz - z0;
+                const bool zdok = (unsigned)zd < (unsigned)BZ;
+#pragma unroll
+                for (int ly = 0; ly < 4; ++ly) {
+                    const int yd = sy + ly - y0;
+                    const bool dok = zdok && (unsigned)yd < (unsigned)BY;
+                    const int offd = zd * L.SZ + yd * L.SY;
+                    const float tyz = __fmul_rn(ty[ly], tz[lz]);       // TY = TABI(Y) * TZ
+#pragma unroll
+                    for (int lx = 0; lx < 4; ++lx) {
+                        const float wg = __fmul_rn(tx[lx], tyz);       // WG = TABI(X) * TY
+                        if (xok[lx] && dok && wg != 0.0f) {
+                            const int o = offd + xo[lx];
+                            const double wd = (double)wg;
+                            atomicAdd(&L.re[o], to_fixed_prod(vxs, wd));
+                            atomicAdd(&L.im[o], to_fixed_prod(vys, wd));
+                            atomicAdd(&L.w[o], wd);
+                        }
                     }
                 }
             }
@@ -523,9 +561,12 @@ void insert_brick_kernel(const BrickJob* __restrict__ jobs, int njobs, int* __re
             const int p = lst[s];
             const Geom g = J.geom[p];
             const float2* spec_p = J.spec + (size_t)(g.img - J.img0) * N * H;
-            wave_project<BX, BY, BZ, P>(L, g, spec_p, nx, N, x0, y0, z0, bx, false, bx, tab, fltb, vscale);
-            if (x0 <= LN2)
-                wave_project<BX, BY, BZ, P>(L, g, spec_p, nx, N, x0, y0, z0, mb, true, bx, tab, fltb, vscale);
+            if (x0 <= LN2) {
+                wave_project<BX, BY, BZ, true, P>(L, g, spec_p, nx, N, x0, y0, z0, bx, false, bx, tab, fltb, vscale);
+                wave_project<BX, BY, BZ, true, P>(L, g, spec_p, nx, N, x0, y0, z0, mb, true, bx, tab, fltb, vscale);
+            } else {
+                wave_project<BX, BY, BZ, false, P>(L, g, spec_p, nx, N, x0, y0, z0, bx, false, bx, tab, fltb, vscale);
+            }
         }
         __syncthreads();
         // add the brick to the half grid
