@@ -55,6 +55,10 @@ extern "C" {
 #define SPBP_GEOM_F64     0x20  /* FP64 rotations (from FP64 angles), sample positions, nearest voxels
                                    and table indices: departs from the reference's fp32 geometry
                                    (oneline.f:56-105) to agree with an FP64 computation          */
+#define SPBP_PLAIN_THRESHOLD 0x40 /* NRMW2's W > 0.1 test on the GPU sums alone.  By default the
+                                   voxels whose W lies within 2^-11 of 0.1 are decided by
+                                   re-summing their weights in the reference's order (see
+                                   spbp_threshold_stats)                                       */
 
 /* kernels reported by spbp_kernel_stats */
 #define SPBP_K_INSERT   0   /* ONELINE scatter (units: projections)       */
@@ -63,6 +67,7 @@ extern "C" {
 #define SPBP_K_FINALIZE 3   /* one whole finalisation (units: volumes)    */
 #define SPBP_K_BIN      4   /* brick work-list build (units: projections) */
 #define SPBP_K_GATHER   5   /* central-slice KB gather (units: projections) */
+#define SPBP_K_RESOLVE  6   /* NRMW2 threshold re-summation (units: finalisations) */
 
 /* insertion strategies */
 #define SPBP_STRAT_AUTO   0   /* the library's choice                                     */
@@ -151,6 +156,29 @@ int spbp_export_raw(spbp_ctx* ctx, int half, float* X, float* W, void* stream);
 int spbp_kernel_stats(spbp_ctx* ctx, int kernel, double* total_ms, long* launches, long* units);
 int spbp_reset_stats(spbp_ctx* ctx);
 
+/* NRMW2 (bp3f.f:1355-1366) divides where W > 0.1 and zeroes elsewhere, a
+ * test that two summation orders can decide differently when W lies within
+ * rounding distance of 0.1.  The library keeps the geometry of every section
+ * inserted since the last reset and, when finalising, re-sums the weights of
+ * each voxel whose W lies within 2^-11 of 0.1 in the reference's order --
+ * images in input order, symmetries, rows J ascending, I ascending, direct
+ * before reflected taps (bp32f.f:866-898, oneline.f:56-152), in fp32 (FP64
+ * with SPBP_GEOM_F64) -- and takes the decision from that sum.  After
+ * spbp_import_partial the history no longer describes the accumulators and
+ * the GPU sums decide.  For the last finalised volume: voxels re-summed,
+ * decisions that differ from the GPU sums' own, and re-sums that disagreed
+ * with the GPU weight beyond rounding (left to the GPU sums; 0 unless the
+ * accumulators were changed outside the library).  No counterpart in the
+ * reference (its sums are the order). */
+int spbp_threshold_stats(const spbp_ctx* ctx, long long* listed, long long* changed, long long* inconsistent);
+/* The same re-summation for any n half-grid voxels (index ((kz mod N) N +
+ * (ky mod N)) (NX+1) + kx): W as NRMW2 reads it -- SYMPLANE0 pair and, for
+ * which = -1, both halves added -- summed in the reference's order, returned
+ * as double (the fp32 sum widened, or the FP64 sum with SPBP_GEOM_F64).
+ * which: -1 all halves, 0 or 1 one half.  Synchronous.  No counterpart in
+ * the reference; for diagnostics and tests. */
+int spbp_reference_weights(spbp_ctx* ctx, int which, const long long* voxels, int n, double* w_out, void* stream);
+
 /* Sizes (elements) of one half-grid: complex X count = (NX+1) N N. */
 size_t spbp_grid_elems(const spbp_ctx* ctx);
 
@@ -213,10 +241,13 @@ int spbp_graph_launch(spbp_graph* g, void* stream);
 int spbp_graph_destroy(spbp_graph* g);
 /* Host-side tables, exactly as the kernels use them (no GPU needed):
  * the KB interpolation table TABI(0:4999) and FLTB for padded size n2
- * (bp32f.f:193-214, fillbessi0.f:88-117), evaluated in FP64 and stored fp32;
+ * (bp32f.f:193-214, fillbessi0.f:88-117) -- evaluated in REAL arithmetic as
+ * the reference does (bit-identical, I1 by Cody's CALCI1 as besi1.f), or with
+ * flags & SPBP_GEOM_F64 the FP64 evaluation rounded to fp32 that contexts
+ * with FP64 geometry use;
  * the CANG matrices (cang.f:44-82) of n doc-angle triples (PSI, THETA, PHI),
  * n x 9 fp32, with flags & SPBP_EXACT_PI selecting exact pi. */
-int spbp_kb_table(int n2, float* tab, float* fltb);
+int spbp_kb_table(int n2, int flags, float* tab, float* fltb);
 int spbp_rotations(int n, const double* angles_deg, int flags, float* dm);
 /* Library version string. */
 const char* spbp_version(void);

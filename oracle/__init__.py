@@ -76,7 +76,7 @@ def lib():
         L.or_insert_f64.argtypes = [C.c_int, _f64p, _f64p, _f64p, _f64p, _f64p, C.c_double]
         L.or_insert_f32_acc64.argtypes = [C.c_int, _f64p, _f64p, _f32p, _f32p, _f32p, C.c_float]
         L.or_gather.argtypes = [C.c_int, _f64p, C.c_int, _f32p, _f32p, C.c_float, _f64p]
-        L.or_bp_sym_f32.argtypes = [C.c_int, C.c_int, _f32p, _f64p, vp, vp, vp, vp, C.c_int, _f64p]
+        L.or_bp_sym_f32.argtypes = [C.c_int, C.c_int, _f32p, _f64p, vp, vp, vp, vp, vp, C.c_int, _f64p]
         L.or_bp_sym_f32.restype = C.c_int
         L.or_sym_matmul_f32.argtypes = [_f32p, _f32p, _f32p]
         L.or_triquad.argtypes = [C.c_double, C.c_double, C.c_double, _f32p]
@@ -222,12 +222,15 @@ def insert_f32_acc64(X, W, bi, dm, tab, fltb) -> None:
                               np.ascontiguousarray(dm, np.float32), tab, np.float32(fltb))
 
 
-def lib_table(n2: int):
-    """TABI evaluated in FP64 and rounded to fp32, FLTB = 4999/3 in fp32: the
-    table libspbp builds (its kernels' weights), for checks of functions that
-    have no reference counterpart (the projector)."""
-    t, _ = kb_table(n2, "f64")
-    return t.astype(np.float32), np.float32(np.float32(4999.0) / np.float32(3.0))
+def lib_table(n2: int, geom_f64: bool = False):
+    """The table libspbp's kernels weight with: TABI in REAL arithmetic as the
+    reference evaluates it (== kb_table(n2, "f32")), or with FP64 geometry the
+    FP64 evaluation rounded to fp32; FLTB = 4999/3 in fp32.  For checks of
+    functions that have no reference counterpart (the projector)."""
+    f = np.float32(np.float32(4999.0) / np.float32(3.0))
+    if geom_f64:
+        return kb_table(n2, "f64")[0].astype(np.float32), f
+    return kb_table(n2, "f32")[0], f
 
 
 def gather(X, dm, table=None):
@@ -345,10 +348,11 @@ def rtsq(img, theta, shx, shy, scale=1.0):
     return out
 
 
-def bp_sym(imgs, angles, sym_angles, half=None):
+def bp_sym(imgs, angles, sym_angles, half=None, want_xw=False):
     """FP32-faithful BP 3F / BP 32F with a symmetry document (sym_angles:
     (nsym, 3) PSI, THETA, PHI rows): every projection inserted once per
-    DMS = SM DM, and SYMVOL applied to every output volume."""
+    DMS = SM DM, and SYMVOL applied to every output volume ('X', 'W' as in
+    bp() when want_xw)."""
     imgs = np.ascontiguousarray(imgs, np.float32)
     nimg, nx, _ = imgs.shape
     ang = np.ascontiguousarray(np.asarray(angles, np.float64).reshape(nimg, 3))
@@ -360,10 +364,16 @@ def bp_sym(imgs, angles, sym_angles, half=None):
         out["h1"] = np.zeros((nx, nx, nx), np.float32)
         out["h2"] = np.zeros((nx, nx, nx), np.float32)
     ptr = lambda a: None if a is None else a.ctypes.data
+    n = 2 * nx
+    xw = np.zeros(3 * (nx + 1) * n * n, np.float32) if want_xw else None
     rc = lib().or_bp_sym_f32(nx, nimg, imgs, ang, ptr(hp), ptr(out["full"]), ptr(out.get("h1")),
-                             ptr(out.get("h2")), sym.shape[0], sym)
+                             ptr(out.get("h2")), ptr(xw), sym.shape[0], sym)
     if rc != 0:
         raise RuntimeError(f"oracle bp failed: {rc}")
+    if want_xw:
+        nh = (nx + 1) * n * n
+        out["X"] = xw[: 2 * nh].view(np.complex64).reshape(n, n, nx + 1)
+        out["W"] = xw[2 * nh:].reshape(n, n, nx + 1)
     return out
 
 

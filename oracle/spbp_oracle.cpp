@@ -39,8 +39,7 @@ constexpr int LTAB = 4999;    // KB table length - 1 (bp32f.f:81, oneline.f:44)
 
 // ---------------------------------------------------------------- Bessel
 // Modified Bessel I1 by its power series; converges to double precision for
-// the arguments used here (0 <= x <= 6.5).  Replaces Cody's CALCI1/BESI1
-// (besi1.f:3-251), which evaluates the same function in single precision.
+// the arguments used here (0 <= x <= 6.5): the FP64 form of BESI1.
 double bessel_i1(double x) {
     double h = 0.5 * x, term = h, sum = h, h2 = h * h;
     for (int k = 1; k < 200; ++k) {
@@ -49,6 +48,36 @@ double bessel_i1(double x) {
         if (term < 1e-18 * sum) break;
     }
     return sum;
+}
+// I1 in single precision as BESI1 -> CALCI1 computes it for |x| < 15
+// (besi1.f:3-230): W. J. Cody's published rational approximation (SPECFUN,
+// ACM TOMS Algorithm 715), numerator in x^2 (15 terms) and denominator in
+// x^2 - 225 (monic, 5 further terms), both by Horner, value (P / Q) |x|.
+// Every step rounds to float as the REAL code does.
+float cody_i1_f32(float x) {
+    static const float num[15] = {
+        -1.9705291802535139930E-19f, -6.5245515583151902910E-16f, -1.1928788903603238754E-12f,
+        -1.4831904935994647675E-09f, -1.3466829827635152875E-06f, -9.1746443287817501309E-04f,
+        -4.7207090827310162436E-01f, -1.8225946631657315931E+02f, -5.1894091982308017540E+04f,
+        -1.0588550724769347106E+07f, -1.4828267606612366099E+09f, -1.3357437682275493024E+11f,
+        -6.9876779648010090070E+12f, -1.7732037840791591320E+14f, -1.4577180278143463643E+15f};
+    static const float den[5] = {-4.0076864679904189921E+03f, 7.4810580356655069138E+06f,
+                                 -8.0059518998619764991E+09f, 4.8544714258273622913E+12f,
+                                 -1.3218168307321442305E+15f};
+    const float a = fabsf(x);
+    float r;
+    if (a < 2.98E-8f) {
+        r = 0.5f * a;
+    } else {
+        const float t = a * a;
+        float pn = num[0];
+        for (float c : std::vector<float>(num + 1, num + 15)) pn = pn * t + c;
+        const float u = t - 225.0f;
+        float qd = u + den[0];
+        for (int k = 1; k < 5; ++k) qd = qd * u + den[k];
+        r = (pn / qd) * a;
+    }
+    return x < 0.0f ? -r : r;
 }
 // I_{3/2}(x) = sqrt(2/(pi x)) (cosh x - sinh x / x): closed form of the
 // half-integer order used by WINDKB2A via RIBSL (besi1.f:793-800).  Small x by
@@ -78,7 +107,7 @@ template <> struct Num<float> {
     static float fabs_(float v) { return fabsf(v); }
     static float sqrt_(float v) { return sqrtf(v); }
     static float pow_(float a, float b) { return powf(a, b); }
-    static float i1(float x) { return (float)bessel_i1((double)x); }
+    static float i1(float x) { return cody_i1_f32(x); }
     static float i15(float x) { return (float)bessel_i15((double)x); }
     static float j15(float x) { return (float)bessel_j15((double)x); }
     static float twopi() { return 2.0f * (float)M_PI; }          // REAL parameter (bp32f.f:950-951)
@@ -986,8 +1015,8 @@ int or_bp_f64(int nx, int nimg, const float* imgs, const double* angles, const i
     return bp_run<double>(nx, nimg, imgs, angles, half, out_full, out_h1, out_h2, xw_out, exact_pi);
 }
 int or_bp_sym_f32(int nx, int nimg, const float* imgs, const double* angles, const int8_t* half,
-                  float* out_full, float* out_h1, float* out_h2, int nsym, const double* sym) {
-    return bp_run<float>(nx, nimg, imgs, angles, half, out_full, out_h1, out_h2, nullptr, 0, nsym, sym);
+                  float* out_full, float* out_h1, float* out_h2, float* xw_out, int nsym, const double* sym) {
+    return bp_run<float>(nx, nimg, imgs, angles, half, out_full, out_h1, out_h2, xw_out, 0, nsym, sym);
 }
 void or_sym_matmul_f32(const float* sm, const float* dm, float* out) { sym_matmul<float>(sm, dm, out); }
 float or_triquad(double r, double s, double t, const float* f) { return triquad_ref(r, s, t, f); }

@@ -31,6 +31,7 @@ TWOSUM_W = 0x04
 CHECK_FINITE = 0x08
 TIMING = 0x10
 GEOM_F64 = 0x20
+PLAIN_THRESHOLD = 0x40
 
 K_INSERT = 0
 K_PAD = 1
@@ -38,6 +39,7 @@ K_FFT2 = 2
 K_FINALIZE = 3
 K_BIN = 4
 K_GATHER = 5
+K_RESOLVE = 6
 
 STRAT_AUTO = 0
 STRAT_DIRECT = 1
@@ -95,7 +97,7 @@ def lib():
         L.spbp_reset_stats.argtypes = [vp]
         L.spbp_gather.argtypes = [vp, vp, C.c_int, vp, vp, vp, vp]
         L.spbp_scatter.argtypes = [vp, vp, C.c_int, vp, vp, vp, vp, vp]
-        L.spbp_kb_table.argtypes = [C.c_int, vp, C.POINTER(C.c_float)]
+        L.spbp_kb_table.argtypes = [C.c_int, C.c_int, vp, C.POINTER(C.c_float)]
         L.spbp_tune.argtypes = [vp, vp, C.c_int, C.c_int, vp, vp, vp]
         L.spbp_set_symmetries.argtypes = [vp, C.c_int, vp]
         L.spbp_insert_rb.argtypes = [vp, vp, C.c_int, C.c_int, vp, vp, vp, vp]
@@ -108,13 +110,15 @@ def lib():
         L.spbp_graph_launch.argtypes = [vp, vp]
         L.spbp_graph_destroy.argtypes = [vp]
         L.spbp_rotations.argtypes = [C.c_int, vp, C.c_int, vp]
+        L.spbp_threshold_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+        L.spbp_reference_weights.argtypes = [vp, C.c_int, vp, C.c_int, vp, vp]
         for name in ("spbp_create", "spbp_destroy", "spbp_reset", "spbp_insert", "spbp_insert_dm",
                      "spbp_insert_host", "spbp_finalize", "spbp_finalize_host", "spbp_export_partial",
                      "spbp_import_partial", "spbp_export_raw", "spbp_kernel_stats", "spbp_reset_stats",
                      "spbp_gather", "spbp_scatter", "spbp_kb_table", "spbp_rotations", "spbp_tune",
                      "spbp_get_plan", "spbp_set_plan", "spbp_insert_batch", "spbp_graph_create",
                      "spbp_graph_launch", "spbp_graph_destroy", "spbp_set_symmetries", "spbp_insert_rb",
-                     "spbp_rtsq", "spbp_insert_rb_host"):
+                     "spbp_rtsq", "spbp_insert_rb_host", "spbp_threshold_stats", "spbp_reference_weights"):
             getattr(L, name).restype = C.c_int
         _LIB = L
     return _LIB
@@ -134,12 +138,13 @@ def check(rc: int, ctx=None) -> None:
         raise SpbpError(rc, msg.decode() if msg else "")
 
 
-def kb_table(n2: int):
-    """(TABI(0:4999) float32, FLTB) for padded size n2, from the library."""
+def kb_table(n2: int, geom_f64: bool = False):
+    """(TABI(0:4999) float32, FLTB) for padded size n2, from the library: the
+    reference's REAL evaluation, or the FP64 one that FP64-geometry contexts use."""
     import numpy as np
     tab = np.zeros(5000, np.float32)
     f = C.c_float()
-    check(lib().spbp_kb_table(int(n2), tab.ctypes.data_as(C.c_void_p), C.byref(f)))
+    check(lib().spbp_kb_table(int(n2), GEOM_F64 if geom_f64 else 0, tab.ctypes.data_as(C.c_void_p), C.byref(f)))
     return tab, np.float32(f.value)
 
 

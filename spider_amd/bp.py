@@ -44,14 +44,15 @@ class Backprojector:
     def __init__(self, nx: int, two_halves: bool = False, exact_pi: bool = False,
                  twosum: bool = False, check_finite: bool = False, device=None, batch: int = 0,
                  timing: bool = False, strategy: str = "auto", brick_proj: int = 0, symmetries=None,
-                 geom_f64: bool = False):
+                 geom_f64: bool = False, plain_threshold: bool = False):
         torch = _torch()
         if not torch.cuda.is_available():
             raise RuntimeError("spider_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
         self.device = torch.device("cuda", torch.cuda.current_device() if device is None else int(device))
         flags = (_lib.TWO_HALVES if two_halves else 0) | (_lib.EXACT_PI if exact_pi else 0) | \
                 (_lib.TWOSUM_W if twosum else 0) | (_lib.CHECK_FINITE if check_finite else 0) | \
-                (_lib.TIMING if timing else 0) | (_lib.GEOM_F64 if geom_f64 else 0)
+                (_lib.TIMING if timing else 0) | (_lib.GEOM_F64 if geom_f64 else 0) | \
+                (_lib.PLAIN_THRESHOLD if plain_threshold else 0)
         strat = {"auto": _lib.STRAT_AUTO, "direct": _lib.STRAT_DIRECT, "brick": _lib.STRAT_BRICK}[strategy]
         opts = _lib.Opts(int(nx), flags, self.device.index, int(batch), strat, int(brick_proj))
         ctx = C.c_void_p()
@@ -91,6 +92,28 @@ class Backprojector:
 
     def reset(self) -> None:
         self._check(_lib.lib().spbp_reset(self._ctx, _stream(self.device)))
+
+    def threshold_stats(self) -> dict:
+        """Last finalisation's NRMW2 re-summation: voxels re-summed in the
+        reference's order, decisions that differ from the GPU sums' own, and
+        re-sums that disagreed with the GPU weight (left to the GPU)."""
+        v = [C.c_longlong() for _ in range(3)]
+        self._check(_lib.lib().spbp_threshold_stats(self._ctx, *[C.byref(x) for x in v]))
+        return {"listed": v[0].value, "changed": v[1].value, "inconsistent": v[2].value}
+
+    def reference_weights(self, voxels, which: int = -1) -> np.ndarray:
+        """W as NRMW2 reads it at half-grid voxels (flat indices into
+        grid_shape, or (kz, ky, kx) index triples), summed in the reference's
+        order: fp32 values (FP64 with geom_f64) as float64."""
+        v = np.asarray(voxels, np.int64)
+        if v.ndim == 2:
+            n = 2 * self.nx
+            v = (v[:, 0] * n + v[:, 1]) * (self.nx + 1) + v[:, 2]
+        v = np.ascontiguousarray(v.ravel())
+        out = np.zeros(v.size, np.float64)
+        self._check(_lib.lib().spbp_reference_weights(self._ctx, int(which), v.ctypes.data_as(C.c_void_p), int(v.size),
+                                                      out.ctypes.data_as(C.c_void_p), _stream(self.device)))
+        return out
 
     def set_symmetries(self, angles) -> None:
         """Symmetry operations as doc rows (PSI, THETA, PHI), (nsym, 3); one

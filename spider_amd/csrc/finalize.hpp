@@ -10,6 +10,7 @@ struct Acc {
     const float* W[2];
     const float* Wc[2];   // compensation terms of the weights, or null
     int n;                // halves summed: 1 or 2
+    const unsigned char* mask = nullptr;   // NRMW2 decisions fixed by the reference's sums (1 zero, 2 divide), or null
 };
 
 // (j, k) of the kx = 0 plane is paired by SYMPLANE0 with (N-j, N-k) unless it
@@ -50,13 +51,16 @@ __device__ inline float2 norm_val(const Acc& a, int nx, int N, int i, int j, int
     double wd;
     load_sym(a, nx, N, i, j, k, x, w, wd);
     const float s = ((i + j + k) & 1) ? -1.0f : 1.0f;
+    // NRMW2 (bp3f.f:1359): divide where W > 0.1; near 0.1 the decision may
+    // come from the reference's own sums
+    const unsigned char m = a.mask ? a.mask[((size_t)k * N + j) * (nx + 1) + i] : 0;
     if (a.Wc[0]) {
         // compensated weights: threshold and division in FP64
-        if (wd > (double)0.1f)
+        if (m ? m == 2 : wd > (double)0.1f)
             return make_float2((float)((double)(x.x * s) / wd), (float)((double)(x.y * s) / wd));
         return make_float2(0.f, 0.f);
     }
-    if (w > 0.1f)                                      // NRMW2 (bp3f.f:1359)
+    if (m ? m == 2 : w > 0.1f)
         return make_float2(__fdiv_rn(x.x * s, w), __fdiv_rn(x.y * s, w));
     return make_float2(0.f, 0.f);
 }

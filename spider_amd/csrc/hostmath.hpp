@@ -20,20 +20,72 @@ double bessel_i1(double x) {
 double bessel_i15(double x) { return std::sqrt(2.0 / (M_PI * x)) * (std::cosh(x) - std::sinh(x) / x); }
 double bessel_j15(double x) { return std::sqrt(2.0 / (M_PI * x)) * (std::sin(x) / x - std::cos(x)); }
 
-// KB interpolation table TABI(0:4999) (bp32f.f:193-214), evaluated in FP64.
-void kb_table(int n2, std::vector<float>& tab, float& fltb_f) {
+// KB interpolation table TABI(0:4999) (bp32f.f:193-214), evaluated in FP64
+// (used with FP64 geometry, SPBP_GEOM_F64).
+void kb_table_f64(int n2, std::vector<double>& tab) {
     const double alpha = 6.5, V = 4.0 / 2.0 / (double)n2, aaaa = 0.9 * V;
     const double b0 = std::sqrt(alpha) * bessel_i1(alpha);
     const double fltb = (double)LTAB / (double)(LN2 + 1);
-    tab.assign(LTAB + 1, 0.0f);
+    tab.assign(LTAB + 1, 0.0);
     for (int i = 0; i <= LTAB; ++i) {
         const double s = (double)i / fltb / (double)n2;
         if (s <= aaaa) {
             const double q = s / aaaa, x = std::sqrt(1.0 - q * q);
-            tab[i] = (float)(std::sqrt(alpha * x) * bessel_i1(alpha * x) / b0);
+            tab[i] = std::sqrt(alpha * x) * bessel_i1(alpha * x) / b0;
         }
     }
-    fltb_f = (float)LTAB / (float)(LN2 + 1);   // REAL(LTAB)/REAL(LN2+1)
+}
+
+// I1(x), 0 <= x < 15, in single precision as the reference evaluates it:
+// W. J. Cody's rational approximation (SPECFUN routine CALCI1, ACM TOMS 19
+// (1993) 22-32, Algorithm 715), reached through BESI1 (besi1.f:3-230).  The
+// published coefficients are given as decimal REAL constants; the operation
+// order is Cody's: Horner in x^2 for the numerator, in x^2 - 225 for the
+// denominator, then (P / Q) x.
+float bessel_i1_real(float x) {
+    static const float P[15] = {-1.9705291802535139930E-19f, -6.5245515583151902910E-16f, -1.1928788903603238754E-12f,
+                                -1.4831904935994647675E-09f, -1.3466829827635152875E-06f, -9.1746443287817501309E-04f,
+                                -4.7207090827310162436E-01f, -1.8225946631657315931E+02f, -5.1894091982308017540E+04f,
+                                -1.0588550724769347106E+07f, -1.4828267606612366099E+09f, -1.3357437682275493024E+11f,
+                                -6.9876779648010090070E+12f, -1.7732037840791591320E+14f, -1.4577180278143463643E+15f};
+    static const float Q[5] = {-4.0076864679904189921E+03f, 7.4810580356655069138E+06f, -8.0059518998619764991E+09f,
+                               4.8544714258273622913E+12f, -1.3218168307321442305E+15f};
+    const float ax = std::fabs(x);
+    if (ax < 2.98E-8f) return 0.5f * x;
+    float xx = ax * ax;
+    float sp = P[0];
+    for (int j = 1; j < 15; ++j) sp = sp * xx + P[j];
+    xx = xx - 225.0f;
+    float sq = xx + Q[0];
+    for (int j = 1; j < 5; ++j) sq = sq * xx + Q[j];
+    const float r = (sp / sq) * ax;
+    return x < 0.0f ? -r : r;
+}
+
+// TABI(0:4999) and FLTB in REAL arithmetic, as bp32f.f:193-214 computes them
+// (the default table: every per-tap weight is then the reference's).
+void kb_table(int n2, std::vector<float>& tab, float& fltb_f) {
+    const float alpha = 6.5f, V = (float)(5 - 1) / 2.0f / (float)n2, aaaa = 0.9f * V;
+    const float b0 = std::sqrt(alpha) * bessel_i1_real(alpha);
+    const float fltb = (float)LTAB / (float)(LN2 + 1);   // REAL(LTAB)/REAL(LN2+1)
+    tab.assign(LTAB + 1, 0.0f);
+    for (int i = 0; i <= LTAB; ++i) {
+        const float s = (float)i / fltb / (float)n2;
+        if (s <= aaaa) {
+            const float q = s / aaaa;
+            const float x = std::sqrt(1.0f - q * q);
+            tab[i] = std::sqrt(alpha * x) * bessel_i1_real(alpha * x) / b0;
+        }
+    }
+    fltb_f = fltb;
+}
+
+// The FP64 table rounded to fp32 (with FP64 geometry).
+void kb_table_f64_f32(int n2, std::vector<float>& tab, float& fltb_f) {
+    std::vector<double> t;
+    kb_table_f64(n2, t);
+    tab.assign(t.begin(), t.end());
+    fltb_f = (float)LTAB / (float)(LN2 + 1);
 }
 
 // |WKB| of WINDKB2A per integer squared radius LR = 0..(L/2)^2, FP64.

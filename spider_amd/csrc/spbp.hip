@@ -41,6 +41,7 @@
 #include "hostmath.hpp"
 #include "insert.hpp"
 #include "project.hpp"
+#include "resolve.hpp"
 #include "rotate.hpp"
 
 using namespace spbp;
@@ -122,9 +123,30 @@ struct spbp_ctx {
     int* blist = nullptr;   // projection lists
     int* bikey = nullptr;   // work item -> key
     size_t blist_cap = 0;
+    // NRMW2 weight test by the reference's sums: every section inserted since
+    // the last reset, in insertion order (valid while the accumulators hold
+    // only insertions), its device copy, and the re-summation buffers
+    std::vector<Geom> hist;
+    bool hist_ok = true;
+    Geom* hist_d = nullptr;
+    size_t hist_dcap = 0, hist_dn = 0;
+    double* tab64 = nullptr;          // FP64 table (FP64 geometry)
+    unsigned char* tmask = nullptr;   // per voxel: 0 = GPU sums decide, 1 = zero, 2 = divide
+    bool tmask_dirty = true;
+    long long* rlist = nullptr; size_t rlist_cap = 0;
+    int* rcnt = nullptr; size_t rcnt_cap = 0;
+    long long* roff = nullptr; size_t roff_cap = 0;
+    long long* rbsum = nullptr; size_t rbsum_cap = 0;
+    int2* rhits = nullptr; size_t rhits_cap = 0;
+    int* rtcnt = nullptr; size_t rtcnt_cap = 0;
+    long long* rtoff = nullptr; size_t rtoff_cap = 0;
+    char* rterms = nullptr; size_t rterms_cap = 0;
+    char* rsums = nullptr; size_t rsums_cap = 0;
+    unsigned long long* rctr = nullptr;   // [listed, changed, inconsistent]
+    long long res_stats[3] = {0, 0, 0};
     // SPBP_TIMING: (start, stop, units) per recorded launch, per kernel kind
     struct Rec { hipEvent_t a, b; long units; };
-    std::vector<Rec> recs[6];
+    std::vector<Rec> recs[7];
     std::vector<hipEvent_t> evpool;
     std::string err;
 };
@@ -539,6 +561,11 @@ int stage_rtsq(spbp_ctx* c, int nimg, const float* inplane, hipStream_t st) {
     return SPBP_OK;
 }
 
+// Append the n sections just staged (c->geom_h) to the insertion history.
+void record_history(spbp_ctx* c, size_t n) {
+    if (c->hist_ok) c->hist.insert(c->hist.end(), c->geom_h, c->geom_h + n);
+}
+
 int insert_impl(spbp_ctx* c, const void* imgs, int dtype, int nimg, const float* dm_in,
                 const double* ang, const int8_t* half, hipStream_t st, const float* inplane = nullptr) {
     if (!c) return fail(nullptr, SPBP_ERR_ARG, "NULL context");
@@ -574,8 +601,9 @@ int insert_impl(spbp_ctx* c, const void* imgs, int dtype, int nimg, const float*
         } else if ((rc = pad_fft(c, src, dtype, b, st))) {
             return rc;
         }
-        if ((rc = run_insert(c, p0, c->geom_d + (size_t)p0 * ns, b * ns, t, st, b))) return rc;
+        if ((rc = run_insert(c, p0, c->geom_d + (size_t)p0 * ns, b * ns, t, st, b))) { c->hist_ok = false; return rc; }
     }
+    record_history(c, (size_t)nimg * ns);
     return SPBP_OK;
 }
 
@@ -635,11 +663,190 @@ Acc make_acc(const spbp_ctx* c, int which) {
     return a;
 }
 
+// ------------------------------------------------------------------ NRMW2 weight test
+// Half-width of the band around 0.1 whose voxels are re-summed: 2^-11 of 0.1.
+// A sequential fp32 sum of n positive terms is within (n - 1) 2^-24 of the
+// exact sum, so the decisions are exact for voxels of up to ~4000 weight
+// terms, and in practice (errors grow like sqrt(n)) for far more.
+constexpr double THRESH_BAND = 0.1 * (1.0 / 2048.0);
+
+template <typename T>
+int grow(spbp_ctx* c, T*& p, size_t& cap, size_t n, const char* what) {
+    if (n <= cap && p) return SPBP_OK;
+    if (p) { (void)hipFree(p); p = nullptr; cap = 0; }
+    const size_t m = std::max<size_t>(std::max<size_t>(n, 2 * cap), 1024);
+    if (hipMalloc((void**)&p, m * sizeof(T)) != hipSuccess) { p = nullptr; return fail(c, SPBP_ERR_ALLOC, "%s: %zu bytes", what, m * sizeof(T)); }
+    cap = m;
+    return SPBP_OK;
+}
+
+// out[0..n] = exclusive scan of n counts (out[n] = total), on st.
+template <typename C>
+int scan_counts(spbp_ctx* c, const C* in, long long n, long long* out, hipStream_t st) {
+    const int nb = (int)((n + 1023) / 1024);
+    int rc = grow(c, c->rbsum, c->rbsum_cap, (size_t)nb + 1, "scan");
+    if (rc) return rc;
+    if (nb > 0) scan_blocks_kernel<C><<<nb, 1024, 0, st>>>(in, n, out, c->rbsum);
+    scan_sums_kernel<<<1, 1024, 0, st>>>(c->rbsum, nb);
+    if (nb > 0) scan_add_kernel<<<nb, 1024, 0, st>>>(out, n, c->rbsum, nb);
+    else HIPCHK(c, hipMemcpyAsync(out, c->rbsum, sizeof(long long), hipMemcpyDeviceToDevice, st));
+    HIPCHK(c, hipGetLastError());
+    return SPBP_OK;
+}
+
+template <typename T>
+int read_back(spbp_ctx* c, const T* d, T* h, hipStream_t st) {
+    HIPCHK(c, hipMemcpyAsync(h, d, sizeof(T), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    return SPBP_OK;
+}
+
+// Steps 3-5 for positions / terms of type P (float: the reference's fp32
+// geometry and sums; double: the FP64 computation).
+template <typename P>
+int resolve_terms(spbp_ctx* c, const Acc& a, int hmask, long long nv, int nchunks, long long nhits, double* wout,
+                  hipStream_t st) {
+    const long long nitems = 2 * nv;
+    const P* tab = sizeof(P) == 4 ? (const P*)c->tab : (const P*)c->tab64;
+    const P fltb = sizeof(P) == 4 ? (P)c->fltb : (P)(4999.0 / 3.0);
+    int rc;
+    if ((rc = grow(c, c->rtcnt, c->rtcnt_cap, (size_t)nhits, "term counts")) ||
+        (rc = grow(c, c->rtoff, c->rtoff_cap, (size_t)nhits + 1, "term offsets")))
+        return rc;
+    const unsigned hb = (unsigned)((nhits + RES_TPB - 1) / RES_TPB);
+    if (nhits > 0)
+        term_kernel<P, false><<<hb, RES_TPB, 0, st>>>(c->rhits, nhits, c->rlist, c->hist_d, tab, fltb, c->nx, c->N,
+                                                      c->rtcnt, nullptr, nullptr);
+    if ((rc = scan_counts(c, c->rtcnt, nhits, c->rtoff, st))) return rc;
+    long long nterms = 0;
+    if ((rc = read_back(c, c->rtoff + nhits, &nterms, st))) return rc;
+    if ((rc = grow(c, c->rterms, c->rterms_cap, (size_t)std::max<long long>(nterms, 1) * sizeof(P), "weight terms")) ||
+        (rc = grow(c, c->rsums, c->rsums_cap, (size_t)nitems * 2 * sizeof(P), "weight sums")))
+        return rc;
+    if (nhits > 0)
+        term_kernel<P, true><<<hb, RES_TPB, 0, st>>>(c->rhits, nhits, c->rlist, c->hist_d, tab, fltb, c->nx, c->N,
+                                                     nullptr, c->rtoff, (P*)c->rterms);
+    const unsigned ib = (unsigned)((nitems + RES_TPB - 1) / RES_TPB);
+    sum_kernel<P><<<ib, RES_TPB, 0, st>>>(nitems, nchunks, c->roff, c->rhits, c->rtoff, (const P*)c->rterms, c->hist_d,
+                                          (P*)c->rsums);
+    const unsigned vb = (unsigned)((nv + RES_TPB - 1) / RES_TPB);
+    if (wout) {
+        weights_out_kernel<P><<<vb, RES_TPB, 0, st>>>(c->rlist, nv, hmask, c->nx, c->N, (const P*)c->rsums, wout);
+    } else {
+        c->tmask_dirty = true;
+        decide_kernel<P><<<vb, RES_TPB, 0, st>>>(c->rlist, nv, a, hmask, c->nx, c->N, (const P*)c->rsums,
+                                                 0.1 * (1.0 / 512.0), c->tmask, c->rctr);
+    }
+    HIPCHK(c, hipGetLastError());
+    return SPBP_OK;
+}
+
+// History on the device (only what is new since the last upload).
+int upload_history(spbp_ctx* c, hipStream_t st) {
+    const size_t nhist = c->hist.size();
+    int rc;
+    if (c->hist_dn > nhist) c->hist_dn = 0;
+    if (nhist > c->hist_dcap) {
+        if ((rc = grow(c, c->hist_d, c->hist_dcap, nhist, "insertion history"))) return rc;
+        c->hist_dn = 0;
+    }
+    if (nhist > c->hist_dn)
+        HIPCHK(c, hipMemcpyAsync(c->hist_d + c->hist_dn, c->hist.data() + c->hist_dn, (nhist - c->hist_dn) * sizeof(Geom),
+                                 hipMemcpyHostToDevice, st));
+    c->hist_dn = nhist;
+    return SPBP_OK;
+}
+
+// Steps 2-5 for the nv voxels in c->rlist.
+int resolve_list(spbp_ctx* c, int which, const Acc& a, long long nv, double* wout, hipStream_t st) {
+    const size_t nhist = c->hist.size();
+    const int hmask = which < 0 ? (1 << c->nvol) - 1 : 1 << which;
+    const long long nitems = 2 * nv;
+    const int nchunks = (int)((nhist + RES_TPB - 1) / RES_TPB);
+    const long long nblk = nitems * nchunks;
+    if (nblk > (long long)INT32_MAX) return fail(c, SPBP_ERR_SIZE, "threshold re-summation too large (%lld blocks)", nblk);
+    int rc;
+    if ((rc = grow(c, c->rcnt, c->rcnt_cap, (size_t)nblk, "hit counts")) ||
+        (rc = grow(c, c->roff, c->roff_cap, (size_t)nblk + 1, "hit offsets")))
+        return rc;
+    hit_kernel<0><<<(unsigned)nblk, RES_TPB, 0, st>>>(c->rlist, nitems, nchunks, c->hist_d, (int)nhist, hmask, c->nx, c->N,
+                                                      c->rcnt, nullptr, nullptr);
+    if ((rc = scan_counts(c, c->rcnt, nblk, c->roff, st))) return rc;
+    long long nhits = 0;
+    if ((rc = read_back(c, c->roff + nblk, &nhits, st))) return rc;
+    if ((rc = grow(c, c->rhits, c->rhits_cap, (size_t)std::max<long long>(nhits, 1), "hits"))) return rc;
+    hit_kernel<1><<<(unsigned)nblk, RES_TPB, 0, st>>>(c->rlist, nitems, nchunks, c->hist_d, (int)nhist, hmask, c->nx, c->N,
+                                                      nullptr, c->roff, c->rhits);
+    HIPCHK(c, hipGetLastError());
+    return (c->flags & SPBP_GEOM_F64) ? resolve_terms<double>(c, a, hmask, nv, nchunks, nhits, wout, st)
+                                      : resolve_terms<float>(c, a, hmask, nv, nchunks, nhits, wout, st);
+}
+
+// Decide NRMW2's test for the voxels near 0.1 by the reference's sums (the
+// finalisation of `which`: -1 all halves, else one half); on success with
+// *active, a.mask holds the decisions until clear_decisions.
+int resolve_threshold(spbp_ctx* c, int which, Acc& a, bool* active, long long* nv_out, hipStream_t st) {
+    *active = false;
+    *nv_out = 0;
+    c->res_stats[0] = c->res_stats[1] = c->res_stats[2] = 0;
+    if (!c->hist_ok || (c->flags & SPBP_PLAIN_THRESHOLD) || c->hist.empty()) return SPBP_OK;
+    const size_t nhist = c->hist.size();
+    if (nhist > (size_t)INT32_MAX) return SPBP_OK;
+    int rc;
+    Timed tm(c, SPBP_K_RESOLVE, 1, st);
+    if ((rc = upload_history(c, st))) return rc;
+    if (!c->tmask) {
+        HIPCHK(c, hipMalloc(&c->tmask, (c->nh + 3) / 4 * 4));
+        c->tmask_dirty = true;
+    }
+    if (c->tmask_dirty) {
+        fill_kernel<<<nblocks((c->nh + 3) / 4), TPB, 0, st>>>((float*)c->tmask, (c->nh + 3) / 4, 0.0f);   // zero bits
+        c->tmask_dirty = false;
+    }
+    if (!c->rctr) HIPCHK(c, hipMalloc(&c->rctr, 4 * sizeof(unsigned long long)));
+    // 1. voxels in the band
+    if ((rc = grow(c, c->rlist, c->rlist_cap, 4096, "threshold list"))) return rc;
+    unsigned long long nv = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        fill_kernel<<<1, TPB, 0, st>>>((float*)c->rctr, 8, 0.0f);   // zero bits
+        thresh_list_kernel<<<nblocks(c->nh), TPB, 0, st>>>(a, c->nx, c->N, THRESH_BAND, c->rlist, (long long)c->rlist_cap,
+                                                           c->rctr);
+        HIPCHK(c, hipGetLastError());
+        if ((rc = read_back(c, c->rctr, &nv, st))) return rc;
+        if (nv <= c->rlist_cap) break;
+        if ((rc = grow(c, c->rlist, c->rlist_cap, (size_t)nv, "threshold list"))) return rc;
+    }
+    c->res_stats[0] = (long long)nv;
+    *nv_out = (long long)nv;
+    if (nv == 0) return SPBP_OK;
+    // 2-5. hits, terms, sums, decisions
+    if ((rc = resolve_list(c, which, a, (long long)nv, nullptr, st))) return rc;
+    unsigned long long ctr[4] = {0, 0, 0, 0};
+    HIPCHK(c, hipMemcpyAsync(ctr, c->rctr, sizeof ctr, hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    c->res_stats[1] = (long long)ctr[1];
+    c->res_stats[2] = (long long)ctr[2];
+    a.mask = c->tmask;
+    *active = true;
+    return SPBP_OK;
+}
+
+void clear_decisions(spbp_ctx* c, long long nv, hipStream_t st) {
+    mask_clear_kernel<<<(unsigned)((nv + RES_TPB - 1) / RES_TPB), RES_TPB, 0, st>>>(c->rlist, nv, c->tmask);
+    if (hipGetLastError() == hipSuccess) c->tmask_dirty = false;
+}
+
 int finalize_one(spbp_ctx* c, int which, float* out, hipStream_t st) {
     const int nx = c->nx, N = c->N;
+    Acc a = make_acc(c, which);
+    bool active = false;
+    long long nlisted = 0;
+    int rc = resolve_threshold(c, which, a, &active, &nlisted, st);
+    if (rc) return rc;
     Timed t(c, SPBP_K_FINALIZE, 1, st);
-    normalize_kernel<<<nblocks(c->nh), TPB, 0, st>>>(make_acc(c, which), nx, N, c->work);
+    normalize_kernel<<<nblocks(c->nh), TPB, 0, st>>>(a, nx, N, c->work);
     HIPCHK(c, hipGetLastError());
+    if (active) clear_decisions(c, nlisted, st);
     FFTCHK(c, hipfftSetStream(c->inv, st));
     FFTCHK(c, hipfftExecC2R(c->inv, (hipfftComplex*)c->work, (hipfftReal*)c->real));
     const size_t nv = (size_t)nx * nx * nx;
@@ -713,7 +920,16 @@ int spbp_create(const spbp_opts* o, spbp_ctx** out) {
             return bail(fail(nullptr, SPBP_ERR_ALLOC, "BP 32F; XE, WE, WO, & XO: %zu bytes", c->nh * 12 * (size_t)c->nvol));
     }
     std::vector<float> tab;
-    kb_table(c->N, tab, c->fltb);
+    if (c->flags & SPBP_GEOM_F64) {
+        kb_table_f64_f32(c->N, tab, c->fltb);
+        std::vector<double> t64;
+        kb_table_f64(c->N, t64);
+        if (hipMalloc(&c->tab64, t64.size() * sizeof(double)) != hipSuccess ||
+            hipMemcpy(c->tab64, t64.data(), t64.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(nullptr, SPBP_ERR_ALLOC, "FP64 table"));
+    } else {
+        kb_table(c->N, tab, c->fltb);
+    }
     if (hipMalloc(&c->tab, tab.size() * sizeof(float)) != hipSuccess ||
         hipMemcpy(c->tab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
         hipMalloc(&c->flag, sizeof(int)) != hipSuccess ||
@@ -743,7 +959,8 @@ int spbp_destroy(spbp_ctx* c) {
     }
     void* dev_ptrs[] = {c->tab, c->padbuf, c->spec, c->vmax, c->geom_d, c->work, c->real, c->wkb, c->psum, c->pcnt, c->mean, c->flag,
                         c->bcnt, c->boff, c->bioff, c->bcur, c->bnext, c->blist, c->bikey, c->jobs_d,
-                        c->sm_d, c->symtmp, c->rotbuf, c->rprm_d};
+                        c->sm_d, c->symtmp, c->rotbuf, c->rprm_d, c->hist_d, c->tab64, c->tmask, c->rlist, c->rcnt,
+                        c->roff, c->rbsum, c->rhits, c->rtcnt, c->rtoff, c->rterms, c->rsums, c->rctr};
     for (void* p : dev_ptrs) if (p) (void)hipFree(p);
     if (c->geom_h) (void)hipHostFree(c->geom_h);
     if (c->jobs_h) (void)hipHostFree(c->jobs_h);
@@ -765,6 +982,8 @@ int spbp_destroy(spbp_ctx* c) {
 int spbp_reset(spbp_ctx* c, void* stream) {
     if (!c) return fail(nullptr, SPBP_ERR_ARG, "NULL context");
     DeviceGuard dg(c->device);
+    c->hist.clear();
+    c->hist_ok = true;
     hipStream_t st = (hipStream_t)stream;
     for (int h = 0; h < c->nvol; ++h) {
         fill_kernel<<<nblocks(2 * c->nh), TPB, 0, st>>>((float*)c->X[h], 2 * c->nh, 0.0f);
@@ -1041,7 +1260,9 @@ int stage_batch(const spbp_job* jobs, int njobs, hipStream_t st) {
 // ------------------------------------------------------------------ batch graph
 struct spbp_graph {
     int device = 0;
+    int reset_first = 0;
     std::vector<spbp_job> jobs;
+    std::vector<std::vector<Geom>> hgeom;   // per job, the sections one launch inserts
     std::vector<Geom*> geom;          // per job, device (owned)
     std::vector<BrickJob*> tables;    // per round, device (owned)
     hipStream_t stream = nullptr;
@@ -1059,7 +1280,12 @@ int spbp_insert_batch(const spbp_job* jobs, int njobs, void* stream) {
     if ((rc = stage_batch(jobs, njobs, st))) return rc;
     std::vector<Geom*> geom(njobs);
     for (int j = 0; j < njobs; ++j) geom[j] = jobs[j].ctx->geom_d;
-    return enqueue_batch(jobs, njobs, geom.data(), nullptr, st);
+    rc = enqueue_batch(jobs, njobs, geom.data(), nullptr, st);
+    for (int j = 0; j < njobs; ++j) {
+        if (rc) jobs[j].ctx->hist_ok = false;
+        else record_history(jobs[j].ctx, (size_t)jobs[j].nimg * jobs[j].ctx->nsym);
+    }
+    return rc;
 }
 
 int spbp_graph_destroy(spbp_graph* g) {
@@ -1095,10 +1321,13 @@ int spbp_graph_create(const spbp_job* jobs, int njobs, int reset_first, spbp_gra
     // pass over the chunk sizes), job tables
     if ((rc = stage_batch(jobs, njobs, g->stream))) return bail(rc);
     g->geom.assign(njobs, nullptr);
+    g->hgeom.assign(njobs, {});
+    g->reset_first = reset_first;
     for (int j = 0; j < njobs; ++j) {
         spbp_ctx* c = jobs[j].ctx;
         if (jobs[j].nimg == 0) continue;
         const size_t ng = (size_t)jobs[j].nimg * c->nsym;
+        g->hgeom[j].assign(c->geom_h, c->geom_h + ng);
         if (hipMalloc(&g->geom[j], ng * sizeof(Geom)) != hipSuccess) return bail(fail(c, SPBP_ERR_ALLOC, "graph geometry"));
         if (hipMemcpyAsync(g->geom[j], c->geom_d, ng * sizeof(Geom), hipMemcpyDeviceToDevice, g->stream) != hipSuccess)
             return bail(fail(c, SPBP_ERR_DEVICE, "graph geometry copy"));
@@ -1157,6 +1386,12 @@ int spbp_graph_launch(spbp_graph* g, void* stream) {
     if (!g || !g->exec) return fail(nullptr, SPBP_ERR_ARG, "NULL graph");
     DeviceGuard dg(g->device);
     const hipError_t e = hipGraphLaunch(g->exec, (hipStream_t)stream);
+    for (size_t j = 0; j < g->jobs.size(); ++j) {
+        spbp_ctx* c = g->jobs[j].ctx;
+        if (e != hipSuccess) { c->hist_ok = false; continue; }
+        if (g->reset_first) { c->hist.clear(); c->hist_ok = true; }
+        if (c->hist_ok) c->hist.insert(c->hist.end(), g->hgeom[j].begin(), g->hgeom[j].end());
+    }
     if (e != hipSuccess) return fail(nullptr, SPBP_ERR_DEVICE, "hipGraphLaunch: %s", hipGetErrorString(e));
     return SPBP_OK;
 }
@@ -1330,11 +1565,12 @@ int spbp_set_symmetries(spbp_ctx* c, int nsym, const double* angles_deg) {
     return c->fwd_batch > 0 ? ensure_lists(c, c->fwd_batch) : SPBP_OK;
 }
 
-int spbp_kb_table(int n2, float* tab, float* fltb) {
+int spbp_kb_table(int n2, int flags, float* tab, float* fltb) {
     if (n2 < 2 || !tab || !fltb) return fail(nullptr, SPBP_ERR_ARG, "bad table arguments");
     std::vector<float> t;
     float f = 0.f;
-    kb_table(n2, t, f);
+    if (flags & SPBP_GEOM_F64) kb_table_f64_f32(n2, t, f);
+    else kb_table(n2, t, f);
     std::memcpy(tab, t.data(), t.size() * sizeof(float));
     *fltb = f;
     return SPBP_OK;
@@ -1393,6 +1629,44 @@ int spbp_import_partial(spbp_ctx* c, int half, const float* X, const float* W, i
     if (!accumulate && c->Wc[half]) fill_kernel<<<nblocks(c->nh), TPB, 0, st>>>(c->Wc[half], c->nh, 0.0f);
     import_kernel<<<nblocks(c->nh), TPB, 0, st>>>((const float2*)X, W, c->nx, c->N, c->X[half], c->W[half], accumulate);
     HIPCHK(c, hipGetLastError());
+    // the accumulators now hold sums the insertion history does not describe
+    c->hist_ok = false;
+    return SPBP_OK;
+}
+
+int spbp_reference_weights(spbp_ctx* c, int which, const long long* voxels, int n, double* w_out, void* stream) {
+    if (!c) return fail(nullptr, SPBP_ERR_ARG, "NULL context");
+    if (n < 0 || (n > 0 && (!voxels || !w_out))) return fail(c, SPBP_ERR_ARG, "bad voxel list");
+    if (which < -1 || which >= c->nvol) return fail(c, SPBP_ERR_ARG, "no half %d", which);
+    if (!c->hist_ok) return fail(c, SPBP_ERR_ARG, "the accumulators hold imported sums: no insertion history");
+    for (int i = 0; i < n; ++i)
+        if (voxels[i] < 0 || (size_t)voxels[i] >= c->nh) return fail(c, SPBP_ERR_ARG, "voxel %lld outside the half grid", voxels[i]);
+    if (n == 0) return SPBP_OK;
+    if (c->hist.empty()) {
+        for (int i = 0; i < n; ++i) w_out[i] = 0.0;
+        return SPBP_OK;
+    }
+    if (c->hist.size() > (size_t)INT32_MAX) return fail(c, SPBP_ERR_SIZE, "insertion history too long");
+    DeviceGuard dg(c->device);
+    hipStream_t st = (hipStream_t)stream;
+    int rc;
+    if ((rc = upload_history(c, st)) || (rc = grow(c, c->rlist, c->rlist_cap, (size_t)n, "voxel list"))) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->rlist, voxels, (size_t)n * sizeof(long long), hipMemcpyHostToDevice, st));
+    double* d = nullptr;
+    HIPCHK(c, hipMalloc(&d, (size_t)n * sizeof(double)));
+    rc = resolve_list(c, which, make_acc(c, which), n, d, st);
+    if (rc == SPBP_OK && (hipMemcpyAsync(w_out, d, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+                          hipStreamSynchronize(st) != hipSuccess))
+        rc = fail(c, SPBP_ERR_DEVICE, "reference weights read-back failed");
+    (void)hipFree(d);
+    return rc;
+}
+
+int spbp_threshold_stats(const spbp_ctx* c, long long* listed, long long* changed, long long* inconsistent) {
+    if (!c) return fail(nullptr, SPBP_ERR_ARG, "NULL context");
+    if (listed) *listed = c->res_stats[0];
+    if (changed) *changed = c->res_stats[1];
+    if (inconsistent) *inconsistent = c->res_stats[2];
     return SPBP_OK;
 }
 

@@ -251,11 +251,9 @@ def test_twosum_weights(bpmod, strategy):
     nx, m = 32, 600
     imgs = rng.standard_normal((m, nx, nx)).astype(np.float32)
     ang = synth.uniform_angles(m, rng)
-    # the library evaluates the KB table in FP64 (rounded to fp32); the
-    # reference's single-precision table differs at the support edge, where
-    # sqrt(1 - q^2) cancels, so the exact sum is formed with the FP64 table
-    _, fltb = oracle.kb_table(2 * nx, "f32")
-    tab = oracle.kb_table(2 * nx, "f64")[0].astype(np.float32)
+    # the library weights with the reference's REAL table, so the exact sum
+    # is formed from the same fp32 terms
+    tab, fltb = oracle.kb_table(2 * nx, "f32")
     n = 2 * nx
     X64 = np.zeros((n, n, nx + 1), np.complex128)
     W64 = np.zeros((n, n, nx + 1), np.float64)

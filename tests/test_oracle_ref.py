@@ -25,13 +25,24 @@ def test_kb_table_vs_reference(golden):
         tab, f = oracle.kb_table(n2, "f32")
         assert np.float32(f) == np.float32(fltb) and ln2 == 2 and nnn == 3
         assert alpha == pytest.approx(6.5) and aaaa == pytest.approx(1.8 / n2, rel=1e-6)
-        # same support (zero beyond |d| = 1.8 voxels, index 2999) ...
-        np.testing.assert_array_equal(tab != 0, ref != 0)
-        assert (ref != 0).sum() == 3000
-        # ... and values to single-precision Bessel accuracy (Cody I1 vs series)
-        assert np.abs(tab - ref).max() <= 4e-7
+        # the REAL evaluation (Cody's I1 as besi1.f) is the reference's bit for bit
+        np.testing.assert_array_equal(tab, ref)
+        assert (ref != 0).sum() == 3000      # zero beyond |d| = 1.8 voxels, index 2999
+        # the FP64 evaluation differs only by single-precision rounding
         t64, _ = oracle.kb_table(n2, "f64")
         assert np.abs(t64 - ref).max() <= 1e-6
+
+
+def test_library_kb_table_is_the_reference(golden):
+    """libspbp's default table (host code, no GPU) equals the reference's TABI."""
+    from spider_amd import _lib
+    g = golden("kbtab.npz")
+    for n2 in (128, 150, 256):
+        tab, f = _lib.kb_table(n2)
+        np.testing.assert_array_equal(tab, g[f"tab{n2}"])
+        assert np.float32(f) == np.float32(g[f"meta{n2}"][0])
+        t64, _ = _lib.kb_table(n2, geom_f64=True)
+        np.testing.assert_array_equal(t64, oracle.kb_table(n2, "f64")[0].astype(np.float32))
 
 
 def test_bessel_vs_reference(golden):
