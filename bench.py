@@ -392,6 +392,8 @@ def main():
     fft_ms, _, _ = b.kernel_stats(_lib.K_FFT2)
     fin_ms, nfin, _ = b.kernel_stats(_lib.K_FINALIZE)
     bin_ms, _, _ = b.kernel_stats(_lib.K_BIN)
+    res_ms, nres, _ = b.kernel_stats(_lib.K_RESOLVE)
+    thresh = b.threshold_stats()
     b.close()
 
     U = distinct_voxels(bpmod, nx, synth.uniform_angles(16, np.random.default_rng(5)))
@@ -421,7 +423,9 @@ def main():
                       "twosum_weights": twosum},
            "breakdown_ms_per_step": {"pad": round(pad_ms / args.steps, 3), "fft2": round(fft_ms / args.steps, 3),
                                      "insert": round(ins_ms / args.steps, 3), "bin": round(bin_ms / args.steps, 3),
-                                     "finalize": round(fin_ms / max(nfin, 1), 3)},
+                                     "finalize": round(fin_ms / max(nfin, 1), 3),
+                                     "nrmw2_resum": round(res_ms / max(nres, 1), 3)},
+           "nrmw2_threshold": thresh,
            "roofline": roof,
            "plan": {"strategy": plan["strategy"], "brick": list(plan["brick"]), "brick_proj": plan["brick_proj"],
                     "tune_seconds": None if t_tune is None else round(t_tune, 3)}}

@@ -198,6 +198,15 @@ template <typename P>
 __device__ inline int sample_terms(const Geom& g, int I, int J, int nx, int N, const P* __restrict__ tab, P fltb,
                                    int tx, int ty, int tz, P out[2]) {
     if (!((I * I + J * J) < nx * nx) || (I == 0 && J < 0)) return 0;   // oneline.f:53-55
+    {
+        // cheap test first: a term needs the sample (at q or -q) within 2.5
+        // voxels per axis of t, i.e. q within 2.5 of t or of -t
+        const float fI = (float)I, fJ = (float)J;
+        const float qx = fI * g.r[0] + fJ * g.r[3], qy = fI * g.r[1] + fJ * g.r[4], qz = fI * g.r[2] + fJ * g.r[5];
+        const float dm = fmaxf(fabsf(qx - tx), fmaxf(fabsf(qy - ty), fabsf(qz - tz)));
+        const float dp = fmaxf(fabsf(qx + tx), fmaxf(fabsf(qy + ty), fabsf(qz + tz)));
+        if (fminf(dm, dp) > 2.6f) return 0;
+    }
     P x, y, z;
     sample_pos(g, I, J, x, y, z);
     if (x < (P)0) { x = -x; y = -y; z = -z; }
@@ -267,7 +276,8 @@ __device__ inline void for_candidates(const Geom& g, int nx, int tx, int ty, int
 }
 
 // Thread per hit: WRITE = false counts its terms into tcnt, WRITE = true
-// writes them in order at toff.
+// writes them in order at toff, negated for sections of half 1 (weights are
+// positive, so the sign carries the half to the summation).
 template <typename P, bool WRITE>
 __global__ __launch_bounds__(RES_TPB) void term_kernel(const int2* __restrict__ hits, long long nhits,
                                                        const long long* __restrict__ list, const Geom* __restrict__ hist,
@@ -282,11 +292,12 @@ __global__ __launch_bounds__(RES_TPB) void term_kernel(const int2* __restrict__ 
     const Geom g = hist[hv.x];
     long long q = WRITE ? toff[h] : 0;
     int n = 0;
+    const bool neg = g.half != 0;
     for_candidates<P>(g, nx, tx, ty, tz, [&](int I, int J) {
         P t[2];
         const int k = sample_terms<P>(g, I, J, nx, N, tab, fltb, tx, ty, tz, t);
         for (int u = 0; u < k; ++u) {
-            if (WRITE) terms[q++] = t[u];
+            if (WRITE) terms[q++] = neg ? -t[u] : t[u];
             ++n;
         }
     });
@@ -294,30 +305,37 @@ __global__ __launch_bounds__(RES_TPB) void term_kernel(const int2* __restrict__ 
 }
 
 // ------------------------------------------------------------------ sums, decisions
-// Thread per item: its terms in section order, one sequential sum per half.
+// Thread per item: its terms in section order (contiguous, since hits and
+// terms are both laid out item-major), one sequential sum per half.
 template <typename P>
 __global__ __launch_bounds__(RES_TPB) void sum_kernel(long long nitems, int nchunks, const long long* __restrict__ hoff,
-                                                      const int2* __restrict__ hits, const long long* __restrict__ toff,
-                                                      const P* __restrict__ terms, const Geom* __restrict__ hist,
+                                                      const long long* __restrict__ toff, const P* __restrict__ terms,
                                                       P* __restrict__ sums) {
     const long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (it >= nitems) return;
-    P acc[2] = {(P)0, (P)0};
-    const long long h0 = hoff[it * nchunks], h1 = hoff[(it + 1) * nchunks];
-    for (long long h = h0; h < h1; ++h) {
-        const int half = hist[hits[h].x].half;
-        for (long long q = toff[h]; q < toff[h + 1]; ++q) acc[half] = acc[half] + terms[q];
+    P a0 = (P)0, a1 = (P)0;
+    long long q = toff[hoff[it * nchunks]];
+    const long long q1 = toff[hoff[(it + 1) * nchunks]];
+    // the loads of a group are independent; only the additions are serial
+    for (; q + 8 <= q1; q += 8) {
+        P v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = terms[q + k];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            if (v[k] > (P)0) a0 = a0 + v[k];
+            else a1 = a1 + (-v[k]);
+        }
     }
-    sums[2 * it] = acc[0];
-    sums[2 * it + 1] = acc[1];
+    for (; q < q1; ++q) {
+        const P v = terms[q];
+        if (v > (P)0) a0 = a0 + v;
+        else a1 = a1 + (-v);
+    }
+    sums[2 * it] = a0;
+    sums[2 * it + 1] = a1;
 }
 
-// Thread per listed voxel: W as NRMW2 sees it (SYMPLANE0 pair added per half,
-// then the included halves in order; bp32f.f:925-930, 345-346) and the
-// decision into mask (2 = divide, 1 = zero).  ctr[1] counts decisions that
-// differ from the GPU sums' own; a re-summation farther than `tol` from the
-// GPU's weight (a history that does not describe the accumulators) leaves the
-// voxel to the GPU sums and counts in ctr[2].
 template <typename P>
 __device__ inline P combined_weight(const long long* __restrict__ list, long long s, int hmask, int nx, int N,
                                     const P* __restrict__ sums) {

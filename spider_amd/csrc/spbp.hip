@@ -727,8 +727,7 @@ int resolve_terms(spbp_ctx* c, const Acc& a, int hmask, long long nv, int nchunk
         term_kernel<P, true><<<hb, RES_TPB, 0, st>>>(c->rhits, nhits, c->rlist, c->hist_d, tab, fltb, c->nx, c->N,
                                                      nullptr, c->rtoff, (P*)c->rterms);
     const unsigned ib = (unsigned)((nitems + RES_TPB - 1) / RES_TPB);
-    sum_kernel<P><<<ib, RES_TPB, 0, st>>>(nitems, nchunks, c->roff, c->rhits, c->rtoff, (const P*)c->rterms, c->hist_d,
-                                          (P*)c->rsums);
+    sum_kernel<P><<<ib, RES_TPB, 0, st>>>(nitems, nchunks, c->roff, c->rtoff, (const P*)c->rterms, (P*)c->rsums);
     const unsigned vb = (unsigned)((nv + RES_TPB - 1) / RES_TPB);
     if (wout) {
         weights_out_kernel<P><<<vb, RES_TPB, 0, st>>>(c->rlist, nv, hmask, c->nx, c->N, (const P*)c->rsums, wout);
@@ -1589,7 +1588,7 @@ int spbp_rotations(int n, const double* ang, int flags, float* dm) {
 
 int spbp_kernel_stats(spbp_ctx* c, int kernel, double* total_ms, long* launches, long* units) {
     if (!c) return fail(nullptr, SPBP_ERR_ARG, "NULL context");
-    if (kernel < 0 || kernel > SPBP_K_GATHER) return fail(c, SPBP_ERR_ARG, "no kernel kind %d", kernel);
+    if (kernel < 0 || kernel > SPBP_K_RESOLVE) return fail(c, SPBP_ERR_ARG, "no kernel kind %d", kernel);
     DeviceGuard dg(c->device);
     double ms = 0.0;
     long u = 0;
