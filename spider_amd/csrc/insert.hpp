@@ -205,18 +205,42 @@ __global__ __launch_bounds__(1024) void brick_scan_kernel(const int* __restrict_
     if (threadIdx.x == 0) *next_item = 0;
 }
 
-// Write the projection lists of every (brick, half) key.
+// Write the projection lists of every (brick, half) key.  Block: one brick x
+// 1024 projections; each half's hits of a round of TPB projections take one
+// cursor reservation per block (wave ballots give the ranks).
 __global__ __launch_bounds__(TPB) void brick_fill_kernel(const Geom* __restrict__ geom, int nproj, BrickGrid bg, int nx,
                                                          const int* __restrict__ off, int* __restrict__ cursor,
                                                          int* __restrict__ list) {
+    __shared__ int wc[2][TPB / 64];
+    __shared__ int base[2];
     const int b = blockIdx.x;
     const int p0 = blockIdx.y * 1024;
-    for (int p = p0 + threadIdx.x; p < min(nproj, p0 + 1024); p += TPB) {
-        const Geom g = geom[p];
-        if (brick_hit(bg, b, nx, g)) {
-            const int key = b + (g.half ? bg.nb : 0);
-            list[off[key] + atomicAdd(&cursor[key], 1)] = p;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (int r = p0; r < min(nproj, p0 + 1024); r += TPB) {
+        const int p = r + threadIdx.x;
+        bool h0 = false, h1 = false;
+        if (p < nproj) {
+            const Geom& g = geom[p];
+            if (brick_hit(bg, b, nx, g)) { h1 = g.half != 0; h0 = !h1; }
         }
+        const unsigned long long m0 = __ballot(h0), m1 = __ballot(h1);
+        if (lane == 0) { wc[0][w] = __popcll(m0); wc[1][w] = __popcll(m1); }
+        __syncthreads();
+        if (threadIdx.x < 2) {
+            int t = 0;
+            for (int k = 0; k < TPB / 64; ++k) t += wc[threadIdx.x][k];
+            base[threadIdx.x] = t ? atomicAdd(&cursor[b + threadIdx.x * bg.nb], t) : 0;
+        }
+        __syncthreads();
+        if (h0 || h1) {
+            const int hh = h1 ? 1 : 0;
+            int rank = __popcll((h1 ? m1 : m0) & below);
+            for (int k = 0; k < w; ++k) rank += wc[hh][k];
+            const int key = b + hh * bg.nb;
+            list[off[key] + base[hh] + rank] = p;
+        }
+        __syncthreads();
     }
 }
 

@@ -91,7 +91,7 @@ struct spbp_ctx {
     // brick strategy
     int strategy = SPBP_STRAT_BRICK;
     int shape = 3;   // index into SHAPES
-    int C = 32;
+    int C = 64;
     bool tuned = false;
     bool capturing = false;   // a batch graph is being captured: no host syncs, no timing events
     // symmetries (BUILDS): each image is inserted once per matrix, SYMVOL after the window
@@ -905,7 +905,7 @@ int spbp_create(const spbp_opts* o, spbp_ctx** out) {
     c->strategy = o->strategy == SPBP_STRAT_DIRECT ? SPBP_STRAT_DIRECT : SPBP_STRAT_BRICK;
     if (o->strategy < 0 || o->strategy > SPBP_STRAT_BRICK) { delete c; return fail(nullptr, SPBP_ERR_ARG, "unknown strategy %d", o->strategy); }
     if (o->brick_proj > 128) { delete c; return fail(nullptr, SPBP_ERR_ARG, "brick_proj %d above 128", o->brick_proj); }
-    c->C = o->brick_proj > 0 ? o->brick_proj : 32;
+    c->C = o->brick_proj > 0 ? o->brick_proj : 64;
     {
         int n = 0;
         if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0) c->ncu = n;
@@ -936,8 +936,8 @@ int spbp_create(const spbp_opts* o, spbp_ctx** out) {
         hipEventCreateWithFlags(&c->jobs_evt, hipEventDisableTiming) != hipSuccess)
         return bail(fail(nullptr, SPBP_ERR_DEVICE, "context set-up failed"));
     if (hipMalloc(&c->bnext, sizeof(int)) != hipSuccess) return bail(fail(nullptr, SPBP_ERR_ALLOC, "brick queue"));
-    // default plan: 16x12x8 bricks, 32 projections per item (fastest measured at
-    // 128^2 into 256^3; spbp_tune picks per workload)
+    // default plan: 16x12x8 bricks, 64 projections per item (fastest measured at 128^2 into 256^3 and
+    // 256^2 into 512^3; spbp_tune picks per workload)
     if ((rc = configure_bricks(c, 3))) { g_last_error = c->err; return bail(rc); }
     rc = spbp_reset(c, nullptr);
     if (rc) return bail(rc);
