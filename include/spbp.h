@@ -55,6 +55,9 @@ extern "C" {
 #define SPBP_GEOM_F64     0x20  /* FP64 rotations (from FP64 angles), sample positions, nearest voxels
                                    and table indices: departs from the reference's fp32 geometry
                                    (oneline.f:56-105) to agree with an FP64 computation          */
+#define SPBP_HIPFFT       0x80  /* forward 2D transforms by hipFFT on the fully padded images instead
+                                   of the library's own two-pass transforms (power-of-two N only;
+                                   other sizes always use hipFFT)                              */
 #define SPBP_PLAIN_THRESHOLD 0x40 /* NRMW2's W > 0.1 test on the GPU sums alone.  By default the
                                    voxels whose W lies within 2^-11 of 0.1 are decided by
                                    re-summing their weights in the reference's order (see

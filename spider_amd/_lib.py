@@ -32,6 +32,7 @@ CHECK_FINITE = 0x08
 TIMING = 0x10
 GEOM_F64 = 0x20
 PLAIN_THRESHOLD = 0x40
+HIPFFT = 0x80
 
 K_INSERT = 0
 K_PAD = 1

@@ -44,7 +44,7 @@ class Backprojector:
     def __init__(self, nx: int, two_halves: bool = False, exact_pi: bool = False,
                  twosum: bool = False, check_finite: bool = False, device=None, batch: int = 0,
                  timing: bool = False, strategy: str = "auto", brick_proj: int = 0, symmetries=None,
-                 geom_f64: bool = False, plain_threshold: bool = False):
+                 geom_f64: bool = False, plain_threshold: bool = False, hipfft: bool = False):
         torch = _torch()
         if not torch.cuda.is_available():
             raise RuntimeError("spider_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
@@ -52,7 +52,7 @@ class Backprojector:
         flags = (_lib.TWO_HALVES if two_halves else 0) | (_lib.EXACT_PI if exact_pi else 0) | \
                 (_lib.TWOSUM_W if twosum else 0) | (_lib.CHECK_FINITE if check_finite else 0) | \
                 (_lib.TIMING if timing else 0) | (_lib.GEOM_F64 if geom_f64 else 0) | \
-                (_lib.PLAIN_THRESHOLD if plain_threshold else 0)
+                (_lib.PLAIN_THRESHOLD if plain_threshold else 0) | (_lib.HIPFFT if hipfft else 0)
         strat = {"auto": _lib.STRAT_AUTO, "direct": _lib.STRAT_DIRECT, "brick": _lib.STRAT_BRICK}[strategy]
         opts = _lib.Opts(int(nx), flags, self.device.index, int(batch), strat, int(brick_proj))
         ctx = C.c_void_p()

@@ -37,6 +37,7 @@
 
 
 #include "common.hpp"
+#include "fft.hpp"
 #include "finalize.hpp"
 #include "hostmath.hpp"
 #include "insert.hpp"
@@ -76,6 +77,10 @@ struct spbp_ctx {
     int fwd_batch = 0;
     hipfftHandle fwd_tail = 0;
     int tail_batch = 0;
+    // own forward transforms (power-of-two N): e^{-2 pi i m / N}, m < N
+    float2* tw = nullptr;
+    int fft_log2 = 0;   // log2 N, or 0 when hipFFT does the forward transforms
+    double* fmean = nullptr;   // per chunk image: PADD2's mean
     // finalisation
     hipfftHandle inv = 0;
     float2* work = nullptr;
@@ -260,23 +265,26 @@ int ensure_lists(spbp_ctx* c, int B) {
 }
 
 int ensure_chunk(spbp_ctx* c, int want_batch) {
-    if (c->padbuf && c->fwd_batch >= want_batch) return ensure_lists(c, c->fwd_batch);
+    if (c->spec && c->fwd_batch >= want_batch) return ensure_lists(c, c->fwd_batch);
     if (c->padbuf) { (void)hipFree(c->padbuf); c->padbuf = nullptr; }
     if (c->spec) { (void)hipFree(c->spec); c->spec = nullptr; }
     if (c->fwd) { hipfftDestroy(c->fwd); c->fwd = 0; }
     if (c->fwd_tail) { hipfftDestroy(c->fwd_tail); c->fwd_tail = 0; c->tail_batch = 0; }
     const int N = c->N, H = c->H, B = want_batch;
-    HIPCHK(c, hipMalloc(&c->padbuf, (size_t)B * N * N * sizeof(float)));
+    if (!c->fft_log2) HIPCHK(c, hipMalloc(&c->padbuf, (size_t)B * N * N * sizeof(float)));
     HIPCHK(c, hipMalloc(&c->spec, (size_t)B * N * H * sizeof(float2)));
     if (c->vmax) { (void)hipFree(c->vmax); c->vmax = nullptr; }
     HIPCHK(c, hipMalloc(&c->vmax, (size_t)B * sizeof(float)));
+    if (c->fmean) { (void)hipFree(c->fmean); c->fmean = nullptr; }
+    if (c->fft_log2) HIPCHK(c, hipMalloc(&c->fmean, (size_t)B * sizeof(double)));
     int rc = ensure_lists(c, B);
     if (rc) return rc;
+    c->fwd_batch = B;
+    if (c->fft_log2) return SPBP_OK;   // own transforms: no plan, and the pad buffer is not used
     int n[2] = {N, N};
     int inembed[2] = {N, N};
     int onembed[2] = {N, H};
     FFTCHK(c, hipfftPlanMany(&c->fwd, 2, n, inembed, 1, N * N, onembed, 1, N * H, HIPFFT_R2C, B));
-    c->fwd_batch = B;
     return SPBP_OK;
 }
 
@@ -496,6 +504,31 @@ int run_insert(spbp_ctx* c, int img0, const Geom* geom_d, int ng, const Targets&
 
 // PADD2 + batched forward FFT of b images into c->spec.
 int pad_fft(spbp_ctx* c, const void* src, int dtype, int b, hipStream_t st) {
+    if (c->fft_log2) {
+        const size_t lds = fft_lds_bytes(c->N);
+        const int M = std::min(FFT_ELEMS / c->N, c->nx / 2);
+        const dim3 rgrid((c->nx / 2 + M - 1) / M, b);
+        {
+            Timed tm(c, SPBP_K_PAD, b, st);
+            if (dtype == SPBP_BF16) {
+                pad_mean_kernel<uint16_t><<<b, TPB, 0, st>>>((const uint16_t*)src, c->nx, c->fmean, c->vmax);
+                row_fft_kernel<uint16_t><<<rgrid, TPB, lds, st>>>((const uint16_t*)src, c->nx, c->N, c->fft_log2, c->tw,
+                                                                  c->fmean, c->spec);
+            } else {
+                pad_mean_kernel<float><<<b, TPB, 0, st>>>((const float*)src, c->nx, c->fmean, c->vmax);
+                row_fft_kernel<float><<<rgrid, TPB, lds, st>>>((const float*)src, c->nx, c->N, c->fft_log2, c->tw,
+                                                               c->fmean, c->spec);
+            }
+        }
+        {
+            Timed tm(c, SPBP_K_FFT2, b, st);
+            const int cw = FFT_ELEMS / c->N;
+            col_fft_kernel<<<dim3((c->H + cw - 1) / cw, b), TPB, lds, st>>>(c->nx, c->N, c->fft_log2, c->tw, c->spec,
+                                                                           c->vmax);
+        }
+        HIPCHK(c, hipGetLastError());
+        return SPBP_OK;
+    }
     {
         Timed tm(c, SPBP_K_PAD, b, st);
         if (dtype == SPBP_BF16)
@@ -936,6 +969,19 @@ int spbp_create(const spbp_opts* o, spbp_ctx** out) {
         hipEventCreateWithFlags(&c->jobs_evt, hipEventDisableTiming) != hipSuccess)
         return bail(fail(nullptr, SPBP_ERR_DEVICE, "context set-up failed"));
     if (hipMalloc(&c->bnext, sizeof(int)) != hipSuccess) return bail(fail(nullptr, SPBP_ERR_ALLOC, "brick queue"));
+    if ((c->N & (c->N - 1)) == 0 && c->N >= 8 && c->N <= 2048 && !(o->flags & SPBP_HIPFFT)) {
+        int L = 0;
+        while ((1 << L) < c->N) ++L;
+        std::vector<float2> t(c->N);
+        for (int m = 0; m < c->N; ++m) {
+            const double a = -2.0 * M_PI * (double)m / (double)c->N;
+            t[m] = make_float2((float)std::cos(a), (float)std::sin(a));
+        }
+        if (hipMalloc(&c->tw, t.size() * sizeof(float2)) != hipSuccess ||
+            hipMemcpy(c->tw, t.data(), t.size() * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(nullptr, SPBP_ERR_ALLOC, "twiddle table"));
+        c->fft_log2 = L;
+    }
     // default plan: 16x12x8 bricks, 64 projections per item (fastest measured at 128^2 into 256^3 and
     // 256^2 into 512^3; spbp_tune picks per workload)
     if ((rc = configure_bricks(c, 3))) { g_last_error = c->err; return bail(rc); }
@@ -958,7 +1004,7 @@ int spbp_destroy(spbp_ctx* c) {
     }
     void* dev_ptrs[] = {c->tab, c->padbuf, c->spec, c->vmax, c->geom_d, c->work, c->real, c->wkb, c->psum, c->pcnt, c->mean, c->flag,
                         c->bcnt, c->boff, c->bioff, c->bcur, c->bnext, c->blist, c->bikey, c->jobs_d,
-                        c->sm_d, c->symtmp, c->rotbuf, c->rprm_d, c->hist_d, c->tab64, c->tmask, c->rlist, c->rcnt,
+                        c->sm_d, c->symtmp, c->rotbuf, c->rprm_d, c->hist_d, c->tab64, c->tmask, c->rlist, c->rcnt, c->tw, c->fmean,
                         c->roff, c->rbsum, c->rhits, c->rtcnt, c->rtoff, c->rterms, c->rsums, c->rctr};
     for (void* p : dev_ptrs) if (p) (void)hipFree(p);
     if (c->geom_h) (void)hipHostFree(c->geom_h);

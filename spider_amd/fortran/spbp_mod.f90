@@ -14,7 +14,7 @@ module spbp
       SPBP_ERR_NONFINITE = 205
    integer(c_int), parameter, public :: SPBP_F32 = 0, SPBP_BF16 = 1
    integer(c_int), parameter, public :: SPBP_TWO_HALVES = 1, SPBP_EXACT_PI = 2, SPBP_TWOSUM_W = 4, &
-      SPBP_CHECK_FINITE = 8, SPBP_TIMING = 16, SPBP_GEOM_F64 = 32, SPBP_PLAIN_THRESHOLD = 64
+      SPBP_CHECK_FINITE = 8, SPBP_TIMING = 16, SPBP_GEOM_F64 = 32, SPBP_PLAIN_THRESHOLD = 64, SPBP_HIPFFT = 128
    integer(c_int), parameter, public :: SPBP_STRAT_AUTO = 0, SPBP_STRAT_DIRECT = 1, SPBP_STRAT_BRICK = 2
 
    type, bind(c), public :: spbp_opts
