@@ -124,15 +124,16 @@ __global__ __launch_bounds__(TPB) void pad_mean_kernel(const T* __restrict__ img
     const float r2 = (float)c * (float)c;
     double s = 0.0;
     unsigned cnt = 0;
-    for (int i0 = threadIdx.x; i0 < n; i0 += FFT_BATCH * TPB) {
-        float v[FFT_BATCH];
+    constexpr int MB = FFT_BATCH;   // 4 FFT_BATCH measured slower (2.44 vs 2.11 ms per 20k images of 128^2)
+    for (int i0 = threadIdx.x; i0 < n; i0 += MB * TPB) {
+        float v[MB];
 #pragma unroll
-        for (int u = 0; u < FFT_BATCH; ++u) {
+        for (int u = 0; u < MB; ++u) {
             const int idx = i0 + u * TPB;
             v[u] = idx < n ? load_img(img, idx) : 0.0f;
         }
 #pragma unroll
-        for (int u = 0; u < FFT_BATCH; ++u) {
+        for (int u = 0; u < MB; ++u) {
             const int idx = i0 + u * TPB;
             const int j = idx / nx, i = idx - j * nx;
             const float tj = (float)(j - c), ti = (float)(i - c);

@@ -172,37 +172,66 @@ __global__ __launch_bounds__(TPB) void brick_count_kernel(const Geom* __restrict
     }
 }
 
-// Exclusive scans of the counts and of the work items per key; zero cursors.
-// One block of 1024 threads.  item_off[nkeys] = number of work items.
-__global__ __launch_bounds__(1024) void brick_scan_kernel(const int* __restrict__ cnt, int nkeys, int C,
-                                                          int* __restrict__ off, int* __restrict__ item_off,
-                                                          int* __restrict__ cursor, int* __restrict__ next_item) {
-    __shared__ int s_off[1024], s_it[1024];
-    const int per = (nkeys + 1023) / 1024;
-    const int k0 = threadIdx.x * per, k1 = min(nkeys, k0 + per);
-    int so = 0, si = 0;
-    for (int k = k0; k < k1; ++k) { so += cnt[k]; si += (cnt[k] + C - 1) / C; }
-    s_off[threadIdx.x] = so;
-    s_it[threadIdx.x] = si;
+// Exclusive scans of the counts and of the work items per key, zeroed
+// cursors, queue head zeroed: three kernels (block per 1024 keys; one block
+// over the block totals; offsets added).  off[nkeys] and item_off[nkeys] are
+// the totals.
+__device__ inline int2 block_scan_pair(int2 v, int2* total) {
+    __shared__ int2 wsum[1024 / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    int2 inc = v;
+    for (int d = 1; d < 64; d <<= 1) {
+        const int ux = __shfl_up(inc.x, d, 64), uy = __shfl_up(inc.y, d, 64);
+        if (lane >= d) { inc.x += ux; inc.y += uy; }
+    }
+    if (lane == 63) wsum[w] = inc;
     __syncthreads();
-    for (int d = 1; d < 1024; d <<= 1) {   // inclusive Hillis-Steele scan
-        int a = 0, b = 0;
-        if ((int)threadIdx.x >= d) { a = s_off[threadIdx.x - d]; b = s_it[threadIdx.x - d]; }
-        __syncthreads();
-        s_off[threadIdx.x] += a;
-        s_it[threadIdx.x] += b;
-        __syncthreads();
+    int2 base = make_int2(0, 0), all = make_int2(0, 0);
+    for (int k = 0; k < nw; ++k) {
+        if (k < w) { base.x += wsum[k].x; base.y += wsum[k].y; }
+        all.x += wsum[k].x;
+        all.y += wsum[k].y;
     }
-    int ro = s_off[threadIdx.x] - so, ri = s_it[threadIdx.x] - si;
-    for (int k = k0; k < k1; ++k) {
-        off[k] = ro;
-        item_off[k] = ri;
-        cursor[k] = 0;
-        ro += cnt[k];
-        ri += (cnt[k] + C - 1) / C;
+    __syncthreads();
+    *total = all;
+    return make_int2(base.x + inc.x - v.x, base.y + inc.y - v.y);
+}
+
+__global__ __launch_bounds__(1024) void key_scan_blocks_kernel(const int* __restrict__ cnt, int nkeys, int C,
+                                                               int* __restrict__ off, int* __restrict__ item_off,
+                                                               int2* __restrict__ bsum) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = k < nkeys ? cnt[k] : 0;
+    int2 tot;
+    const int2 ex = block_scan_pair(make_int2(c, (c + C - 1) / C), &tot);
+    if (k < nkeys) { off[k] = ex.x; item_off[k] = ex.y; }
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(1024) void key_scan_sums_kernel(int2* __restrict__ bsum, int nb, int nkeys,
+                                                             int* __restrict__ off, int* __restrict__ item_off,
+                                                             int* __restrict__ next_item) {
+    int2 carry = make_int2(0, 0);
+    for (int b0 = 0; b0 < nb; b0 += blockDim.x) {
+        const int b = b0 + threadIdx.x;
+        int2 tot;
+        const int2 ex = block_scan_pair(b < nb ? bsum[b] : make_int2(0, 0), &tot);
+        if (b < nb) bsum[b] = make_int2(carry.x + ex.x, carry.y + ex.y);
+        carry.x += tot.x;
+        carry.y += tot.y;
     }
-    if (threadIdx.x == 1023) { off[nkeys] = s_off[1023]; item_off[nkeys] = s_it[1023]; }
-    if (threadIdx.x == 0) *next_item = 0;
+    if (threadIdx.x == 0) { off[nkeys] = carry.x; item_off[nkeys] = carry.y; *next_item = 0; }
+}
+
+__global__ __launch_bounds__(1024) void key_scan_add_kernel(int nkeys, const int2* __restrict__ bsum,
+                                                            int* __restrict__ off, int* __restrict__ item_off,
+                                                            int* __restrict__ cursor) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nkeys) return;
+    const int2 b = bsum[blockIdx.x];
+    off[k] += b.x;
+    item_off[k] += b.y;
+    cursor[k] = 0;
 }
 
 // Write the projection lists of every (brick, half) key.  Block: one brick x

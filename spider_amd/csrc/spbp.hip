@@ -124,6 +124,7 @@ struct spbp_ctx {
     int* boff = nullptr;    // [2 nb + 1] list offsets
     int* bioff = nullptr;   // [2 nb + 1] work-item offsets
     int* bcur = nullptr;    // [2 nb] fill cursors
+    int2* bscan = nullptr;  // [ceil(2 nb / 1024)] key-scan block totals
     int* bnext = nullptr;   // work-queue head
     int* blist = nullptr;   // projection lists
     int* bikey = nullptr;   // work item -> key
@@ -238,6 +239,7 @@ int configure_bricks(spbp_ctx* c, int shape) {
     int* bufs[4] = {c->bcnt, c->boff, c->bioff, c->bcur};
     for (int* q : bufs) if (q) (void)hipFree(q);
     c->bcnt = c->boff = c->bioff = c->bcur = nullptr;
+    if (c->bscan) { (void)hipFree(c->bscan); c->bscan = nullptr; }
     if (c->blist) { (void)hipFree(c->blist); c->blist = nullptr; }
     if (c->bikey) { (void)hipFree(c->bikey); c->bikey = nullptr; }
     c->blist_cap = 0;
@@ -248,6 +250,7 @@ int configure_bricks(spbp_ctx* c, int shape) {
     HIPCHK(c, hipMalloc(&c->boff, (nk + 1) * sizeof(int)));
     HIPCHK(c, hipMalloc(&c->bioff, (nk + 1) * sizeof(int)));
     HIPCHK(c, hipMalloc(&c->bcur, nk * sizeof(int)));
+    HIPCHK(c, hipMalloc(&c->bscan, ((nk + 1023) / 1024 + 1) * sizeof(int2)));
     return SPBP_OK;
 }
 
@@ -468,7 +471,10 @@ int build_lists(spbp_ctx* c, const Geom* geom_d, int b, hipStream_t st) {
     Timed tm(c, SPBP_K_BIN, b, st);
     fill_kernel<<<nblocks(nkeys), TPB, 0, st>>>((float*)c->bcnt, nkeys, 0.0f);   // zero bits
     brick_count_kernel<<<lgrid, TPB, 0, st>>>(geom_d, b, c->bg, c->nx, c->bcnt);
-    brick_scan_kernel<<<1, 1024, 0, st>>>(c->bcnt, nkeys, c->C, c->boff, c->bioff, c->bcur, c->bnext);
+    const int kb = (nkeys + 1023) / 1024;
+    key_scan_blocks_kernel<<<kb, 1024, 0, st>>>(c->bcnt, nkeys, c->C, c->boff, c->bioff, c->bscan);
+    key_scan_sums_kernel<<<1, 1024, 0, st>>>(c->bscan, kb, nkeys, c->boff, c->bioff, c->bnext);
+    key_scan_add_kernel<<<kb, 1024, 0, st>>>(nkeys, c->bscan, c->boff, c->bioff, c->bcur);
     brick_fill_kernel<<<lgrid, TPB, 0, st>>>(geom_d, b, c->bg, c->nx, c->boff, c->bcur, c->blist);
     brick_itemkey_kernel<<<nblocks(nkeys), TPB, 0, st>>>(c->bioff, nkeys, c->bikey);
     HIPCHK(c, hipGetLastError());
@@ -1003,7 +1009,7 @@ int spbp_destroy(spbp_ctx* c) {
         if (c->Wc[h]) (void)hipFree(c->Wc[h]);
     }
     void* dev_ptrs[] = {c->tab, c->padbuf, c->spec, c->vmax, c->geom_d, c->work, c->real, c->wkb, c->psum, c->pcnt, c->mean, c->flag,
-                        c->bcnt, c->boff, c->bioff, c->bcur, c->bnext, c->blist, c->bikey, c->jobs_d,
+                        c->bcnt, c->boff, c->bioff, c->bcur, c->bscan, c->bnext, c->blist, c->bikey, c->jobs_d,
                         c->sm_d, c->symtmp, c->rotbuf, c->rprm_d, c->hist_d, c->tab64, c->tmask, c->rlist, c->rcnt, c->tw, c->fmean,
                         c->roff, c->rbsum, c->rhits, c->rtcnt, c->rtoff, c->rterms, c->rsums, c->rctr};
     for (void* p : dev_ptrs) if (p) (void)hipFree(p);
