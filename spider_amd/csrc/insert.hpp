@@ -312,16 +312,19 @@ struct BrickLds {
     // spectrum value and the fp32 weight (two 24-bit mantissas fit FP64),
     // scaled by a power of two and rounded to an integer unit of 2^-48 of the
     // work item's largest spectrum magnitude; the brick sum is exact and only
-    // that rounding and the final conversion round.  Weights span many orders
-    // of magnitude (products of three table values near the support edge), so
-    // they keep FP64 partial sums, which preserve their relative accuracy.
-    unsigned long long re[NS], im[NS];
-    double w[NS];
+    // that rounding and the final conversion round.  Weights are exact
+    // 64-bit fixed point too, in units of 2^-WBITS: every weight term is a
+    // product of three table values, at most 1, rounded to 2^-(WBITS+1)
+    // absolute (relative 10^-14 where NRMW2 divides, W > 0.1).
+    unsigned long long re[NS], im[NS], w[NS];
     int item;
     float vmax[TPB / 64];
 };
 
 constexpr int VBITS = 48;                   // |term| < 2^48 units; item sums < 2^62
+constexpr int WBITS = 50;                   // weight terms <= 1 < 2^51 / 2^WBITS; item sums < 2^63
+constexpr double WSCALE = 1125899906842624.0;            // 2^WBITS
+constexpr double WINV = 1.0 / 1125899906842624.0;        // 2^-WBITS
 
 // a * b (|a b| < 2^51) rounded to the nearest integer, as a two's-complement
 // 64-bit value: add 1.5 * 2^52 and read the mantissa bits.  The constant's low
@@ -500,7 +503,7 @@ __device__ __forceinline__ void wave_project(BrickLds<BX, BY, BZ>& L, const Geom
                             const double wd = (double)wg;
                             atomicAdd(&L.re[o], to_fixed_prod(vxs, wd));
                             atomicAdd(&L.im[o], to_fixed_prod(vim[lx], wd));
-                            atomicAdd(&L.w[o], wd);
+                            atomicAdd(&L.w[o], to_fixed_prod(wd, WSCALE));
                         }
                     }
                 }
@@ -531,7 +534,7 @@ __device__ __forceinline__ void wave_project(BrickLds<BX, BY, BZ>& L, const Geom
                             const double wd = (double)wg;
                             atomicAdd(&L.re[o], to_fixed_prod(vxs, wd));
                             atomicAdd(&L.im[o], to_fixed_prod(vys, wd));
-                            atomicAdd(&L.w[o], wd);
+                            atomicAdd(&L.w[o], to_fixed_prod(wd, WSCALE));
                         }
                     }
                 }
@@ -607,7 +610,7 @@ void insert_brick_kernel(const BrickJob* __restrict__ jobs, int njobs, int* __re
         const int* lst = J.list + J.off[key] + s0;
         int x0, y0, z0;
         brick_origin(bg, b, nx, x0, y0, z0);
-        for (int i = threadIdx.x; i < L.NS; i += blockDim.x) { L.re[i] = 0ull; L.im[i] = 0ull; L.w[i] = 0.0; }
+        for (int i = threadIdx.x; i < L.NS; i += blockDim.x) { L.re[i] = 0ull; L.im[i] = 0ull; L.w[i] = 0ull; }
         // fixed-point unit of the item: 2^-VBITS of its largest spectrum
         // magnitude (rounded up to a power of two), so every term is below
         // 2^VBITS units and a voxel's sum over the item below 2^62
@@ -647,13 +650,13 @@ void insert_brick_kernel(const BrickJob* __restrict__ jobs, int njobs, int* __re
         for (int v = threadIdx.x; v < NV; v += blockDim.x) {
             const int lx = v % BX, ly = (v / BX) % BY, lz = v / (BX * BY);
             const int i = L.at(L.rowbase(ly, lz), L.rot(ly, lz), lx);
-            if (L.w[i] == 0.0) continue;
+            if (L.w[i] == 0ull) continue;
             const int kx = x0 + lx, ky = y0 + ly, kz = z0 + lz;
             if (kx > nx || ky > nx || kz > nx) continue;   // brick overhangs the grid (partial last brick)
             const size_t o = ((size_t)(kz >= 0 ? kz : N + kz) * N + (ky >= 0 ? ky : N + ky)) * H + kx;
             unsafeAtomicAdd(&X[o].x, (float)__dmul_rn(from_fixed(L.re[i]), vinv));
             unsafeAtomicAdd(&X[o].y, (float)__dmul_rn(from_fixed(L.im[i]), vinv));
-            const double wd = L.w[i];
+            const double wd = __dmul_rn(from_fixed(L.w[i]), WINV);
             const float wi = (float)wd;
             if (Wc) twosum_atomic_add(W, Wc, o, wi, (float)(wd - (double)wi));
             else if (W) unsafeAtomicAdd(&W[o], wi);
