@@ -36,6 +36,52 @@ __global__ __launch_bounds__(TPB) void gather_kernel(const float2* __restrict__ 
     if (flip) { x = -x; y = -y; z = -z; }
     const int ix = nearest_voxel(x, N), iy = nearest_voxel(y, N), iz = nearest_voxel(z, N);
     if (!in_bounds(ix, iy, iz, nx)) { *o = make_float2(0.f, 0.f); return; }
+    const size_t plane = (size_t)N * H;
+    if (ix >= LN2) {
+        // no reflected taps (every tap has p_x >= ix - 2 >= 0); the nonzero
+        // taps of each axis are a run of at most 4 (the table vanishes beyond
+        // 1.8 voxels): from i - 2 when its weight is nonzero, else from i - 1.
+        // Same nonzero terms in the same order as the general loop below.
+        float tx[4], ty[4], tz[4];
+        const float wx0 = kb_weight(tab, __fsub_rn(x, (float)(ix - 2)), fltb);
+        const float wy0 = kb_weight(tab, __fsub_rn(y, (float)(iy - 2)), fltb);
+        const float wz0 = kb_weight(tab, __fsub_rn(z, (float)(iz - 2)), fltb);
+        const int sx = wx0 != 0.0f ? ix - 2 : ix - 1;
+        const int sy = wy0 != 0.0f ? iy - 2 : iy - 1;
+        const int sz = wz0 != 0.0f ? iz - 2 : iz - 1;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            tx[t] = (t == 0 && sx == ix - 2) ? wx0 : kb_weight(tab, __fsub_rn(x, (float)(sx + t)), fltb);
+            ty[t] = (t == 0 && sy == iy - 2) ? wy0 : kb_weight(tab, __fsub_rn(y, (float)(sy + t)), fltb);
+            tz[t] = (t == 0 && sz == iz - 2) ? wz0 : kb_weight(tab, __fsub_rn(z, (float)(sz + t)), fltb);
+        }
+        double dre = 0.0, dim = 0.0;
+#pragma unroll
+        for (int lz = 0; lz < 4; ++lz) {
+            const int izp = sz + lz;
+            const float2* Xz = X + (size_t)(izp >= 0 ? izp : N + izp) * plane;
+#pragma unroll
+            for (int ly = 0; ly < 4; ++ly) {
+                const float tyz = __fmul_rn(ty[ly], tz[lz]);
+                const int iyp = sy + ly;
+                const float2* Xy = Xz + (size_t)(iyp >= 0 ? iyp : N + iyp) * H + sx;
+#pragma unroll
+                for (int lx = 0; lx < 4; ++lx) {
+                    const float wg = __fmul_rn(tx[lx], tyz);
+                    if (wg != 0.0f) {
+                        const float2 v = Xy[lx];
+                        const double w = (double)wg;
+                        dre += w * (double)v.x;
+                        dim += w * (double)v.y;
+                    }
+                }
+            }
+        }
+        // the general form adds the (empty) reflected sums: + 0 and - 0
+        const double re = dre + 0.0, im = dim - 0.0;
+        *o = make_float2((float)re, (float)(flip ? -im : im));
+        return;
+    }
     float tx[5], ty[5], tz[5];
 #pragma unroll
     for (int l = 0; l < 5; ++l) {
@@ -43,7 +89,6 @@ __global__ __launch_bounds__(TPB) void gather_kernel(const float2* __restrict__ 
         ty[l] = kb_weight(tab, __fsub_rn(y, (float)(iy + l - LN2)), fltb);
         tz[l] = kb_weight(tab, __fsub_rn(z, (float)(iz + l - LN2)), fltb);
     }
-    const size_t plane = (size_t)N * H;
     double dre = 0.0, dim = 0.0, mre = 0.0, mim = 0.0;
     for (int lz = 0; lz < 5; ++lz) {
         if (tz[lz] == 0.0f) continue;
