@@ -96,7 +96,7 @@ struct spbp_ctx {
     // brick strategy
     int strategy = SPBP_STRAT_BRICK;
     int shape = 3;   // index into SHAPES
-    int C = 64;
+    int C = 128;
     bool tuned = false;
     bool capturing = false;   // a batch graph is being captured: no host syncs, no timing events
     // symmetries (BUILDS): each image is inserted once per matrix, SYMVOL after the window
@@ -513,24 +513,38 @@ int pad_fft(spbp_ctx* c, const void* src, int dtype, int b, hipStream_t st) {
     if (c->fft_log2) {
         const size_t lds = fft_lds_bytes(c->N);
         const int M = std::min(FFT_ELEMS / c->N, c->nx / 2);
-        const dim3 rgrid((c->nx / 2 + M - 1) / M, b);
+        const int cw = FFT_ELEMS / c->N;
+        const size_t npix = (size_t)c->nx * c->nx, nspec = (size_t)c->N * c->H;
+        const size_t esz = dtype == SPBP_BF16 ? 2 : 4;
+        // sub-batches whose row-pass output (~128 MiB) is still in the
+        // last-level cache when the column pass reads it back
+        const int sb = (int)std::max<size_t>(1, ((size_t)128 << 20) / ((size_t)c->nx * c->H * sizeof(float2)));
         {
-            Timed tm(c, SPBP_K_PAD, b, st);
-            if (dtype == SPBP_BF16) {
+            Timed tm(c, SPBP_K_PAD, b, st);   // the means of the whole chunk (one block per image)
+            if (dtype == SPBP_BF16)
                 pad_mean_kernel<uint16_t><<<b, TPB, 0, st>>>((const uint16_t*)src, c->nx, c->fmean, c->vmax);
-                row_fft_kernel<uint16_t><<<rgrid, TPB, lds, st>>>((const uint16_t*)src, c->nx, c->N, c->fft_log2, c->tw,
-                                                                  c->fmean, c->spec);
-            } else {
+            else
                 pad_mean_kernel<float><<<b, TPB, 0, st>>>((const float*)src, c->nx, c->fmean, c->vmax);
-                row_fft_kernel<float><<<rgrid, TPB, lds, st>>>((const float*)src, c->nx, c->N, c->fft_log2, c->tw,
-                                                               c->fmean, c->spec);
-            }
         }
-        {
-            Timed tm(c, SPBP_K_FFT2, b, st);
-            const int cw = FFT_ELEMS / c->N;
-            col_fft_kernel<<<dim3((c->H + cw - 1) / cw, b), TPB, lds, st>>>(c->nx, c->N, c->fft_log2, c->tw, c->spec,
-                                                                           c->vmax);
+        for (int s0 = 0; s0 < b; s0 += sb) {
+            const int n = std::min(sb, b - s0);
+            const void* im = (const char*)src + s0 * npix * esz;
+            float2* sp = c->spec + s0 * nspec;
+            {
+                Timed tm(c, SPBP_K_PAD, n, st);
+                const dim3 rgrid((c->nx / 2 + M - 1) / M, n);
+                if (dtype == SPBP_BF16)
+                    row_fft_kernel<uint16_t><<<rgrid, TPB, lds, st>>>((const uint16_t*)im, c->nx, c->N, c->fft_log2, c->tw,
+                                                                      c->fmean + s0, sp);
+                else
+                    row_fft_kernel<float><<<rgrid, TPB, lds, st>>>((const float*)im, c->nx, c->N, c->fft_log2, c->tw,
+                                                                   c->fmean + s0, sp);
+            }
+            {
+                Timed tm(c, SPBP_K_FFT2, n, st);
+                col_fft_kernel<<<dim3((c->H + cw - 1) / cw, n), TPB, lds, st>>>(c->nx, c->N, c->fft_log2, c->tw, sp,
+                                                                               c->vmax + s0);
+            }
         }
         HIPCHK(c, hipGetLastError());
         return SPBP_OK;
@@ -936,15 +950,17 @@ int spbp_create(const spbp_opts* o, spbp_ctx** out) {
     if (o->batch > 0) {
         c->batch = o->batch;
     } else {
-        const size_t per = (size_t)c->N * c->N * 4 + (size_t)c->N * c->H * 8;
-        // 2 GiB of padded images + spectra per chunk (of 288 GB): larger chunks
-        // mean fewer list builds and fewer persistent-kernel tails
-        c->batch = (int)std::max<size_t>(16, std::min<size_t>(4096, ((size_t)2048 << 20) / per));
+        // 2 GiB of chunk buffers (of 288 GB): the spectra, plus the padded
+        // images when hipFFT does the transforms.  Larger chunks mean fewer
+        // list builds and fewer persistent-kernel tails.
+        const bool own = (c->N & (c->N - 1)) == 0 && c->N >= 8 && c->N <= 2048 && !(o->flags & SPBP_HIPFFT);
+        const size_t per = (own ? 0 : (size_t)c->N * c->N * 4) + (size_t)c->N * c->H * 8;
+        c->batch = (int)std::max<size_t>(16, std::min<size_t>(8192, ((size_t)2048 << 20) / per));
     }
     c->strategy = o->strategy == SPBP_STRAT_DIRECT ? SPBP_STRAT_DIRECT : SPBP_STRAT_BRICK;
     if (o->strategy < 0 || o->strategy > SPBP_STRAT_BRICK) { delete c; return fail(nullptr, SPBP_ERR_ARG, "unknown strategy %d", o->strategy); }
     if (o->brick_proj > 128) { delete c; return fail(nullptr, SPBP_ERR_ARG, "brick_proj %d above 128", o->brick_proj); }
-    c->C = o->brick_proj > 0 ? o->brick_proj : 64;
+    c->C = o->brick_proj > 0 ? o->brick_proj : 128;
     {
         int n = 0;
         if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0) c->ncu = n;
@@ -988,7 +1004,7 @@ int spbp_create(const spbp_opts* o, spbp_ctx** out) {
             return bail(fail(nullptr, SPBP_ERR_ALLOC, "twiddle table"));
         c->fft_log2 = L;
     }
-    // default plan: 16x12x8 bricks, 64 projections per item (fastest measured at 128^2 into 256^3 and
+    // default plan: 16x12x8 bricks, 128 projections per item (fastest measured at 128^2 into 256^3 and
     // 256^2 into 512^3; spbp_tune picks per workload)
     if ((rc = configure_bricks(c, 3))) { g_last_error = c->err; return bail(rc); }
     rc = spbp_reset(c, nullptr);
