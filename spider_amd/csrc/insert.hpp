@@ -321,6 +321,13 @@ struct BrickLds {
     float vmax[TPB / 64];
 };
 
+#ifndef SPBP_LANE_SPREAD
+#define SPBP_LANE_SPREAD 2
+#endif
+// power of two <= 64.  Measured at 128^2 into 256^3 (insertion ms per 20k
+// projections): 1 (consecutive lanes, consecutive candidates) 55.0, 2 53.2,
+// 4 53.6, 8 54.2, 16 54.3
+constexpr int LANE_SPREAD = SPBP_LANE_SPREAD;
 constexpr int VBITS = 48;                   // |term| < 2^48 units; item sums < 2^62
 constexpr int WBITS = 50;                   // weight terms <= 1 < 2^51 / 2^WBITS; item sums < 2^63
 constexpr double WSCALE = 1125899906842624.0;            // 2^WBITS
@@ -420,7 +427,9 @@ __device__ __forceinline__ void wave_project(BrickLds<BX, BY, BZ>& L, const Geom
     }
     const int total = __shfl(pref, 63, 64);
     for (int q0 = 0; q0 < total; q0 += 64) {
-        const int q = q0 + lane;
+        // lanes of one 16-lane LDS pass take candidates LANE_SPREAD apart
+        // (samples further apart in the brick: fewer bank and address conflicts)
+        const int q = q0 + (lane & (64 / LANE_SPREAD - 1)) * LANE_SPREAD + lane / (64 / LANE_SPREAD);
         // line of candidate q: smallest l with pref[l] > q (every lane searches)
         int l = 0;
 #pragma unroll
