@@ -116,8 +116,9 @@ __device__ inline void lds_fft(float2* buf, int M, int N, int L, int S, const fl
 // PADD2's mean: FP64 mean of the pixels outside the inscribed circle
 // (asta.f, bp3f.f:1316); block per image.  Also zeroes the image's vmax.
 template <typename T>
-__global__ __launch_bounds__(TPB) void pad_mean_kernel(const T* __restrict__ imgs, int nx, double* __restrict__ mean,
-                                                       float* __restrict__ vmax) {
+__global__ __launch_bounds__(TPB) void pad_mean_kernel(const T* __restrict__ imgs, int nx, int lnx,
+                                                       double* __restrict__ mean, float* __restrict__ vmax) {
+    // nx = 2^lnx
     const int b = blockIdx.x;
     const T* img = imgs + (size_t)b * nx * nx;
     const int c = nx / 2, n = nx * nx;
@@ -135,7 +136,7 @@ __global__ __launch_bounds__(TPB) void pad_mean_kernel(const T* __restrict__ img
 #pragma unroll
         for (int u = 0; u < MB; ++u) {
             const int idx = i0 + u * TPB;
-            const int j = idx / nx, i = idx - j * nx;
+            const int j = idx >> lnx, i = idx & (nx - 1);
             const float tj = (float)(j - c), ti = (float)(i - c);
             if (idx < n && tj * tj + ti * ti > r2) { s += (double)v[u]; ++cnt; }
         }
@@ -167,7 +168,7 @@ __global__ __launch_bounds__(TPB) void row_fft_kernel(const T* __restrict__ imgs
 #pragma unroll
         for (int u = 0; u < FFT_BATCH; ++u) {
             const int idx = i0 + u * TPB;
-            const int m = idx / N, x = idx - m * N - ip;
+            const int m = idx >> L, x = (idx & (N - 1)) - ip;
             const int r = 2 * (p0 + m);
             v[u] = make_float2(0.f, 0.f);
             if (idx < total && (unsigned)x < (unsigned)nx) {
@@ -184,8 +185,11 @@ __global__ __launch_bounds__(TPB) void row_fft_kernel(const T* __restrict__ imgs
     // Z = FFT(x_r + i x_{r+1}): X_r[k] = (Z[k] + conj Z[N-k]) / 2,
     // X_{r+1}[k] = -i (Z[k] - conj Z[N-k]) / 2
     float2* out = spec + (size_t)b * N * H;
-    for (int idx = threadIdx.x; idx < mc * H; idx += blockDim.x) {
-        const int m = idx / H, k = idx - m * H;
+    const int hn = N >> 1;   // k = 0 .. N/2 - 1 by shifts, then the N/2 column of each pair
+    for (int idx = threadIdx.x; idx < mc * (hn + 1); idx += blockDim.x) {
+        int m, k;
+        if (idx < mc * hn) { m = idx >> (L - 1); k = idx & (hn - 1); }
+        else { m = idx - mc * hn; k = hn; }
         const float2 z = buf[m * N + k], w = buf[m * N + ((N - k) & (N - 1))];
         const int r = 2 * (p0 + m);
         out[(size_t)(ip + r) * H + k] = make_float2(0.5f * __fadd_rn(z.x, w.x), 0.5f * __fsub_rn(z.y, w.y));
@@ -208,31 +212,37 @@ __global__ __launch_bounds__(TPB) void col_fft_kernel(int nx, int N, int L, cons
     const int CW = FFT_ELEMS / N, c0 = blockIdx.x * CW, cw = min(CW, H - c0);
     const int S = N + 1;   // odd column stride: a row of the slab spreads over the banks
     float2* sp = spec + (size_t)b * N * H + c0;
-    const int total = cw * nx;   // only rows IP .. IP + NX - 1 are nonzero
+    // element idx of the slab: row idx >> lc, column idx & (cp - 1), with cp
+    // the power of two >= cw (columns >= cw idle)
+    int lc = 0;
+    while ((1 << lc) < cw) ++lc;
+    const int cp = 1 << lc;
+    const int total = cp * nx;   // only rows IP .. IP + NX - 1 are nonzero
     for (int i0 = threadIdx.x; i0 < total; i0 += FFT_BATCH * TPB) {
         float2 v[FFT_BATCH];
 #pragma unroll
         for (int u = 0; u < FFT_BATCH; ++u) {
             const int idx = i0 + u * TPB;
-            const int n = idx / cw, m = idx - n * cw;
-            v[u] = idx < total ? sp[(size_t)(ip + n) * H + m] : make_float2(0.f, 0.f);
+            const int n = idx >> lc, m = idx & (cp - 1);
+            v[u] = (idx < total && m < cw) ? sp[(size_t)(ip + n) * H + m] : make_float2(0.f, 0.f);
         }
 #pragma unroll
         for (int u = 0; u < FFT_BATCH; ++u) {
             const int idx = i0 + u * TPB;
-            const int n = idx / cw, m = idx - n * cw;
-            if (idx < total) buf[m * S + ip + n] = v[u];
+            const int n = idx >> lc, m = idx & (cp - 1);
+            if (idx < total && m < cw) buf[m * S + ip + n] = v[u];
         }
     }
-    for (int idx = threadIdx.x; idx < cw * (N - nx); idx += blockDim.x) {   // the zero rows
-        const int n = idx / cw, m = idx - n * cw;
-        buf[m * S + (n < ip ? n : n + nx)] = make_float2(0.f, 0.f);
+    for (int idx = threadIdx.x; idx < cp * (N - nx); idx += blockDim.x) {   // the zero rows
+        const int n = idx >> lc, m = idx & (cp - 1);
+        if (m < cw) buf[m * S + (n < ip ? n : n + nx)] = make_float2(0.f, 0.f);
     }
     __syncthreads();
     lds_fft(buf, cw, N, L, S, stw);
     float mx = 0.0f;
-    for (int idx = threadIdx.x; idx < cw * N; idx += blockDim.x) {
-        const int n = idx / cw, m = idx - n * cw;
+    for (int idx = threadIdx.x; idx < cp * N; idx += blockDim.x) {
+        const int n = idx >> lc, m = idx & (cp - 1);
+        if (m >= cw) continue;
         const float2 v = buf[m * S + n];
         sp[(size_t)n * H + m] = v;
         mx = fmaxf(mx, fmaxf(fabsf(v.x), fabsf(v.y)));

@@ -522,9 +522,9 @@ int pad_fft(spbp_ctx* c, const void* src, int dtype, int b, hipStream_t st) {
         {
             Timed tm(c, SPBP_K_PAD, b, st);   // the means of the whole chunk (one block per image)
             if (dtype == SPBP_BF16)
-                pad_mean_kernel<uint16_t><<<b, TPB, 0, st>>>((const uint16_t*)src, c->nx, c->fmean, c->vmax);
+                pad_mean_kernel<uint16_t><<<b, TPB, 0, st>>>((const uint16_t*)src, c->nx, c->fft_log2 - 1, c->fmean, c->vmax);
             else
-                pad_mean_kernel<float><<<b, TPB, 0, st>>>((const float*)src, c->nx, c->fmean, c->vmax);
+                pad_mean_kernel<float><<<b, TPB, 0, st>>>((const float*)src, c->nx, c->fft_log2 - 1, c->fmean, c->vmax);
         }
         for (int s0 = 0; s0 < b; s0 += sb) {
             const int n = std::min(sb, b - s0);
