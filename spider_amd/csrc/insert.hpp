@@ -306,6 +306,10 @@ struct BrickLds {
     __device__ static int rowbase(int y, int z) { return z * SZ + y * SY; }
     __device__ static int rot(int y, int z) { return ROT ? RA * y + RB * z : 0; }
     __device__ static int at(int base, int r, int x) { return ROT ? base + ((x + r) & (BX - 1)) : base + x; }
+    // the same in bytes (8-byte elements): the tap loop's addresses need no scaling
+    __device__ static int rowbase8(int y, int z) { return (z * SZ + y * SY) * 8; }
+    __device__ static int rot8(int y, int z) { return ROT ? (RA * y + RB * z) * 8 : 0; }
+    __device__ static int at8(int base8, int r8, int x8) { return ROT ? base8 + ((x8 + r8) & ((BX - 1) * 8)) : base8 + x8; }
     // gfx950's LDS runs ds_add_u64 at 0.14 cycles per lane, ds_add_f64 at
     // 0.26 and ds_add_f32 at ~3 (measured).  X partial sums are exact 64-bit
     // fixed point: each per-tap term is the exact product of the fp32
@@ -345,6 +349,10 @@ __device__ inline unsigned long long to_fixed_prod(double a, double b) {
     return ((unsigned long long)hi << 32) | (unsigned)r;
 }
 __device__ inline double from_fixed(unsigned long long a) { return (double)(long long)a; }
+// LDS add at a byte offset into one of the brick arrays
+__device__ inline void lds_add(unsigned long long* arr, int ob, unsigned long long v) {
+    atomicAdd((unsigned long long*)((char*)arr + ob), v);
+}
 
 // Per image of a chunk: the largest |Re| or |Im| of its raw R2C spectrum
 // (the magnitudes the insertion's terms are formed from).  Block per image.
@@ -381,7 +389,8 @@ struct Box {
 template <int BX, int BY, int BZ, bool MIR, typename P>
 __device__ __forceinline__ void wave_project(BrickLds<BX, BY, BZ>& L, const Geom& g, const float2* __restrict__ spec_p,
                                              int nx, int N, int x0, int y0, int z0, const Box box, bool exclude,
-                                             const Box ex, const float* __restrict__ tab, P fltb, double vscale) {
+                                             const Box ex, const float* __restrict__ tab, P fltb, double vscale,
+                                             double wscale) {
     const float* lo = box.lo;
     const float* hi = box.hi;
     const float* elo = ex.lo;
@@ -478,7 +487,7 @@ __device__ __forceinline__ void wave_project(BrickLds<BX, BY, BZ>& L, const Geom
             // Targets inside the brick, per axis: a tap p with p_x >= 0 is written
             // at p, a tap with p_x < 0 at -p (conjugated).  Offsets are
             // brick-local; out-of-brick positions get a flag instead of a branch.
-            int xo[4];
+            int xo[4], xo8[4];
             bool xok[4], xmir[4];
             double vim[4];
 #pragma unroll
@@ -486,6 +495,7 @@ __device__ __forceinline__ void wave_project(BrickLds<BX, BY, BZ>& L, const Geom
                 const int px = sx + t;
                 xmir[t] = px < 0;
                 xo[t] = (xmir[t] ? -px : px) - x0;
+                xo8[t] = xo[t] * 8;
                 xok[t] = (unsigned)xo[t] < (unsigned)BX;
                 vim[t] = xmir[t] ? -vys : vys;
             }
@@ -500,30 +510,31 @@ __device__ __forceinline__ void wave_project(BrickLds<BX, BY, BZ>& L, const Geom
                     const int yd = py - y0, ym = -py - y0;
                     const bool dok = zdok && (unsigned)yd < (unsigned)BY;
                     const bool mok = zmok && (unsigned)ym < (unsigned)BY;
-                    const int offd = L.rowbase(yd, zd), offm = L.rowbase(ym, zm);
-                    const int rd = L.rot(yd, zd), rm = L.rot(ym, zm);
+                    const int offd = L.rowbase8(yd, zd), offm = L.rowbase8(ym, zm);
+                    const int rd = L.rot8(yd, zd), rm = L.rot8(ym, zm);
                     const float tyz = __fmul_rn(ty[ly], tz[lz]);       // TY = TABI(Y) * TZ
 #pragma unroll
                     for (int lx = 0; lx < 4; ++lx) {
                         const float wg = __fmul_rn(tx[lx], tyz);       // WG = TABI(X) * TY
                         const bool ok = xok[lx] && (xmir[lx] ? mok : dok) && wg != 0.0f;
                         if (ok) {
-                            const int o = xmir[lx] ? L.at(offm, rm, xo[lx]) : L.at(offd, rd, xo[lx]);
+                            const int ob = xmir[lx] ? L.at8(offm, rm, xo8[lx]) : L.at8(offd, rd, xo8[lx]);
                             const double wd = (double)wg;
-                            atomicAdd(&L.re[o], to_fixed_prod(vxs, wd));
-                            atomicAdd(&L.im[o], to_fixed_prod(vim[lx], wd));
-                            atomicAdd(&L.w[o], to_fixed_prod(wd, WSCALE));
+                            lds_add(L.re, ob, to_fixed_prod(vxs, wd));
+                            lds_add(L.im, ob, to_fixed_prod(vim[lx], wd));
+                            lds_add(L.w, ob, to_fixed_prod(wd, wscale));
                         }
                     }
                 }
             }
         } else {
-            int xo[4];
+            int xo8[4];
             bool xok[4];
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
-                xo[t] = sx + t - x0;
-                xok[t] = (unsigned)xo[t] < (unsigned)BX;
+                const int xo = sx + t - x0;
+                xo8[t] = xo * 8;
+                xok[t] = (unsigned)xo < (unsigned)BX;
             }
 #pragma unroll
             for (int lz = 0; lz < 4; ++lz) {
@@ -533,17 +544,17 @@ __device__ __forceinline__ void wave_project(BrickLds<BX, BY, BZ>& L, const Geom
                 for (int ly = 0; ly < 4; ++ly) {
                     const int yd = sy + ly - y0;
                     const bool dok = zdok && (unsigned)yd < (unsigned)BY;
-                    const int offd = L.rowbase(yd, zd), rd = L.rot(yd, zd);
+                    const int offd = L.rowbase8(yd, zd), rd = L.rot8(yd, zd);
                     const float tyz = __fmul_rn(ty[ly], tz[lz]);       // TY = TABI(Y) * TZ
 #pragma unroll
                     for (int lx = 0; lx < 4; ++lx) {
                         const float wg = __fmul_rn(tx[lx], tyz);       // WG = TABI(X) * TY
                         if (xok[lx] && dok && wg != 0.0f) {
-                            const int o = L.at(offd, rd, xo[lx]);
+                            const int ob = L.at8(offd, rd, xo8[lx]);
                             const double wd = (double)wg;
-                            atomicAdd(&L.re[o], to_fixed_prod(vxs, wd));
-                            atomicAdd(&L.im[o], to_fixed_prod(vys, wd));
-                            atomicAdd(&L.w[o], to_fixed_prod(wd, WSCALE));
+                            lds_add(L.re, ob, to_fixed_prod(vxs, wd));
+                            lds_add(L.im, ob, to_fixed_prod(vys, wd));
+                            lds_add(L.w, ob, to_fixed_prod(wd, wscale));
                         }
                     }
                 }
@@ -555,6 +566,7 @@ __device__ __forceinline__ void wave_project(BrickLds<BX, BY, BZ>& L, const Geom
 // One insertion job for the brick kernel: a context's chunk of projections
 // (spectra, geometry), its table, brick grid and lists, and its half grids.
 struct BrickJob {
+    double wscale;        // 2^WBITS, the weights' fixed-point scale (a kernel argument: kept in registers)
     const float2* spec;   // spectra of images img0, img0 + 1, ...
     const float* vmax;    // largest |Re|, |Im| of each of those spectra
     int img0;
@@ -639,16 +651,17 @@ void insert_brick_kernel(const BrickJob* __restrict__ jobs, int njobs, int* __re
         Box mb;
         for (int k = 0; k < 3; ++k) { mb.lo[k] = -bx.hi[k]; mb.hi[k] = -bx.lo[k]; }
         const float* __restrict__ tab = J.tab;
+        const double wsc = J.wscale;
         const P fltb = fltb_of<P>(J.fltb);
         for (int s = wave; s < ns; s += nwave) {
             const int p = lst[s];
             const Geom g = J.geom[p];
             const float2* spec_p = J.spec + (size_t)(g.img - J.img0) * N * H;
             if (x0 <= LN2) {
-                wave_project<BX, BY, BZ, true, P>(L, g, spec_p, nx, N, x0, y0, z0, bx, false, bx, tab, fltb, vscale);
-                wave_project<BX, BY, BZ, true, P>(L, g, spec_p, nx, N, x0, y0, z0, mb, true, bx, tab, fltb, vscale);
+                wave_project<BX, BY, BZ, true, P>(L, g, spec_p, nx, N, x0, y0, z0, bx, false, bx, tab, fltb, vscale, wsc);
+                wave_project<BX, BY, BZ, true, P>(L, g, spec_p, nx, N, x0, y0, z0, mb, true, bx, tab, fltb, vscale, wsc);
             } else {
-                wave_project<BX, BY, BZ, false, P>(L, g, spec_p, nx, N, x0, y0, z0, bx, false, bx, tab, fltb, vscale);
+                wave_project<BX, BY, BZ, false, P>(L, g, spec_p, nx, N, x0, y0, z0, bx, false, bx, tab, fltb, vscale, wsc);
             }
         }
         __syncthreads();

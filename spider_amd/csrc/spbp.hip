@@ -400,6 +400,7 @@ struct Targets {
 
 BrickJob brick_job(const spbp_ctx* c, int img0, const Geom* geom_d, const Targets& t) {
     BrickJob j;
+    j.wscale = WSCALE;
     j.spec = c->spec;
     j.vmax = c->vmax;
     j.img0 = img0;
