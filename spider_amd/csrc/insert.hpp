@@ -291,7 +291,7 @@ __global__ __launch_bounds__(TPB) void brick_itemkey_kernel(const int* __restric
 #define SPBP_LDS_ROT_B 3
 #endif
 
-template <int BX, int BY, int BZ>
+template <int BX, int BY, int BZ, int NT = TPB>
 struct BrickLds {
     static_assert(BX <= 31 && BY <= 31 && BZ <= 31, "brick edge too long for the 64-lane line table");
     // Layout of the brick's voxels.  Padded: rows of BX + 1, layers of
@@ -322,7 +322,7 @@ struct BrickLds {
     // absolute (relative 10^-14 where NRMW2 divides, W > 0.1).
     unsigned long long re[NS], im[NS], w[NS];
     int item;
-    float vmax[TPB / 64];
+    float vmax[NT / 64];
 };
 
 #ifndef SPBP_LANE_SPREAD
@@ -386,8 +386,8 @@ struct Box {
 // MIR: the brick starts at kx <= LN2, so taps with p_x < 0 can land in it
 // mirrored (at -p_x <= LN2); elsewhere they never do and the tap loop drops
 // the per-tap mirror selects.
-template <int BX, int BY, int BZ, bool MIR, typename P>
-__device__ __forceinline__ void wave_project(BrickLds<BX, BY, BZ>& L, const Geom& g, const float2* __restrict__ spec_p,
+template <int BX, int BY, int BZ, bool MIR, typename P, int NT>
+__device__ __forceinline__ void wave_project(BrickLds<BX, BY, BZ, NT>& L, const Geom& g, const float2* __restrict__ spec_p,
                                              int nx, int N, int x0, int y0, int z0, const Box box, bool exclude,
                                              const Box ex, const float* __restrict__ tab, P fltb, double vscale,
                                              double wscale) {
@@ -590,16 +590,16 @@ struct BrickJob {
 constexpr int MAX_JOBS = 64;
 
 // Persistent workgroups pull (job, key, projection slice) items from one
-// queue spanning every job of the launch (job-major order); the four waves of
+// queue spanning every job of the launch (job-major order); the waves of
 // a workgroup take the item's projections in turn and share the brick
 // through LDS atomics.  The queue head must be zero at launch.
 // The fp32-geometry instantiations are held to 128 VGPRs (4 waves per SIMD);
 // the FP64-geometry ones need more and run 3.
-template <int BX, int BY, int BZ, typename P>
-__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(sizeof(P) == 4 ? 4 : 3)))
+template <int BX, int BY, int BZ, typename P, int NT = TPB>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(P) == 4 ? 4 : 3)))
 void insert_brick_kernel(const BrickJob* __restrict__ jobs, int njobs, int* __restrict__ next_item) {
     constexpr int NV = BX * BY * BZ;
-    __shared__ BrickLds<BX, BY, BZ> L;
+    __shared__ BrickLds<BX, BY, BZ, NT> L;
     __shared__ int jbase[MAX_JOBS + 1];
     if (threadIdx.x == 0) {
         int acc = 0;
@@ -658,10 +658,10 @@ void insert_brick_kernel(const BrickJob* __restrict__ jobs, int njobs, int* __re
             const Geom g = J.geom[p];
             const float2* spec_p = J.spec + (size_t)(g.img - J.img0) * N * H;
             if (x0 <= LN2) {
-                wave_project<BX, BY, BZ, true, P>(L, g, spec_p, nx, N, x0, y0, z0, bx, false, bx, tab, fltb, vscale, wsc);
-                wave_project<BX, BY, BZ, true, P>(L, g, spec_p, nx, N, x0, y0, z0, mb, true, bx, tab, fltb, vscale, wsc);
+                wave_project<BX, BY, BZ, true, P, NT>(L, g, spec_p, nx, N, x0, y0, z0, bx, false, bx, tab, fltb, vscale, wsc);
+                wave_project<BX, BY, BZ, true, P, NT>(L, g, spec_p, nx, N, x0, y0, z0, mb, true, bx, tab, fltb, vscale, wsc);
             } else {
-                wave_project<BX, BY, BZ, false, P>(L, g, spec_p, nx, N, x0, y0, z0, bx, false, bx, tab, fltb, vscale, wsc);
+                wave_project<BX, BY, BZ, false, P, NT>(L, g, spec_p, nx, N, x0, y0, z0, bx, false, bx, tab, fltb, vscale, wsc);
             }
         }
         __syncthreads();

@@ -50,8 +50,13 @@ using namespace spbp;
 namespace {
 thread_local std::string g_last_error;
 // brick shapes of the LDS-binned strategy (voxels; 3 x 8 B per voxel in LDS)
-constexpr int NSHAPES = 4;
-constexpr int SHAPES[NSHAPES][3] = {{16, 16, 8}, {16, 8, 8}, {8, 8, 8}, {16, 12, 8}};
+constexpr int NSHAPES = 7;
+constexpr int SHAPES[NSHAPES][3] = {{16, 16, 8}, {16, 8, 8}, {8, 8, 8}, {16, 12, 8}, {16, 16, 10}, {16, 16, 12}, {16, 16, 13}};
+// brick workgroups: 16x16x10, 16x16x12 and 16x16x13 (61.7, 73.8 and 80.2 KB of
+// LDS) run two 512-thread groups per CU; the others four (three for 16x16x8)
+// 256-thread groups.  The FP64-geometry instantiations need more VGPRs than
+// 16 waves per CU allow: those contexts use 16x16x10 for the two largest
+// shapes.
 constexpr int NCVALS = 4;
 constexpr int CVALS[NCVALS] = {16, 32, 64, 128};   // projections per work item tried by the tuner
 }  // namespace
@@ -226,6 +231,7 @@ struct Timed {
 
 // Brick grid of shape `shape` and its per-key arrays.
 int configure_bricks(spbp_ctx* c, int shape) {
+    if ((c->flags & SPBP_GEOM_F64) && shape >= 5) shape = 4;
     const int bx = SHAPES[shape][0], by = SHAPES[shape][1], bz = SHAPES[shape][2];
     BrickGrid g;
     g.bx = bx;
@@ -424,7 +430,8 @@ unsigned brick_grid(const spbp_ctx* c) {
     // workgroups resident per CU: LDS-bound (52.7 KB) for the largest brick,
     // VGPR-bound (4 waves of 256-thread groups per SIMD) for the others
     // (the FP64-geometry instantiations need ~165 VGPRs: 3 per CU for every shape)
-    return (unsigned)c->ncu * ((c->shape == 0 || (c->flags & SPBP_GEOM_F64)) ? 3u : 4u);   // 16x12x8: 39.4 KB, 4 fit
+    if (c->shape >= 4) return (unsigned)c->ncu * ((c->flags & SPBP_GEOM_F64) ? 1u : 2u);
+    return (unsigned)c->ncu * ((c->shape == 0 || (c->flags & SPBP_GEOM_F64)) ? 3u : 4u);   // 16x12x8: 36.9 KB, 4 fit
 }
 
 // One persistent brick-kernel launch over the device job table table_d[0..n),
@@ -440,7 +447,12 @@ int launch_bricks_dev(spbp_ctx* c, const BrickJob* table_d, int n, int* counter,
         case 4: insert_brick_kernel<8, 8, 8, float><<<grid, TPB, 0, st>>>(table_d, n, counter); break;
         case 5: insert_brick_kernel<8, 8, 8, double><<<grid, TPB, 0, st>>>(table_d, n, counter); break;
         case 6: insert_brick_kernel<16, 12, 8, float><<<grid, TPB, 0, st>>>(table_d, n, counter); break;
-        default: insert_brick_kernel<16, 12, 8, double><<<grid, TPB, 0, st>>>(table_d, n, counter); break;
+        case 7: insert_brick_kernel<16, 12, 8, double><<<grid, TPB, 0, st>>>(table_d, n, counter); break;
+        case 8: insert_brick_kernel<16, 16, 10, float, 2 * TPB><<<grid, 2 * TPB, 0, st>>>(table_d, n, counter); break;
+        case 9: insert_brick_kernel<16, 16, 10, double, 2 * TPB><<<grid, 2 * TPB, 0, st>>>(table_d, n, counter); break;
+        case 10: insert_brick_kernel<16, 16, 12, float, 2 * TPB><<<grid, 2 * TPB, 0, st>>>(table_d, n, counter); break;
+        case 12: insert_brick_kernel<16, 16, 13, float, 2 * TPB><<<grid, 2 * TPB, 0, st>>>(table_d, n, counter); break;
+        default: return fail(c, SPBP_ERR_ARG, "brick shape %d has no FP64-geometry kernel", c->shape);
     }
     HIPCHK(c, hipGetLastError());
     return SPBP_OK;
@@ -1005,9 +1017,9 @@ int spbp_create(const spbp_opts* o, spbp_ctx** out) {
             return bail(fail(nullptr, SPBP_ERR_ALLOC, "twiddle table"));
         c->fft_log2 = L;
     }
-    // default plan: 16x12x8 bricks, 128 projections per item (fastest measured at 128^2 into 256^3 and
-    // 256^2 into 512^3; spbp_tune picks per workload)
-    if ((rc = configure_bricks(c, 3))) { g_last_error = c->err; return bail(rc); }
+    // default plan: 16x16x13 bricks (the largest that runs two 512-thread groups per CU), 128 projections
+    // per item: fastest measured at 128^2 into 256^3 and 256^2 into 512^3; spbp_tune picks per workload
+    if ((rc = configure_bricks(c, NSHAPES - 1))) { g_last_error = c->err; return bail(rc); }
     rc = spbp_reset(c, nullptr);
     if (rc) return bail(rc);
     if (hipDeviceSynchronize() != hipSuccess) return bail(fail(nullptr, SPBP_ERR_DEVICE, "zeroing accumulators failed"));
