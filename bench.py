@@ -90,6 +90,82 @@ def measured_traffic(kernel, nx, per_launch, images_dtype="f32", twosum=False):
     return None
 
 
+def measured_counters(kernel, nx, images_dtype="f32", twosum=False):
+    """The committed rocprofv3 counter record (profiles/atomics.json) of
+    `kernel` for the same NX, image storage and weight compensation: LDS
+    wave-instructions and global atomic requests per projection; None when no
+    measurement matches."""
+    path = os.path.join(ROOT, "profiles", "atomics.json")
+    try:
+        with open(path) as fh:
+            t = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    for rec in t.get("records", []):
+        if (rec.get("kernel") == kernel and rec.get("nx") == nx
+                and rec.get("images_dtype", "f32") == images_dtype and bool(rec.get("twosum", False)) == twosum):
+            return rec
+    return None
+
+
+def copy_rates():
+    """Device-to-device copy rates, read + write bytes per second (GB/s): one
+    2 GiB buffer copied 4 times (beyond the 256 MB last-level cache: HBM) and
+    one 64 MiB buffer copied 64 times (cache-resident).  Timed with events on
+    torch's current stream, the stream the copies run on."""
+    import torch
+    out = {}
+    for name, mib, reps in (("hbm", 2048, 4), ("cache", 64, 64)):
+        src = torch.empty(mib << 20, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            dst.copy_(src)
+        e1.record()
+        e1.synchronize()
+        out[name] = round(2.0 * (mib << 20) * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9, 1)
+        del src, dst
+    return out
+
+
+def atomic_roofline(kname, nx, per_launch, avg_launch_s, U, images_dtype, twosum):
+    """Atomic throughput of the insertion against measured peaks.  LDS: the
+    kernel's LDS wave-instructions per launch (rocprofv3 SQ_INSTS_LDS, from
+    the committed counter record) over the live launch time, against the
+    library's calibration of 64-bit LDS atomic adds.  Global: the algorithmic
+    atomic count per projection, 3 U (complex64 value and fp32 weight of each
+    distinct voxel; 4 U with compensated weights), against non-returning fp32
+    global atomics spread over 1 GiB; the kernel issues fewer (one flush per
+    brick per work item), as the counter record's L2 atomic count shows."""
+    from spider_amd import _lib
+    import ctypes as C
+    L = _lib.lib()
+    peaks = {}
+    for name, what, nbytes in (("lds", _lib.CAL_LDS_U64, 0), ("global", _lib.CAL_GLOBAL_F32, 1 << 30)):
+        r = C.c_double()
+        _lib.check(L.spbp_calibrate(what, nbytes, C.byref(r), None))
+        peaks[name] = r.value
+    rec = measured_counters(kname, nx, images_dtype, twosum)
+    a_unit = (4.0 if twosum else 3.0) * U
+    g_ach = a_unit * per_launch / avg_launch_s
+    out = {"global": {"unit": "atomics/s", "algorithmic_per_unit": round(a_unit), "achieved": round(g_ach, -6),
+                      "peak": round(peaks["global"], -6), "frac": round(g_ach / peaks["global"], 4),
+                      "peak_source": "spbp_calibrate: fp32 non-returning atomics over 1 GiB",
+                      "measured_l2_atomics_per_unit": None if rec is None else rec.get("l2_atomics_per_unit"),
+                      "measured_l2_atomics_per_s": None if rec is None else round(
+                          rec["l2_atomics_per_unit"] * per_launch / avg_launch_s, -6)},
+           "lds": {"unit": "wave-instructions/s", "peak": round(peaks["lds"], -6),
+                   "peak_source": "spbp_calibrate: ds_add_u64, 64 active lanes, no bank conflicts",
+                   "achieved": None, "frac": None, "per_unit": None}}
+    if rec is not None and rec.get("lds_instr_per_unit"):
+        l_ach = rec["lds_instr_per_unit"] * per_launch / avg_launch_s
+        out["lds"].update(achieved=round(l_ach, -6), frac=round(l_ach / peaks["lds"], 4),
+                          per_unit=rec["lds_instr_per_unit"])
+    return out
+
+
 def parity_subset(bpmod, imgs, ang, m, **kw):
     import numpy as np
     import oracle
@@ -313,6 +389,7 @@ def main():
     ap.add_argument("--bf16", action="store_true", help="store the images as BF16 (default for config 5)")
     ap.add_argument("--twosum", action="store_true", help="compensated weights (default for config 5)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-calib", action="store_true", help="skip the copy-rate and atomic-rate calibrations")
     ap.add_argument("--no-parity", action="store_true")
     args = ap.parse_args()
 
@@ -410,6 +487,12 @@ def main():
             "kernel": kname, "bytes_per_unit": round(b_alg), "unit_of_work": "projection",
             "units_per_launch": round(per_launch, 1), "avg_launch_ms": round(avg_launch_s * 1e3, 4),
             "U_distinct_voxels": round(U, 1)}
+    if not args.no_calib:
+        cr = copy_rates()
+        roof.update(copy_GBps=cr, frac_of_copy_hbm=round(achieved / cr["hbm"], 5),
+                    frac_of_copy_cache=round(achieved / cr["cache"], 5))
+        if plan["strategy"] == "brick":
+            roof["atomics"] = atomic_roofline(kname, nx, per_launch, avg_launch_s, U, "bf16" if bf16 else "f32", twosum)
 
     out = {"metric": METRIC, "value": round(value, 2), "unit": "projections/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 3),

@@ -158,6 +158,17 @@ int spbp_export_raw(spbp_ctx* ctx, int half, float* X, float* W, void* stream);
  * recorded events.  No counterpart in the reference. */
 int spbp_kernel_stats(spbp_ctx* ctx, int kernel, double* total_ms, long* launches, long* units);
 int spbp_reset_stats(spbp_ctx* ctx);
+/* Measured peaks for the insertion's atomic roofline (no counterpart in the
+ * reference), timed on the current device with HIP events on `stream`:
+ * SPBP_CAL_LDS_U64: 64-bit LDS atomic adds as the brick kernel issues them
+ * (512-thread groups, 64 active lanes, no bank conflicts), in wave-
+ * instructions per second; SPBP_CAL_GLOBAL_F32: non-returning fp32 global
+ * atomic adds, consecutive lanes on consecutive words, spread over `bytes`
+ * of device memory (at least 2 MiB per 256 CUs), in lane operations per
+ * second.  Allocates and frees its own buffer; synchronises the stream. */
+#define SPBP_CAL_LDS_U64    0
+#define SPBP_CAL_GLOBAL_F32 1
+int spbp_calibrate(int what, size_t bytes, double* rate, void* stream);
 
 /* NRMW2 (bp3f.f:1355-1366) divides where W > 0.1 and zeroes elsewhere, a
  * test that two summation orders can decide differently when W lies within

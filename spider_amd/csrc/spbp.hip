@@ -36,6 +36,7 @@
 #include "spbp.h"
 
 
+#include "calib.hpp"
 #include "common.hpp"
 #include "fft.hpp"
 #include "finalize.hpp"
@@ -1665,6 +1666,55 @@ int spbp_rotations(int n, const double* ang, int flags, float* dm) {
         cang((float)a[2], (float)a[1], (float)a[0], (flags & SPBP_EXACT_PI) != 0, dm + 9 * (size_t)p);
     }
     return SPBP_OK;
+}
+
+int spbp_calibrate(int what, size_t bytes, double* rate, void* stream) {
+    if (!rate || (what != SPBP_CAL_LDS_U64 && what != SPBP_CAL_GLOBAL_F32))
+        return fail(nullptr, SPBP_ERR_ARG, "bad calibration arguments (what %d)", what);
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        ncu <= 0)
+        return fail(nullptr, SPBP_ERR_DEVICE, "no device");
+    hipStream_t st = (hipStream_t)stream;
+    constexpr int LNT = 2 * TPB;                 // the brick kernel's default group size
+    const unsigned lblocks = 8u * (unsigned)ncu;  // four rounds of two groups per CU
+    const unsigned gblocks = 8u * (unsigned)ncu;
+    const size_t gwidth = (size_t)gblocks * TPB;
+    const size_t n = bytes / sizeof(float);
+    if (what == SPBP_CAL_GLOBAL_F32 && bytes > ((size_t)1 << 34))
+        return fail(nullptr, SPBP_ERR_ARG, "calibration buffer of %zu bytes above 16 GiB", bytes);
+    if (what == SPBP_CAL_GLOBAL_F32 && n < gwidth)
+        return fail(nullptr, SPBP_ERR_ARG, "calibration buffer of %zu bytes below %zu", bytes, gwidth * sizeof(float));
+    void* buf = nullptr;
+    const size_t nbytes = what == SPBP_CAL_LDS_U64 ? lblocks * sizeof(unsigned long long) : n * sizeof(float);
+    if (hipMalloc(&buf, nbytes) != hipSuccess) return fail(nullptr, SPBP_ERR_ALLOC, "calibration buffer");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = SPBP_OK;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = fail(nullptr, SPBP_ERR_DEVICE, "events");
+    float ms = 0.f;
+    if (rc == SPBP_OK && what == SPBP_CAL_GLOBAL_F32) {
+        fill_kernel<<<(unsigned)((n + TPB - 1) / TPB), TPB, 0, st>>>((float*)buf, n, 0.f);
+        cal_global_kernel<<<gblocks, TPB, 0, st>>>((float*)buf, n);   // warm-up
+    } else if (rc == SPBP_OK) {
+        cal_lds_kernel<LNT><<<lblocks, LNT, 0, st>>>((unsigned long long*)buf);
+    }
+    if (rc == SPBP_OK && hipEventRecord(e0, st) != hipSuccess) rc = fail(nullptr, SPBP_ERR_DEVICE, "event record");
+    if (rc == SPBP_OK) {
+        if (what == SPBP_CAL_GLOBAL_F32) cal_global_kernel<<<gblocks, TPB, 0, st>>>((float*)buf, n);
+        else cal_lds_kernel<LNT><<<lblocks, LNT, 0, st>>>((unsigned long long*)buf);
+        if (hipGetLastError() != hipSuccess || hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess || !(ms > 0.f))
+            rc = fail(nullptr, SPBP_ERR_DEVICE, "calibration launch failed");
+    }
+    if (rc == SPBP_OK) {
+        const double ops = what == SPBP_CAL_LDS_U64 ? (double)lblocks * (LNT / 64) * CAL_LDS_ITER
+                                                   : (double)gwidth * CAL_GLOBAL_ITER;
+        *rate = ops / (ms * 1e-3);
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(buf);
+    return rc;
 }
 
 int spbp_kernel_stats(spbp_ctx* c, int kernel, double* total_ms, long* launches, long* units) {

@@ -38,6 +38,9 @@ def test_create_rejects_bad_sizes_without_touching_a_device():
     assert L.spbp_finalize(None, None, None, None, None) == _lib.ERR_ARG
     assert L.spbp_export_partial(None, 0, None, None, None) == _lib.ERR_ARG
     assert L.spbp_kernel_stats(None, 0, None, None, None) == _lib.ERR_ARG
+    r = C.c_double()
+    assert L.spbp_calibrate(7, 0, C.byref(r), None) == _lib.ERR_ARG
+    assert L.spbp_calibrate(_lib.CAL_LDS_U64, 0, None, None) == _lib.ERR_ARG
     assert L.spbp_destroy(None) == 0
 
 

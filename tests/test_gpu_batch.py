@@ -151,3 +151,19 @@ def test_graph_replays_match_eager_batch(bpmod):
             g.close()
         for c in ctxs + eager:
             c.close()
+
+
+def test_calibrations_return_rates(bpmod):
+    """spbp_calibrate: positive, finite rates in a physically possible range
+    (256 CUs at 2.4 GHz issue at most 6.1e11 wave-instructions per second,
+    1.6e14 lane operations); undersized and oversized buffers refused."""
+    import ctypes as C
+    from spider_amd import _lib
+    L = _lib.lib()
+    r = C.c_double()
+    assert L.spbp_calibrate(_lib.CAL_LDS_U64, 0, C.byref(r), None) == 0
+    assert 1e9 < r.value < 6.2e11
+    assert L.spbp_calibrate(_lib.CAL_GLOBAL_F32, 1 << 28, C.byref(r), None) == 0
+    assert 1e9 < r.value < 1.6e14
+    assert L.spbp_calibrate(_lib.CAL_GLOBAL_F32, 4096, C.byref(r), None) == _lib.ERR_ARG
+    assert L.spbp_calibrate(_lib.CAL_GLOBAL_F32, (1 << 34) + 4, C.byref(r), None) == _lib.ERR_ARG
