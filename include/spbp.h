@@ -220,7 +220,8 @@ int spbp_set_symmetries(spbp_ctx* ctx, int nsym, const double* angles_deg);
 /* Plan = insertion strategy (SPBP_STRAT_DIRECT / _BRICK), brick shape
  * (0: 16x16x8, 1: 16x8x8, 2: 8x8x8, 3: 16x12x8, 4: 16x16x10, 5: 16x16x12,
  * 6: 16x16x13 voxels = the LDS bin; 6 is the default) and projections per
- * brick work item (default 128).  spbp_tune times every plan on a sample of
+ * brick work item (default 128; a launch of fewer than 8 x that many
+ * sections halves it, down to 16, so its busiest bricks still split).  spbp_tune times every plan on a sample of
  * up to 1024 of the given images (into scratch grids: the accumulators are
  * untouched) and keeps the fastest; spbp_get_plan reports the plan and, when
  * ms_per_proj is not NULL, the 29 measured ms per projection ([shape][C = 16,
