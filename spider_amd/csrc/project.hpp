@@ -6,6 +6,10 @@
 
 namespace spbp {
 
+// four floats at an 8-byte aligned address (global loads need only dword
+// alignment on gfx950)
+typedef float float4a8 __attribute__((ext_vector_type(4), aligned(8)));
+
 // For lattice sample (I, J) of projection p, ONELINE adds w * v to X at each
 // direct tap t (t_x >= 0) and w * conj(v) at -t for each reflected tap
 // (t_x < 0), where v is BI(I, J), conjugated when the sample is Friedel-
@@ -16,18 +20,22 @@ namespace spbp {
 // The taps, the table lookups and the fp32 weights w are computed exactly as
 // the insertion kernels compute them, and the sum is accumulated in FP64.
 // One thread per output entry; out is nproj x N x H in the BI layout (row
-// jp = J mod N, I fastest).
-__global__ __launch_bounds__(TPB) void gather_kernel(const float2* __restrict__ X, const Geom* __restrict__ geom,
-                                                     int nx, int N, const float* __restrict__ tab, float fltb,
-                                                     float2* __restrict__ out) {
+// jp = J mod N, I fastest).  A workgroup covers a GT x GT tile of (I, jp), so
+// its samples form a patch of the section and share most of their taps in
+// the caches; blockIdx.x = tile (I tiles fastest), blockIdx.y = projection.
+constexpr int GT = 16;
+__global__ __launch_bounds__(GT * GT) void gather_kernel(const float2* __restrict__ X, const Geom* __restrict__ geom,
+                                                         int nx, int N, const float* __restrict__ tab, float fltb,
+                                                         float2* __restrict__ out) {
     const int H = nx + 1;
     const int p = blockIdx.y;
-    const int lin = blockIdx.x * blockDim.x + threadIdx.x;
-    if (lin >= H * N) return;
-    const int jp = lin / H;
-    const int I = lin - jp * H;
+    const int ti = (H + GT - 1) / GT;
+    const int tj = blockIdx.x / ti;
+    const int I = (blockIdx.x - tj * ti) * GT + (threadIdx.x & (GT - 1));
+    const int jp = tj * GT + threadIdx.x / GT;
+    if (I >= H || jp >= N) return;
     const int J = jp <= nx ? jp : jp - N;
-    float2* o = out + (size_t)p * N * H + lin;
+    float2* o = out + ((size_t)p * N + jp) * H + I;
     if (!(I * I + J * J < (N * N) / 4) || (I == 0 && J < 0)) { *o = make_float2(0.f, 0.f); return; }
     const Geom g = geom[p];
     float x, y, z;
@@ -55,6 +63,9 @@ __global__ __launch_bounds__(TPB) void gather_kernel(const float2* __restrict__ 
             ty[t] = (t == 0 && sy == iy - 2) ? wy0 : kb_weight(tab, __fsub_rn(y, (float)(sy + t)), fltb);
             tz[t] = (t == 0 && sz == iz - 2) ? wz0 : kb_weight(tab, __fsub_rn(z, (float)(sz + t)), fltb);
         }
+        // each tap row is read whole (columns sx .. sx + 3 <= ix + 2 <= nx - 1,
+        // inside the row) as two 16-byte loads; zero-weight taps add exact
+        // zeros, so the sums equal those over the nonzero taps alone
         double dre = 0.0, dim = 0.0;
 #pragma unroll
         for (int lz = 0; lz < 4; ++lz) {
@@ -65,15 +76,13 @@ __global__ __launch_bounds__(TPB) void gather_kernel(const float2* __restrict__ 
                 const float tyz = __fmul_rn(ty[ly], tz[lz]);
                 const int iyp = sy + ly;
                 const float2* Xy = Xz + (size_t)(iyp >= 0 ? iyp : N + iyp) * H + sx;
+                const float4a8 a = *(const float4a8*)Xy, b = *(const float4a8*)(Xy + 2);
+                const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 #pragma unroll
                 for (int lx = 0; lx < 4; ++lx) {
-                    const float wg = __fmul_rn(tx[lx], tyz);
-                    if (wg != 0.0f) {
-                        const float2 v = Xy[lx];
-                        const double w = (double)wg;
-                        dre += w * (double)v.x;
-                        dim += w * (double)v.y;
-                    }
+                    const double w = (double)__fmul_rn(tx[lx], tyz);
+                    dre += w * (double)v[2 * lx];
+                    dim += w * (double)v[2 * lx + 1];
                 }
             }
         }

@@ -1108,11 +1108,11 @@ int spbp_gather(spbp_ctx* c, const float* X, int nimg, const double* angles_deg,
     int rc = stage_geom(c, nimg, dm, angles_deg, nullptr, st, false);
     if (rc) return rc;
     const int N = c->N, H = c->H;
-    const unsigned gx = nblocks((size_t)H * N);
+    const unsigned gx = (unsigned)(((H + GT - 1) / GT) * ((N + GT - 1) / GT));
     for (int p0 = 0; p0 < nimg; p0 += 65535) {
         const int b = std::min(65535, nimg - p0);
         Timed tm(c, SPBP_K_GATHER, b, st);
-        gather_kernel<<<dim3(gx, b), TPB, 0, st>>>((const float2*)X, c->geom_d + p0, c->nx, N, c->tab, c->fltb,
+        gather_kernel<<<dim3(gx, b), GT * GT, 0, st>>>((const float2*)X, c->geom_d + p0, c->nx, N, c->tab, c->fltb,
                                                    (float2*)out_bi + (size_t)p0 * N * H);
     }
     HIPCHK(c, hipGetLastError());
