@@ -322,7 +322,7 @@ def run_recon(args):
     res_halves = F.resolution(F.fsc(vols["h1"], vols["h2"]), 0.143)
     log(f"recon: BP 32F {t_bp:.3f} s, FSC(0.5) vs phantom at {res_bp:.4f}")
 
-    P = project.Projector(nx, angles=ang, batch=256, timing=True)
+    P = project.Projector(nx, angles=ang, batch=args.proj_batch, timing=True)
     curve = []
 
     def cb(it, x):
@@ -376,6 +376,8 @@ def main():
                          "4: CG reconstruction with the gather projector beside BP 32F")
     ap.add_argument("--volumes", type=int, default=12, help="config 3: number of volumes")
     ap.add_argument("--cg-iters", type=int, default=30)
+    ap.add_argument("--proj-batch", type=int, default=2048,
+                    help="config 4: projections per gather / scatter call of the explicit A and A^T")
     ap.add_argument("--nproj", type=int, default=None, help="override the projection count")
     ap.add_argument("--batch", type=int, default=0, help="projections per internal chunk (0 = auto)")
     ap.add_argument("--strategy", default="auto", choices=["auto", "direct", "brick"])

@@ -114,7 +114,7 @@ class Projector:
     (n, 9), or doc angles (n, 3) PSI, THETA, PHI)."""
 
     def __init__(self, nx: int, angles=None, dm=None, exact_pi: bool = False, device=None,
-                 strategy: str = "auto", batch: int = 256, timing: bool = False):
+                 strategy: str = "auto", batch: int = 2048, timing: bool = False):
         torch = _torch()
         self.nx = int(nx)
         self.N = 2 * self.nx
