@@ -483,7 +483,7 @@ def main():
     per_launch = units / launches
     achieved = b_alg * per_launch / avg_launch_s / 1e9
     kname = ("insert_direct_kernel" if plan["strategy"] == "direct"
-             else "insert_brick_kernel<%d,%d,%d>" % tuple(plan["brick"]))
+             else "insert_brick_kernel<%d,%d,%d%s>" % (*plan["brick"], ",own" if plan.get("owner_computes") else ""))
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(kname, nx, per_launch, "bf16" if bf16 else "f32", twosum),
             "kernel": kname, "bytes_per_unit": round(b_alg), "unit_of_work": "projection",
@@ -513,6 +513,7 @@ def main():
            "nrmw2_threshold": thresh,
            "roofline": roof,
            "plan": {"strategy": plan["strategy"], "brick": list(plan["brick"]), "brick_proj": plan["brick_proj"],
+                    "owner_computes": plan.get("owner_computes", False),
                     "tune_seconds": None if t_tune is None else round(t_tune, 3)}}
     if not args.no_parity:
         log("bench: parity subset vs oracle")

@@ -219,12 +219,15 @@ int spbp_set_symmetries(spbp_ctx* ctx, int nsym, const double* angles_deg);
  * projection loop is one OpenMP loop per operation, bp32f.f:802-913) ---- */
 /* Plan = insertion strategy (SPBP_STRAT_DIRECT / _BRICK), brick shape
  * (0: 16x16x8, 1: 16x8x8, 2: 8x8x8, 3: 16x12x8, 4: 16x16x10, 5: 16x16x12,
- * 6: 16x16x13 voxels = the LDS bin; 6 is the default) and projections per
+ * 6: 16x16x13 voxels = the LDS bin, every sample evaluated in each bin its
+ * taps reach; 7, the default: owner-computes bins, a 16x16x13 LDS region
+ * around a 12x12x9 core, each sample evaluated only in the bin whose core
+ * holds its nearest voxel) and projections per
  * brick work item (default 128; a launch of fewer than 8 x that many
  * sections halves it, down to 16, so its busiest bricks still split).  spbp_tune times every plan on a sample of
  * up to 1024 of the given images (into scratch grids: the accumulators are
  * untouched) and keeps the fastest; spbp_get_plan reports the plan and, when
- * ms_per_proj is not NULL, the 29 measured ms per projection ([shape][C = 16,
+ * ms_per_proj is not NULL, the 33 measured ms per projection ([shape][C = 16,
  * 32, 64, 128], then direct). */
 int spbp_tune(spbp_ctx* ctx, const void* imgs, int dtype, int nimg, const double* angles_deg, const float* dm,
               void* stream);

@@ -61,7 +61,9 @@ class Job(C.Structure):
                 ("angles_deg", C.c_void_p), ("dm", C.c_void_p), ("half", C.c_void_p)]
 
 
-SHAPES = [(16, 16, 8), (16, 8, 8), (8, 8, 8), (16, 12, 8), (16, 16, 10), (16, 16, 12), (16, 16, 13)]
+SHAPES = [(16, 16, 8), (16, 8, 8), (8, 8, 8), (16, 12, 8), (16, 16, 10), (16, 16, 12), (16, 16, 13), (16, 16, 13)]
+# owner-computes bricks (SHAPES gives the LDS region; each sample is evaluated in one brick only)
+SHAPE_OWN = [False] * 7 + [True]
 TUNE_C = [16, 32, 64, 128]
 
 

@@ -275,7 +275,8 @@ class Backprojector:
         self._check(_lib.lib().spbp_get_plan(self._ctx, C.byref(st), C.byref(sh), C.byref(cp), ms))
         table = np.array(ms[:-1], np.float64).reshape(len(_lib.SHAPES), len(_lib.TUNE_C))
         return {"strategy": {_lib.STRAT_DIRECT: "direct", _lib.STRAT_BRICK: "brick"}[st.value],
-                "brick": _lib.SHAPES[sh.value], "shape": sh.value, "brick_proj": cp.value,
+                "brick": _lib.SHAPES[sh.value], "owner_computes": _lib.SHAPE_OWN[sh.value], "shape": sh.value,
+                "brick_proj": cp.value,
                 "ms_per_proj": table, "direct_ms_per_proj": float(ms[-1])}
 
     def set_plan(self, strategy="brick", shape: int = 0, brick_proj: int = 0) -> None:

@@ -123,9 +123,10 @@ __global__ __launch_bounds__(TPB) void insert_direct_kernel(
 constexpr float BRICK_MARGIN = 1.86f;   // > 1.8: support of the KB table in voxels
 
 struct BrickGrid {
-    int bx, by, bz;      // brick edge (voxels)
+    int bx, by, bz;      // brick edge (voxels); the core's with owner-computes bricks
     int nbx, nby, nbz;   // bricks per axis
     int nb;              // nbx * nby * nbz
+    int own;             // owner-computes bricks (LDS region = core + 2 per side)
 };
 
 __device__ inline void brick_origin(const BrickGrid& bg, int b, int nx, int& x0, int& y0, int& z0) {
@@ -140,8 +141,11 @@ __device__ inline void brick_origin(const BrickGrid& bg, int b, int nx, int& x0,
 __device__ inline bool brick_hit(const BrickGrid& bg, int b, int nx, const Geom& g) {
     int x0, y0, z0;
     brick_origin(bg, b, nx, x0, y0, z0);
-    const float hx = 0.5f * (bg.bx - 1) + BRICK_MARGIN, hy = 0.5f * (bg.by - 1) + BRICK_MARGIN,
-                hz = 0.5f * (bg.bz - 1) + BRICK_MARGIN;
+    // owner-computes: samples whose nearest voxel is in the core (within 0.5
+    // of it), and at kx = 0 the mirrored samples within 2.5 of -core (a plane
+    // through the origin crosses -B exactly when it crosses B)
+    const float m = bg.own ? (x0 == 0 ? 2.52f : 0.52f) : BRICK_MARGIN;
+    const float hx = 0.5f * (bg.bx - 1) + m, hy = 0.5f * (bg.by - 1) + m, hz = 0.5f * (bg.bz - 1) + m;
     const float cx = x0 + 0.5f * (bg.bx - 1), cy = y0 + 0.5f * (bg.by - 1), cz = z0 + 0.5f * (bg.bz - 1);
     const float d = fabsf(g.r[6] * cx + g.r[7] * cy + g.r[8] * cz);
     const float reach = hx * fabsf(g.r[6]) + hy * fabsf(g.r[7]) + hz * fabsf(g.r[8]) + 0.02f;
@@ -563,6 +567,171 @@ __device__ __forceinline__ void wave_project(BrickLds<BX, BY, BZ, NT>& L, const 
     }
 }
 
+// Owner-computes bricks.  The LDS region is BX x BY x BZ voxels; its core
+// (BX - 4) x (BY - 4) x (BZ - 4) starts 2 voxels in from the region's origin.
+// A sample belongs to the brick whose core holds its nearest voxel (ix, iy,
+// iz) and is evaluated there only: its direct taps (p_x >= 0, p within 2 of
+// the nearest voxel per axis) all land in the region, so no tap is masked by
+// the brick's edge.  Reflected taps (p_x < 0, written at -p) go to the brick
+// whose core holds -p: bricks at kx = 0 run a second pass over the samples
+// whose nearest voxel lies within 2 of -core with ix <= 1.  Every (sample,
+// tap) pair is written once; regions overlap, and the flush adds each region
+// to the half grid.
+//
+// MIRPASS false: owned samples, direct taps (XCHK: drop taps with p_x < 0,
+// needed only at kx = 0).  MIRPASS true: the reflected taps whose target lies
+// in the core.  (x0, y0, z0): the core's origin.
+template <int BX, int BY, int BZ, bool MIRPASS, bool XCHK, typename P, int NT>
+__device__ __forceinline__ void wave_project_own(BrickLds<BX, BY, BZ, NT>& L, const Geom& g,
+                                                 const float2* __restrict__ spec_p, int nx, int N, int x0, int y0,
+                                                 int z0, const Box box, const float* __restrict__ tab, P fltb,
+                                                 double vscale, double wscale) {
+    constexpr int CX = BX - 4, CY = BY - 4, CZ = BZ - 4;
+    const float* lo = box.lo;
+    const float* hi = box.hi;
+    const int H = nx + 1;
+    const int lane = threadIdx.x & 63;
+    float bmn = 0.f, bmx = 0.f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float c = g.r[3 + k];
+        bmn += fminf(c * lo[k], c * hi[k]);
+        bmx += fmaxf(c * lo[k], c * hi[k]);
+    }
+    const int b0 = (int)ceilf(bmn - 0.01f), b1 = (int)floorf(bmx + 0.01f);
+    const int nl = b1 - b0 + 1;   // <= 64: box edge (<= 17) times sqrt(3), plus 2
+    int cnt = 0, alo = 0;
+    if (lane < nl) {
+        const float bb = (float)(b0 + lane);
+        float amin = -1e30f, amax = 1e30f;
+        bool empty = false;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float r1 = g.r[k], off = bb * g.r[3 + k];
+            if (fabsf(r1) > 1e-6f) {
+                const float u = (lo[k] - off) / r1, w = (hi[k] - off) / r1;
+                amin = fmaxf(amin, fminf(u, w));
+                amax = fminf(amax, fmaxf(u, w));
+            } else if (off < lo[k] - 0.01f || off > hi[k] + 0.01f) {
+                empty = true;
+            }
+        }
+        if (!empty) {
+            const int a0 = (int)ceilf(amin - 0.01f), a1 = (int)floorf(amax + 0.01f);
+            if (a1 >= a0) { alo = a0; cnt = a1 - a0 + 1; }
+        }
+    }
+    int pref = cnt;
+    for (int d = 1; d < 64; d <<= 1) {
+        const int u = __shfl_up(pref, d, 64);
+        if (lane >= d) pref += u;
+    }
+    const int total = __shfl(pref, 63, 64);
+    for (int q0 = 0; q0 < total; q0 += 64) {
+        const int q = q0 + (lane & (64 / LANE_SPREAD - 1)) * LANE_SPREAD + lane / (64 / LANE_SPREAD);
+        int l = 0;
+#pragma unroll
+        for (int step = 32; step > 0; step >>= 1) {
+            const int t = __shfl(pref, l + step - 1, 64);
+            if (t <= q) l += step;
+        }
+        const int lpref = __shfl(pref, l, 64), lcnt = __shfl(cnt, l, 64), lalo = __shfl(alo, l, 64);
+        if (q >= total) continue;
+        const int bb = b0 + l;
+        const int aa = lalo + (q - (lpref - lcnt));
+        const bool canon = aa > 0 || (aa == 0 && bb >= 0);
+        const int I = canon ? aa : -aa, J = canon ? bb : -bb;
+        if (I > nx || J < -nx + 1 || J > nx || !(I * I + J * J < nx * nx)) continue;   // oneline.f:53-55
+        P x, y, z;
+        sample_pos(g, I, J, x, y, z);
+        const bool flip = x < (P)0;                                    // oneline.f:60-67
+        if (flip == canon) continue;
+        if (flip) { x = -x; y = -y; z = -z; }
+        const int ix = nearest_voxel(x, N), iy = nearest_voxel(y, N), iz = nearest_voxel(z, N);
+        if (!in_bounds(ix, iy, iz, nx)) continue;                      // oneline.f:73-75
+        if constexpr (MIRPASS) {
+            // reflected taps exist only for ix <= 1; their targets -p lie within 2 of -(ix, iy, iz)
+            if (ix > 1 || -iy + 2 < y0 || -iy - 2 >= y0 + CY || -iz + 2 < z0 || -iz - 2 >= z0 + CZ) continue;
+        } else {
+            if (ix < x0 || ix >= x0 + CX || iy < y0 || iy >= y0 + CY || iz < z0 || iz >= z0 + CZ) continue;
+        }
+        float2 v = spectrum_value(spec_p, I, J, N, H);
+        if (flip) v.y = -v.y;
+        float tx[4], ty[4], tz[4];
+        const float wx0 = kb_weight(tab, dsub(x, ix - 2), fltb);
+        const float wy0 = kb_weight(tab, dsub(y, iy - 2), fltb);
+        const float wz0 = kb_weight(tab, dsub(z, iz - 2), fltb);
+        const int sx = wx0 != 0.0f ? ix - 2 : ix - 1;
+        const int sy = wy0 != 0.0f ? iy - 2 : iy - 1;
+        const int sz = wz0 != 0.0f ? iz - 2 : iz - 1;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            tx[t] = (t == 0 && sx == ix - 2) ? wx0 : kb_weight(tab, dsub(x, sx + t), fltb);
+            ty[t] = (t == 0 && sy == iy - 2) ? wy0 : kb_weight(tab, dsub(y, sy + t), fltb);
+            tz[t] = (t == 0 && sz == iz - 2) ? wz0 : kb_weight(tab, dsub(z, sz + t), fltb);
+        }
+        const double vxs = __dmul_rn((double)v.x, vscale), vys = __dmul_rn((double)v.y, vscale);
+        if constexpr (MIRPASS) {
+            // target -p, region-local: (-p) - (core origin - 2)
+            const double vim = -vys;
+#pragma unroll
+            for (int lz = 0; lz < 4; ++lz) {
+                const int tzl = -(sz + lz) - z0;
+                const bool zok = (unsigned)tzl < (unsigned)CZ;
+#pragma unroll
+                for (int ly = 0; ly < 4; ++ly) {
+                    const int tyl = -(sy + ly) - y0;
+                    const bool ok = zok && (unsigned)tyl < (unsigned)CY;
+                    const int offm = L.rowbase8(tyl + 2, tzl + 2), rm = L.rot8(tyl + 2, tzl + 2);
+                    const float tyz = __fmul_rn(ty[ly], tz[lz]);
+#pragma unroll
+                    for (int lx = 0; lx < 2; ++lx) {   // p_x = sx + lx < 0 only for the first two taps
+                        const int px = sx + lx;
+                        const float wg = __fmul_rn(tx[lx], tyz);
+                        if (ok && px < 0 && -px - x0 < CX && wg != 0.0f) {
+                            const int ob = L.at8(offm, rm, (-px - x0 + 2) * 8);
+                            const double wd = (double)wg;
+                            lds_add(L.re, ob, to_fixed_prod(vxs, wd));
+                            lds_add(L.im, ob, to_fixed_prod(vim, wd));
+                            lds_add(L.w, ob, to_fixed_prod(wd, wscale));
+                        }
+                    }
+                }
+            }
+        } else {
+            // every direct tap is inside the region: p - (core origin - 2) in [0, B)
+            int xo8[4];
+            bool xok[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                xo8[t] = (sx + t - x0 + 2) * 8;
+                xok[t] = !XCHK || sx + t >= 0;
+            }
+#pragma unroll
+            for (int lz = 0; lz < 4; ++lz) {
+                const int zl = sz + lz - z0 + 2;
+#pragma unroll
+                for (int ly = 0; ly < 4; ++ly) {
+                    const int yl = sy + ly - y0 + 2;
+                    const int offd = L.rowbase8(yl, zl), rd = L.rot8(yl, zl);
+                    const float tyz = __fmul_rn(ty[ly], tz[lz]);
+#pragma unroll
+                    for (int lx = 0; lx < 4; ++lx) {
+                        const float wg = __fmul_rn(tx[lx], tyz);
+                        if (xok[lx] && wg != 0.0f) {
+                            const int ob = L.at8(offd, rd, xo8[lx]);
+                            const double wd = (double)wg;
+                            lds_add(L.re, ob, to_fixed_prod(vxs, wd));
+                            lds_add(L.im, ob, to_fixed_prod(vys, wd));
+                            lds_add(L.w, ob, to_fixed_prod(wd, wscale));
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
 // One insertion job for the brick kernel: a context's chunk of projections
 // (spectra, geometry), its table, brick grid and lists, and its half grids.
 struct BrickJob {
@@ -595,7 +764,7 @@ constexpr int MAX_JOBS = 64;
 // through LDS atomics.  The queue head must be zero at launch.
 // The fp32-geometry instantiations are held to 128 VGPRs (4 waves per SIMD);
 // the FP64-geometry ones need more and run 3.
-template <int BX, int BY, int BZ, typename P, int NT = TPB>
+template <int BX, int BY, int BZ, typename P, int NT = TPB, bool OWN = false>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(P) == 4 ? 4 : 3)))
 void insert_brick_kernel(const BrickJob* __restrict__ jobs, int njobs, int* __restrict__ next_item) {
     constexpr int NV = BX * BY * BZ;
@@ -653,7 +822,30 @@ void insert_brick_kernel(const BrickJob* __restrict__ jobs, int njobs, int* __re
         const float* __restrict__ tab = J.tab;
         const double wsc = J.wscale;
         const P fltb = fltb_of<P>(J.fltb);
-        for (int s = wave; s < ns; s += nwave) {
+        if constexpr (OWN) {
+            constexpr int CX = BX - 4, CY = BY - 4, CZ = BZ - 4;
+            // E whose nearest voxel lies in the core; at kx = 0 also E within
+            // 2.5 of -core with ix <= 1 (E_x < 1.5)
+            Box oc;
+            oc.lo[0] = x0 - 0.51f; oc.lo[1] = y0 - 0.51f; oc.lo[2] = z0 - 0.51f;
+            oc.hi[0] = x0 + CX - 0.49f; oc.hi[1] = y0 + CY - 0.49f; oc.hi[2] = z0 + CZ - 0.49f;
+            Box om;
+            om.lo[0] = -0.01f; om.hi[0] = 1.51f;
+            om.lo[1] = -(y0 + CY - 1) - 2.51f; om.hi[1] = -y0 + 2.51f;
+            om.lo[2] = -(z0 + CZ - 1) - 2.51f; om.hi[2] = -z0 + 2.51f;
+            for (int s = wave; s < ns; s += nwave) {
+                const int p = lst[s];
+                const Geom g = J.geom[p];
+                const float2* spec_p = J.spec + (size_t)(g.img - J.img0) * N * H;
+                if (x0 == 0) {
+                    wave_project_own<BX, BY, BZ, false, true, P, NT>(L, g, spec_p, nx, N, x0, y0, z0, oc, tab, fltb, vscale, wsc);
+                    wave_project_own<BX, BY, BZ, true, false, P, NT>(L, g, spec_p, nx, N, x0, y0, z0, om, tab, fltb, vscale, wsc);
+                } else {
+                    wave_project_own<BX, BY, BZ, false, false, P, NT>(L, g, spec_p, nx, N, x0, y0, z0, oc, tab, fltb, vscale, wsc);
+                }
+            }
+        }
+        for (int s = wave; s < ns && !OWN; s += nwave) {
             const int p = lst[s];
             const Geom g = J.geom[p];
             const float2* spec_p = J.spec + (size_t)(g.img - J.img0) * N * H;
@@ -673,8 +865,10 @@ void insert_brick_kernel(const BrickJob* __restrict__ jobs, int njobs, int* __re
             const int lx = v % BX, ly = (v / BX) % BY, lz = v / (BX * BY);
             const int i = L.at(L.rowbase(ly, lz), L.rot(ly, lz), lx);
             if (L.w[i] == 0ull) continue;
-            const int kx = x0 + lx, ky = y0 + ly, kz = z0 + lz;
-            if (kx > nx || ky > nx || kz > nx) continue;   // brick overhangs the grid (partial last brick)
+            const int rg = OWN ? 2 : 0;   // owner-computes: the region starts 2 voxels before the core
+            const int kx = x0 - rg + lx, ky = y0 - rg + ly, kz = z0 - rg + lz;
+            // brick overhangs the grid (partial last brick; a region's halo)
+            if (kx < 0 || ky < -nx + 1 || kz < -nx + 1 || kx > nx || ky > nx || kz > nx) continue;
             const size_t o = ((size_t)(kz >= 0 ? kz : N + kz) * N + (ky >= 0 ? ky : N + ky)) * H + kx;
             unsafeAtomicAdd(&X[o].x, (float)__dmul_rn(from_fixed(L.re[i]), vinv));
             unsafeAtomicAdd(&X[o].y, (float)__dmul_rn(from_fixed(L.im[i]), vinv));

@@ -51,12 +51,15 @@ using namespace spbp;
 namespace {
 thread_local std::string g_last_error;
 // brick shapes of the LDS-binned strategy (voxels; 3 x 8 B per voxel in LDS)
-constexpr int NSHAPES = 7;
-constexpr int SHAPES[NSHAPES][3] = {{16, 16, 8}, {16, 8, 8}, {8, 8, 8}, {16, 12, 8}, {16, 16, 10}, {16, 16, 12}, {16, 16, 13}};
+// {x, y, z, owner-computes}; for owner-computes bricks the LDS region, whose
+// core (each sample's owner) is 4 voxels shorter per axis
+constexpr int NSHAPES = 8;
+constexpr int SHAPES[NSHAPES][4] = {{16, 16, 8, 0},  {16, 8, 8, 0},   {8, 8, 8, 0},    {16, 12, 8, 0},
+                                    {16, 16, 10, 0}, {16, 16, 12, 0}, {16, 16, 13, 0}, {16, 16, 13, 1}};
 // brick workgroups: 16x16x10, 16x16x12 and 16x16x13 (61.7, 73.8 and 80.2 KB of
 // LDS) run two 512-thread groups per CU; the others four (three for 16x16x8)
 // 256-thread groups.  The FP64-geometry instantiations need more VGPRs than
-// 16 waves per CU allow: those contexts use 16x16x10 for the two largest
+// 16 waves per CU allow: those contexts use 16x16x10 for the three largest
 // shapes.
 constexpr int NCVALS = 4;
 constexpr int CVALS[NCVALS] = {16, 32, 64, 128};   // projections per work item tried by the tuner
@@ -233,11 +236,13 @@ struct Timed {
 // Brick grid of shape `shape` and its per-key arrays.
 int configure_bricks(spbp_ctx* c, int shape) {
     if ((c->flags & SPBP_GEOM_F64) && shape >= 5) shape = 4;
-    const int bx = SHAPES[shape][0], by = SHAPES[shape][1], bz = SHAPES[shape][2];
+    const int own = SHAPES[shape][3], halo = own ? 4 : 0;
+    const int bx = SHAPES[shape][0] - halo, by = SHAPES[shape][1] - halo, bz = SHAPES[shape][2] - halo;
     BrickGrid g;
     g.bx = bx;
     g.by = by;
     g.bz = bz;
+    g.own = own;
     g.nbx = (c->nx + bx - 1) / bx;
     g.nby = (2 * c->nx - 1 + by - 1) / by;
     g.nbz = (2 * c->nx - 1 + bz - 1) / bz;
@@ -464,6 +469,7 @@ int launch_bricks_dev(spbp_ctx* c, const BrickJob* table_d, int n, int* counter,
         case 9: insert_brick_kernel<16, 16, 10, double, 2 * TPB><<<grid, 2 * TPB, 0, st>>>(table_d, n, counter); break;
         case 10: insert_brick_kernel<16, 16, 12, float, 2 * TPB><<<grid, 2 * TPB, 0, st>>>(table_d, n, counter); break;
         case 12: insert_brick_kernel<16, 16, 13, float, 2 * TPB><<<grid, 2 * TPB, 0, st>>>(table_d, n, counter); break;
+        case 14: insert_brick_kernel<16, 16, 13, float, 2 * TPB, true><<<grid, 2 * TPB, 0, st>>>(table_d, n, counter); break;
         default: return fail(c, SPBP_ERR_ARG, "brick shape %d has no FP64-geometry kernel", c->shape);
     }
     HIPCHK(c, hipGetLastError());
@@ -1029,8 +1035,9 @@ int spbp_create(const spbp_opts* o, spbp_ctx** out) {
             return bail(fail(nullptr, SPBP_ERR_ALLOC, "twiddle table"));
         c->fft_log2 = L;
     }
-    // default plan: 16x16x13 bricks (the largest that runs two 512-thread groups per CU), 128 projections
-    // per item: fastest measured at 128^2 into 256^3 and 256^2 into 512^3; spbp_tune picks per workload
+    // default plan: owner-computes bricks, 16x16x13 LDS regions around 12x12x9 cores (the largest region
+    // that runs two 512-thread groups per CU), 128 projections per item: fastest measured at 128^2 into
+    // 256^3 and 256^2 into 512^3; spbp_tune picks per workload
     if ((rc = configure_bricks(c, NSHAPES - 1))) { g_last_error = c->err; return bail(rc); }
     rc = spbp_reset(c, nullptr);
     if (rc) return bail(rc);

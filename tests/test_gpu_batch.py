@@ -53,7 +53,7 @@ def _export(ctx, half=0):
     return X.cpu().numpy(), W.cpu().numpy()
 
 
-@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4, 5, 6, 7])
 @pytest.mark.parametrize("cproj", [16, 128])
 def test_every_plan_matches_oracle(bpmod, shape, cproj):
     nx, m = 20, 90
@@ -62,7 +62,7 @@ def test_every_plan_matches_oracle(bpmod, shape, cproj):
     with bpmod.Backprojector(nx, batch=32) as b:
         b.set_plan("brick", shape, cproj)
         assert b.plan()["brick"] == [(16, 16, 8), (16, 8, 8), (8, 8, 8), (16, 12, 8), (16, 16, 10), (16, 16, 12),
-                                    (16, 16, 13)][shape]
+                                    (16, 16, 13), (16, 16, 13)][shape]
         b.insert(torch.from_numpy(imgs).cuda(), ang)
         X, W = _export(b)
     np.testing.assert_array_equal(W != 0, Wr != 0)

@@ -2,8 +2,10 @@
 passes, power-of-two N) against hipFFT on the fully padded images and against
 the oracle's FP64 spectrum.
 
-Bars: weights bit-identical (they do not depend on the spectra); accumulated
-X within rel-L2 1e-6 of the hipFFT path (two fp32 FFTs, different rounding);
+Bars: weights with identical support and within 1e-6 relative per voxel
+(they do not depend on the spectra; only the order of the fp32 atomic adds
+that flush overlapping brick regions differs); accumulated X within rel-L2
+1e-6 of the hipFFT path (two fp32 FFTs, different rounding);
 the raw spectrum of a power-of-two size within 2e-7 relative (RMS) of an FP64
 FFT of the same padded image.
 """
@@ -47,7 +49,10 @@ def test_own_transforms_match_hipfft(bpmod, nx, bf16):
             b.insert(imgs, ang)
             X, W = b.export_raw(0)
             out.append((X.cpu().numpy(), W.cpu().numpy()))
-    np.testing.assert_array_equal(out[0][1], out[1][1])
+    W0, W1 = out[0][1], out[1][1]
+    np.testing.assert_array_equal(W0 != 0, W1 != 0)
+    nz = W1 != 0
+    assert np.max(np.abs(W0[nz] - W1[nz]) / W1[nz]) <= 1e-6
     assert rel(out[0][0], out[1][0]) <= 1e-6
 
 
