@@ -390,6 +390,7 @@ def main():
                     help="MiB: run one device-to-device copy of this size first (calibrates PMC byte counters)")
     ap.add_argument("--bf16", action="store_true", help="store the images as BF16 (default for config 5)")
     ap.add_argument("--twosum", action="store_true", help="compensated weights (default for config 5)")
+    ap.add_argument("--geom-f64", action="store_true", help="FP64 sample geometry (SPBP_GEOM_F64) in the timed job")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-calib", action="store_true", help="skip the copy-rate and atomic-rate calibrations")
     ap.add_argument("--no-parity", action="store_true")
@@ -435,7 +436,7 @@ def main():
         log(f"bench: calibration copy of {args.copy_calib} MiB done")
 
     b = bpmod.Backprojector(nx, timing=True, batch=args.batch, strategy=args.strategy, brick_proj=args.brick_proj,
-                            twosum=twosum)
+                            twosum=twosum, geom_f64=args.geom_f64)
     t_tune = None
     if args.shape is not None:
         b.set_plan("direct" if args.strategy == "direct" else "brick", args.shape, args.brick_proj)
@@ -505,7 +506,7 @@ def main():
                                   f"{2 * nx}^3 padded Fourier grid{', compensated weights' if twosum else ''} "
                                   f"(config {args.config})",
                       "nx": nx, "nproj": nproj, "grid": 2 * nx, "images_dtype": "bf16" if bf16 else "f32",
-                      "twosum_weights": twosum},
+                      "twosum_weights": twosum, "geom_f64": bool(args.geom_f64)},
            "breakdown_ms_per_step": {"pad": round(pad_ms / args.steps, 3), "fft2": round(fft_ms / args.steps, 3),
                                      "insert": round(ins_ms / args.steps, 3), "bin": round(bin_ms / args.steps, 3),
                                      "finalize": round(fin_ms / max(nfin, 1), 3),

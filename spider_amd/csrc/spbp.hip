@@ -59,8 +59,8 @@ constexpr int SHAPES[NSHAPES][4] = {{16, 16, 8, 0},  {16, 8, 8, 0},   {8, 8, 8, 
 // brick workgroups: 16x16x10, 16x16x12 and 16x16x13 (61.7, 73.8 and 80.2 KB of
 // LDS) run two 512-thread groups per CU; the others four (three for 16x16x8)
 // 256-thread groups.  The FP64-geometry instantiations need more VGPRs than
-// 16 waves per CU allow: those contexts use 16x16x10 for the three largest
-// shapes.
+// 16 waves per CU allow: those contexts run one 512-thread group per CU, and
+// 16x16x10 in place of shared-evaluation 16x16x12 and 16x16x13.
 constexpr int NCVALS = 4;
 constexpr int CVALS[NCVALS] = {16, 32, 64, 128};   // projections per work item tried by the tuner
 }  // namespace
@@ -235,7 +235,7 @@ struct Timed {
 
 // Brick grid of shape `shape` and its per-key arrays.
 int configure_bricks(spbp_ctx* c, int shape) {
-    if ((c->flags & SPBP_GEOM_F64) && shape >= 5) shape = 4;
+    if ((c->flags & SPBP_GEOM_F64) && (shape == 5 || shape == 6)) shape = 4;
     const int own = SHAPES[shape][3], halo = own ? 4 : 0;
     const int bx = SHAPES[shape][0] - halo, by = SHAPES[shape][1] - halo, bz = SHAPES[shape][2] - halo;
     BrickGrid g;
@@ -470,6 +470,7 @@ int launch_bricks_dev(spbp_ctx* c, const BrickJob* table_d, int n, int* counter,
         case 10: insert_brick_kernel<16, 16, 12, float, 2 * TPB><<<grid, 2 * TPB, 0, st>>>(table_d, n, counter); break;
         case 12: insert_brick_kernel<16, 16, 13, float, 2 * TPB><<<grid, 2 * TPB, 0, st>>>(table_d, n, counter); break;
         case 14: insert_brick_kernel<16, 16, 13, float, 2 * TPB, true><<<grid, 2 * TPB, 0, st>>>(table_d, n, counter); break;
+        case 15: insert_brick_kernel<16, 16, 13, double, 2 * TPB, true><<<grid, 2 * TPB, 0, st>>>(table_d, n, counter); break;
         default: return fail(c, SPBP_ERR_ARG, "brick shape %d has no FP64-geometry kernel", c->shape);
     }
     HIPCHK(c, hipGetLastError());
