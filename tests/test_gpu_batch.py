@@ -167,3 +167,26 @@ def test_calibrations_return_rates(bpmod):
     assert 1e9 < r.value < 1.6e14
     assert L.spbp_calibrate(_lib.CAL_GLOBAL_F32, 4096, C.byref(r), None) == _lib.ERR_ARG
     assert L.spbp_calibrate(_lib.CAL_GLOBAL_F32, (1 << 34) + 4, C.byref(r), None) == _lib.ERR_ARG
+
+
+@pytest.mark.parametrize("nx,two,sym,geom_f64", [(64, True, False, False), (25, False, True, False),
+                                                 (48, True, True, False), (40, True, False, True)])
+def test_owner_bricks_match_shared_evaluation(bpmod, nx, two, sym, geom_f64):
+    """Owner-computes bricks (plan 7) against shared-evaluation 16x16x13 bricks
+    (plan 6, 16x16x10 under FP64 geometry) on larger and odd grids, two
+    halves, symmetries: the same taps, so identical weight support and sums
+    equal up to the fp32 order of the flush atomics."""
+    m = 150
+    imgs, ang = dataset(nx, m, 40 + nx)
+    half = bpmod.random_halves(m, seed=nx) if two else None
+    syms = np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 180.0], [180.0, 0.0, 0.0]]) if sym else None
+    out = []
+    for shape in (6, 7):
+        with bpmod.Backprojector(nx, two_halves=two, batch=64, symmetries=syms, geom_f64=geom_f64) as b:
+            b.set_plan("brick", shape, 0)
+            b.insert(torch.from_numpy(imgs).cuda(), ang, half)
+            out.append([_export(b, h) for h in ((0, 1) if two else (0,))])
+    for (Xs, Ws), (Xo, Wo) in zip(out[0], out[1]):
+        np.testing.assert_array_equal(Wo != 0, Ws != 0)
+        assert rel(Wo, Ws) <= 1e-6
+        assert rel(Xo, Xs) <= 1e-6
