@@ -31,6 +31,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "spbp.h"
@@ -353,6 +354,51 @@ int ensure_final(spbp_ctx* c) {
 // From dm (nimg x 9 fp32) when given, else from the doc angles (PSI, THETA, PHI).
 // With use_sym and symmetries set, image p gets entries p*nsym + k with
 // DMS = MATMUL(SM_k, DM) (bp32f.f:866-872), all reading spectrum p.
+// Geometry of projection p (its ns symmetry copies) into c->geom_h.  Returns
+// 0, or the failure: 1 missing angle, 2 non-finite rotation, 3 half flag not
+// 0/1, 4 half 1 without two halves.  Touches only p's entries (the callers
+// run it from several threads).
+int geom_one(spbp_ctx* c, int p, int ns, const float* dm_in, const double* ang, const int8_t* half, bool exact,
+             bool f64) {
+    float dm[9];
+    double dd[9];
+    if (dm_in) {
+        std::memcpy(dm, dm_in + 9 * (size_t)p, sizeof dm);
+        for (int q = 0; q < 9; ++q) dd[q] = dm[q];
+    } else {
+        const double* a = ang + 3 * (size_t)p;
+        if (!std::isfinite(a[0]) || !std::isfinite(a[1]) || !std::isfinite(a[2])) return 1;
+        cang((float)a[2], (float)a[1], (float)a[0], exact, dm);   // CANG(PHI, THETA, PSI)
+        if (f64) cang_d(a[2], a[1], a[0], exact, dd);
+    }
+    for (int q = 0; q < 9; ++q)
+        if (!std::isfinite(dm[q])) return 2;
+    int h = 0;
+    if (half) {
+        if (half[p] != 0 && half[p] != 1) return 3;
+        if (half[p] == 1 && c->nvol < 2) return 4;
+        h = half[p];
+    }
+    for (int k = 0; k < ns; ++k) {
+        Geom& g = c->geom_h[(size_t)p * ns + k];
+        if (ns > 1) sym_matmul(c->sm.data() + 9 * k, dm, g.r);
+        else std::memcpy(g.r, dm, sizeof dm);
+        g.half = h;
+        g.img = p;
+        g.pad_ = 0;
+        if (f64) {
+            double dms[9];
+            if (ns > 1) sym_matmul_d(c->smd.data() + 9 * k, dd, dms);
+            else std::memcpy(dms, dd, sizeof dms);
+            for (int q = 0; q < 6; ++q) g.d[q] = dms[q];
+            for (int q = 0; q < 9; ++q) g.r[q] = (float)dms[q];   // for the brick tests
+        } else {
+            for (int q = 0; q < 6; ++q) g.d[q] = g.r[q];
+        }
+    }
+    return 0;
+}
+
 int stage_geom(spbp_ctx* c, int nimg, const float* dm_in, const double* ang, const int8_t* half, hipStream_t st,
                bool use_sym = true) {
     if (!dm_in && !ang) return fail(c, SPBP_ERR_ANGLE, "MISSING ANGLE FOR IMAGE (no angles given)");
@@ -361,43 +407,36 @@ int stage_geom(spbp_ctx* c, int nimg, const float* dm_in, const double* ang, con
     if (rc) return rc;
     const bool exact = (c->flags & SPBP_EXACT_PI) != 0;
     const bool f64 = (c->flags & SPBP_GEOM_F64) != 0;
-    for (int p = 0; p < nimg; ++p) {
-        float dm[9];
-        double dd[9];
-        if (dm_in) {
-            std::memcpy(dm, dm_in + 9 * (size_t)p, sizeof dm);
-            for (int q = 0; q < 9; ++q) dd[q] = dm[q];
-        } else {
-            const double* a = ang + 3 * (size_t)p;
-            if (!std::isfinite(a[0]) || !std::isfinite(a[1]) || !std::isfinite(a[2]))
-                return fail(c, SPBP_ERR_ANGLE, "MISSING ANGLE FOR IMAGE %d", p + 1);
-            cang((float)a[2], (float)a[1], (float)a[0], exact, dm);   // CANG(PHI, THETA, PSI)
-            if (f64) cang_d(a[2], a[1], a[0], exact, dd);
+    // CANG in FP64 costs ~0.1 us per projection: large batches are split over
+    // host threads (contiguous ranges; each stops at its first failure, and
+    // the first failure in projection order is reported)
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int nt = (int)std::min<unsigned>(std::min(hw, 16u), (unsigned)std::max(1, nimg / 2048));
+    std::vector<int> bad_p(nt, -1), bad_k(nt, 0);
+    auto work = [&](int t) {
+        const int p0 = (int)((long long)nimg * t / nt), p1 = (int)((long long)nimg * (t + 1) / nt);
+        for (int p = p0; p < p1; ++p) {
+            const int k = geom_one(c, p, ns, dm_in, ang, half, exact, f64);
+            if (k) { bad_p[t] = p; bad_k[t] = k; return; }
         }
-        for (int q = 0; q < 9; ++q)
-            if (!std::isfinite(dm[q])) return fail(c, SPBP_ERR_ANGLE, "non-finite rotation for image %d", p + 1);
-        int h = 0;
-        if (half) {
-            if (half[p] != 0 && half[p] != 1) return fail(c, SPBP_ERR_ARG, "half flag %d of image %d is not 0/1", half[p], p + 1);
-            if (half[p] == 1 && c->nvol < 2) return fail(c, SPBP_ERR_ARG, "half 1 needs SPBP_TWO_HALVES");
-            h = half[p];
-        }
-        for (int k = 0; k < ns; ++k) {
-            Geom& g = c->geom_h[(size_t)p * ns + k];
-            if (ns > 1) sym_matmul(c->sm.data() + 9 * k, dm, g.r);
-            else std::memcpy(g.r, dm, sizeof dm);
-            g.half = h;
-            g.img = p;
-            g.pad_ = 0;
-            if (f64) {
-                double dms[9];
-                if (ns > 1) sym_matmul_d(c->smd.data() + 9 * k, dd, dms);
-                else std::memcpy(dms, dd, sizeof dms);
-                for (int q = 0; q < 6; ++q) g.d[q] = dms[q];
-                for (int q = 0; q < 9; ++q) g.r[q] = (float)dms[q];   // for the brick tests
-            } else {
-                for (int q = 0; q < 6; ++q) g.d[q] = g.r[q];
-            }
+    };
+    if (nt > 1) {
+        std::vector<std::thread> th;
+        th.reserve(nt - 1);
+        for (int t = 1; t < nt; ++t) th.emplace_back(work, t);
+        work(0);
+        for (auto& x : th) x.join();
+    } else {
+        work(0);
+    }
+    for (int t = 0; t < nt; ++t) {
+        if (bad_p[t] < 0) continue;
+        const int p = bad_p[t];
+        switch (bad_k[t]) {
+            case 1: return fail(c, SPBP_ERR_ANGLE, "MISSING ANGLE FOR IMAGE %d", p + 1);
+            case 2: return fail(c, SPBP_ERR_ANGLE, "non-finite rotation for image %d", p + 1);
+            case 3: return fail(c, SPBP_ERR_ARG, "half flag %d of image %d is not 0/1", half[p], p + 1);
+            default: return fail(c, SPBP_ERR_ARG, "half 1 needs SPBP_TWO_HALVES");
         }
     }
     HIPCHK(c, hipMemcpyAsync(c->geom_d, c->geom_h, (size_t)nimg * ns * sizeof(Geom), hipMemcpyHostToDevice, st));
