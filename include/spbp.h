@@ -125,7 +125,8 @@ int spbp_scatter(spbp_ctx* ctx, const float* bi, int nimg, const double* angles_
  * first).  angles_deg and half are host arrays. */
 int spbp_insert(spbp_ctx* ctx, const void* imgs, int dtype, int nimg,
                 const double* angles_deg, const int8_t* half, void* stream);
-/* Same with precomputed rotations: dm is nimg x 9 fp32 CANG matrices (host). */
+/* Same with precomputed rotations: dm is nimg x 9 fp32 CANG matrices (host);
+ * rows 1-2 of each must be orthonormal to 1e-4 (SPBP_ERR_ANGLE otherwise). */
 int spbp_insert_dm(spbp_ctx* ctx, const void* imgs, int dtype, int nimg,
                    const float* dm, const int8_t* half, void* stream);
 /* Same with host-resident images (staged through pinned memory). */

@@ -581,28 +581,48 @@ __device__ __forceinline__ void wave_project(BrickLds<BX, BY, BZ, NT>& L, const 
 // MIRPASS false: owned samples, direct taps (XCHK: drop taps with p_x < 0,
 // needed only at kx = 0).  MIRPASS true: the reflected taps whose target lies
 // in the core.  (x0, y0, z0): the core's origin.
+// Lattice lines (rows b of the sample lattice) of projection g crossing the box: [b0, b0 + nl).
+__device__ inline void box_lines(const Geom& g, const Box& box, int& b0, int& nl) {
+    float bmn = 0.f, bmx = 0.f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float c = g.r[3 + k];
+        bmn += fminf(c * box.lo[k], c * box.hi[k]);
+        bmx += fmaxf(c * box.lo[k], c * box.hi[k]);
+    }
+    b0 = (int)ceilf(bmn - 0.01f);
+    nl = (int)floorf(bmx + 0.01f) - b0 + 1;
+}
+
+// Two projections, A and B, share the wave's rounds of candidates: lanes
+// 0..31 hold A's lattice lines and lanes 32..63 B's (each projection's lines
+// across a box of this size number at most edge x sqrt(3) + 2 <= 32), so the
+// candidates of both fill the rounds of 64 lanes.  Without B (hasB false)
+// its lines are empty.  The caller checks nlA, nlB <= 32.
 template <int BX, int BY, int BZ, bool MIRPASS, bool XCHK, typename P, int NT>
-__device__ __forceinline__ void wave_project_own(BrickLds<BX, BY, BZ, NT>& L, const Geom& g,
-                                                 const float2* __restrict__ spec_p, int nx, int N, int x0, int y0,
-                                                 int z0, const Box box, const float* __restrict__ tab, P fltb,
-                                                 double vscale, double wscale) {
+__device__ __forceinline__ void wave_project_own(BrickLds<BX, BY, BZ, NT>& L, const Geom& gA, const Geom& gB,
+                                                 bool hasB, const float2* __restrict__ specA,
+                                                 const float2* __restrict__ specB, int b0A, int nlA, int b0B, int nlB,
+                                                 int nx, int N, int x0, int y0, int z0, const Box box,
+                                                 const float* __restrict__ tab, P fltb, double vscale, double wscale) {
     constexpr int CX = BX - 4, CY = BY - 4, CZ = BZ - 4;
     const float* lo = box.lo;
     const float* hi = box.hi;
     const int H = nx + 1;
     const int lane = threadIdx.x & 63;
-    float bmn = 0.f, bmx = 0.f;
+    // this lane's line: projection `side` (0: A, 1: B), line b0 + (lane & 31)
+    const bool side = lane >= 32;
+    Geom g;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const float c = g.r[3 + k];
-        bmn += fminf(c * lo[k], c * hi[k]);
-        bmx += fmaxf(c * lo[k], c * hi[k]);
+    for (int k = 0; k < 6; ++k) {
+        g.r[k] = side ? gB.r[k] : gA.r[k];
+        g.d[k] = side ? gB.d[k] : gA.d[k];
     }
-    const int b0 = (int)ceilf(bmn - 0.01f), b1 = (int)floorf(bmx + 0.01f);
-    const int nl = b1 - b0 + 1;   // <= 64: box edge (<= 17) times sqrt(3), plus 2
+    const int nl = side ? (hasB ? nlB : 0) : nlA;
+    const int b0 = side ? b0B : b0A;
     int cnt = 0, alo = 0;
-    if (lane < nl) {
-        const float bb = (float)(b0 + lane);
+    if ((lane & 31) < nl) {
+        const float bb = (float)(b0 + (lane & 31));
         float amin = -1e30f, amax = 1e30f;
         bool empty = false;
 #pragma unroll
@@ -637,13 +657,22 @@ __device__ __forceinline__ void wave_project_own(BrickLds<BX, BY, BZ, NT>& L, co
         }
         const int lpref = __shfl(pref, l, 64), lcnt = __shfl(cnt, l, 64), lalo = __shfl(alo, l, 64);
         if (q >= total) continue;
-        const int bb = b0 + l;
+        // the candidate's projection: its line's side
+        const bool cside = l >= 32;
+        Geom gc;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            gc.r[k] = cside ? gB.r[k] : gA.r[k];
+            gc.d[k] = cside ? gB.d[k] : gA.d[k];
+        }
+        const float2* __restrict__ spec_p = cside ? specB : specA;
+        const int bb = (cside ? b0B : b0A) + (l & 31);
         const int aa = lalo + (q - (lpref - lcnt));
         const bool canon = aa > 0 || (aa == 0 && bb >= 0);
         const int I = canon ? aa : -aa, J = canon ? bb : -bb;
         if (I > nx || J < -nx + 1 || J > nx || !(I * I + J * J < nx * nx)) continue;   // oneline.f:53-55
         P x, y, z;
-        sample_pos(g, I, J, x, y, z);
+        sample_pos(gc, I, J, x, y, z);
         const bool flip = x < (P)0;                                    // oneline.f:60-67
         if (flip == canon) continue;
         if (flip) { x = -x; y = -y; z = -z; }
@@ -833,15 +862,33 @@ void insert_brick_kernel(const BrickJob* __restrict__ jobs, int njobs, int* __re
             om.lo[0] = -0.01f; om.hi[0] = 1.51f;
             om.lo[1] = -(y0 + CY - 1) - 2.51f; om.hi[1] = -y0 + 2.51f;
             om.lo[2] = -(z0 + CZ - 1) - 2.51f; om.hi[2] = -z0 + 2.51f;
-            for (int s = wave; s < ns; s += nwave) {
-                const int p = lst[s];
-                const Geom g = J.geom[p];
-                const float2* spec_p = J.spec + (size_t)(g.img - J.img0) * N * H;
-                if (x0 == 0) {
-                    wave_project_own<BX, BY, BZ, false, true, P, NT>(L, g, spec_p, nx, N, x0, y0, z0, oc, tab, fltb, vscale, wsc);
-                    wave_project_own<BX, BY, BZ, true, false, P, NT>(L, g, spec_p, nx, N, x0, y0, z0, om, tab, fltb, vscale, wsc);
-                } else {
-                    wave_project_own<BX, BY, BZ, false, false, P, NT>(L, g, spec_p, nx, N, x0, y0, z0, oc, tab, fltb, vscale, wsc);
+            // each wave takes the item's projections two at a time (s, s + 1).
+            // Lines per projection across the core box (edges C + 0.02) or the
+            // mirror box (1.52, BY + 0.02, BZ + 0.02): at most the box diagonal
+            // + 2 for a rotation (orthonormal rows, checked on the host), <= 32.
+            static_assert((BX - 3.98) * (BX - 3.98) + (BY - 3.98) * (BY - 3.98) + (BZ - 3.98) * (BZ - 3.98) <= 900.0 &&
+                              1.52 * 1.52 + (BY + 0.02) * (BY + 0.02) + (BZ + 0.02) * (BZ + 0.02) <= 900.0,
+                          "owner-computes region too large for 32 lines per projection");
+            for (int s = 2 * wave; s < ns; s += 2 * nwave) {
+                const bool hasB = s + 1 < ns;
+                const Geom gA = J.geom[lst[s]];
+                const Geom gB = J.geom[lst[hasB ? s + 1 : s]];
+                const float2* specA = J.spec + (size_t)(gA.img - J.img0) * N * H;
+                const float2* specB = J.spec + (size_t)(gB.img - J.img0) * N * H;
+#pragma unroll
+                for (int pass = 0; pass < 2; ++pass) {
+                    if (pass == 1 && x0 != 0) break;
+                    const Box& bx2 = pass ? om : oc;
+                    int b0A, nlA, b0B, nlB;
+                    box_lines(gA, bx2, b0A, nlA);
+                    box_lines(gB, bx2, b0B, nlB);
+                    if (pass) {
+                        wave_project_own<BX, BY, BZ, true, false, P, NT>(L, gA, gB, hasB, specA, specB, b0A, nlA, b0B, nlB, nx, N, x0, y0, z0, bx2, tab, fltb, vscale, wsc);
+                    } else if (x0 == 0) {
+                        wave_project_own<BX, BY, BZ, false, true, P, NT>(L, gA, gB, hasB, specA, specB, b0A, nlA, b0B, nlB, nx, N, x0, y0, z0, bx2, tab, fltb, vscale, wsc);
+                    } else {
+                        wave_project_own<BX, BY, BZ, false, false, P, NT>(L, gA, gB, hasB, specA, specB, b0A, nlA, b0B, nlB, nx, N, x0, y0, z0, bx2, tab, fltb, vscale, wsc);
+                    }
                 }
             }
         }

@@ -356,7 +356,7 @@ int ensure_final(spbp_ctx* c) {
 // DMS = MATMUL(SM_k, DM) (bp32f.f:866-872), all reading spectrum p.
 // Geometry of projection p (its ns symmetry copies) into c->geom_h.  Returns
 // 0, or the failure: 1 missing angle, 2 non-finite rotation, 3 half flag not
-// 0/1, 4 half 1 without two halves.  Touches only p's entries (the callers
+// 0/1, 4 half 1 without two halves, 5 given rows 1-2 not orthonormal.  Touches only p's entries (the callers
 // run it from several threads).
 int geom_one(spbp_ctx* c, int p, int ns, const float* dm_in, const double* ang, const int8_t* half, bool exact,
              bool f64) {
@@ -373,6 +373,14 @@ int geom_one(spbp_ctx* c, int p, int ns, const float* dm_in, const double* ang, 
     }
     for (int q = 0; q < 9; ++q)
         if (!std::isfinite(dm[q])) return 2;
+    if (dm_in) {
+        // the section's lattice is enumerated along DM's first two rows, which
+        // must be orthonormal (a rotation, as CANG builds)
+        const double a = (double)dm[0] * dm[0] + (double)dm[1] * dm[1] + (double)dm[2] * dm[2];
+        const double b = (double)dm[3] * dm[3] + (double)dm[4] * dm[4] + (double)dm[5] * dm[5];
+        const double ab = (double)dm[0] * dm[3] + (double)dm[1] * dm[4] + (double)dm[2] * dm[5];
+        if (std::fabs(a - 1.0) > 1e-4 || std::fabs(b - 1.0) > 1e-4 || std::fabs(ab) > 1e-4) return 5;
+    }
     int h = 0;
     if (half) {
         if (half[p] != 0 && half[p] != 1) return 3;
@@ -436,6 +444,7 @@ int stage_geom(spbp_ctx* c, int nimg, const float* dm_in, const double* ang, con
             case 1: return fail(c, SPBP_ERR_ANGLE, "MISSING ANGLE FOR IMAGE %d", p + 1);
             case 2: return fail(c, SPBP_ERR_ANGLE, "non-finite rotation for image %d", p + 1);
             case 3: return fail(c, SPBP_ERR_ARG, "half flag %d of image %d is not 0/1", half[p], p + 1);
+            case 5: return fail(c, SPBP_ERR_ANGLE, "rotation of image %d is not orthonormal", p + 1);
             default: return fail(c, SPBP_ERR_ARG, "half 1 needs SPBP_TWO_HALVES");
         }
     }
