@@ -190,3 +190,19 @@ def test_owner_bricks_match_shared_evaluation(bpmod, nx, two, sym, geom_f64):
         np.testing.assert_array_equal(Wo != 0, Ws != 0)
         assert rel(Wo, Ws) <= 1e-6
         assert rel(Xo, Xs) <= 1e-6
+
+
+def test_non_orthonormal_rotations_are_refused(bpmod):
+    """The bricks enumerate each section's lattice along DM's first two rows:
+    spbp_insert_dm refuses matrices whose rows 1-2 are not orthonormal."""
+    from spider_amd import _lib
+    nx, m = 16, 4
+    imgs, ang = dataset(nx, m, 77)
+    dm = _lib.rotations(ang).reshape(m, 9).copy()
+    with bpmod.Backprojector(nx) as b:
+        b.insert_rotations(torch.from_numpy(imgs).cuda(), dm)          # CANG matrices pass
+        bad = dm.copy()
+        bad[2, :3] *= 1.01
+        with pytest.raises(_lib.SpbpError) as e:
+            b.insert_rotations(torch.from_numpy(imgs).cuda(), bad)
+        assert e.value.code == _lib.ERR_ANGLE and "image 3" in str(e.value)
