@@ -118,6 +118,15 @@ int spbp_gather(spbp_ctx* ctx, const float* X, int nimg, const double* angles_de
  * bp32f.f:802-913) minus PADD2 and the forward FFT. */
 int spbp_scatter(spbp_ctx* ctx, const float* bi, int nimg, const double* angles_deg, const float* dm,
                  float* X, float* W, void* stream);
+/* The projector's adjoint applied to real images, for power-of-two padded
+ * sizes: the scatter (no weights) of the spectra A^T's chain gives for nimg
+ * device images of NX x NX fp32 -- the R2C transforms of the zero-padded
+ * images (no PADD2 mean) times cx(I) / N^2 (cx = 2 for 0 < I < NX, else 1),
+ * column 0 folded as the transpose of its Hermitian completion -- added into
+ * X.  Equals spbp_scatter of the BI spectra computed from those images; no
+ * counterpart in the reference (it has no projector adjoint). */
+int spbp_scatter_images(spbp_ctx* ctx, const float* imgs, int nimg, const double* angles_deg, const float* dm,
+                        float* X, void* stream);
 /* Insert projections: PADD2 + forward FFT + checkerboard + ONELINE over all
  * rows, per image (the BP32FQ / BP3FQ projection loop, bp32f.f:802-913,
  * bp3f.f:403-493).  half[i] = 0 -> first sample volume (XE), 1 -> second

@@ -229,6 +229,30 @@ class Backprojector:
                                            C.c_void_p(out.data_ptr()), _stream(self.device)))
         return out
 
+    def scatter_images(self, imgs, angles=None, dm=None, X=None):
+        """The central-slice projector's adjoint applied to real images (n, NX,
+        NX) float32 on the context's device, as spectra added into X (a new
+        zero grid when None); power-of-two padded sizes only.  Equals
+        scatter() of the BI spectra the adjoint's chain builds from imgs."""
+        torch = _torch()
+        if not isinstance(imgs, torch.Tensor) or imgs.dtype != torch.float32 or imgs.dim() != 3 or \
+                tuple(imgs.shape[1:]) != (self.nx, self.nx):
+            raise ValueError(f"imgs must be float32 (n, {self.nx}, {self.nx})")
+        if imgs.device != self.device:
+            raise ValueError(f"imgs on {imgs.device}, context on {self.device}")
+        imgs = imgs.contiguous()
+        n = imgs.shape[0]
+        ang, dmc = self._rotation_args(n, angles, dm)
+        if X is None:
+            X = torch.zeros(self.grid_shape, dtype=torch.complex64, device=self.device)
+        X = self._grid_tensor(X, "X")
+        self._check(_lib.lib().spbp_scatter_images(self._ctx, C.c_void_p(imgs.data_ptr()), n,
+                                                   None if ang is None else ang.ctypes.data_as(C.c_void_p),
+                                                   None if dmc is None else dmc.ctypes.data_as(C.c_void_p),
+                                                   C.c_void_p(X.data_ptr()), _stream(self.device)))
+        self._keep = (imgs,)
+        return X
+
     def scatter(self, B, angles=None, dm=None, X=None, W=None):
         """ONELINE scatter of BI-layout spectra B (n, N, NX+1) complex64, added
         into X (a new zero grid when None) and W when given; returns X."""

@@ -132,6 +132,37 @@ __global__ __launch_bounds__(GT * GT) void gather_kernel(const float2* __restric
     *o = make_float2((float)re, (float)(flip ? -im : im));
 }
 
+// Spectra for the projector's adjoint applied to real images r (the transpose
+// of images = crop(irfft2(BI completed by Hermitian symmetry in column 0))):
+// in place on the R2C spectra R of the zero-padded images, the raw values the
+// insertion takes are R(I, jp) cx(I) / N^2 with cx = 2 for 0 < I < NX and 1 at
+// I = 0 and NX (the columns irfft2 counts twice), and in column 0 the
+// transpose of the completion: R(0, jp) + conj(R(0, N - jp)) for 0 < jp < NX,
+// 0 for jp > NX.  One thread per entry; column 0's pairs (jp, N - jp) by the
+// thread of jp.
+__global__ __launch_bounds__(TPB) void adjoint_spectrum_kernel(float2* __restrict__ spec, int N, int H, int nx,
+                                                               size_t n) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    const size_t per = (size_t)N * H;
+    const size_t r = idx % per;
+    const int jp = (int)(r / H), I = (int)(r - (size_t)jp * H);
+    const float inv = 1.0f / (float)((double)N * (double)N);   // a power of two: exact
+    if (I > 0) {
+        const float2 v = spec[idx];
+        const float sc = I < nx ? 2.0f * inv : inv;
+        spec[idx] = make_float2(v.x * sc, v.y * sc);
+    } else if (jp == 0 || jp == nx) {
+        const float2 v = spec[idx];
+        spec[idx] = make_float2(v.x * inv, v.y * inv);
+    } else if (jp < nx) {
+        float2* m = spec + (idx - r) + (size_t)(N - jp) * H;
+        const float2 a = spec[idx], b = *m;
+        spec[idx] = make_float2(__fadd_rn(a.x * inv, b.x * inv), __fsub_rn(a.y * inv, b.y * inv));
+        *m = make_float2(0.0f, 0.0f);
+    }
+}
+
 // BI(I, J) -> the raw R2C value s that spectrum_value() maps to it:
 // BI = c conj(s), c = (-1)^(I + jp), so s = c conj(BI).
 __global__ __launch_bounds__(TPB) void bi_to_raw_kernel(const float2* __restrict__ bi, int N, int H, size_t n,

@@ -589,16 +589,44 @@ int run_insert(spbp_ctx* c, int img0, const Geom* geom_d, int ng, const Targets&
 }
 
 // PADD2 + batched forward FFT of b images into c->spec.
+// The own two-pass R2C of b zero-padded images minus c->fmean[i] into c->spec
+// (power-of-two N); c->vmax receives each spectrum's largest |Re|, |Im| and
+// must be zero on entry.
+int own_fft(spbp_ctx* c, const void* src, int dtype, int b, hipStream_t st) {
+    const size_t lds = fft_lds_bytes(c->N);
+    const int M = std::min(FFT_ELEMS / c->N, c->nx / 2);
+    const int cw = FFT_ELEMS / c->N;
+    const size_t npix = (size_t)c->nx * c->nx, nspec = (size_t)c->N * c->H;
+    const size_t esz = dtype == SPBP_BF16 ? 2 : 4;
+    // sub-batches whose row-pass output (~128 MiB) is still in the
+    // last-level cache when the column pass reads it back
+    const int sb = (int)std::max<size_t>(1, ((size_t)128 << 20) / ((size_t)c->nx * c->H * sizeof(float2)));
+    for (int s0 = 0; s0 < b; s0 += sb) {
+        const int n = std::min(sb, b - s0);
+        const void* im = (const char*)src + s0 * npix * esz;
+        float2* sp = c->spec + s0 * nspec;
+        {
+            Timed tm(c, SPBP_K_PAD, n, st);
+            const dim3 rgrid((c->nx / 2 + M - 1) / M, n);
+            if (dtype == SPBP_BF16)
+                row_fft_kernel<uint16_t><<<rgrid, TPB, lds, st>>>((const uint16_t*)im, c->nx, c->N, c->fft_log2, c->tw,
+                                                                  c->fmean + s0, sp);
+            else
+                row_fft_kernel<float><<<rgrid, TPB, lds, st>>>((const float*)im, c->nx, c->N, c->fft_log2, c->tw,
+                                                               c->fmean + s0, sp);
+        }
+        {
+            Timed tm(c, SPBP_K_FFT2, n, st);
+            col_fft_kernel<<<dim3((c->H + cw - 1) / cw, n), TPB, lds, st>>>(c->nx, c->N, c->fft_log2, c->tw, sp,
+                                                                           c->vmax + s0);
+        }
+    }
+    HIPCHK(c, hipGetLastError());
+    return SPBP_OK;
+}
+
 int pad_fft(spbp_ctx* c, const void* src, int dtype, int b, hipStream_t st) {
     if (c->fft_log2) {
-        const size_t lds = fft_lds_bytes(c->N);
-        const int M = std::min(FFT_ELEMS / c->N, c->nx / 2);
-        const int cw = FFT_ELEMS / c->N;
-        const size_t npix = (size_t)c->nx * c->nx, nspec = (size_t)c->N * c->H;
-        const size_t esz = dtype == SPBP_BF16 ? 2 : 4;
-        // sub-batches whose row-pass output (~128 MiB) is still in the
-        // last-level cache when the column pass reads it back
-        const int sb = (int)std::max<size_t>(1, ((size_t)128 << 20) / ((size_t)c->nx * c->H * sizeof(float2)));
         {
             Timed tm(c, SPBP_K_PAD, b, st);   // the means of the whole chunk (one block per image)
             if (dtype == SPBP_BF16)
@@ -606,28 +634,7 @@ int pad_fft(spbp_ctx* c, const void* src, int dtype, int b, hipStream_t st) {
             else
                 pad_mean_kernel<float><<<b, TPB, 0, st>>>((const float*)src, c->nx, c->fft_log2 - 1, c->fmean, c->vmax);
         }
-        for (int s0 = 0; s0 < b; s0 += sb) {
-            const int n = std::min(sb, b - s0);
-            const void* im = (const char*)src + s0 * npix * esz;
-            float2* sp = c->spec + s0 * nspec;
-            {
-                Timed tm(c, SPBP_K_PAD, n, st);
-                const dim3 rgrid((c->nx / 2 + M - 1) / M, n);
-                if (dtype == SPBP_BF16)
-                    row_fft_kernel<uint16_t><<<rgrid, TPB, lds, st>>>((const uint16_t*)im, c->nx, c->N, c->fft_log2, c->tw,
-                                                                      c->fmean + s0, sp);
-                else
-                    row_fft_kernel<float><<<rgrid, TPB, lds, st>>>((const float*)im, c->nx, c->N, c->fft_log2, c->tw,
-                                                                   c->fmean + s0, sp);
-            }
-            {
-                Timed tm(c, SPBP_K_FFT2, n, st);
-                col_fft_kernel<<<dim3((c->H + cw - 1) / cw, n), TPB, lds, st>>>(c->nx, c->N, c->fft_log2, c->tw, sp,
-                                                                               c->vmax + s0);
-            }
-        }
-        HIPCHK(c, hipGetLastError());
-        return SPBP_OK;
+        return own_fft(c, src, dtype, b, st);
     }
     {
         Timed tm(c, SPBP_K_PAD, b, st);
@@ -1193,6 +1200,37 @@ int spbp_scatter(spbp_ctx* c, const float* bi, int nimg, const double* angles_de
         const int b = std::min(c->fwd_batch, nimg - p0);
         const size_t n = (size_t)b * N * H;
         bi_to_raw_kernel<<<nblocks(n), TPB, 0, st>>>((const float2*)bi + (size_t)p0 * N * H, N, H, n, c->spec);
+        spec_absmax_kernel<<<b, TPB, 0, st>>>(c->spec, N * H, c->vmax);
+        HIPCHK(c, hipGetLastError());
+        if ((rc = run_insert(c, p0, c->geom_d + p0, b, t, st, b))) return rc;
+    }
+    return SPBP_OK;
+}
+
+int spbp_scatter_images(spbp_ctx* c, const float* imgs, int nimg, const double* angles_deg, const float* dm,
+                        float* X, void* stream) {
+    if (!c) return fail(nullptr, SPBP_ERR_ARG, "NULL context");
+    if (nimg < 0) return fail(c, SPBP_ERR_ARG, "negative projection count %d", nimg);
+    if (nimg == 0) return SPBP_OK;
+    if (!imgs || !X) return fail(c, SPBP_ERR_ARG, "NULL image or grid pointer");
+    if (!c->fft_log2) return fail(c, SPBP_ERR_SIZE, "the image adjoint needs a power-of-two padded size (N = %d)", c->N);
+    DeviceGuard dg(c->device);
+    hipStream_t st = (hipStream_t)stream;
+    int rc = stage_geom(c, nimg, dm, angles_deg, nullptr, st, false);
+    if (rc) return rc;
+    if ((rc = ensure_chunk(c, std::min(c->batch, nimg)))) return rc;
+    const int N = c->N, H = c->H;
+    const size_t npix = (size_t)c->nx * c->nx;
+    const Targets t{(float2*)X, nullptr, nullptr, nullptr, nullptr, nullptr};
+    for (int p0 = 0; p0 < nimg; p0 += c->fwd_batch) {
+        const int b = std::min(c->fwd_batch, nimg - p0);
+        const size_t n = (size_t)b * N * H;
+        // no PADD2 mean: zero means (all-zero bits are 0.0) and zero maxima
+        fill_kernel<<<nblocks(2 * (size_t)b), TPB, 0, st>>>((float*)c->fmean, 2 * (size_t)b, 0.0f);
+        fill_kernel<<<nblocks(b), TPB, 0, st>>>(c->vmax, b, 0.0f);
+        HIPCHK(c, hipGetLastError());
+        if ((rc = own_fft(c, imgs + (size_t)p0 * npix, SPBP_F32, b, st))) return rc;
+        adjoint_spectrum_kernel<<<nblocks(n), TPB, 0, st>>>(c->spec, N, H, c->nx, n);
         spec_absmax_kernel<<<b, TPB, 0, st>>>(c->spec, N * H, c->vmax);
         HIPCHK(c, hipGetLastError());
         if ((rc = run_insert(c, p0, c->geom_d + p0, b, t, st, b))) return rc;

@@ -128,6 +128,11 @@ class Projector:
             dm = _lib.rotations(angles, exact_pi=exact_pi)
         self.dm = np.ascontiguousarray(np.asarray(dm, np.float32).reshape(-1, 9))
         self.batch = int(batch)
+        # adjoint(): the library pads, transforms and folds the residual images
+        # itself (power-of-two N); False runs those steps as torch operations
+        # before the library's scatter (the chain the library's version is
+        # checked against)
+        self.library_adjoint = True
         k = torch.as_tensor(kernel_transform(self.nx), dtype=torch.float64)
         self.D = (k[:, None, None] * k[None, :, None] * k[None, None, :]).to(torch.float32).to(self.device)
         N, H = self.N, self.H
@@ -201,6 +206,15 @@ class Projector:
         N, nx, ip = self.N, self.nx, self.ip
         with torch.no_grad():
             X = torch.zeros((N, N, self.H), dtype=torch.complex64, device=self.device)
+            if self.library_adjoint and N & (N - 1) == 0:
+                # the library's own padded transforms and spectrum folding (power-of-two N)
+                rr = r.to(torch.float32).contiguous()
+                for i0 in range(0, rr.shape[0], self.batch):
+                    i1 = min(rr.shape[0], i0 + self.batch)
+                    self.ctx.scatter_images(rr[i0:i1], dm=self.dm[i0:i1], X=X)
+                F = torch.conj(X * self.chk3) / self.cx
+                up = torch.fft.irfftn(F, s=(N, N, N)) * float(N) ** 3
+                return up[ip:ip + nx, ip:ip + nx, ip:ip + nx] / self.D
             for i0 in range(0, r.shape[0], self.batch):
                 i1 = min(r.shape[0], i0 + self.batch)
                 g = torch.zeros((i1 - i0, N, N), dtype=torch.float32, device=self.device)
