@@ -150,6 +150,17 @@ def test_library_adjoint_matches_torch_chain_and_fp64(mods, nx):
         P.close()
 
 
+def test_library_adjoint_refuses_other_sizes(mods):
+    """spbp_scatter_images needs the own transforms (power-of-two N): a 48^2
+    padded size is refused with SPBP_ERR_SIZE, and the projector's adjoint
+    then runs its torch chain."""
+    bp, project = mods
+    with bp.Backprojector(24) as ctx:
+        with pytest.raises(_lib.SpbpError) as e:
+            ctx.scatter_images(torch.zeros((2, 24, 24), device="cuda"), dm=_lib.rotations(np.zeros((2, 3))))
+        assert e.value.code == _lib.ERR_SIZE
+
+
 def test_gradient_against_fp64_finite_differences(mods):
     """32^3: the hand-written backward (scatter) through autograd against
     central differences of the FP64 loss along random directions."""
