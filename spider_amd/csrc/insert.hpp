@@ -567,6 +567,19 @@ __device__ __forceinline__ void wave_project(BrickLds<BX, BY, BZ, NT>& L, const 
     }
 }
 
+// Lattice lines (rows b of the sample lattice) of projection g crossing the box: [b0, b0 + nl).
+__device__ inline void box_lines(const Geom& g, const Box& box, int& b0, int& nl) {
+    float bmn = 0.f, bmx = 0.f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float c = g.r[3 + k];
+        bmn += fminf(c * box.lo[k], c * box.hi[k]);
+        bmx += fmaxf(c * box.lo[k], c * box.hi[k]);
+    }
+    b0 = (int)ceilf(bmn - 0.01f);
+    nl = (int)floorf(bmx + 0.01f) - b0 + 1;
+}
+
 // Owner-computes bricks.  The LDS region is BX x BY x BZ voxels; its core
 // (BX - 4) x (BY - 4) x (BZ - 4) starts 2 voxels in from the region's origin.
 // A sample belongs to the brick whose core holds its nearest voxel (ix, iy,
@@ -581,24 +594,13 @@ __device__ __forceinline__ void wave_project(BrickLds<BX, BY, BZ, NT>& L, const 
 // MIRPASS false: owned samples, direct taps (XCHK: drop taps with p_x < 0,
 // needed only at kx = 0).  MIRPASS true: the reflected taps whose target lies
 // in the core.  (x0, y0, z0): the core's origin.
-// Lattice lines (rows b of the sample lattice) of projection g crossing the box: [b0, b0 + nl).
-__device__ inline void box_lines(const Geom& g, const Box& box, int& b0, int& nl) {
-    float bmn = 0.f, bmx = 0.f;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const float c = g.r[3 + k];
-        bmn += fminf(c * box.lo[k], c * box.hi[k]);
-        bmx += fmaxf(c * box.lo[k], c * box.hi[k]);
-    }
-    b0 = (int)ceilf(bmn - 0.01f);
-    nl = (int)floorf(bmx + 0.01f) - b0 + 1;
-}
-
+//
 // Two projections, A and B, share the wave's rounds of candidates: lanes
-// 0..31 hold A's lattice lines and lanes 32..63 B's (each projection's lines
-// across a box of this size number at most edge x sqrt(3) + 2 <= 32), so the
-// candidates of both fill the rounds of 64 lanes.  Without B (hasB false)
-// its lines are empty.  The caller checks nlA, nlB <= 32.
+// 0..31 hold A's lattice lines and lanes 32..63 B's, so the candidates of
+// both fill the rounds of 64 lanes.  Without B (hasB false) its lines are
+// empty.  nlA, nlB <= 32: a rotation's section crosses at most the box
+// diagonal + 2 lines, which the kernel's static_assert bounds for its region
+// sizes (rotations are checked orthonormal on the host).
 template <int BX, int BY, int BZ, bool MIRPASS, bool XCHK, typename P, int NT>
 __device__ __forceinline__ void wave_project_own(BrickLds<BX, BY, BZ, NT>& L, const Geom& gA, const Geom& gB,
                                                  bool hasB, const float2* __restrict__ specA,
