@@ -118,6 +118,14 @@ int spbp_gather(spbp_ctx* ctx, const float* X, int nimg, const double* angles_de
  * bp32f.f:802-913) minus PADD2 and the forward FFT. */
 int spbp_scatter(spbp_ctx* ctx, const float* bi, int nimg, const double* angles_deg, const float* dm,
                  float* X, float* W, void* stream);
+/* The projector applied to a grid: for nimg rotations, the gather of X
+ * (spbp_gather's BI spectra), completed to the full C2R input (conjugated,
+ * checkerboard (-1)^(I + jp), column 0 by Hermitian symmetry), transformed
+ * by hipFFT C2R, scaled by 1 / N^2 and cropped to NX x NX at the padding
+ * offset: nimg device images of NX x NX fp32.  No counterpart in the
+ * reference (it has no projector matched to ONELINE). */
+int spbp_gather_images(spbp_ctx* ctx, const float* X, int nimg, const double* angles_deg, const float* dm,
+                       float* imgs, void* stream);
 /* The projector's adjoint applied to real images, for power-of-two padded
  * sizes: the scatter (no weights) of the spectra A^T's chain gives for nimg
  * device images of NX x NX fp32 -- the R2C transforms of the zero-padded

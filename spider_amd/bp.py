@@ -229,6 +229,27 @@ class Backprojector:
                                            C.c_void_p(out.data_ptr()), _stream(self.device)))
         return out
 
+    def gather_images(self, X, angles=None, dm=None, out=None):
+        """The central-slice projector applied to the half grid X: the gather,
+        completed by Hermitian symmetry, C2R-transformed and cropped to (n, NX,
+        NX) float32 images (into `out` when given)."""
+        torch = _torch()
+        if isinstance(X, torch.Tensor):
+            X = X.resolve_conj().resolve_neg().contiguous()
+        X = self._grid_tensor(X, "X")
+        n = (np.asarray(dm).reshape(-1, 9) if dm is not None else np.asarray(angles).reshape(-1, 3)).shape[0]
+        ang, dmc = self._rotation_args(n, angles, dm)
+        if out is None:
+            out = torch.empty((n, self.nx, self.nx), dtype=torch.float32, device=self.device)
+        elif out.dtype != torch.float32 or tuple(out.shape) != (n, self.nx, self.nx) or not out.is_contiguous() \
+                or out.device != self.device:
+            raise ValueError(f"out must be contiguous float32 ({n}, {self.nx}, {self.nx}) on {self.device}")
+        self._check(_lib.lib().spbp_gather_images(self._ctx, C.c_void_p(X.data_ptr()), n,
+                                                  None if ang is None else ang.ctypes.data_as(C.c_void_p),
+                                                  None if dmc is None else dmc.ctypes.data_as(C.c_void_p),
+                                                  C.c_void_p(out.data_ptr()), _stream(self.device)))
+        return out
+
     def scatter_images(self, imgs, angles=None, dm=None, X=None):
         """The central-slice projector's adjoint applied to real images (n, NX,
         NX) float32 on the context's device, as spectra added into X (a new

@@ -132,6 +132,38 @@ __global__ __launch_bounds__(GT * GT) void gather_kernel(const float2* __restric
     *o = make_float2((float)re, (float)(flip ? -im : im));
 }
 
+// The projector's image synthesis from gathered BI spectra, before the C2R
+// transform: in place, C(I, jp) = conj(BI(I, jp) (-1)^(I + jp)), and column 0
+// completed by Hermitian symmetry, C(0, N - jp) = conj(C(0, jp)) for
+// 0 < jp < NX (the rows J < 0 of column 0 are not gathered).  One thread per
+// entry; column 0's pairs by the thread of jp.
+__global__ __launch_bounds__(TPB) void bi_complete_kernel(float2* __restrict__ spec, int N, int H, int nx, size_t n) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    const size_t per = (size_t)N * H;
+    const size_t r = idx % per;
+    const int jp = (int)(r / H), I = (int)(r - (size_t)jp * H);
+    if (I == 0 && jp > nx) return;   // written by the thread of N - jp
+    const float sg = ((I + jp) & 1) ? -1.0f : 1.0f;
+    const float2 v = spec[idx];
+    const float2 c = make_float2(v.x * sg, -(v.y * sg));
+    spec[idx] = c;
+    if (I == 0 && jp > 0 && jp < nx) spec[(idx - r) + (size_t)(N - jp) * H] = make_float2(c.x, -c.y);
+}
+
+// Crop of the C2R outputs (N x N each, unnormalised) to the NX x NX images at
+// offset IP, scaled by 1 / N^2 (the inverse transform's normalisation).
+__global__ __launch_bounds__(TPB) void crop_images_kernel(const float* __restrict__ full, int N, int nx, int ip,
+                                                          size_t n, float* __restrict__ out) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    const size_t npix = (size_t)nx * nx;
+    const size_t img = idx / npix, q = idx - img * npix;
+    const int y = (int)(q / nx), x = (int)(q - (size_t)y * nx);
+    const float inv = 1.0f / (float)((double)N * (double)N);
+    out[idx] = full[(img * N + (size_t)(ip + y)) * N + (size_t)(ip + x)] * inv;
+}
+
 // Spectra for the projector's adjoint applied to real images r (the transpose
 // of images = crop(irfft2(BI completed by Hermitian symmetry in column 0))):
 // in place on the R2C spectra R of the zero-padded images, the raw values the

@@ -87,6 +87,11 @@ struct spbp_ctx {
     int fwd_batch = 0;
     hipfftHandle fwd_tail = 0;
     int tail_batch = 0;
+    // projector image synthesis (spbp_gather_images): C2R plan and its output
+    hipfftHandle c2r = 0;
+    int c2r_batch = 0;
+    float* c2rbuf = nullptr;
+    int c2r_cap = 0;
     // own forward transforms (power-of-two N): e^{-2 pi i m / N}, m < N
     float2* tw = nullptr;
     int fft_log2 = 0;   // log2 N, or 0 when hipFFT does the forward transforms
@@ -1126,6 +1131,8 @@ int spbp_destroy(spbp_ctx* c) {
     if (c->fwd) hipfftDestroy(c->fwd);
     if (c->fwd_tail) hipfftDestroy(c->fwd_tail);
     if (c->inv) hipfftDestroy(c->inv);
+    if (c->c2r) hipfftDestroy(c->c2r);
+    if (c->c2rbuf) (void)hipFree(c->c2rbuf);
     if (c->geom_evt) (void)hipEventDestroy(c->geom_evt);
     if (c->jobs_evt) (void)hipEventDestroy(c->jobs_evt);
     spbp_reset_stats(c);
@@ -1203,6 +1210,54 @@ int spbp_scatter(spbp_ctx* c, const float* bi, int nimg, const double* angles_de
         spec_absmax_kernel<<<b, TPB, 0, st>>>(c->spec, N * H, c->vmax);
         HIPCHK(c, hipGetLastError());
         if ((rc = run_insert(c, p0, c->geom_d + p0, b, t, st, b))) return rc;
+    }
+    return SPBP_OK;
+}
+
+int spbp_gather_images(spbp_ctx* c, const float* X, int nimg, const double* angles_deg, const float* dm,
+                       float* imgs, void* stream) {
+    if (!c) return fail(nullptr, SPBP_ERR_ARG, "NULL context");
+    if (nimg < 0) return fail(c, SPBP_ERR_ARG, "negative projection count %d", nimg);
+    if (nimg == 0) return SPBP_OK;
+    if (!X || !imgs) return fail(c, SPBP_ERR_ARG, "NULL grid or image pointer");
+    DeviceGuard dg(c->device);
+    hipStream_t st = (hipStream_t)stream;
+    int rc = stage_geom(c, nimg, dm, angles_deg, nullptr, st, false);
+    if (rc) return rc;
+    if ((rc = ensure_chunk(c, std::min(c->batch, nimg)))) return rc;
+    const int N = c->N, H = c->H, B = c->fwd_batch;
+    if (c->c2r_cap < B) {
+        if (c->c2rbuf) { (void)hipFree(c->c2rbuf); c->c2rbuf = nullptr; }
+        HIPCHK(c, hipMalloc(&c->c2rbuf, (size_t)B * N * N * sizeof(float)));
+        c->c2r_cap = B;
+    }
+    const int ip = (N - c->nx) / 2 + c->nx % 2;
+    const unsigned gx = (unsigned)(((H + GT - 1) / GT) * ((N + GT - 1) / GT));
+    for (int p0 = 0; p0 < nimg; p0 += B) {
+        const int b = std::min(B, nimg - p0);
+        if (!c->c2r || c->c2r_batch != b) {
+            if (c->c2r) { hipfftDestroy(c->c2r); c->c2r = 0; }
+            int n2[2] = {N, N}, inembed[2] = {N, H}, onembed[2] = {N, N};
+            const hipfftResult r = hipfftPlanMany(&c->c2r, 2, n2, inembed, 1, N * H, onembed, 1, N * N, HIPFFT_C2R, b);
+            if (r != HIPFFT_SUCCESS) { c->c2r = 0; return fail(c, SPBP_ERR_FFT, "hipfftPlanMany(C2R %d): %d", b, (int)r); }
+            c->c2r_batch = b;
+        }
+        {
+            Timed tm(c, SPBP_K_GATHER, b, st);
+            for (int q0 = 0; q0 < b; q0 += 65535) {
+                const int bq = std::min(65535, b - q0);
+                gather_kernel<<<dim3(gx, bq), GT * GT, 0, st>>>((const float2*)X, c->geom_d + p0 + q0, c->nx, N, c->tab,
+                                                                 c->fltb, c->spec + (size_t)q0 * N * H);
+            }
+        }
+        const size_t n = (size_t)b * N * H;
+        bi_complete_kernel<<<nblocks(n), TPB, 0, st>>>(c->spec, N, H, c->nx, n);
+        HIPCHK(c, hipGetLastError());
+        FFTCHK(c, hipfftSetStream(c->c2r, st));
+        FFTCHK(c, hipfftExecC2R(c->c2r, (hipfftComplex*)c->spec, (hipfftReal*)c->c2rbuf));
+        const size_t m = (size_t)b * c->nx * c->nx;
+        crop_images_kernel<<<nblocks(m), TPB, 0, st>>>(c->c2rbuf, N, c->nx, ip, m, imgs + (size_t)p0 * c->nx * c->nx);
+        HIPCHK(c, hipGetLastError());
     }
     return SPBP_OK;
 }

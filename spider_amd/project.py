@@ -128,11 +128,13 @@ class Projector:
             dm = _lib.rotations(angles, exact_pi=exact_pi)
         self.dm = np.ascontiguousarray(np.asarray(dm, np.float32).reshape(-1, 9))
         self.batch = int(batch)
-        # adjoint(): the library pads, transforms and folds the residual images
-        # itself (power-of-two N); False runs those steps as torch operations
-        # before the library's scatter (the chain the library's version is
+        # apply() and adjoint(): the library synthesises the images from the
+        # gather (completion, C2R, crop) and pads, transforms and folds the
+        # residual images before the scatter (the adjoint for power-of-two N
+        # only); False runs those steps as torch operations around
+        # spbp_gather / spbp_scatter (the chain the library's versions are
         # checked against)
-        self.library_adjoint = True
+        self.library_chain = True
         k = torch.as_tensor(kernel_transform(self.nx), dtype=torch.float64)
         self.D = (k[:, None, None] * k[None, :, None] * k[None, None, :]).to(torch.float32).to(self.device)
         N, H = self.N, self.H
@@ -196,7 +198,10 @@ class Projector:
                 out = torch.empty((n, self.nx, self.nx), dtype=torch.float32, device=self.device)
             for i0 in range(0, n, self.batch):
                 i1 = min(n, i0 + self.batch)
-                out[i0:i1] = self.images_from_bi(self.ctx.gather(X, dm=self.dm[i0:i1]))
+                if self.library_chain:
+                    self.ctx.gather_images(X, dm=self.dm[i0:i1], out=out[i0:i1])
+                else:
+                    out[i0:i1] = self.images_from_bi(self.ctx.gather(X, dm=self.dm[i0:i1]))
         return out
 
     def adjoint(self, r):
@@ -206,7 +211,7 @@ class Projector:
         N, nx, ip = self.N, self.nx, self.ip
         with torch.no_grad():
             X = torch.zeros((N, N, self.H), dtype=torch.complex64, device=self.device)
-            if self.library_adjoint and N & (N - 1) == 0:
+            if self.library_chain and N & (N - 1) == 0:
                 # the library's own padded transforms and spectrum folding (power-of-two N)
                 rr = r.to(torch.float32).contiguous()
                 for i0 in range(0, rr.shape[0], self.batch):

@@ -140,12 +140,35 @@ def test_library_adjoint_matches_torch_chain_and_fp64(mods, nx):
         D = P.D.double().cpu().numpy()
         r = rng.standard_normal((P.nproj, nx, nx)).astype(np.float32)
         rt = torch.from_numpy(r).cuda()
-        P.library_adjoint = True
+        P.library_chain = True
         lib = P.adjoint(rt).cpu().numpy()
-        P.library_adjoint = False
+        P.library_chain = False
         chain = P.adjoint(rt).cpu().numpy()
         assert rel(lib, chain) <= 1e-6
         assert rel(lib, op.adjoint64(r, P.dm, D)) <= 1e-5
+    finally:
+        P.close()
+
+
+@pytest.mark.parametrize("nx", [16, 24, 64])
+def test_library_forward_matches_torch_chain_and_fp64(mods, nx):
+    """A v with the library's image synthesis (spbp_gather_images: completion,
+    C2R, crop) against the same steps as torch operations after spbp_gather
+    (rel-L2 <= 1e-6) and the FP64 forward chain (<= 1e-5), over several
+    library calls (batch 16); also for a padded size that is not a power of
+    two."""
+    _, project = mods
+    P, rng = _projector(project, nx, 40, 51 + nx)
+    try:
+        D = P.D.double().cpu().numpy()
+        v = rng.standard_normal((nx,) * 3).astype(np.float32)
+        vt = torch.from_numpy(v).cuda()
+        P.library_chain = True
+        lib = P.apply(vt).cpu().numpy()
+        P.library_chain = False
+        chain = P.apply(vt).cpu().numpy()
+        assert rel(lib, chain) <= 1e-6
+        assert rel(lib, op.forward64(v, P.dm, D)) <= 1e-5
     finally:
         P.close()
 
