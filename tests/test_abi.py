@@ -42,6 +42,7 @@ def test_create_rejects_bad_sizes_without_touching_a_device():
     assert L.spbp_calibrate(7, 0, C.byref(r), None) == _lib.ERR_ARG
     assert L.spbp_calibrate(_lib.CAL_LDS_U64, 0, None, None) == _lib.ERR_ARG
     assert L.spbp_scatter_images(None, None, 1, None, None, None, None) == _lib.ERR_ARG
+    assert L.spbp_gather_images(None, None, 1, None, None, None, None) == _lib.ERR_ARG
     assert L.spbp_destroy(None) == 0
 
 
