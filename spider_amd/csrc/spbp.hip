@@ -87,9 +87,10 @@ struct spbp_ctx {
     int fwd_batch = 0;
     hipfftHandle fwd_tail = 0;
     int tail_batch = 0;
-    // projector image synthesis (spbp_gather_images): C2R plan and its output
-    hipfftHandle c2r = 0;
-    int c2r_batch = 0;
+    // projector image synthesis (spbp_gather_images): C2R plans (two batch
+    // sizes: a call's full chunks and its tail) and their output
+    hipfftHandle c2r = 0, c2r2 = 0;
+    int c2r_batch = 0, c2r2_batch = 0;
     float* c2rbuf = nullptr;
     int c2r_cap = 0;
     // own forward transforms (power-of-two N): e^{-2 pi i m / N}, m < N
@@ -1132,6 +1133,7 @@ int spbp_destroy(spbp_ctx* c) {
     if (c->fwd_tail) hipfftDestroy(c->fwd_tail);
     if (c->inv) hipfftDestroy(c->inv);
     if (c->c2r) hipfftDestroy(c->c2r);
+    if (c->c2r2) hipfftDestroy(c->c2r2);
     if (c->c2rbuf) (void)hipFree(c->c2rbuf);
     if (c->geom_evt) (void)hipEventDestroy(c->geom_evt);
     if (c->jobs_evt) (void)hipEventDestroy(c->jobs_evt);
@@ -1235,8 +1237,15 @@ int spbp_gather_images(spbp_ctx* c, const float* X, int nimg, const double* angl
     const unsigned gx = (unsigned)(((H + GT - 1) / GT) * ((N + GT - 1) / GT));
     for (int p0 = 0; p0 < nimg; p0 += B) {
         const int b = std::min(B, nimg - p0);
-        if (!c->c2r || c->c2r_batch != b) {
-            if (c->c2r) { hipfftDestroy(c->c2r); c->c2r = 0; }
+        // a plan per batch size, two kept: the most recent becomes c2r
+        if (c->c2r2 && c->c2r2_batch == b) {
+            std::swap(c->c2r, c->c2r2);
+            std::swap(c->c2r_batch, c->c2r2_batch);
+        } else if (!c->c2r || c->c2r_batch != b) {
+            if (c->c2r2) hipfftDestroy(c->c2r2);
+            c->c2r2 = c->c2r;
+            c->c2r2_batch = c->c2r_batch;
+            c->c2r = 0;
             int n2[2] = {N, N}, inembed[2] = {N, H}, onembed[2] = {N, N};
             const hipfftResult r = hipfftPlanMany(&c->c2r, 2, n2, inembed, 1, N * H, onembed, 1, N * N, HIPFFT_C2R, b);
             if (r != HIPFFT_SUCCESS) { c->c2r = 0; return fail(c, SPBP_ERR_FFT, "hipfftPlanMany(C2R %d): %d", b, (int)r); }
