@@ -1,11 +1,15 @@
 #!/usr/bin/env python3
-"""Throughput of the Fourier-gridding backprojection (SPIDER `BP 3F`) on one GPU.
+"""Throughput of the Fourier-gridding backprojection (SPIDER `BP 32F`) on one GPU.
 
-A step = one whole reconstruction job over HBM-resident synthetic projections:
-reset the accumulators, insert every projection (pad + batched 2D FFT +
-KB scatter), finalise the volume (normalise + 3D inverse FFT + window).
-Default workload: 20,000 projections of 128x128 fp32 into the 256^3 padded
-grid (the recipe: 60-blob phantom, uniform orientations, SNR 0.1).
+A step = one whole `BP 32F` job over HBM-resident synthetic projections:
+reset the two half-set accumulator pairs, insert every projection into the
+half the RANDLIST split gives it (pad + 2D FFT + KB scatter), finalise the
+overall volume and the two sample volumes (SYMPLANE0 + normalise + 3D inverse
+FFT + window, bp32f.f:275-410).  Default workload: 20,000 projections of
+128x128 fp32 into the 256^3 padded grid (the recipe: 60-blob phantom, uniform
+orientations, SNR 0.1); the same line carries a `geom_f64` record (the job
+with FP64 sample geometry) and a `config5` record (100,000 projections of
+256x256 stored as BF16 into the 512^3 grid with compensated weights).
 
 Prints one JSON line.  `roofline` is for the insertion kernel: algorithmic
 bytes per projection B = 4 NX^2 + 24 U(NX) (image read + read/write of the
@@ -166,17 +170,165 @@ def atomic_roofline(kname, nx, per_launch, avg_launch_s, U, images_dtype, twosum
     return out
 
 
-def parity_subset(bpmod, imgs, ang, m, **kw):
+def parity_subset(bpmod, imgs, ang, m, half=None, **kw):
+    """rel-L2 of the library's volumes on the first m projections against the
+    FP64 and the FP32-faithful oracles: `BP 3F` (half None: the full volume)
+    or `BP 32F` (half given: the worst of the full and the two sample volumes)."""
     import numpy as np
     import oracle
     sub = imgs[:m].contiguous()
-    vol = bpmod.bp3f(sub, ang[:m], **kw).cpu().numpy().astype(np.float64)
+    if half is None:
+        vols = {"full": bpmod.bp3f(sub, ang[:m], **kw)}
+        hp = None
+    else:
+        hp = np.asarray(half[:m], np.int8)
+        f, h1, h2 = bpmod.bp32f(sub, ang[:m], half=hp, **kw)
+        vols = {"full": f, "h1": h1, "h2": h2}
+    vols = {k: v.cpu().numpy().astype(np.float64) for k, v in vols.items()}
     host = sub.float().cpu().numpy()   # BF16 images widen exactly
-    o64 = oracle.bp(host, ang[:m], prec="f64")["full"]
-    o32 = oracle.bp(host, ang[:m], prec="f32")["full"].astype(np.float64)
-    r64 = float(np.linalg.norm(vol - o64) / np.linalg.norm(o64))
-    r32 = float(np.linalg.norm(vol - o32) / np.linalg.norm(o32))
+    o64 = oracle.bp(host, ang[:m], half=hp, prec="f64")
+    o32 = oracle.bp(host, ang[:m], half=hp, prec="f32")
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    r64 = max(rel(v, o64[k]) for k, v in vols.items())
+    r32 = max(rel(v, o32[k].astype(np.float64)) for k, v in vols.items())
     return r64, r32
+
+
+def run_throughput(args, config, nproj, steps, warmup, op="bp32f", bf16=False, twosum=False, geom_f64=False,
+                   parity_proj=100, cpu=True, calib=True, tune=False, extras=True, cpu_seconds=None):
+    """One workload's throughput record.  A step is one whole job over the
+    HBM-resident synthetic projections: reset the accumulators, insert every
+    projection (pad + 2D FFT + lists + KB scatter; `BP 32F`: each into the
+    half RANDLIST gives it), finalise (`BP 32F`: the overall volume and the
+    two sample volumes)."""
+    import numpy as np
+    import torch
+    from spider_amd import _lib, synth
+    from spider_amd import bp as bpmod
+
+    rec = synth.CONFIGS[config]
+    nx = rec["nx"]
+    nproj = nproj or rec["nproj"]
+    log(f"bench: generating config {config}: {nproj} projections {nx}x{nx}")
+    ang, ph, imgs = synth.make_dataset(config, nproj=nproj, device="cuda")
+    if bf16:
+        imgs = imgs.to(torch.bfloat16)
+    halves = op == "bp32f"
+    half = bpmod.random_halves(nproj, seed=synth.config_seed(config)) if halves else None
+    torch.cuda.synchronize()
+
+    if args.copy_calib:
+        src = torch.empty(args.copy_calib << 20, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        log(f"bench: calibration copy of {args.copy_calib} MiB done")
+
+    batch = args.batch if args.batch else (nproj if args.one_launch else 0)
+    b = bpmod.Backprojector(nx, two_halves=halves, timing=True, batch=batch, strategy=args.strategy,
+                            brick_proj=args.brick_proj, twosum=twosum, geom_f64=geom_f64)
+    t_tune = None
+    if args.shape is not None:
+        b.set_plan("direct" if args.strategy == "direct" else "brick", args.shape, args.brick_proj)
+    if tune:
+        t0 = time.perf_counter()
+        b.tune(imgs[:1024], ang[:1024])
+        torch.cuda.synchronize()
+        t_tune = time.perf_counter() - t0
+    plan = b.plan()
+    log(f"bench: plan {plan['strategy']} brick {plan['brick']} C={plan['brick_proj']}")
+
+    def step():
+        b.reset()
+        b.insert(imgs, ang, half)
+        return b.finalize(full=True, halves=halves)
+
+    for w in range(warmup):
+        step()
+        torch.cuda.synchronize()
+        log(f"bench: warmup step {w + 1} done")
+    b.reset_stats()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    log(f"bench: {steps} timed steps done")
+    dt = t1 - t0
+    value = nproj * steps / dt
+    ins_ms, launches, units = b.kernel_stats(_lib.K_INSERT)
+    pad_ms, _, _ = b.kernel_stats(_lib.K_PAD)
+    fft_ms, _, _ = b.kernel_stats(_lib.K_FFT2)
+    fin_ms, nfin, _ = b.kernel_stats(_lib.K_FINALIZE)
+    bin_ms, _, _ = b.kernel_stats(_lib.K_BIN)
+    res_ms, nres, _ = b.kernel_stats(_lib.K_RESOLVE)
+    thresh = b.threshold_stats()
+    b.close()
+    opname = "BP 32F" if halves else "BP 3F"
+    out = {"value": round(value, 2), "unit": "projections/s", "n_gpus": 1,
+           "steps": steps, "warmup": warmup, "ms_per_step": round(dt / steps * 1e3, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "bf16 images, f32 accumulation" if bf16 else "f32",
+           "data": "synthetic (seeded blob phantom, analytic projections, white noise)",
+           "config": {"workload": f"{opname} plain, {nx}x{nx} {'bf16' if bf16 else 'fp32'} images, {nproj} projections, "
+                                  f"{2 * nx}^3 padded Fourier grid{', compensated weights' if twosum else ''}"
+                                  f"{', FP64 sample geometry' if geom_f64 else ''}"
+                                  f"{' (RANDLIST halves; overall + 2 sample volumes)' if halves else ''}"
+                                  f" (config {config})",
+                      "operation": opname, "nx": nx, "nproj": nproj, "grid": 2 * nx,
+                      "images_dtype": "bf16" if bf16 else "f32", "twosum_weights": twosum, "geom_f64": bool(geom_f64),
+                      "insert_launches_per_step": round(launches / steps, 2)},
+           "breakdown_ms_per_step": {"pad": round(pad_ms / steps, 3), "fft2": round(fft_ms / steps, 3),
+                                     "insert": round(ins_ms / steps, 3), "bin": round(bin_ms / steps, 3),
+                                     "finalize": round(fin_ms / steps, 3),
+                                     "nrmw2_resum": round(res_ms / steps, 3)},
+           "nrmw2_threshold": thresh,
+           "plan": {"strategy": plan["strategy"], "brick": list(plan["brick"]), "brick_proj": plan["brick_proj"],
+                    "owner_computes": plan.get("owner_computes", False),
+                    "tune_seconds": None if t_tune is None else round(t_tune, 3)}}
+    if not extras:
+        return out
+
+    U = distinct_voxels(bpmod, nx, synth.uniform_angles(16, np.random.default_rng(5)))
+    # bytes per projection: the image once, and the read + write of each
+    # distinct voxel's complex64 value and fp32 weight (+ fp32 compensation)
+    b_alg = (2.0 if bf16 else 4.0) * nx * nx + (32.0 if twosum else 24.0) * U
+    avg_launch_s = ins_ms / 1e3 / launches
+    per_launch = units / launches
+    achieved = b_alg * per_launch / avg_launch_s / 1e9
+    kname = ("insert_direct_kernel" if plan["strategy"] == "direct"
+             else "insert_brick_kernel<%d,%d,%d%s>" % (*plan["brick"], ",own" if plan.get("owner_computes") else ""))
+    idt = "bf16" if bf16 else "f32"
+    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(kname, nx, per_launch, idt, twosum),
+            "kernel": kname, "bytes_per_unit": round(b_alg), "unit_of_work": "projection",
+            "units_per_launch": round(per_launch, 1), "avg_launch_ms": round(avg_launch_s * 1e3, 4),
+            "U_distinct_voxels": round(U, 1)}
+    if calib:
+        cr = copy_rates()
+        roof.update(copy_GBps=cr, frac_of_copy_hbm=round(achieved / cr["hbm"], 5),
+                    frac_of_copy_cache=round(achieved / cr["cache"], 5))
+        if plan["strategy"] == "brick":
+            roof["atomics"] = atomic_roofline(kname, nx, per_launch, avg_launch_s, U, idt, twosum)
+    out["roofline"] = roof
+    if parity_proj:
+        log("bench: parity subset vs oracle")
+        m = min(parity_proj, nproj)
+        r64, r32 = parity_subset(bpmod, imgs, ang, m, half=half, twosum=twosum)
+        out["rel_l2_vs_fp64"] = r64
+        out["rel_l2_vs_ref_f32"] = r32
+        g64, _ = parity_subset(bpmod, imgs, ang, m, half=half, twosum=twosum, geom_f64=True)
+        out["rel_l2_vs_fp64_with_geom_f64"] = g64
+        out["parity_subset_proj"] = m
+    if cpu:
+        log("bench: cpu baseline")
+        m = min(nproj, 20000 if nx <= 128 else 2000)
+        out["cpu_baseline"] = cpu_baseline(imgs[:m].float().cpu().numpy(), ang[:m], cpu_seconds or args.cpu_seconds)
+        out["gpu_over_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
+    del imgs
+    return out
 
 
 def run_batched(args):
@@ -371,7 +523,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4, 5],
-                    help="2: plain BP 3F throughput (default); 3: 12 uneven BP 32F volumes, "
+                    help="2: plain BP 32F throughput (default; with an FP64-geometry record and a config-5 record: "
+                         "256^2 BF16 x 100,000 into 512^3, compensated weights); 1 / 5: that workload alone; 3: 12 uneven BP 32F volumes, "
                          "plain one-by-one vs batched (tuned plans, BF16 images, one captured graph); "
                          "4: CG reconstruction with the gather projector beside BP 32F")
     ap.add_argument("--volumes", type=int, default=12, help="config 3: number of volumes")
@@ -394,7 +547,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-calib", action="store_true", help="skip the copy-rate and atomic-rate calibrations")
     ap.add_argument("--no-parity", action="store_true")
+    ap.add_argument("--bp3f", action="store_true", help="time BP 3F (one volume) instead of BP 32F")
+    ap.add_argument("--one-launch", action="store_true",
+                    help="one chunk per job: every projection's spectrum resident, one persistent scatter launch")
+    ap.add_argument("--no-extra", action="store_true", help="config 2: skip the FP64-geometry and config-5 records")
+    ap.add_argument("--no-config5", action="store_true", help="config 2: skip the config-5 record")
+    ap.add_argument("--config5-nproj", type=int, default=None, help="config-5 record: projection count")
+    ap.add_argument("--config5-steps", type=int, default=3)
+    ap.add_argument("--config5-parity-proj", type=int, default=60)
     args = ap.parse_args()
+    if args.no_parity:
+        args.parity_proj = 0
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -416,119 +579,27 @@ def main():
     if args.config == 3:
         run_batched(args)
         return
-    rec = synth.CONFIGS[args.config]
-    nx = rec["nx"]
-    nproj = args.nproj or rec["nproj"]
-    log(f"bench: generating config {args.config}: {nproj} projections {nx}x{nx}")
-    ang, ph, imgs = synth.make_dataset(args.config, nproj=nproj, device="cuda")
-    bf16 = args.bf16 or args.config == 5
-    twosum = args.twosum or args.config == 5
-    if bf16:
-        imgs = imgs.to(torch.bfloat16)
-    torch.cuda.synchronize()
-
-    if args.copy_calib:
-        src = torch.empty(args.copy_calib << 20, dtype=torch.uint8, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
-        log(f"bench: calibration copy of {args.copy_calib} MiB done")
-
-    b = bpmod.Backprojector(nx, timing=True, batch=args.batch, strategy=args.strategy, brick_proj=args.brick_proj,
-                            twosum=twosum, geom_f64=args.geom_f64)
-    t_tune = None
-    if args.shape is not None:
-        b.set_plan("direct" if args.strategy == "direct" else "brick", args.shape, args.brick_proj)
-    if args.tune:
-        t0 = time.perf_counter()
-        b.tune(imgs[:1024], ang[:1024])
-        torch.cuda.synchronize()
-        t_tune = time.perf_counter() - t0
-    plan = b.plan()
-    log(f"bench: plan {plan['strategy']} brick {plan['brick']} C={plan['brick_proj']}")
-
-    def step():
-        b.reset()
-        b.insert(imgs, ang)
-        return b.finalize()["full"]
-
-    for w in range(args.warmup):
-        step()
-        torch.cuda.synchronize()
-        log(f"bench: warmup step {w + 1} done")
-    b.reset_stats()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for s in range(args.steps):
-        vol = step()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    log(f"bench: {args.steps} timed steps done")
-    dt = t1 - t0
-    value = nproj * args.steps / dt
-    ins_ms, launches, units = b.kernel_stats(_lib.K_INSERT)
-    pad_ms, _, _ = b.kernel_stats(_lib.K_PAD)
-    fft_ms, _, _ = b.kernel_stats(_lib.K_FFT2)
-    fin_ms, nfin, _ = b.kernel_stats(_lib.K_FINALIZE)
-    bin_ms, _, _ = b.kernel_stats(_lib.K_BIN)
-    res_ms, nres, _ = b.kernel_stats(_lib.K_RESOLVE)
-    thresh = b.threshold_stats()
-    b.close()
-
-    U = distinct_voxels(bpmod, nx, synth.uniform_angles(16, np.random.default_rng(5)))
-    # bytes per projection: the image once, and the read + write of each
-    # distinct voxel's complex64 value and fp32 weight (+ fp32 compensation)
-    b_alg = (2.0 if bf16 else 4.0) * nx * nx + (32.0 if twosum else 24.0) * U
-    avg_launch_s = ins_ms / 1e3 / launches
-    per_launch = units / launches
-    achieved = b_alg * per_launch / avg_launch_s / 1e9
-    kname = ("insert_direct_kernel" if plan["strategy"] == "direct"
-             else "insert_brick_kernel<%d,%d,%d%s>" % (*plan["brick"], ",own" if plan.get("owner_computes") else ""))
-    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(kname, nx, per_launch, "bf16" if bf16 else "f32", twosum),
-            "kernel": kname, "bytes_per_unit": round(b_alg), "unit_of_work": "projection",
-            "units_per_launch": round(per_launch, 1), "avg_launch_ms": round(avg_launch_s * 1e3, 4),
-            "U_distinct_voxels": round(U, 1)}
-    if not args.no_calib:
-        cr = copy_rates()
-        roof.update(copy_GBps=cr, frac_of_copy_hbm=round(achieved / cr["hbm"], 5),
-                    frac_of_copy_cache=round(achieved / cr["cache"], 5))
-        if plan["strategy"] == "brick":
-            roof["atomics"] = atomic_roofline(kname, nx, per_launch, avg_launch_s, U, "bf16" if bf16 else "f32", twosum)
-
-    out = {"metric": METRIC, "value": round(value, 2), "unit": "projections/s", "n_gpus": 1,
-           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 3),
-           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": "bf16 images, f32 accumulation" if bf16 else "f32",
-           "data": "synthetic (seeded blob phantom, analytic projections, white noise)",
-           "config": {"workload": f"BP 3F plain, {nx}x{nx} {'bf16' if bf16 else 'fp32'} images, {nproj} projections, "
-                                  f"{2 * nx}^3 padded Fourier grid{', compensated weights' if twosum else ''} "
-                                  f"(config {args.config})",
-                      "nx": nx, "nproj": nproj, "grid": 2 * nx, "images_dtype": "bf16" if bf16 else "f32",
-                      "twosum_weights": twosum, "geom_f64": bool(args.geom_f64)},
-           "breakdown_ms_per_step": {"pad": round(pad_ms / args.steps, 3), "fft2": round(fft_ms / args.steps, 3),
-                                     "insert": round(ins_ms / args.steps, 3), "bin": round(bin_ms / args.steps, 3),
-                                     "finalize": round(fin_ms / max(nfin, 1), 3),
-                                     "nrmw2_resum": round(res_ms / max(nres, 1), 3)},
-           "nrmw2_threshold": thresh,
-           "roofline": roof,
-           "plan": {"strategy": plan["strategy"], "brick": list(plan["brick"]), "brick_proj": plan["brick_proj"],
-                    "owner_computes": plan.get("owner_computes", False),
-                    "tune_seconds": None if t_tune is None else round(t_tune, 3)}}
-    if not args.no_parity:
-        log("bench: parity subset vs oracle")
-        r64, r32 = parity_subset(bpmod, imgs, ang, min(args.parity_proj, nproj), twosum=twosum)
-        out["rel_l2_vs_fp64"] = r64
-        out["rel_l2_vs_ref_f32"] = r32
-        g64, _ = parity_subset(bpmod, imgs, ang, min(args.parity_proj, nproj), twosum=twosum, geom_f64=True)
-        out["rel_l2_vs_fp64_with_geom_f64"] = g64
-        out["parity_subset_proj"] = min(args.parity_proj, nproj)
-    if not args.no_cpu:
-        log("bench: cpu baseline")
-        m = min(nproj, 20000 if nx <= 128 else 2000)
-        out["cpu_baseline"] = cpu_baseline(imgs[:m].float().cpu().numpy(), ang[:m], args.cpu_seconds)
-        out["gpu_over_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
+    op = "bp3f" if args.bp3f else "bp32f"
+    out = run_throughput(args, args.config, args.nproj, args.steps, args.warmup, op=op,
+                         bf16=args.bf16 or args.config == 5, twosum=args.twosum or args.config == 5,
+                         geom_f64=args.geom_f64, parity_proj=args.parity_proj, cpu=not args.no_cpu,
+                         calib=not args.no_calib, tune=args.tune)
+    out = {"metric": METRIC, **out}
+    if args.config == 2 and not args.no_extra:
+        # the same BP 32F job with FP64 sample geometry (the mode that meets the
+        # FP64 bar), and the 256^2 BF16 -> 512^3 compensated-weights workload
+        log("bench: FP64-geometry BP 32F")
+        g = run_throughput(args, 2, args.nproj, max(2, min(args.steps, 5)), 1, op=op, bf16=False, twosum=False,
+                           geom_f64=True, parity_proj=0, cpu=False, calib=False, tune=False, extras=False)
+        out["geom_f64"] = {k: g[k] for k in ("value", "unit", "ms_per_step", "steps", "config")}
+        out["geom_f64"]["rel_l2_vs_fp64"] = out.get("rel_l2_vs_fp64_with_geom_f64")
+        if not args.no_config5:
+            torch.cuda.empty_cache()
+            log("bench: config 5")
+            c5 = run_throughput(args, 5, args.config5_nproj, args.config5_steps, 1, op=op, bf16=True, twosum=True,
+                                geom_f64=False, parity_proj=args.config5_parity_proj, cpu=not args.no_cpu,
+                                calib=False, tune=False, cpu_seconds=min(args.cpu_seconds, 12.0))
+            out["config5"] = c5
     print(json.dumps(out), flush=True)
 
 
