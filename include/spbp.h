@@ -230,8 +230,10 @@ int spbp_rtsq(const void* imgs, int dtype, int nimg, int nx, const float* params
  * With nsym > 1 every image is inserted once per DMS = MATMUL(SM_k, DM)
  * (bp32f.f:866-872) and every finalised volume goes through SYMVOL
  * (symvol.f:39, bp32f.f:278-303).  nsym <= 1 switches symmetries off, as a
- * one-row or absent document does in the reference.  Applies to inserts made
- * after the call; the gather / scatter entry points ignore it. */
+ * one-row or absent document does in the reference.  One set of symmetries
+ * serves a whole operation (insertion and SYMVOL): the call fails with
+ * SPBP_ERR_ARG once the accumulators hold sums (after an insert or import,
+ * until spbp_reset).  The gather / scatter entry points ignore it. */
 int spbp_set_symmetries(spbp_ctx* ctx, int nsym, const double* angles_deg);
 /* ---- batching, tuning and graphs (no counterpart in the reference: its
  * projection loop is one OpenMP loop per operation, bp32f.f:802-913) ---- */
@@ -269,9 +271,14 @@ int spbp_insert_batch(const spbp_job* jobs, int njobs, void* stream);
 /* A batch captured as a HIP graph (pad, FFT, list build, persistent kernel;
  * with reset_first, zeroing of the accumulators first).  The graph keeps its
  * own copy of the geometry; images are read from the jobs' device pointers at
- * every launch.  The contexts must stay alive and keep their plan and chunk
- * size while the graph exists; other calls on them in between are allowed.
- * spbp_graph_launch replays it on `stream`. */
+ * every launch.  The graph replays work on its contexts' chunk buffers,
+ * brick lists and forward FFT plans, so while it exists every call that
+ * would free or replace them fails with SPBP_ERR_ARG: spbp_destroy,
+ * spbp_tune, spbp_set_symmetries, spbp_set_plan with another brick shape,
+ * and insert / scatter / gather calls that need larger chunk buffers,
+ * longer lists or a forward FFT plan for a new chunk size.  Calls that fit
+ * the existing buffers may run between launches.  spbp_graph_launch replays
+ * it on `stream`; spbp_graph_destroy releases the contexts. */
 typedef struct spbp_graph spbp_graph;
 int spbp_graph_create(const spbp_job* jobs, int njobs, int reset_first, spbp_graph** out);
 int spbp_graph_launch(spbp_graph* g, void* stream);
@@ -286,6 +293,13 @@ int spbp_graph_destroy(spbp_graph* g);
  * n x 9 fp32, with flags & SPBP_EXACT_PI selecting exact pi. */
 int spbp_kb_table(int n2, int flags, float* tab, float* fltb);
 int spbp_rotations(int n, const double* angles_deg, int flags, float* dm);
+/* Make the k-th device allocation of the library from now on (k = 0: the
+ * next one) fail as if the device were out of memory; k < 0 cancels.  A
+ * failed allocation leaves the context usable: the call returns
+ * SPBP_ERR_ALLOC and a later call allocates again.  No counterpart in the
+ * reference; for tests of the allocation error path (ERRT 46,
+ * bp32f.f:180-185). */
+int spbp_debug_fail_alloc(long k);
 /* Library version string. */
 const char* spbp_version(void);
 

@@ -66,8 +66,10 @@ class Backprojector:
 
     # -------------------------------------------------------------- lifecycle
     def close(self) -> None:
+        """Release the context (refused, with SpbpError, while a BatchGraph
+        that replays work on it exists)."""
         if getattr(self, "_ctx", None) is not None and self._ctx.value:
-            _lib.lib().spbp_destroy(self._ctx)
+            self._check(_lib.lib().spbp_destroy(self._ctx))
             self._ctx = None
 
     def __del__(self):
@@ -555,7 +557,7 @@ def _jobs(items):
             ctx, imgs, angles = it[0], it[1], it[2]
             j, k = ctx._job(imgs, angles, None, it[3] if len(it) > 3 else None)
         jobs.append(j)
-        keep.append(k)
+        keep.append((it["ctx"] if isinstance(it, dict) else it[0], k))   # contexts outlive the jobs' users
     arr = (_lib.Job * len(jobs))(*jobs)
     return arr, keep
 
@@ -585,9 +587,12 @@ class BatchGraph:
         _lib.check(_lib.lib().spbp_graph_launch(self._g, _stream(self.device)))
 
     def close(self) -> None:
+        """Destroy the graph; its contexts become free to reallocate again.
+        The graph holds references to its Backprojectors until then."""
         if getattr(self, "_g", None) is not None and self._g.value:
             _lib.lib().spbp_graph_destroy(self._g)
             self._g = None
+        self._keep = None
 
     def __del__(self):
         try:

@@ -295,7 +295,7 @@ __global__ __launch_bounds__(TPB) void brick_itemkey_kernel(const int* __restric
 #define SPBP_LDS_ROT_B 3
 #endif
 
-template <int BX, int BY, int BZ, int NT = TPB>
+template <int BX, int BY, int BZ, int NT = TPB, bool WGT = true>
 struct BrickLds {
     static_assert(BX <= 31 && BY <= 31 && BZ <= 31, "brick edge too long for the 64-lane line table");
     // Layout of the brick's voxels.  Padded: rows of BX + 1, layers of
@@ -313,18 +313,23 @@ struct BrickLds {
     // the same in bytes (8-byte elements): the tap loop's addresses need no scaling
     __device__ static int rowbase8(int y, int z) { return (z * SZ + y * SY) * 8; }
     __device__ static int rot8(int y, int z) { return ROT ? (RA * y + RB * z) * 8 : 0; }
-    __device__ static int at8(int base8, int r8, int x8) { return ROT ? base8 + ((x8 + r8) & ((BX - 1) * 8)) : base8 + x8; }
+    // rotated rows start at multiples of BX * 8 bytes, so the row offset is
+    // OR-ed onto the rotated x (one v_and_or_b32)
+    __device__ static int at8(int base8, int r8, int x8) { return ROT ? base8 | ((x8 + r8) & ((BX - 1) * 8)) : base8 + x8; }
     // gfx950's LDS runs ds_add_u64 at 0.14 cycles per lane, ds_add_f64 at
     // 0.26 and ds_add_f32 at ~3 (measured).  X partial sums are exact 64-bit
     // fixed point: each per-tap term is the exact product of the fp32
     // spectrum value and the fp32 weight (two 24-bit mantissas fit FP64),
-    // scaled by a power of two and rounded to an integer unit of 2^-48 of the
-    // work item's largest spectrum magnitude; the brick sum is exact and only
-    // that rounding and the final conversion round.  Weights are exact
+    // scaled by a power of two and rounded to an integer unit of 2^-VBITS of
+    // the work item's largest spectrum magnitude; the brick sum is exact and
+    // only that rounding and the final conversion round.  Weights are exact
     // 64-bit fixed point too, in units of 2^-WBITS: every weight term is a
     // product of three table values, at most 1, rounded to 2^-(WBITS+1)
-    // absolute (relative 10^-14 where NRMW2 divides, W > 0.1).
-    unsigned long long re[NS], im[NS], w[NS];
+    // absolute (relative 10^-14 where NRMW2 divides, W > 0.1).  Each LDS
+    // word holds the sum of its terms' raw encodings (fixed_term); the
+    // flush decodes them (brick_sums).  Without weights (WGT false: the
+    // adjoint scatter) the w array is not used.
+    unsigned long long re[NS], im[NS], w[WGT ? NS : 1];
     int item;
     float vmax[NT / 64];
 };
@@ -336,23 +341,43 @@ struct BrickLds {
 // projections): 1 (consecutive lanes, consecutive candidates) 55.0, 2 53.2,
 // 4 53.6, 8 54.2, 16 54.3
 constexpr int LANE_SPREAD = SPBP_LANE_SPREAD;
-constexpr int VBITS = 48;                   // |term| < 2^48 units; item sums < 2^62
+// Fixed-point accumulation in LDS.  A term t (an integer, |t| < 2^51) is
+// encoded as the bit pattern of the double 1.5 * 2^52 + t, which is the
+// 64-bit integer FIXED_OFF + t: one fused multiply-add forms it from the
+// exact product (fixed_term).  A voxel's LDS word accumulates the encodings of
+// its n terms: n FIXED_OFF + sum(t) mod 2^64.  FIXED_OFF's low 48 bits are
+// zero, so a sum with |sum(t)| < 2^47 is its low 48 bits, sign-extended, and
+// the high 16 bits then hold n * 0x4338 mod 2^16 = 8 (n * 2151 mod 2^13),
+// which gives n back (n < 2^13; 2151 is odd).  The weights' word, whose terms
+// need all 63 bits, is decoded by subtracting n FIXED_OFF.
+//
+// Bound on n: a work item holds at most 128 sections; one section reaches a
+// voxel p through the lattice samples of its plane within 1.8 voxels of p
+// (direct taps) or of -p (reflected taps), at most 28 each (a plane section
+// of a cube of edge 3.6 has area <= 18.4 and perimeter <= 17.4, and a convex
+// set holds at most area + perimeter / 2 + 1 lattice points), so n <= 7168.
+// With terms below 2^VBITS, |sum(t)| < 7168 * 2^34 < 2^47.
+constexpr int VBITS = 34;                   // |term| < 2^34 units; item sums < 2^47
 constexpr int WBITS = 50;                   // weight terms <= 1 < 2^51 / 2^WBITS; item sums < 2^63
 constexpr double WSCALE = 1125899906842624.0;            // 2^WBITS
 constexpr double WINV = 1.0 / 1125899906842624.0;        // 2^-WBITS
+constexpr unsigned long long FIXED_OFF = 0x4338000000000000ull;   // bits of 1.5 * 2^52
+constexpr unsigned INV_2151 = 4951;         // 2151 * 4951 = 1 mod 2^13
 
-// a * b (|a b| < 2^51) rounded to the nearest integer, as a two's-complement
-// 64-bit value: add 1.5 * 2^52 and read the mantissa bits.  The constant's low
-// word is zero, so only the high word needs its offset removed.  The callers'
-// products are exact in FP64 (two fp32 mantissas, one scaled by a power of
-// two), so the fused multiply-add rounds once, exactly as the rounded product
-// plus the offset would.
-__device__ inline unsigned long long to_fixed_prod(double a, double b) {
-    const unsigned long long r = (unsigned long long)__double_as_longlong(__fma_rn(a, b, 6755399441055744.0));
-    const unsigned hi = (unsigned)(r >> 32) - 0x43380000u;
-    return ((unsigned long long)hi << 32) | (unsigned)r;
+// Encoding of round(a b) (|a b| < 2^51): the bits of fma(a, b, 1.5 * 2^52).
+// The callers' products are exact in FP64 (two fp32 mantissas, one scaled by
+// a power of two), so the fused multiply-add rounds once, exactly as the
+// rounded product plus the offset would.
+__device__ inline unsigned long long fixed_term(double a, double b) {
+    return (unsigned long long)__double_as_longlong(__fma_rn(a, b, 6755399441055744.0));
 }
-__device__ inline double from_fixed(unsigned long long a) { return (double)(long long)a; }
+// sum(t) of an accumulated word whose |sum(t)| < 2^47
+__device__ inline long long fixed_sum48(unsigned long long s) { return (long long)(s << 16) >> 16; }
+// n of an accumulated word with n < 2^13 terms, given its sum(t)
+__device__ inline unsigned fixed_count(unsigned long long s, long long t) {
+    const unsigned k = (unsigned)((s - (unsigned long long)t) >> 48);   // n * 0x4338 mod 2^16
+    return ((k >> 3) * INV_2151) & 0x1FFFu;
+}
 // LDS add at a byte offset into one of the brick arrays
 __device__ inline void lds_add(unsigned long long* arr, int ob, unsigned long long v) {
     atomicAdd((unsigned long long*)((char*)arr + ob), v);
@@ -390,8 +415,8 @@ struct Box {
 // MIR: the brick starts at kx <= LN2, so taps with p_x < 0 can land in it
 // mirrored (at -p_x <= LN2); elsewhere they never do and the tap loop drops
 // the per-tap mirror selects.
-template <int BX, int BY, int BZ, bool MIR, typename P, int NT>
-__device__ __forceinline__ void wave_project(BrickLds<BX, BY, BZ, NT>& L, const Geom& g, const float2* __restrict__ spec_p,
+template <int BX, int BY, int BZ, bool MIR, typename P, int NT, bool WGT>
+__device__ __forceinline__ void wave_project(BrickLds<BX, BY, BZ, NT, WGT>& L, const Geom& g, const float2* __restrict__ spec_p,
                                              int nx, int N, int x0, int y0, int z0, const Box box, bool exclude,
                                              const Box ex, const float* __restrict__ tab, P fltb, double vscale,
                                              double wscale) {
@@ -524,9 +549,9 @@ __device__ __forceinline__ void wave_project(BrickLds<BX, BY, BZ, NT>& L, const 
                         if (ok) {
                             const int ob = xmir[lx] ? L.at8(offm, rm, xo8[lx]) : L.at8(offd, rd, xo8[lx]);
                             const double wd = (double)wg;
-                            lds_add(L.re, ob, to_fixed_prod(vxs, wd));
-                            lds_add(L.im, ob, to_fixed_prod(vim[lx], wd));
-                            lds_add(L.w, ob, to_fixed_prod(wd, wscale));
+                            lds_add(L.re, ob, fixed_term(vxs, wd));
+                            lds_add(L.im, ob, fixed_term(vim[lx], wd));
+                            if constexpr (WGT) lds_add(L.w, ob, fixed_term(wd, wscale));
                         }
                     }
                 }
@@ -556,9 +581,9 @@ __device__ __forceinline__ void wave_project(BrickLds<BX, BY, BZ, NT>& L, const 
                         if (xok[lx] && dok && wg != 0.0f) {
                             const int ob = L.at8(offd, rd, xo8[lx]);
                             const double wd = (double)wg;
-                            lds_add(L.re, ob, to_fixed_prod(vxs, wd));
-                            lds_add(L.im, ob, to_fixed_prod(vys, wd));
-                            lds_add(L.w, ob, to_fixed_prod(wd, wscale));
+                            lds_add(L.re, ob, fixed_term(vxs, wd));
+                            lds_add(L.im, ob, fixed_term(vys, wd));
+                            if constexpr (WGT) lds_add(L.w, ob, fixed_term(wd, wscale));
                         }
                     }
                 }
@@ -594,168 +619,88 @@ __device__ inline void box_lines(const Geom& g, const Box& box, int& b0, int& nl
 // MIRPASS false: owned samples, direct taps (XCHK: drop taps with p_x < 0,
 // needed only at kx = 0).  MIRPASS true: the reflected taps whose target lies
 // in the core.  (x0, y0, z0): the core's origin.
-//
-// Two projections, A and B, share the wave's rounds of candidates: lanes
-// 0..31 hold A's lattice lines and lanes 32..63 B's, so the candidates of
-// both fill the rounds of 64 lanes.  Without B (hasB false) its lines are
-// empty.  nlA, nlB <= 32: a rotation's section crosses at most the box
-// diagonal + 2 lines, which the kernel's static_assert bounds for its region
-// sizes (rotations are checked orthonormal on the host).
-template <int BX, int BY, int BZ, bool MIRPASS, bool XCHK, typename P, int NT>
-__device__ __forceinline__ void wave_project_own(BrickLds<BX, BY, BZ, NT>& L, const Geom& gA, const Geom& gB,
-                                                 bool hasB, const float2* __restrict__ specA,
-                                                 const float2* __restrict__ specB, int b0A, int nlA, int b0B, int nlB,
-                                                 int nx, int N, int x0, int y0, int z0, const Box box,
-                                                 const float* __restrict__ tab, P fltb, double vscale, double wscale) {
+
+template <int BX, int BY, int BZ, bool MIRPASS, bool XCHK, typename P, int NT, bool WGT>
+__device__ __forceinline__ void own_taps(BrickLds<BX, BY, BZ, NT, WGT>& L, const Geom& gc,
+                                         const float2* __restrict__ spec_p, int I, int J, int nx, int N, int x0,
+                                         int y0, int z0, const float* __restrict__ tab, P fltb, double vscale,
+                                         double wscale) {
     constexpr int CX = BX - 4, CY = BY - 4, CZ = BZ - 4;
-    const float* lo = box.lo;
-    const float* hi = box.hi;
     const int H = nx + 1;
-    const int lane = threadIdx.x & 63;
-    // this lane's line: projection `side` (0: A, 1: B), line b0 + (lane & 31)
-    const bool side = lane >= 32;
-    Geom g;
+    P x, y, z;
+    sample_pos(gc, I, J, x, y, z);
+    const bool flip = x < (P)0;                                        // oneline.f:60-67
+    if (flip) { x = -x; y = -y; z = -z; }
+    const int ix = nearest_voxel(x, N), iy = nearest_voxel(y, N), iz = nearest_voxel(z, N);
+    float2 v = spectrum_value(spec_p, I, J, N, H);
+    if (flip) v.y = -v.y;
+    float tx[4], ty[4], tz[4];
+    const float wx0 = kb_weight(tab, dsub(x, ix - 2), fltb);
+    const float wy0 = kb_weight(tab, dsub(y, iy - 2), fltb);
+    const float wz0 = kb_weight(tab, dsub(z, iz - 2), fltb);
+    const int sx = wx0 != 0.0f ? ix - 2 : ix - 1;
+    const int sy = wy0 != 0.0f ? iy - 2 : iy - 1;
+    const int sz = wz0 != 0.0f ? iz - 2 : iz - 1;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        g.r[k] = side ? gB.r[k] : gA.r[k];
-        g.d[k] = side ? gB.d[k] : gA.d[k];
+    for (int t = 0; t < 4; ++t) {
+        tx[t] = (t == 0 && sx == ix - 2) ? wx0 : kb_weight(tab, dsub(x, sx + t), fltb);
+        ty[t] = (t == 0 && sy == iy - 2) ? wy0 : kb_weight(tab, dsub(y, sy + t), fltb);
+        tz[t] = (t == 0 && sz == iz - 2) ? wz0 : kb_weight(tab, dsub(z, sz + t), fltb);
     }
-    const int nl = side ? (hasB ? nlB : 0) : nlA;
-    const int b0 = side ? b0B : b0A;
-    int cnt = 0, alo = 0;
-    if ((lane & 31) < nl) {
-        const float bb = (float)(b0 + (lane & 31));
-        float amin = -1e30f, amax = 1e30f;
-        bool empty = false;
+    const double vxs = __dmul_rn((double)v.x, vscale), vys = __dmul_rn((double)v.y, vscale);
+    if constexpr (MIRPASS) {
+        // target -p, region-local: (-p) - (core origin - 2)
+        const double vim = -vys;
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const float r1 = g.r[k], off = bb * g.r[3 + k];
-            if (fabsf(r1) > 1e-6f) {
-                const float u = (lo[k] - off) / r1, w = (hi[k] - off) / r1;
-                amin = fmaxf(amin, fminf(u, w));
-                amax = fminf(amax, fmaxf(u, w));
-            } else if (off < lo[k] - 0.01f || off > hi[k] + 0.01f) {
-                empty = true;
-            }
-        }
-        if (!empty) {
-            const int a0 = (int)ceilf(amin - 0.01f), a1 = (int)floorf(amax + 0.01f);
-            if (a1 >= a0) { alo = a0; cnt = a1 - a0 + 1; }
-        }
-    }
-    int pref = cnt;
-    for (int d = 1; d < 64; d <<= 1) {
-        const int u = __shfl_up(pref, d, 64);
-        if (lane >= d) pref += u;
-    }
-    const int total = __shfl(pref, 63, 64);
-    for (int q0 = 0; q0 < total; q0 += 64) {
-        const int q = q0 + (lane & (64 / LANE_SPREAD - 1)) * LANE_SPREAD + lane / (64 / LANE_SPREAD);
-        int l = 0;
+        for (int lz = 0; lz < 4; ++lz) {
+            const int tzl = -(sz + lz) - z0;
+            const bool zok = (unsigned)tzl < (unsigned)CZ;
 #pragma unroll
-        for (int step = 32; step > 0; step >>= 1) {
-            const int t = __shfl(pref, l + step - 1, 64);
-            if (t <= q) l += step;
-        }
-        const int lpref = __shfl(pref, l, 64), lcnt = __shfl(cnt, l, 64), lalo = __shfl(alo, l, 64);
-        if (q >= total) continue;
-        // the candidate's projection: its line's side
-        const bool cside = l >= 32;
-        Geom gc;
+            for (int ly = 0; ly < 4; ++ly) {
+                const int tyl = -(sy + ly) - y0;
+                const bool ok = zok && (unsigned)tyl < (unsigned)CY;
+                const int offm = L.rowbase8(tyl + 2, tzl + 2), rm = L.rot8(tyl + 2, tzl + 2);
+                const float tyz = __fmul_rn(ty[ly], tz[lz]);
 #pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            gc.r[k] = cside ? gB.r[k] : gA.r[k];
-            gc.d[k] = cside ? gB.d[k] : gA.d[k];
-        }
-        const float2* __restrict__ spec_p = cside ? specB : specA;
-        const int bb = (cside ? b0B : b0A) + (l & 31);
-        const int aa = lalo + (q - (lpref - lcnt));
-        const bool canon = aa > 0 || (aa == 0 && bb >= 0);
-        const int I = canon ? aa : -aa, J = canon ? bb : -bb;
-        if (I > nx || J < -nx + 1 || J > nx || !(I * I + J * J < nx * nx)) continue;   // oneline.f:53-55
-        P x, y, z;
-        sample_pos(gc, I, J, x, y, z);
-        const bool flip = x < (P)0;                                    // oneline.f:60-67
-        if (flip == canon) continue;
-        if (flip) { x = -x; y = -y; z = -z; }
-        const int ix = nearest_voxel(x, N), iy = nearest_voxel(y, N), iz = nearest_voxel(z, N);
-        if (!in_bounds(ix, iy, iz, nx)) continue;                      // oneline.f:73-75
-        if constexpr (MIRPASS) {
-            // reflected taps exist only for ix <= 1; their targets -p lie within 2 of -(ix, iy, iz)
-            if (ix > 1 || -iy + 2 < y0 || -iy - 2 >= y0 + CY || -iz + 2 < z0 || -iz - 2 >= z0 + CZ) continue;
-        } else {
-            if (ix < x0 || ix >= x0 + CX || iy < y0 || iy >= y0 + CY || iz < z0 || iz >= z0 + CZ) continue;
-        }
-        float2 v = spectrum_value(spec_p, I, J, N, H);
-        if (flip) v.y = -v.y;
-        float tx[4], ty[4], tz[4];
-        const float wx0 = kb_weight(tab, dsub(x, ix - 2), fltb);
-        const float wy0 = kb_weight(tab, dsub(y, iy - 2), fltb);
-        const float wz0 = kb_weight(tab, dsub(z, iz - 2), fltb);
-        const int sx = wx0 != 0.0f ? ix - 2 : ix - 1;
-        const int sy = wy0 != 0.0f ? iy - 2 : iy - 1;
-        const int sz = wz0 != 0.0f ? iz - 2 : iz - 1;
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            tx[t] = (t == 0 && sx == ix - 2) ? wx0 : kb_weight(tab, dsub(x, sx + t), fltb);
-            ty[t] = (t == 0 && sy == iy - 2) ? wy0 : kb_weight(tab, dsub(y, sy + t), fltb);
-            tz[t] = (t == 0 && sz == iz - 2) ? wz0 : kb_weight(tab, dsub(z, sz + t), fltb);
-        }
-        const double vxs = __dmul_rn((double)v.x, vscale), vys = __dmul_rn((double)v.y, vscale);
-        if constexpr (MIRPASS) {
-            // target -p, region-local: (-p) - (core origin - 2)
-            const double vim = -vys;
-#pragma unroll
-            for (int lz = 0; lz < 4; ++lz) {
-                const int tzl = -(sz + lz) - z0;
-                const bool zok = (unsigned)tzl < (unsigned)CZ;
-#pragma unroll
-                for (int ly = 0; ly < 4; ++ly) {
-                    const int tyl = -(sy + ly) - y0;
-                    const bool ok = zok && (unsigned)tyl < (unsigned)CY;
-                    const int offm = L.rowbase8(tyl + 2, tzl + 2), rm = L.rot8(tyl + 2, tzl + 2);
-                    const float tyz = __fmul_rn(ty[ly], tz[lz]);
-#pragma unroll
-                    for (int lx = 0; lx < 2; ++lx) {   // p_x = sx + lx < 0 only for the first two taps
-                        const int px = sx + lx;
-                        const float wg = __fmul_rn(tx[lx], tyz);
-                        if (ok && px < 0 && -px - x0 < CX && wg != 0.0f) {
-                            const int ob = L.at8(offm, rm, (-px - x0 + 2) * 8);
-                            const double wd = (double)wg;
-                            lds_add(L.re, ob, to_fixed_prod(vxs, wd));
-                            lds_add(L.im, ob, to_fixed_prod(vim, wd));
-                            lds_add(L.w, ob, to_fixed_prod(wd, wscale));
-                        }
+                for (int lx = 0; lx < 2; ++lx) {   // p_x = sx + lx < 0 only for the first two taps
+                    const int px = sx + lx;
+                    const float wg = __fmul_rn(tx[lx], tyz);
+                    if (ok && px < 0 && -px - x0 < CX && wg != 0.0f) {
+                        const int ob = L.at8(offm, rm, (-px - x0 + 2) * 8);
+                        const double wd = (double)wg;
+                        lds_add(L.re, ob, fixed_term(vxs, wd));
+                        lds_add(L.im, ob, fixed_term(vim, wd));
+                        if constexpr (WGT) lds_add(L.w, ob, fixed_term(wd, wscale));
                     }
                 }
             }
-        } else {
-            // every direct tap is inside the region: p - (core origin - 2) in [0, B)
-            int xo8[4];
-            bool xok[4];
+        }
+    } else {
+        // every direct tap is inside the region: p - (core origin - 2) in [0, B)
+        int xo8[4];
+        bool xok[4];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                xo8[t] = (sx + t - x0 + 2) * 8;
-                xok[t] = !XCHK || sx + t >= 0;
-            }
+        for (int t = 0; t < 4; ++t) {
+            xo8[t] = (sx + t - x0 + 2) * 8;
+            xok[t] = !XCHK || sx + t >= 0;
+        }
 #pragma unroll
-            for (int lz = 0; lz < 4; ++lz) {
-                const int zl = sz + lz - z0 + 2;
+        for (int lz = 0; lz < 4; ++lz) {
+            const int zl = sz + lz - z0 + 2;
 #pragma unroll
-                for (int ly = 0; ly < 4; ++ly) {
-                    const int yl = sy + ly - y0 + 2;
-                    const int offd = L.rowbase8(yl, zl), rd = L.rot8(yl, zl);
-                    const float tyz = __fmul_rn(ty[ly], tz[lz]);
+            for (int ly = 0; ly < 4; ++ly) {
+                const int yl = sy + ly - y0 + 2;
+                const int offd = L.rowbase8(yl, zl), rd = L.rot8(yl, zl);
+                const float tyz = __fmul_rn(ty[ly], tz[lz]);
 #pragma unroll
-                    for (int lx = 0; lx < 4; ++lx) {
-                        const float wg = __fmul_rn(tx[lx], tyz);
-                        if (xok[lx] && wg != 0.0f) {
-                            const int ob = L.at8(offd, rd, xo8[lx]);
-                            const double wd = (double)wg;
-                            lds_add(L.re, ob, to_fixed_prod(vxs, wd));
-                            lds_add(L.im, ob, to_fixed_prod(vys, wd));
-                            lds_add(L.w, ob, to_fixed_prod(wd, wscale));
-                        }
+                for (int lx = 0; lx < 4; ++lx) {
+                    const float wg = __fmul_rn(tx[lx], tyz);
+                    if (xok[lx] && wg != 0.0f) {
+                        const int ob = L.at8(offd, rd, xo8[lx]);
+                        const double wd = (double)wg;
+                        lds_add(L.re, ob, fixed_term(vxs, wd));
+                        lds_add(L.im, ob, fixed_term(vys, wd));
+                        if constexpr (WGT) lds_add(L.w, ob, fixed_term(wd, wscale));
                     }
                 }
             }
@@ -789,17 +734,177 @@ struct BrickJob {
 };
 constexpr int MAX_JOBS = 64;
 
+// One pass of a wave over its sections of a work item (s = 2 wave, 2 wave +
+// 1, then every 2 nwave).  Two sections, A and B, share each candidate round:
+// lanes 0..31 hold A's lattice lines and lanes 32..63 B's (without B, its
+// lines are empty).  nlA, nlB <= 32: a rotation's section crosses at most the
+// box diagonal + 2 lines, which the kernel's static_assert bounds for its
+// region sizes (rotations are checked orthonormal on the host).
+//
+// Candidates are tested in rounds of 64 (one per lane), and the ones that
+// pass are compacted into full batches of 64 before the tap loop runs, across
+// rounds and across the wave's sections: every ds_add_u64 wave instruction
+// costs the same whatever its active lanes, so the tap loop runs over full
+// batches only, plus one partial batch at the end of the pass.  A round's
+// accepted candidates are pushed (ds_permute) to the lanes after the pending
+// ones; a round that completes a batch leaves its overflow in lanes 0.. as
+// the next batch's start.  A candidate travels as its sample (I, J) and its
+// section's place s in the item's list (packed in 30 bits); the tap loop
+// reloads that section's geometry and recomputes the sample's position.
+__device__ __forceinline__ unsigned pack_cand(int I, int J, int s) {
+    return (unsigned)I | ((unsigned)(J + 1024) << 11) | ((unsigned)s << 23);
+}
+
+template <int BX, int BY, int BZ, bool MIRPASS, bool XCHK, typename P, int NT, bool WGT>
+__device__ __forceinline__ void own_pass(BrickLds<BX, BY, BZ, NT, WGT>& L, const BrickJob& Jb, const int* __restrict__ lst,
+                                         int ns, int wave, int nwave, int x0, int y0, int z0, const Box box,
+                                         const float* __restrict__ tab, P fltb, double vscale, double wscale) {
+    constexpr int CX = BX - 4, CY = BY - 4, CZ = BZ - 4;
+    const int nx = Jb.nx, N = Jb.N, H = nx + 1;
+    const float* lo = box.lo;
+    const float* hi = box.hi;
+    const int lane = threadIdx.x & 63;
+    const bool side = lane >= 32;   // this lane's line: section A (0) or B (1), line b0 + (lane & 31)
+    const unsigned long long below = (1ull << lane) - 1ull;
+    Geom gA, gB;
+    int b0A = 0, b0B = 0;
+    int pref = 0, cnt = 0, alo = 0, total = 0, q0 = 0;
+    int s = 2 * wave - 2 * nwave;   // current pair
+    unsigned pend = 0;   // this lane's pending candidate (valid for lane < npend)
+    int npend = 0;       // wave-uniform
+    for (;;) {
+        unsigned cur = 0;
+        bool full = false;
+        for (;;) {
+            if (q0 >= total) {   // next pair of sections: their lattice lines across the box
+                s += 2 * nwave;
+                if (s >= ns) break;
+                const bool hasB = s + 1 < ns;
+                gA = Jb.geom[lst[s]];
+                gB = Jb.geom[lst[hasB ? s + 1 : s]];
+                int nlA, nlB;
+                box_lines(gA, box, b0A, nlA);
+                box_lines(gB, box, b0B, nlB);
+                Geom g;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) g.r[k] = side ? gB.r[k] : gA.r[k];
+                const int nl = side ? (hasB ? nlB : 0) : nlA;
+                const int b0 = side ? b0B : b0A;
+                cnt = 0;
+                alo = 0;
+                if ((lane & 31) < nl) {
+                    const float bb = (float)(b0 + (lane & 31));
+                    float amin = -1e30f, amax = 1e30f;
+                    bool empty = false;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        const float r1 = g.r[k], off = bb * g.r[3 + k];
+                        if (fabsf(r1) > 1e-6f) {
+                            const float u = (lo[k] - off) / r1, w = (hi[k] - off) / r1;
+                            amin = fmaxf(amin, fminf(u, w));
+                            amax = fminf(amax, fmaxf(u, w));
+                        } else if (off < lo[k] - 0.01f || off > hi[k] + 0.01f) {
+                            empty = true;
+                        }
+                    }
+                    if (!empty) {
+                        const int a0 = (int)ceilf(amin - 0.01f), a1 = (int)floorf(amax + 0.01f);
+                        if (a1 >= a0) { alo = a0; cnt = a1 - a0 + 1; }
+                    }
+                }
+                pref = cnt;
+                for (int d = 1; d < 64; d <<= 1) {
+                    const int u = __shfl_up(pref, d, 64);
+                    if (lane >= d) pref += u;
+                }
+                total = __shfl(pref, 63, 64);
+                q0 = 0;
+                continue;
+            }
+            const int q = q0 + (lane & (64 / LANE_SPREAD - 1)) * LANE_SPREAD + lane / (64 / LANE_SPREAD);
+            q0 += 64;
+            // line of candidate q: smallest l with pref[l] > q (every lane searches)
+            int l = 0;
+#pragma unroll
+            for (int step = 32; step > 0; step >>= 1) {
+                const int t = __shfl(pref, l + step - 1, 64);
+                if (t <= q) l += step;
+            }
+            const int lpref = __shfl(pref, l, 64), lcnt = __shfl(cnt, l, 64), lalo = __shfl(alo, l, 64);
+            bool ok = q < total;
+            const bool cside = l >= 32;
+            const int bb = (cside ? b0B : b0A) + (l & 31);
+            const int aa = lalo + (q - (lpref - lcnt));
+            // canonical sample (I, J) of lattice point (aa, bb): I > 0 or I = 0, J >= 0
+            const bool canon = aa > 0 || (aa == 0 && bb >= 0);
+            const int I = canon ? aa : -aa, J = canon ? bb : -bb;
+            ok = ok && !(I > nx || J < -nx + 1 || J > nx || !(I * I + J * J < nx * nx));   // oneline.f:53-55
+            if (ok) {
+                Geom gc;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) {
+                    gc.r[k] = cside ? gB.r[k] : gA.r[k];
+                    gc.d[k] = cside ? gB.d[k] : gA.d[k];
+                }
+                P x, y, z;
+                sample_pos(gc, I, J, x, y, z);
+                const bool flip = x < (P)0;                            // oneline.f:60-67
+                if (flip) { x = -x; y = -y; z = -z; }
+                const int ix = nearest_voxel(x, N), iy = nearest_voxel(y, N), iz = nearest_voxel(z, N);
+                // (aa, bb) must be the sample's effective lattice point; oneline.f:73-75
+                ok = flip != canon && in_bounds(ix, iy, iz, nx);
+                if constexpr (MIRPASS) {
+                    // reflected taps exist only for ix <= 1; their targets -p lie within 2 of -(ix, iy, iz)
+                    ok = ok && !(ix > 1 || -iy + 2 < y0 || -iy - 2 >= y0 + CY || -iz + 2 < z0 || -iz - 2 >= z0 + CZ);
+                } else {
+                    ok = ok && !(ix < x0 || ix >= x0 + CX || iy < y0 || iy >= y0 + CY || iz < z0 || iz >= z0 + CZ);
+                }
+            }
+            const unsigned long long m = __ballot(ok);
+            const int nv = __popcll(m);
+            if (nv == 0) continue;
+            // a permutation of the lanes: accepted ones first, in lane order
+            const int rank = ok ? __popcll(m & below) : nv + __popcll(~m & below);
+            const unsigned got = (unsigned)__builtin_amdgcn_ds_permute(((npend + rank) & 63) << 2,
+                                                                        (int)pack_cand(I, J, s + (cside ? 1 : 0)));
+            if (npend + nv >= 64) {
+                cur = lane >= npend ? got : pend;
+                pend = got;   // lanes < npend + nv - 64: the overflow
+                npend += nv - 64;
+                full = true;
+                break;
+            }
+            if (lane >= npend && lane < npend + nv) pend = got;
+            npend += nv;
+        }
+        if (!full) {
+            if (npend == 0) break;
+            cur = pend;
+        }
+        if (full || lane < npend) {
+            const int I = (int)(cur & 0x7FFu), J = (int)((cur >> 11) & 0xFFFu) - 1024;
+            const Geom& gc = Jb.geom[lst[cur >> 23]];
+            const float2* spec_p = Jb.spec + (size_t)(gc.img - Jb.img0) * N * H;
+            own_taps<BX, BY, BZ, MIRPASS, XCHK, P, NT, WGT>(L, gc, spec_p, I, J, nx, N, x0, y0, z0, tab, fltb, vscale,
+                                                            wscale);
+        }
+        if (!full) break;
+    }
+}
+
+
 // Persistent workgroups pull (job, key, projection slice) items from one
 // queue spanning every job of the launch (job-major order); the waves of
 // a workgroup take the item's projections in turn and share the brick
 // through LDS atomics.  The queue head must be zero at launch.
 // The fp32-geometry instantiations are held to 128 VGPRs (4 waves per SIMD);
-// the FP64-geometry ones need more and run 3.
-template <int BX, int BY, int BZ, typename P, int NT = TPB, bool OWN = false>
+// the FP64-geometry ones need more and run 3.  WGT false: no weights are
+// accumulated (every job's W0 and W1 are null: the adjoint scatter).
+template <int BX, int BY, int BZ, typename P, int NT = TPB, bool OWN = false, bool WGT = true>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(P) == 4 ? 4 : 3)))
 void insert_brick_kernel(const BrickJob* __restrict__ jobs, int njobs, int* __restrict__ next_item) {
     constexpr int NV = BX * BY * BZ;
-    __shared__ BrickLds<BX, BY, BZ, NT> L;
+    __shared__ BrickLds<BX, BY, BZ, NT, WGT> L;
     __shared__ int jbase[MAX_JOBS + 1];
     if (threadIdx.x == 0) {
         int acc = 0;
@@ -831,10 +936,14 @@ void insert_brick_kernel(const BrickJob* __restrict__ jobs, int njobs, int* __re
         const int* lst = J.list + J.off[key] + s0;
         int x0, y0, z0;
         brick_origin(bg, b, nx, x0, y0, z0);
-        for (int i = threadIdx.x; i < L.NS; i += blockDim.x) { L.re[i] = 0ull; L.im[i] = 0ull; L.w[i] = 0ull; }
+        for (int i = threadIdx.x; i < L.NS; i += blockDim.x) {
+            L.re[i] = 0ull;
+            L.im[i] = 0ull;
+            if constexpr (WGT) L.w[i] = 0ull;
+        }
         // fixed-point unit of the item: 2^-VBITS of its largest spectrum
         // magnitude (rounded up to a power of two), so every term is below
-        // 2^VBITS units and a voxel's sum over the item below 2^62
+        // 2^VBITS units and a voxel's sum over the item below 2^47
         float m = 0.0f;
         for (int s = threadIdx.x; s < ns; s += blockDim.x) m = fmaxf(m, J.vmax[J.geom[lst[s]].img - J.img0]);
         for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_down(m, off, 64));
@@ -871,27 +980,11 @@ void insert_brick_kernel(const BrickJob* __restrict__ jobs, int njobs, int* __re
             static_assert((BX - 3.98) * (BX - 3.98) + (BY - 3.98) * (BY - 3.98) + (BZ - 3.98) * (BZ - 3.98) <= 900.0 &&
                               1.52 * 1.52 + (BY + 0.02) * (BY + 0.02) + (BZ + 0.02) * (BZ + 0.02) <= 900.0,
                           "owner-computes region too large for 32 lines per projection");
-            for (int s = 2 * wave; s < ns; s += 2 * nwave) {
-                const bool hasB = s + 1 < ns;
-                const Geom gA = J.geom[lst[s]];
-                const Geom gB = J.geom[lst[hasB ? s + 1 : s]];
-                const float2* specA = J.spec + (size_t)(gA.img - J.img0) * N * H;
-                const float2* specB = J.spec + (size_t)(gB.img - J.img0) * N * H;
-#pragma unroll
-                for (int pass = 0; pass < 2; ++pass) {
-                    if (pass == 1 && x0 != 0) break;
-                    const Box& bx2 = pass ? om : oc;
-                    int b0A, nlA, b0B, nlB;
-                    box_lines(gA, bx2, b0A, nlA);
-                    box_lines(gB, bx2, b0B, nlB);
-                    if (pass) {
-                        wave_project_own<BX, BY, BZ, true, false, P, NT>(L, gA, gB, hasB, specA, specB, b0A, nlA, b0B, nlB, nx, N, x0, y0, z0, bx2, tab, fltb, vscale, wsc);
-                    } else if (x0 == 0) {
-                        wave_project_own<BX, BY, BZ, false, true, P, NT>(L, gA, gB, hasB, specA, specB, b0A, nlA, b0B, nlB, nx, N, x0, y0, z0, bx2, tab, fltb, vscale, wsc);
-                    } else {
-                        wave_project_own<BX, BY, BZ, false, false, P, NT>(L, gA, gB, hasB, specA, specB, b0A, nlA, b0B, nlB, nx, N, x0, y0, z0, bx2, tab, fltb, vscale, wsc);
-                    }
-                }
+            if (x0 == 0) {
+                own_pass<BX, BY, BZ, false, true, P, NT, WGT>(L, J, lst, ns, wave, nwave, x0, y0, z0, oc, tab, fltb, vscale, wsc);
+                own_pass<BX, BY, BZ, true, false, P, NT, WGT>(L, J, lst, ns, wave, nwave, x0, y0, z0, om, tab, fltb, vscale, wsc);
+            } else {
+                own_pass<BX, BY, BZ, false, false, P, NT, WGT>(L, J, lst, ns, wave, nwave, x0, y0, z0, oc, tab, fltb, vscale, wsc);
             }
         }
         for (int s = wave; s < ns && !OWN; s += nwave) {
@@ -899,32 +992,37 @@ void insert_brick_kernel(const BrickJob* __restrict__ jobs, int njobs, int* __re
             const Geom g = J.geom[p];
             const float2* spec_p = J.spec + (size_t)(g.img - J.img0) * N * H;
             if (x0 <= LN2) {
-                wave_project<BX, BY, BZ, true, P, NT>(L, g, spec_p, nx, N, x0, y0, z0, bx, false, bx, tab, fltb, vscale, wsc);
-                wave_project<BX, BY, BZ, true, P, NT>(L, g, spec_p, nx, N, x0, y0, z0, mb, true, bx, tab, fltb, vscale, wsc);
+                wave_project<BX, BY, BZ, true, P, NT, WGT>(L, g, spec_p, nx, N, x0, y0, z0, bx, false, bx, tab, fltb, vscale, wsc);
+                wave_project<BX, BY, BZ, true, P, NT, WGT>(L, g, spec_p, nx, N, x0, y0, z0, mb, true, bx, tab, fltb, vscale, wsc);
             } else {
-                wave_project<BX, BY, BZ, false, P, NT>(L, g, spec_p, nx, N, x0, y0, z0, bx, false, bx, tab, fltb, vscale, wsc);
+                wave_project<BX, BY, BZ, false, P, NT, WGT>(L, g, spec_p, nx, N, x0, y0, z0, bx, false, bx, tab, fltb, vscale, wsc);
             }
         }
         __syncthreads();
         // add the brick to the half grid
         float2* __restrict__ X = half ? J.X1 : J.X0;
-        float* __restrict__ W = half ? J.W1 : J.W0;
-        float* __restrict__ Wc = half ? J.Wc1 : J.Wc0;   // null unless compensated weights
+        [[maybe_unused]] float* __restrict__ W = half ? J.W1 : J.W0;
+        [[maybe_unused]] float* __restrict__ Wc = half ? J.Wc1 : J.Wc0;   // null unless compensated weights
         for (int v = threadIdx.x; v < NV; v += blockDim.x) {
             const int lx = v % BX, ly = (v / BX) % BY, lz = v / (BX * BY);
             const int i = L.at(L.rowbase(ly, lz), L.rot(ly, lz), lx);
-            if (L.w[i] == 0ull) continue;
+            const unsigned long long sre = L.re[i];
+            if (sre == 0ull) continue;   // no tap reached the voxel (n = 0)
             const int rg = OWN ? 2 : 0;   // owner-computes: the region starts 2 voxels before the core
             const int kx = x0 - rg + lx, ky = y0 - rg + ly, kz = z0 - rg + lz;
             // brick overhangs the grid (partial last brick; a region's halo)
             if (kx < 0 || ky < -nx + 1 || kz < -nx + 1 || kx > nx || ky > nx || kz > nx) continue;
             const size_t o = ((size_t)(kz >= 0 ? kz : N + kz) * N + (ky >= 0 ? ky : N + ky)) * H + kx;
-            unsafeAtomicAdd(&X[o].x, (float)__dmul_rn(from_fixed(L.re[i]), vinv));
-            unsafeAtomicAdd(&X[o].y, (float)__dmul_rn(from_fixed(L.im[i]), vinv));
-            const double wd = __dmul_rn(from_fixed(L.w[i]), WINV);
-            const float wi = (float)wd;
-            if (Wc) twosum_atomic_add(W, Wc, o, wi, (float)(wd - (double)wi));
-            else if (W) unsafeAtomicAdd(&W[o], wi);
+            const long long tre = fixed_sum48(sre);
+            unsafeAtomicAdd(&X[o].x, (float)__dmul_rn((double)tre, vinv));
+            unsafeAtomicAdd(&X[o].y, (float)__dmul_rn((double)fixed_sum48(L.im[i]), vinv));
+            if constexpr (WGT) {
+                const unsigned long long tw = L.w[i] - (unsigned long long)fixed_count(sre, tre) * FIXED_OFF;
+                const double wd = __dmul_rn((double)(long long)tw, WINV);
+                const float wi = (float)wd;
+                if (Wc) twosum_atomic_add(W, Wc, o, wi, (float)(wd - (double)wi));
+                else if (W) unsafeAtomicAdd(&W[o], wi);
+            }
         }
         __syncthreads();
     }

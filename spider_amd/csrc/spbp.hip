@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 #include <hipfft/hipfft.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -99,6 +100,7 @@ struct spbp_ctx {
     double* fmean = nullptr;   // per chunk image: PADD2's mean
     // finalisation
     hipfftHandle inv = 0;
+    bool final_ok = false;    // every finalisation buffer and the inverse plan exist
     float2* work = nullptr;
     float* real = nullptr;
     float* wkb = nullptr;
@@ -115,6 +117,8 @@ struct spbp_ctx {
     int C = 128;
     bool tuned = false;
     bool capturing = false;   // a batch graph is being captured: no host syncs, no timing events
+    int graphs = 0;           // live batch graphs that replay work on this context's buffers
+    bool filled = false;      // the accumulators hold inserted or imported sums (since the last reset)
     // symmetries (BUILDS): each image is inserted once per matrix, SYMVOL after the window
     int nsym = 1;
     std::vector<float> sm;    // nsym x 9, Fortran (3,3) order
@@ -200,6 +204,23 @@ int fail(spbp_ctx* ctx, int code, const char* fmt, ...) {
         if (r_ != HIPFFT_SUCCESS) return fail((ctx), SPBP_ERR_FFT, "%s: hipfft error %d", #call, (int)r_); \
     } while (0)
 
+// Every device allocation of the library goes through dmalloc.  Tests make
+// the k-th one from now on fail as if out of memory (spbp_debug_fail_alloc),
+// so that the error paths of the lazily allocated buffers can be exercised.
+std::atomic<long> g_fail_after{-1};
+
+template <typename T>
+hipError_t dmalloc(T** p, size_t bytes) {
+    long k = g_fail_after.load();
+    while (k >= 0) {
+        if (g_fail_after.compare_exchange_weak(k, k - 1)) {
+            if (k == 0) { *p = nullptr; return hipErrorOutOfMemory; }
+            break;
+        }
+    }
+    return hipMalloc((void**)p, bytes);
+}
+
 struct DeviceGuard {
     int prev = -1;
     explicit DeviceGuard(int dev) {
@@ -240,7 +261,17 @@ struct Timed {
     }
 };
 
-// Brick grid of shape `shape` and its per-key arrays.
+// A captured batch graph (spbp_graph_create) replays launches that read the
+// context's chunk buffers, brick lists and forward FFT plans: while one
+// exists, nothing may free or replace them.
+int refuse_captured(spbp_ctx* c, const char* what) {
+    return fail(c, SPBP_ERR_ARG, "%s: a captured batch graph uses this context's buffers "
+                "(spbp_graph_destroy it first)", what);
+}
+
+// Brick grid of shape `shape` and its per-key arrays.  On failure the
+// context holds no brick grid (c->bg.nb = 0) and the next call allocates
+// again; the shape and grid are committed only once every array exists.
 int configure_bricks(spbp_ctx* c, int shape) {
     if ((c->flags & SPBP_GEOM_F64) && (shape == 5 || shape == 6)) shape = 4;
     const int own = SHAPES[shape][3], halo = own ? 4 : 0;
@@ -254,7 +285,9 @@ int configure_bricks(spbp_ctx* c, int shape) {
     g.nby = (2 * c->nx - 1 + by - 1) / by;
     g.nbz = (2 * c->nx - 1 + bz - 1) / bz;
     g.nb = g.nbx * g.nby * g.nbz;
-    if (c->bcnt && g.nb == c->bg.nb && shape == c->shape) return SPBP_OK;
+    if (c->bg.nb > 0 && g.nb == c->bg.nb && shape == c->shape) return SPBP_OK;
+    if (c->graphs > 0) return refuse_captured(c, "changing the brick shape");
+    c->bg.nb = 0;   // no valid grid until every array below exists
     int* bufs[4] = {c->bcnt, c->boff, c->bioff, c->bcur};
     for (int* q : bufs) if (q) (void)hipFree(q);
     c->bcnt = c->boff = c->bioff = c->bcur = nullptr;
@@ -262,51 +295,66 @@ int configure_bricks(spbp_ctx* c, int shape) {
     if (c->blist) { (void)hipFree(c->blist); c->blist = nullptr; }
     if (c->bikey) { (void)hipFree(c->bikey); c->bikey = nullptr; }
     c->blist_cap = 0;
+    const size_t nk = 2 * (size_t)g.nb;
+    HIPCHK(c, dmalloc(&c->bcnt, nk * sizeof(int)));
+    HIPCHK(c, dmalloc(&c->boff, (nk + 1) * sizeof(int)));
+    HIPCHK(c, dmalloc(&c->bioff, (nk + 1) * sizeof(int)));
+    HIPCHK(c, dmalloc(&c->bcur, nk * sizeof(int)));
+    HIPCHK(c, dmalloc(&c->bscan, ((nk + 1023) / 1024 + 1) * sizeof(int2)));
     c->bg = g;
     c->shape = shape;
-    const size_t nk = 2 * (size_t)g.nb;
-    HIPCHK(c, hipMalloc(&c->bcnt, nk * sizeof(int)));
-    HIPCHK(c, hipMalloc(&c->boff, (nk + 1) * sizeof(int)));
-    HIPCHK(c, hipMalloc(&c->bioff, (nk + 1) * sizeof(int)));
-    HIPCHK(c, hipMalloc(&c->bcur, nk * sizeof(int)));
-    HIPCHK(c, hipMalloc(&c->bscan, ((nk + 1023) / 1024 + 1) * sizeof(int2)));
     return SPBP_OK;
 }
 
-// Projection lists for chunks of up to B projections.
+// Projection lists for chunks of up to B projections.  The capacity is
+// committed only once both arrays exist.
 int ensure_lists(spbp_ctx* c, int B) {
+    if (c->bg.nb <= 0) {   // an earlier allocation failed: the last good shape again
+        const int rc = configure_bricks(c, c->shape);
+        if (rc) return rc;
+    }
     const size_t cap = (size_t)B * c->nsym * c->bg.nb;   // B images, nsym sections each
-    if (c->blist && c->blist_cap >= cap) return SPBP_OK;
+    if (c->blist && c->bikey && c->blist_cap >= cap) return SPBP_OK;
+    if (c->graphs > 0) return refuse_captured(c, "growing the brick lists");
+    c->blist_cap = 0;
     if (c->blist) { (void)hipFree(c->blist); c->blist = nullptr; }
     if (c->bikey) { (void)hipFree(c->bikey); c->bikey = nullptr; }
-    HIPCHK(c, hipMalloc(&c->blist, cap * sizeof(int)));
+    HIPCHK(c, dmalloc(&c->blist, cap * sizeof(int)));
     // items <= sum over keys of ceil(cnt / C) <= pairs + keys
-    HIPCHK(c, hipMalloc(&c->bikey, (cap + 2 * (size_t)c->bg.nb) * sizeof(int)));
+    HIPCHK(c, dmalloc(&c->bikey, (cap + 2 * (size_t)c->bg.nb) * sizeof(int)));
     c->blist_cap = cap;
     return SPBP_OK;
 }
 
+// Chunk buffers (and, for hipFFT transforms, the forward plan) for chunks of
+// want_batch projections.  c->fwd_batch is the committed chunk size: zero
+// until every buffer of the set exists.
 int ensure_chunk(spbp_ctx* c, int want_batch) {
-    if (c->spec && c->fwd_batch >= want_batch) return ensure_lists(c, c->fwd_batch);
+    const bool set = c->spec && c->vmax && (c->fft_log2 ? c->fmean != nullptr : (c->padbuf && c->fwd));
+    if (set && c->fwd_batch >= want_batch) return ensure_lists(c, c->fwd_batch);
+    if (c->graphs > 0) return refuse_captured(c, "growing the chunk buffers");
+    c->fwd_batch = 0;
     if (c->padbuf) { (void)hipFree(c->padbuf); c->padbuf = nullptr; }
     if (c->spec) { (void)hipFree(c->spec); c->spec = nullptr; }
     if (c->fwd) { hipfftDestroy(c->fwd); c->fwd = 0; }
     if (c->fwd_tail) { hipfftDestroy(c->fwd_tail); c->fwd_tail = 0; c->tail_batch = 0; }
-    const int N = c->N, H = c->H, B = want_batch;
-    if (!c->fft_log2) HIPCHK(c, hipMalloc(&c->padbuf, (size_t)B * N * N * sizeof(float)));
-    HIPCHK(c, hipMalloc(&c->spec, (size_t)B * N * H * sizeof(float2)));
     if (c->vmax) { (void)hipFree(c->vmax); c->vmax = nullptr; }
-    HIPCHK(c, hipMalloc(&c->vmax, (size_t)B * sizeof(float)));
     if (c->fmean) { (void)hipFree(c->fmean); c->fmean = nullptr; }
-    if (c->fft_log2) HIPCHK(c, hipMalloc(&c->fmean, (size_t)B * sizeof(double)));
+    const int N = c->N, H = c->H, B = want_batch;
+    if (!c->fft_log2) HIPCHK(c, dmalloc(&c->padbuf, (size_t)B * N * N * sizeof(float)));
+    HIPCHK(c, dmalloc(&c->spec, (size_t)B * N * H * sizeof(float2)));
+    HIPCHK(c, dmalloc(&c->vmax, (size_t)B * sizeof(float)));
+    if (c->fft_log2) HIPCHK(c, dmalloc(&c->fmean, (size_t)B * sizeof(double)));
     int rc = ensure_lists(c, B);
     if (rc) return rc;
+    if (!c->fft_log2) {   // own transforms need no plan, and the pad buffer is not used
+        int n[2] = {N, N};
+        int inembed[2] = {N, N};
+        int onembed[2] = {N, H};
+        hipfftResult r = hipfftPlanMany(&c->fwd, 2, n, inembed, 1, N * N, onembed, 1, N * H, HIPFFT_R2C, B);
+        if (r != HIPFFT_SUCCESS) { c->fwd = 0; return fail(c, SPBP_ERR_FFT, "hipfftPlanMany(%d): hipfft error %d", B, (int)r); }
+    }
     c->fwd_batch = B;
-    if (c->fft_log2) return SPBP_OK;   // own transforms: no plan, and the pad buffer is not used
-    int n[2] = {N, N};
-    int inembed[2] = {N, N};
-    int onembed[2] = {N, H};
-    FFTCHK(c, hipfftPlanMany(&c->fwd, 2, n, inembed, 1, N * N, onembed, 1, N * H, HIPFFT_R2C, B));
     return SPBP_OK;
 }
 
@@ -314,7 +362,8 @@ hipfftHandle plan_for(spbp_ctx* c, int b, int* rc) {
     *rc = SPBP_OK;
     if (b == c->fwd_batch) return c->fwd;
     if (c->fwd_tail && c->tail_batch == b) return c->fwd_tail;
-    if (c->fwd_tail) { hipfftDestroy(c->fwd_tail); c->fwd_tail = 0; }
+    if (c->graphs > 0) { *rc = refuse_captured(c, "a new forward FFT plan"); return 0; }
+    if (c->fwd_tail) { hipfftDestroy(c->fwd_tail); c->fwd_tail = 0; c->tail_batch = 0; }
     const int N = c->N, H = c->H;
     int n[2] = {N, N};
     int inembed[2] = {N, N};
@@ -330,29 +379,39 @@ int ensure_geom(spbp_ctx* c, size_t n) {
         HIPCHK(c, hipEventSynchronize(c->geom_evt));   // previous upload has left the staging buffer
         return SPBP_OK;
     }
+    c->geom_cap = 0;   // committed again only once both buffers exist
     if (c->geom_h) { (void)hipEventSynchronize(c->geom_evt); (void)hipHostFree(c->geom_h); c->geom_h = nullptr; }
     if (c->geom_d) { (void)hipFree(c->geom_d); c->geom_d = nullptr; }
     const size_t cap = std::max<size_t>(n, 1024);
     HIPCHK(c, hipHostMalloc((void**)&c->geom_h, cap * sizeof(Geom), hipHostMallocDefault));
-    HIPCHK(c, hipMalloc(&c->geom_d, cap * sizeof(Geom)));
+    HIPCHK(c, dmalloc(&c->geom_d, cap * sizeof(Geom)));
     c->geom_cap = cap;
     return SPBP_OK;
 }
 
 int ensure_final(spbp_ctx* c) {
-    if (c->work) return SPBP_OK;
+    if (c->final_ok) return SPBP_OK;
+    // a failed earlier attempt may have left part of the set: start over
+    void* old[] = {c->work, c->real, c->wkb, c->psum, c->pcnt, c->mean};
+    for (void* q : old) if (q) (void)hipFree(q);
+    c->work = nullptr; c->real = nullptr; c->wkb = nullptr; c->psum = nullptr; c->pcnt = nullptr; c->mean = nullptr;
+    if (c->inv) { hipfftDestroy(c->inv); c->inv = 0; }
     const int N = c->N, nx = c->nx;
-    HIPCHK(c, hipMalloc(&c->work, c->nh * sizeof(float2)));
-    HIPCHK(c, hipMalloc(&c->real, (size_t)N * N * N * sizeof(float)));
+    HIPCHK(c, dmalloc(&c->work, c->nh * sizeof(float2)));
+    HIPCHK(c, dmalloc(&c->real, (size_t)N * N * N * sizeof(float)));
     std::vector<float> w;
     wkb_table(nx, N, w);
-    HIPCHK(c, hipMalloc(&c->wkb, w.size() * sizeof(float)));
+    HIPCHK(c, dmalloc(&c->wkb, w.size() * sizeof(float)));
     HIPCHK(c, hipMemcpy(c->wkb, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice));
     const size_t nb = nblocks((size_t)nx * nx * nx);
-    HIPCHK(c, hipMalloc(&c->psum, nb * sizeof(double)));
-    HIPCHK(c, hipMalloc(&c->pcnt, nb * sizeof(unsigned)));
-    HIPCHK(c, hipMalloc(&c->mean, sizeof(float)));
-    FFTCHK(c, hipfftPlan3d(&c->inv, N, N, N, HIPFFT_C2R));
+    HIPCHK(c, dmalloc(&c->psum, nb * sizeof(double)));
+    HIPCHK(c, dmalloc(&c->pcnt, nb * sizeof(unsigned)));
+    HIPCHK(c, dmalloc(&c->mean, sizeof(float)));
+    if (hipfftPlan3d(&c->inv, N, N, N, HIPFFT_C2R) != HIPFFT_SUCCESS) {
+        c->inv = 0;
+        return fail(c, SPBP_ERR_FFT, "hipfftPlan3d(%d) failed", N);
+    }
+    c->final_ok = true;
     return SPBP_OK;
 }
 
@@ -505,26 +564,33 @@ unsigned brick_grid(const spbp_ctx* c) {
     return (unsigned)c->ncu * ((c->shape == 0 || (c->flags & SPBP_GEOM_F64)) ? 3u : 4u);   // 16x12x8: 36.9 KB, 4 fit
 }
 
+template <int BX, int BY, int BZ, typename P, int NT = TPB, bool OWN = false>
+void launch_shape(bool wgt, unsigned grid, const BrickJob* table_d, int n, int* counter, hipStream_t st) {
+    if (wgt) insert_brick_kernel<BX, BY, BZ, P, NT, OWN, true><<<grid, NT, 0, st>>>(table_d, n, counter);
+    else insert_brick_kernel<BX, BY, BZ, P, NT, OWN, false><<<grid, NT, 0, st>>>(table_d, n, counter);
+}
+
 // One persistent brick-kernel launch over the device job table table_d[0..n),
 // all jobs with c's brick shape; `counter` (the queue head) must be zero on st.
-int launch_bricks_dev(spbp_ctx* c, const BrickJob* table_d, int n, int* counter, hipStream_t st) {
+// wgt false: no job accumulates weights (W0 and W1 null in every job).
+int launch_bricks_dev(spbp_ctx* c, const BrickJob* table_d, int n, int* counter, hipStream_t st, bool wgt = true) {
     const unsigned grid = brick_grid(c);
     const bool f64 = (c->flags & SPBP_GEOM_F64) != 0;
     switch (c->shape * 2 + (f64 ? 1 : 0)) {
-        case 0: insert_brick_kernel<16, 16, 8, float><<<grid, TPB, 0, st>>>(table_d, n, counter); break;
-        case 1: insert_brick_kernel<16, 16, 8, double><<<grid, TPB, 0, st>>>(table_d, n, counter); break;
-        case 2: insert_brick_kernel<16, 8, 8, float><<<grid, TPB, 0, st>>>(table_d, n, counter); break;
-        case 3: insert_brick_kernel<16, 8, 8, double><<<grid, TPB, 0, st>>>(table_d, n, counter); break;
-        case 4: insert_brick_kernel<8, 8, 8, float><<<grid, TPB, 0, st>>>(table_d, n, counter); break;
-        case 5: insert_brick_kernel<8, 8, 8, double><<<grid, TPB, 0, st>>>(table_d, n, counter); break;
-        case 6: insert_brick_kernel<16, 12, 8, float><<<grid, TPB, 0, st>>>(table_d, n, counter); break;
-        case 7: insert_brick_kernel<16, 12, 8, double><<<grid, TPB, 0, st>>>(table_d, n, counter); break;
-        case 8: insert_brick_kernel<16, 16, 10, float, 2 * TPB><<<grid, 2 * TPB, 0, st>>>(table_d, n, counter); break;
-        case 9: insert_brick_kernel<16, 16, 10, double, 2 * TPB><<<grid, 2 * TPB, 0, st>>>(table_d, n, counter); break;
-        case 10: insert_brick_kernel<16, 16, 12, float, 2 * TPB><<<grid, 2 * TPB, 0, st>>>(table_d, n, counter); break;
-        case 12: insert_brick_kernel<16, 16, 13, float, 2 * TPB><<<grid, 2 * TPB, 0, st>>>(table_d, n, counter); break;
-        case 14: insert_brick_kernel<16, 16, 13, float, 2 * TPB, true><<<grid, 2 * TPB, 0, st>>>(table_d, n, counter); break;
-        case 15: insert_brick_kernel<16, 16, 13, double, 2 * TPB, true><<<grid, 2 * TPB, 0, st>>>(table_d, n, counter); break;
+        case 0: launch_shape<16, 16, 8, float>(wgt, grid, table_d, n, counter, st); break;
+        case 1: launch_shape<16, 16, 8, double>(wgt, grid, table_d, n, counter, st); break;
+        case 2: launch_shape<16, 8, 8, float>(wgt, grid, table_d, n, counter, st); break;
+        case 3: launch_shape<16, 8, 8, double>(wgt, grid, table_d, n, counter, st); break;
+        case 4: launch_shape<8, 8, 8, float>(wgt, grid, table_d, n, counter, st); break;
+        case 5: launch_shape<8, 8, 8, double>(wgt, grid, table_d, n, counter, st); break;
+        case 6: launch_shape<16, 12, 8, float>(wgt, grid, table_d, n, counter, st); break;
+        case 7: launch_shape<16, 12, 8, double>(wgt, grid, table_d, n, counter, st); break;
+        case 8: launch_shape<16, 16, 10, float, 2 * TPB>(wgt, grid, table_d, n, counter, st); break;
+        case 9: launch_shape<16, 16, 10, double, 2 * TPB>(wgt, grid, table_d, n, counter, st); break;
+        case 10: launch_shape<16, 16, 12, float, 2 * TPB>(wgt, grid, table_d, n, counter, st); break;
+        case 12: launch_shape<16, 16, 13, float, 2 * TPB>(wgt, grid, table_d, n, counter, st); break;
+        case 14: launch_shape<16, 16, 13, float, 2 * TPB, true>(wgt, grid, table_d, n, counter, st); break;
+        case 15: launch_shape<16, 16, 13, double, 2 * TPB, true>(wgt, grid, table_d, n, counter, st); break;
         default: return fail(c, SPBP_ERR_ARG, "brick shape %d has no FP64-geometry kernel", c->shape);
     }
     HIPCHK(c, hipGetLastError());
@@ -532,13 +598,13 @@ int launch_bricks_dev(spbp_ctx* c, const BrickJob* table_d, int n, int* counter,
 }
 
 // Same with a host job table, staged through c's pinned buffer.
-int launch_bricks(spbp_ctx* c, const BrickJob* jh, int n, int* counter, hipStream_t st) {
+int launch_bricks(spbp_ctx* c, const BrickJob* jh, int n, int* counter, hipStream_t st, bool wgt = true) {
     if (n < 1 || n > MAX_JOBS) return fail(c, SPBP_ERR_ARG, "%d jobs in one launch (1..%d)", n, MAX_JOBS);
     if (c->jobs_cap < n) {
         if (c->jobs_h) { (void)hipEventSynchronize(c->jobs_evt); (void)hipHostFree(c->jobs_h); c->jobs_h = nullptr; }
         if (c->jobs_d) { (void)hipFree(c->jobs_d); c->jobs_d = nullptr; }
         HIPCHK(c, hipHostMalloc((void**)&c->jobs_h, MAX_JOBS * sizeof(BrickJob), hipHostMallocDefault));
-        HIPCHK(c, hipMalloc(&c->jobs_d, MAX_JOBS * sizeof(BrickJob)));
+        HIPCHK(c, dmalloc(&c->jobs_d, MAX_JOBS * sizeof(BrickJob)));
         c->jobs_cap = MAX_JOBS;
     } else {
         HIPCHK(c, hipEventSynchronize(c->jobs_evt));   // the previous upload has left the staging buffer
@@ -546,12 +612,14 @@ int launch_bricks(spbp_ctx* c, const BrickJob* jh, int n, int* counter, hipStrea
     std::memcpy(c->jobs_h, jh, n * sizeof(BrickJob));
     HIPCHK(c, hipMemcpyAsync(c->jobs_d, c->jobs_h, n * sizeof(BrickJob), hipMemcpyHostToDevice, st));
     HIPCHK(c, hipEventRecord(c->jobs_evt, st));
-    return launch_bricks_dev(c, c->jobs_d, n, counter, st);
+    return launch_bricks_dev(c, c->jobs_d, n, counter, st, wgt);
 }
 
 // Per-(brick, half) projection lists of b projections (geometry at geom_d);
 // also zeroes c's queue head.
 int build_lists(spbp_ctx* c, const Geom* geom_d, int b, hipStream_t st) {
+    if (c->bg.nb <= 0 || !c->bcnt || !c->blist || !c->bikey || c->blist_cap < (size_t)b * c->bg.nb)
+        return fail(c, SPBP_ERR_ARG, "brick lists not allocated for %d sections", b);
     const int nkeys = 2 * c->bg.nb;
     const dim3 lgrid(c->bg.nb, (b + 1023) / 1024);
     Timed tm(c, SPBP_K_BIN, b, st);
@@ -586,12 +654,14 @@ int launch_direct(spbp_ctx* c, int img0, const Geom* geom_d, int ng, const Targe
 // ONELINE over ng sections (geometry at geom_d) of images img0.. whose raw
 // R2C spectra are in c->spec, with the context's strategy; `units` = images.
 int run_insert(spbp_ctx* c, int img0, const Geom* geom_d, int ng, const Targets& t, hipStream_t st, long units) {
+    if (!c->spec || !c->vmax || ng > c->fwd_batch * c->nsym)
+        return fail(c, SPBP_ERR_ARG, "chunk buffers not allocated for %d sections", ng);
     if (c->strategy == SPBP_STRAT_DIRECT) return launch_direct(c, img0, geom_d, ng, t, st, units);
     int rc = build_lists(c, geom_d, ng, st);
     if (rc) return rc;
     const BrickJob job = brick_job(c, img0, geom_d, ng, t);
     Timed tm(c, SPBP_K_INSERT, units, st);
-    return launch_bricks(c, &job, 1, c->bnext, st);
+    return launch_bricks(c, &job, 1, c->bnext, st, t.W0 != nullptr || t.W1 != nullptr);
 }
 
 // PADD2 + batched forward FFT of b images into c->spec.
@@ -684,11 +754,12 @@ int check_images(spbp_ctx* c, const void* imgs, int dtype, int nimg, hipStream_t
 // into c->rprm_d, queued on st.
 int stage_rtsq(spbp_ctx* c, int nimg, const float* inplane, hipStream_t st) {
     if (c->rprm_cap < nimg) {
+        c->rprm_cap = 0;   // committed again only once both buffers exist
         if (c->rprm_h) { (void)hipEventSynchronize(c->rprm_evt); (void)hipHostFree(c->rprm_h); c->rprm_h = nullptr; }
         if (c->rprm_d) { (void)hipFree(c->rprm_d); c->rprm_d = nullptr; }
         const int cap = std::max(nimg, 1024);
         HIPCHK(c, hipHostMalloc((void**)&c->rprm_h, cap * sizeof(RtsqParam), hipHostMallocDefault));
-        HIPCHK(c, hipMalloc(&c->rprm_d, cap * sizeof(RtsqParam)));
+        HIPCHK(c, dmalloc(&c->rprm_d, cap * sizeof(RtsqParam)));
         if (!c->rprm_evt) HIPCHK(c, hipEventCreateWithFlags(&c->rprm_evt, hipEventDisableTiming));
         c->rprm_cap = cap;
     } else {
@@ -727,12 +798,14 @@ int insert_impl(spbp_ctx* c, const void* imgs, int dtype, int nimg, const float*
     const size_t esz = dtype == SPBP_BF16 ? 2 : 4;
     const size_t npix = (size_t)c->nx * c->nx;
     if (inplane && c->rot_cap < c->fwd_batch) {
+        c->rot_cap = 0;
         if (c->rotbuf) { (void)hipFree(c->rotbuf); c->rotbuf = nullptr; }
-        HIPCHK(c, hipMalloc(&c->rotbuf, (size_t)c->fwd_batch * npix * sizeof(float)));
+        HIPCHK(c, dmalloc(&c->rotbuf, (size_t)c->fwd_batch * npix * sizeof(float)));
         c->rot_cap = c->fwd_batch;
     }
     const Targets t{c->X[0], c->W[0], c->X[1], c->W[1], c->Wc[0], c->Wc[1]};
     const int ns = c->nsym;
+    c->filled = true;
     for (int p0 = 0; p0 < nimg; p0 += c->fwd_batch) {
         const int b = std::min(c->fwd_batch, nimg - p0);
         const char* src = (const char*)imgs + (size_t)p0 * npix * esz;
@@ -821,7 +894,7 @@ int grow(spbp_ctx* c, T*& p, size_t& cap, size_t n, const char* what) {
     if (n <= cap && p) return SPBP_OK;
     if (p) { (void)hipFree(p); p = nullptr; cap = 0; }
     const size_t m = std::max<size_t>(std::max<size_t>(n, 2 * cap), 1024);
-    if (hipMalloc((void**)&p, m * sizeof(T)) != hipSuccess) { p = nullptr; return fail(c, SPBP_ERR_ALLOC, "%s: %zu bytes", what, m * sizeof(T)); }
+    if (dmalloc((void**)&p, m * sizeof(T)) != hipSuccess) { p = nullptr; return fail(c, SPBP_ERR_ALLOC, "%s: %zu bytes", what, m * sizeof(T)); }
     cap = m;
     return SPBP_OK;
 }
@@ -941,14 +1014,14 @@ int resolve_threshold(spbp_ctx* c, int which, Acc& a, bool* active, long long* n
     Timed tm(c, SPBP_K_RESOLVE, 1, st);
     if ((rc = upload_history(c, st))) return rc;
     if (!c->tmask) {
-        HIPCHK(c, hipMalloc(&c->tmask, (c->nh + 3) / 4 * 4));
+        HIPCHK(c, dmalloc(&c->tmask, (c->nh + 3) / 4 * 4));
         c->tmask_dirty = true;
     }
     if (c->tmask_dirty) {
         fill_kernel<<<nblocks((c->nh + 3) / 4), TPB, 0, st>>>((float*)c->tmask, (c->nh + 3) / 4, 0.0f);   // zero bits
         c->tmask_dirty = false;
     }
-    if (!c->rctr) HIPCHK(c, hipMalloc(&c->rctr, 4 * sizeof(unsigned long long)));
+    if (!c->rctr) HIPCHK(c, dmalloc(&c->rctr, 4 * sizeof(unsigned long long)));
     // 1. voxels in the band
     if ((rc = grow(c, c->rlist, c->rlist_cap, 4096, "threshold list"))) return rc;
     unsigned long long nv = 0;
@@ -1002,7 +1075,7 @@ int finalize_one(spbp_ctx* c, int which, float* out, hipStream_t st) {
     subtract_kernel<<<nb, TPB, 0, st>>>(out, c->mean, nx);
     HIPCHK(c, hipGetLastError());
     if (c->nsym > 1) {   // real-space symmetrisation after WINDKB2A (bp32f.f:278-303)
-        if (!c->symtmp) HIPCHK(c, hipMalloc(&c->symtmp, nv * sizeof(float)));
+        if (!c->symtmp) HIPCHK(c, dmalloc(&c->symtmp, nv * sizeof(float)));
         HIPCHK(c, hipMemcpyAsync(c->symtmp, out, nv * sizeof(float), hipMemcpyDeviceToDevice, st));
         symvol_kernel<<<nb, TPB, 0, st>>>(c->symtmp, out, nx, c->sm_d, c->nsym);
         HIPCHK(c, hipGetLastError());
@@ -1043,12 +1116,14 @@ int spbp_create(const spbp_opts* o, spbp_ctx** out) {
     if (o->batch > 0) {
         c->batch = o->batch;
     } else {
-        // 2 GiB of chunk buffers (of 288 GB): the spectra, plus the padded
+        // 8 GiB of chunk buffers (of 288 GB): the spectra, plus the padded
         // images when hipFFT does the transforms.  Larger chunks mean fewer
-        // list builds and fewer persistent-kernel tails.
+        // list builds and fewer persistent-kernel tails: 20,000 projections
+        // of 128^2 go through one launch (481.8k proj/s for a BP 32F job,
+        // 473.5k in three launches of 2 GiB chunks).
         const bool own = (c->N & (c->N - 1)) == 0 && c->N >= 8 && c->N <= 2048 && !(o->flags & SPBP_HIPFFT);
         const size_t per = (own ? 0 : (size_t)c->N * c->N * 4) + (size_t)c->N * c->H * 8;
-        c->batch = (int)std::max<size_t>(16, std::min<size_t>(8192, ((size_t)2048 << 20) / per));
+        c->batch = (int)std::max<size_t>(16, std::min<size_t>(32768, ((size_t)8192 << 20) / per));
     }
     c->strategy = o->strategy == SPBP_STRAT_DIRECT ? SPBP_STRAT_DIRECT : SPBP_STRAT_BRICK;
     if (o->strategy < 0 || o->strategy > SPBP_STRAT_BRICK) { delete c; return fail(nullptr, SPBP_ERR_ARG, "unknown strategy %d", o->strategy); }
@@ -1061,9 +1136,9 @@ int spbp_create(const spbp_opts* o, spbp_ctx** out) {
     int rc = SPBP_OK;
     auto bail = [&](int code) { spbp_destroy(c); return code; };
     for (int h = 0; h < c->nvol; ++h) {
-        if (hipMalloc(&c->X[h], c->nh * sizeof(float2)) != hipSuccess ||
-            hipMalloc(&c->W[h], c->nh * sizeof(float)) != hipSuccess ||
-            ((c->flags & SPBP_TWOSUM_W) && hipMalloc(&c->Wc[h], c->nh * sizeof(float)) != hipSuccess))
+        if (dmalloc(&c->X[h], c->nh * sizeof(float2)) != hipSuccess ||
+            dmalloc(&c->W[h], c->nh * sizeof(float)) != hipSuccess ||
+            ((c->flags & SPBP_TWOSUM_W) && dmalloc(&c->Wc[h], c->nh * sizeof(float)) != hipSuccess))
             return bail(fail(nullptr, SPBP_ERR_ALLOC, "BP 32F; XE, WE, WO, & XO: %zu bytes", c->nh * 12 * (size_t)c->nvol));
     }
     std::vector<float> tab;
@@ -1071,19 +1146,19 @@ int spbp_create(const spbp_opts* o, spbp_ctx** out) {
         kb_table_f64_f32(c->N, tab, c->fltb);
         std::vector<double> t64;
         kb_table_f64(c->N, t64);
-        if (hipMalloc(&c->tab64, t64.size() * sizeof(double)) != hipSuccess ||
+        if (dmalloc(&c->tab64, t64.size() * sizeof(double)) != hipSuccess ||
             hipMemcpy(c->tab64, t64.data(), t64.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
             return bail(fail(nullptr, SPBP_ERR_ALLOC, "FP64 table"));
     } else {
         kb_table(c->N, tab, c->fltb);
     }
-    if (hipMalloc(&c->tab, tab.size() * sizeof(float)) != hipSuccess ||
+    if (dmalloc(&c->tab, tab.size() * sizeof(float)) != hipSuccess ||
         hipMemcpy(c->tab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMalloc(&c->flag, sizeof(int)) != hipSuccess ||
+        dmalloc(&c->flag, sizeof(int)) != hipSuccess ||
         hipEventCreateWithFlags(&c->geom_evt, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->jobs_evt, hipEventDisableTiming) != hipSuccess)
         return bail(fail(nullptr, SPBP_ERR_DEVICE, "context set-up failed"));
-    if (hipMalloc(&c->bnext, sizeof(int)) != hipSuccess) return bail(fail(nullptr, SPBP_ERR_ALLOC, "brick queue"));
+    if (dmalloc(&c->bnext, sizeof(int)) != hipSuccess) return bail(fail(nullptr, SPBP_ERR_ALLOC, "brick queue"));
     if ((c->N & (c->N - 1)) == 0 && c->N >= 8 && c->N <= 2048 && !(o->flags & SPBP_HIPFFT)) {
         int L = 0;
         while ((1 << L) < c->N) ++L;
@@ -1092,7 +1167,7 @@ int spbp_create(const spbp_opts* o, spbp_ctx** out) {
             const double a = -2.0 * M_PI * (double)m / (double)c->N;
             t[m] = make_float2((float)std::cos(a), (float)std::sin(a));
         }
-        if (hipMalloc(&c->tw, t.size() * sizeof(float2)) != hipSuccess ||
+        if (dmalloc(&c->tw, t.size() * sizeof(float2)) != hipSuccess ||
             hipMemcpy(c->tw, t.data(), t.size() * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess)
             return bail(fail(nullptr, SPBP_ERR_ALLOC, "twiddle table"));
         c->fft_log2 = L;
@@ -1110,6 +1185,7 @@ int spbp_create(const spbp_opts* o, spbp_ctx** out) {
 
 int spbp_destroy(spbp_ctx* c) {
     if (!c) return SPBP_OK;
+    if (c->graphs > 0) return refuse_captured(c, "spbp_destroy");
     DeviceGuard dg(c->device);
     if (c->geom_evt) (void)hipEventSynchronize(c->geom_evt);
     if (c->jobs_evt) (void)hipEventSynchronize(c->jobs_evt);
@@ -1148,6 +1224,7 @@ int spbp_reset(spbp_ctx* c, void* stream) {
     DeviceGuard dg(c->device);
     c->hist.clear();
     c->hist_ok = true;
+    c->filled = false;
     hipStream_t st = (hipStream_t)stream;
     for (int h = 0; h < c->nvol; ++h) {
         fill_kernel<<<nblocks(2 * c->nh), TPB, 0, st>>>((float*)c->X[h], 2 * c->nh, 0.0f);
@@ -1229,8 +1306,9 @@ int spbp_gather_images(spbp_ctx* c, const float* X, int nimg, const double* angl
     if ((rc = ensure_chunk(c, std::min(c->batch, nimg)))) return rc;
     const int N = c->N, H = c->H, B = c->fwd_batch;
     if (c->c2r_cap < B) {
+        c->c2r_cap = 0;
         if (c->c2rbuf) { (void)hipFree(c->c2rbuf); c->c2rbuf = nullptr; }
-        HIPCHK(c, hipMalloc(&c->c2rbuf, (size_t)B * N * N * sizeof(float)));
+        HIPCHK(c, dmalloc(&c->c2rbuf, (size_t)B * N * N * sizeof(float)));
         c->c2r_cap = B;
     }
     const int ip = (N - c->nx) / 2 + c->nx % 2;
@@ -1307,6 +1385,7 @@ int spbp_tune(spbp_ctx* c, const void* imgs, int dtype, int nimg, const double* 
     if (!c) return fail(nullptr, SPBP_ERR_ARG, "NULL context");
     if (nimg <= 0 || !imgs) return fail(c, SPBP_ERR_ARG, "tuning needs a sample of images");
     if (dtype != SPBP_F32 && dtype != SPBP_BF16) return fail(c, SPBP_ERR_ARG, "unknown image type %d", dtype);
+    if (c->graphs > 0) return refuse_captured(c, "spbp_tune");
     DeviceGuard dg(c->device);
     hipStream_t st = (hipStream_t)stream;
     const int m = std::min(nimg, std::min(c->batch, 1024));
@@ -1317,8 +1396,8 @@ int spbp_tune(spbp_ctx* c, const void* imgs, int dtype, int nimg, const double* 
     if ((rc = pad_fft(c, imgs, dtype, b, st))) return rc;
     float2* Xs = nullptr;
     float* Ws = nullptr;
-    HIPCHK(c, hipMalloc(&Xs, c->nh * sizeof(float2)));
-    if (hipMalloc(&Ws, c->nh * sizeof(float)) != hipSuccess) { (void)hipFree(Xs); return fail(c, SPBP_ERR_ALLOC, "tuning scratch"); }
+    HIPCHK(c, dmalloc(&Xs, c->nh * sizeof(float2)));
+    if (dmalloc(&Ws, c->nh * sizeof(float)) != hipSuccess) { (void)hipFree(Xs); return fail(c, SPBP_ERR_ALLOC, "tuning scratch"); }
     const Targets t{Xs, Ws, nullptr, nullptr, nullptr, nullptr};
     const int s_strat = c->strategy, s_shape = c->shape, s_C = c->C;
     float ms = 0.f;
@@ -1519,6 +1598,7 @@ struct spbp_graph {
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    bool counted = false;   // the contexts' live-graph counts include this graph
 };
 
 extern "C" {
@@ -1530,7 +1610,10 @@ int spbp_insert_batch(const spbp_job* jobs, int njobs, void* stream) {
     hipStream_t st = (hipStream_t)stream;
     if ((rc = stage_batch(jobs, njobs, st))) return rc;
     std::vector<Geom*> geom(njobs);
-    for (int j = 0; j < njobs; ++j) geom[j] = jobs[j].ctx->geom_d;
+    for (int j = 0; j < njobs; ++j) {
+        geom[j] = jobs[j].ctx->geom_d;
+        if (jobs[j].nimg > 0) jobs[j].ctx->filled = true;
+    }
     rc = enqueue_batch(jobs, njobs, geom.data(), nullptr, st);
     for (int j = 0; j < njobs; ++j) {
         if (rc) jobs[j].ctx->hist_ok = false;
@@ -1548,6 +1631,8 @@ int spbp_graph_destroy(spbp_graph* g) {
     for (Geom* p : g->geom) if (p) (void)hipFree(p);
     for (BrickJob* p : g->tables) if (p) (void)hipFree(p);
     if (g->stream) (void)hipStreamDestroy(g->stream);
+    if (g->counted)
+        for (const spbp_job& j : g->jobs) --j.ctx->graphs;
     delete g;
     return SPBP_OK;
 }
@@ -1579,7 +1664,7 @@ int spbp_graph_create(const spbp_job* jobs, int njobs, int reset_first, spbp_gra
         if (jobs[j].nimg == 0) continue;
         const size_t ng = (size_t)jobs[j].nimg * c->nsym;
         g->hgeom[j].assign(c->geom_h, c->geom_h + ng);
-        if (hipMalloc(&g->geom[j], ng * sizeof(Geom)) != hipSuccess) return bail(fail(c, SPBP_ERR_ALLOC, "graph geometry"));
+        if (dmalloc(&g->geom[j], ng * sizeof(Geom)) != hipSuccess) return bail(fail(c, SPBP_ERR_ALLOC, "graph geometry"));
         if (hipMemcpyAsync(g->geom[j], c->geom_d, ng * sizeof(Geom), hipMemcpyDeviceToDevice, g->stream) != hipSuccess)
             return bail(fail(c, SPBP_ERR_DEVICE, "graph geometry copy"));
         // run each FFT plan the batch uses once, outside the capture (lazy
@@ -1603,7 +1688,7 @@ int spbp_graph_create(const spbp_job* jobs, int njobs, int reset_first, spbp_gra
         spbp_ctx* lead = nullptr;
         const int n = round_table(jobs, njobs, g->geom.data(), r, table, &lead);
         if (!n) continue;
-        if (hipMalloc(&g->tables[r], MAX_JOBS * sizeof(BrickJob)) != hipSuccess ||
+        if (dmalloc(&g->tables[r], MAX_JOBS * sizeof(BrickJob)) != hipSuccess ||
             hipMemcpy(g->tables[r], table, n * sizeof(BrickJob), hipMemcpyHostToDevice) != hipSuccess)
             return bail(fail(nullptr, SPBP_ERR_ALLOC, "graph job tables"));
     }
@@ -1629,6 +1714,9 @@ int spbp_graph_create(const spbp_job* jobs, int njobs, int reset_first, spbp_gra
     if (rc) return bail(rc);
     if (hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess)
         return bail(fail(nullptr, SPBP_ERR_DEVICE, "hipGraphInstantiate failed"));
+    // from now on the contexts' buffers, lists and plans must stay in place
+    for (int j = 0; j < njobs; ++j) ++jobs[j].ctx->graphs;
+    g->counted = true;
     *out = g;
     return SPBP_OK;
 }
@@ -1641,6 +1729,7 @@ int spbp_graph_launch(spbp_graph* g, void* stream) {
         spbp_ctx* c = g->jobs[j].ctx;
         if (e != hipSuccess) { c->hist_ok = false; continue; }
         if (g->reset_first) { c->hist.clear(); c->hist_ok = true; }
+        if (g->jobs[j].nimg > 0) c->filled = true;
         if (c->hist_ok) c->hist.insert(c->hist.end(), g->hgeom[j].begin(), g->hgeom[j].end());
     }
     if (e != hipSuccess) return fail(nullptr, SPBP_ERR_DEVICE, "hipGraphLaunch: %s", hipGetErrorString(e));
@@ -1670,7 +1759,7 @@ int spbp_rtsq(const void* imgs, int dtype, int nimg, int nx, const float* params
     }
     hipStream_t st = (hipStream_t)stream;
     RtsqParam* d = nullptr;
-    HIPCHK(nullptr, hipMalloc(&d, nimg * sizeof(RtsqParam)));
+    HIPCHK(nullptr, dmalloc(&d, nimg * sizeof(RtsqParam)));
     hipError_t e = hipMemcpyAsync(d, h.data(), nimg * sizeof(RtsqParam), hipMemcpyHostToDevice, st);
     const size_t npix = (size_t)nx * nx;
     for (int p0 = 0; p0 < nimg && e == hipSuccess; p0 += 65535) {
@@ -1703,7 +1792,7 @@ int insert_host_impl(spbp_ctx* c, const void* imgs_host, int dtype, int nimg, co
     const size_t per = (size_t)c->nx * c->nx * esz;
     const int chunk = std::max(1, std::min(nimg, c->batch));
     void* dimg = nullptr;
-    HIPCHK(c, hipMalloc(&dimg, per * chunk));
+    HIPCHK(c, dmalloc(&dimg, per * chunk));
     int rc = SPBP_OK;
     for (int p0 = 0; p0 < nimg && rc == SPBP_OK; p0 += chunk) {
         const int b = std::min(chunk, nimg - p0);
@@ -1752,7 +1841,7 @@ int spbp_finalize_host(spbp_ctx* c, float* out_full, float* out_h1, float* out_h
     const size_t nv = (size_t)c->nx * c->nx * c->nx;
     float* outs[3] = {out_full, out_h1, out_h2};
     float* d = nullptr;
-    HIPCHK(c, hipMalloc(&d, nv * sizeof(float)));
+    HIPCHK(c, dmalloc(&d, nv * sizeof(float)));
     int rc = SPBP_OK;
     for (int q = 0; q < 3 && rc == SPBP_OK; ++q) {
         if (!outs[q]) continue;
@@ -1791,6 +1880,10 @@ int spbp_export_raw(spbp_ctx* c, int half, float* X, float* W, void* stream) {
 int spbp_set_symmetries(spbp_ctx* c, int nsym, const double* angles_deg) {
     if (!c) return fail(nullptr, SPBP_ERR_ARG, "NULL context");
     if (nsym < 0 || (nsym > 1 && !angles_deg)) return fail(c, SPBP_ERR_ARG, "bad symmetry arguments");
+    if (c->graphs > 0) return refuse_captured(c, "spbp_set_symmetries");
+    // DMS insertion and SYMVOL use one set of symmetries per operation
+    // (bp32f.f:866-872, 278-303): it is fixed while the accumulators hold sums
+    if (c->filled) return fail(c, SPBP_ERR_ARG, "spbp_set_symmetries on accumulators that hold sums (spbp_reset first)");
     DeviceGuard dg(c->device);
     if (nsym <= 1) {   // one operation (the identity) or none: no symmetrisation (bp32f.f:169)
         c->nsym = 1;
@@ -1806,14 +1899,24 @@ int spbp_set_symmetries(spbp_ctx* c, int nsym, const double* angles_deg) {
         cang_d(a[2], a[1], a[0], (c->flags & SPBP_EXACT_PI) != 0, smd.data() + 9 * k);
     }
     HIPCHK(c, hipDeviceSynchronize());   // no queued work may still read the old tables
-    if (c->sm_d) { (void)hipFree(c->sm_d); c->sm_d = nullptr; }
-    HIPCHK(c, hipMalloc(&c->sm_d, sm.size() * sizeof(float)));
-    HIPCHK(c, hipMemcpy(c->sm_d, sm.data(), sm.size() * sizeof(float), hipMemcpyHostToDevice));
+    float* smd_new = nullptr;
+    HIPCHK(c, dmalloc(&smd_new, sm.size() * sizeof(float)));
+    if (hipMemcpy(smd_new, sm.data(), sm.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(smd_new);
+        return fail(c, SPBP_ERR_DEVICE, "symmetry table upload failed");
+    }
+    if (c->sm_d) (void)hipFree(c->sm_d);
+    c->sm_d = smd_new;
     c->sm = sm;
     c->smd = smd;
     c->nsym = nsym;
     c->tuned = false;
     return c->fwd_batch > 0 ? ensure_lists(c, c->fwd_batch) : SPBP_OK;
+}
+
+int spbp_debug_fail_alloc(long k) {
+    g_fail_after.store(k < 0 ? -1 : k);
+    return SPBP_OK;
 }
 
 int spbp_kb_table(int n2, int flags, float* tab, float* fltb) {
@@ -1857,7 +1960,7 @@ int spbp_calibrate(int what, size_t bytes, double* rate, void* stream) {
         return fail(nullptr, SPBP_ERR_ARG, "calibration buffer of %zu bytes below %zu", bytes, gwidth * sizeof(float));
     void* buf = nullptr;
     const size_t nbytes = what == SPBP_CAL_LDS_U64 ? lblocks * sizeof(unsigned long long) : n * sizeof(float);
-    if (hipMalloc(&buf, nbytes) != hipSuccess) return fail(nullptr, SPBP_ERR_ALLOC, "calibration buffer");
+    if (dmalloc(&buf, nbytes) != hipSuccess) return fail(nullptr, SPBP_ERR_ALLOC, "calibration buffer");
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = SPBP_OK;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = fail(nullptr, SPBP_ERR_DEVICE, "events");
@@ -1931,6 +2034,7 @@ int spbp_import_partial(spbp_ctx* c, int half, const float* X, const float* W, i
     HIPCHK(c, hipGetLastError());
     // the accumulators now hold sums the insertion history does not describe
     c->hist_ok = false;
+    c->filled = true;
     return SPBP_OK;
 }
 
@@ -1953,7 +2057,7 @@ int spbp_reference_weights(spbp_ctx* c, int which, const long long* voxels, int 
     if ((rc = upload_history(c, st)) || (rc = grow(c, c->rlist, c->rlist_cap, (size_t)n, "voxel list"))) return rc;
     HIPCHK(c, hipMemcpyAsync(c->rlist, voxels, (size_t)n * sizeof(long long), hipMemcpyHostToDevice, st));
     double* d = nullptr;
-    HIPCHK(c, hipMalloc(&d, (size_t)n * sizeof(double)));
+    HIPCHK(c, dmalloc(&d, (size_t)n * sizeof(double)));
     rc = resolve_list(c, which, make_acc(c, which), n, d, st);
     if (rc == SPBP_OK && (hipMemcpyAsync(w_out, d, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
                           hipStreamSynchronize(st) != hipSuccess))

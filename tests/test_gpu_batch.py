@@ -206,3 +206,72 @@ def test_non_orthonormal_rotations_are_refused(bpmod):
         with pytest.raises(_lib.SpbpError) as e:
             b.insert_rotations(torch.from_numpy(imgs).cuda(), bad)
         assert e.value.code == _lib.ERR_ANGLE and "image 3" in str(e.value)
+
+
+def test_live_graph_refuses_calls_that_free_its_buffers(bpmod):
+    """While a captured graph exists, calls that would free or replace the
+    chunk buffers, brick lists or forward FFT plans it replays on fail with
+    SPBP_ERR_ARG (203) and change nothing; calls that fit the buffers run,
+    and the graph still replays exactly.  After spbp_graph_destroy the same
+    calls succeed.  No replay follows a refused call that could have
+    invalidated the graph."""
+    from spider_amd import _lib
+    c1 = bpmod.Backprojector(20, batch=64)     # N = 40: hipFFT transforms, tail plan of 36
+    c2 = bpmod.Backprojector(16)               # N = 32: own transforms, chunk of 50
+    imgs1, ang1 = dataset(20, 100, 91)
+    imgs2, ang2 = dataset(16, 50, 92)
+    t1, t2 = torch.from_numpy(imgs1).cuda(), torch.from_numpy(imgs2).cuda()
+    big, bang = dataset(16, 200, 93)
+    g = bpmod.BatchGraph([(c1, t1, ang1), (c2, t2, ang2)], reset_first=True)
+    try:
+        g.launch()
+        ref = [_export(c1), _export(c2)]
+        refused = [
+            lambda: c1.set_plan("brick", 0, 0),                       # another brick shape
+            lambda: c1.tune(t1[:32], ang1[:32]),                     # re-plans every shape
+            lambda: c2.set_symmetries([[0, 0, 0], [0, 0, 120], [0, 0, 240]]),
+            lambda: c2.insert(torch.from_numpy(big).cuda(), bang),   # larger chunk buffers
+            lambda: c1.insert(t1[:50], ang1[:50]),                    # a new hipFFT tail plan
+            lambda: c1.close(),
+        ]
+        for k, call in enumerate(refused):
+            with pytest.raises(bpmod.SpbpError) as e:
+                call()
+            assert e.value.code == _lib.ERR_ARG, (k, str(e.value))
+            assert "captured batch graph" in str(e.value), k
+        c1.insert(t1[:64], ang1[:64])      # fits the chunk and plan: allowed
+        c2.insert(t2[:20], ang2[:20])
+        torch.cuda.synchronize()
+        g.launch()
+        for c, r in zip((c1, c2), ref):
+            X, W = _export(c)
+            np.testing.assert_array_equal(W != 0, r[1] != 0)
+            assert rel(W, r[1]) <= 1e-6 and rel(X, r[0]) <= 1e-6
+    finally:
+        g.close()
+    # released: the refused calls now run
+    c1.set_plan("brick", 0, 0)
+    c1.reset()
+    c1.insert(t1[:50], ang1[:50])
+    c2.reset()
+    c2.insert(torch.from_numpy(big).cuda(), bang)
+    X, W = _export(c2)
+    Xr, Wr = oracle_raw(big, bang)
+    assert rel(W, Wr) <= 1e-6 and rel(X, Xr) <= 1e-6
+    c1.close()
+    c2.close()
+
+
+def test_symmetries_fixed_while_accumulators_hold_sums(bpmod):
+    """One set of symmetries per operation (DMS insertion and SYMVOL,
+    bp32f.f:866-872, 278-303): spbp_set_symmetries is refused between an
+    insert and spbp_reset, and allowed again after the reset."""
+    from spider_amd import _lib
+    imgs, ang = dataset(16, 10, 95)
+    with bpmod.Backprojector(16) as b:
+        b.insert(torch.from_numpy(imgs).cuda(), ang)
+        with pytest.raises(bpmod.SpbpError) as e:
+            b.set_symmetries([[0, 0, 0], [0, 0, 180]])
+        assert e.value.code == _lib.ERR_ARG
+        b.reset()
+        b.set_symmetries([[0, 0, 0], [0, 0, 180]])
