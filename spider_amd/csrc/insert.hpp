@@ -1,5 +1,7 @@
 // insert.hpp -- PADD2 and the ONELINE scatter kernels.
 #pragma once
+#include <type_traits>
+
 #include "common.hpp"
 
 namespace spbp {
@@ -316,6 +318,13 @@ struct BrickLds {
     // rotated rows start at multiples of BX * 8 bytes, so the row offset is
     // OR-ed onto the rotated x (one v_and_or_b32)
     __device__ static int at8(int base8, int r8, int x8) { return ROT ? base8 | ((x8 + r8) & ((BX - 1) * 8)) : base8 + x8; }
+    // brick-local coordinates of the v-th voxel (x fastest) and its element
+    __device__ static int coords(int v, int& x, int& y, int& z) {
+        x = v % BX;
+        y = (v / BX) % BY;
+        z = v / (BX * BY);
+        return at(rowbase(y, z), rot(y, z), x);
+    }
     // gfx950's LDS runs ds_add_u64 at 0.14 cycles per lane, ds_add_f64 at
     // 0.26 and ds_add_f32 at ~3 (measured).  X partial sums are exact 64-bit
     // fixed point: each per-tap term is the exact product of the fp32
@@ -329,6 +338,36 @@ struct BrickLds {
     // word holds the sum of its terms' raw encodings (fixed_term); the
     // flush decodes them (brick_sums).  Without weights (WGT false: the
     // adjoint scatter) the w array is not used.
+    unsigned long long re[NS], im[NS], w[WGT ? NS : 1];
+    int item;
+    float vmax[NT / 64];
+};
+
+// LDS layout of the owner-computes regions, in 4 x 4 tiles of (x, y):
+// element ((z (BY/4) + y/4) (BX/4) + x/4) 16 + 4 (y mod 4) + x mod 4.
+// gfx950's LDS has 32 four-byte banks and serves a 64-bit access for 16
+// lanes per cycle, so 16 lanes whose elements differ mod 16 -- here: whose
+// (x mod 4, y mod 4) differ -- never conflict (measured for ds_add_u64: 6.2
+// CU-cycles per wave instruction, against 12.9 for random elements and 6.35
+// for consecutive ones).  own_taps orders each lane's taps so that at every
+// tap slot the 16 lanes of a group hold the 16 distinct (x mod 4, y mod 4).
+// The element index is the sum of one part per axis (disjoint bit fields of
+// the tile index and the in-tile position).  Sums as in BrickLds.
+template <int BX, int BY, int BZ, int NT = TPB, bool WGT = true>
+struct TileLds {
+    static_assert(BX % 4 == 0 && BY % 4 == 0, "the tile layout needs x and y edges divisible by 4");
+    static constexpr int NS = BX * BY * BZ;
+    static constexpr int TX = BX / 4, TY = BY / 4;
+    __device__ static int xpart(int x) { return ((x >> 2) << 4) | (x & 3); }
+    __device__ static int ypart(int y) { return (((y >> 2) * TX) << 4) | ((y & 3) << 2); }
+    __device__ static int zpart(int z) { return (z * TY * TX) << 4; }
+    __device__ static int coords(int v, int& x, int& y, int& z) {
+        const int t = v >> 4;
+        x = ((t % TX) << 2) | (v & 3);
+        y = (((t / TX) % TY) << 2) | ((v >> 2) & 3);
+        z = t / (TX * TY);
+        return v;
+    }
     unsigned long long re[NS], im[NS], w[WGT ? NS : 1];
     int item;
     float vmax[NT / 64];
@@ -621,10 +660,11 @@ __device__ inline void box_lines(const Geom& g, const Box& box, int& b0, int& nl
 // in the core.  (x0, y0, z0): the core's origin.
 
 template <int BX, int BY, int BZ, bool MIRPASS, bool XCHK, typename P, int NT, bool WGT>
-__device__ __forceinline__ void own_taps(BrickLds<BX, BY, BZ, NT, WGT>& L, const Geom& gc,
+__device__ __forceinline__ void own_taps(TileLds<BX, BY, BZ, NT, WGT>& L, const Geom& gc,
                                          const float2* __restrict__ spec_p, int I, int J, int nx, int N, int x0,
                                          int y0, int z0, const float* __restrict__ tab, P fltb, double vscale,
                                          double wscale) {
+    using Lds = TileLds<BX, BY, BZ, NT, WGT>;
     constexpr int CX = BX - 4, CY = BY - 4, CZ = BZ - 4;
     const int H = nx + 1;
     P x, y, z;
@@ -634,39 +674,50 @@ __device__ __forceinline__ void own_taps(BrickLds<BX, BY, BZ, NT, WGT>& L, const
     const int ix = nearest_voxel(x, N), iy = nearest_voxel(y, N), iz = nearest_voxel(z, N);
     float2 v = spectrum_value(spec_p, I, J, N, H);
     if (flip) v.y = -v.y;
-    float tx[4], ty[4], tz[4];
-    const float wx0 = kb_weight(tab, dsub(x, ix - 2), fltb);
-    const float wy0 = kb_weight(tab, dsub(y, iy - 2), fltb);
-    const float wz0 = kb_weight(tab, dsub(z, iz - 2), fltb);
-    const int sx = wx0 != 0.0f ? ix - 2 : ix - 1;
-    const int sy = wy0 != 0.0f ? iy - 2 : iy - 1;
-    const int sz = wz0 != 0.0f ? iz - 2 : iz - 1;
+    // The nonzero taps of each axis are a run of at most 4 of the 5
+    // reference taps i-2..i+2 (the table vanishes beyond 1.8 voxels, and
+    // i-2 and i+2 are 4 apart): start at i-2 when its weight is nonzero.
+    const int sx = kb_weight(tab, dsub(x, ix - 2), fltb) != 0.0f ? ix - 2 : ix - 1;
+    const int sy = kb_weight(tab, dsub(y, iy - 2), fltb) != 0.0f ? iy - 2 : iy - 1;
+    const int sz = kb_weight(tab, dsub(z, iz - 2), fltb) != 0.0f ? iz - 2 : iz - 1;
+    float tz[4];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        tx[t] = (t == 0 && sx == ix - 2) ? wx0 : kb_weight(tab, dsub(x, sx + t), fltb);
-        ty[t] = (t == 0 && sy == iy - 2) ? wy0 : kb_weight(tab, dsub(y, sy + t), fltb);
-        tz[t] = (t == 0 && sz == iz - 2) ? wz0 : kb_weight(tab, dsub(z, sz + t), fltb);
-    }
+    for (int t = 0; t < 4; ++t) tz[t] = kb_weight(tab, dsub(z, sz + t), fltb);
     const double vxs = __dmul_rn((double)v.x, vscale), vys = __dmul_rn((double)v.y, vscale);
     if constexpr (MIRPASS) {
         // target -p, region-local: (-p) - (core origin - 2)
         const double vim = -vys;
+        float tx[2], ty[4];
+        int mx8[2], my8[4], mz8[4];
+        bool mxok[2], myok[4], mzok[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            ty[t] = kb_weight(tab, dsub(y, sy + t), fltb);
+            const int tyl = -(sy + t) - y0, tzl = -(sz + t) - z0;
+            myok[t] = (unsigned)tyl < (unsigned)CY;
+            mzok[t] = (unsigned)tzl < (unsigned)CZ;
+            my8[t] = Lds::ypart(tyl + 2) * 8;
+            mz8[t] = Lds::zpart(tzl + 2) * 8;
+        }
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {   // p_x = sx + t < 0 only for the first two taps
+            tx[t] = kb_weight(tab, dsub(x, sx + t), fltb);
+            const int px = sx + t;
+            mxok[t] = px < 0 && -px - x0 < CX;
+            mx8[t] = Lds::xpart(-px - x0 + 2) * 8;
+        }
 #pragma unroll
         for (int lz = 0; lz < 4; ++lz) {
-            const int tzl = -(sz + lz) - z0;
-            const bool zok = (unsigned)tzl < (unsigned)CZ;
 #pragma unroll
             for (int ly = 0; ly < 4; ++ly) {
-                const int tyl = -(sy + ly) - y0;
-                const bool ok = zok && (unsigned)tyl < (unsigned)CY;
-                const int offm = L.rowbase8(tyl + 2, tzl + 2), rm = L.rot8(tyl + 2, tzl + 2);
+                const bool ok = mzok[lz] && myok[ly];
+                const int row = my8[ly] + mz8[lz];
                 const float tyz = __fmul_rn(ty[ly], tz[lz]);
 #pragma unroll
-                for (int lx = 0; lx < 2; ++lx) {   // p_x = sx + lx < 0 only for the first two taps
-                    const int px = sx + lx;
+                for (int lx = 0; lx < 2; ++lx) {
                     const float wg = __fmul_rn(tx[lx], tyz);
-                    if (ok && px < 0 && -px - x0 < CX && wg != 0.0f) {
-                        const int ob = L.at8(offm, rm, (-px - x0 + 2) * 8);
+                    if (ok && mxok[lx] && wg != 0.0f) {
+                        const int ob = row + mx8[lx];
                         const double wd = (double)wg;
                         lds_add(L.re, ob, fixed_term(vxs, wd));
                         lds_add(L.im, ob, fixed_term(vim, wd));
@@ -676,27 +727,38 @@ __device__ __forceinline__ void own_taps(BrickLds<BX, BY, BZ, NT, WGT>& L, const
             }
         }
     } else {
-        // every direct tap is inside the region: p - (core origin - 2) in [0, B)
-        int xo8[4];
+        // every direct tap is inside the region: p - (core origin - 2) in [0, B).
+        // This lane visits its x and y taps in the cyclic order that puts
+        // (x mod 4, y mod 4) = ((lane + kx) mod 4, (lane / 4 + ky) mod 4) at
+        // tap slot (kx, ky): the 16 lanes of a group then hold 16 distinct
+        // banks at every slot.
+        const int lane = threadIdx.x & 63;
+        const int xl = sx - x0 + 2, yl = sy - y0 + 2, zl = sz - z0 + 2;
+        const int rx = (lane - xl) & 3, ry = ((lane >> 2) - yl) & 3;
+        float tx[4], ty[4];
+        int x8[4], y8[4], z8[4];
         bool xok[4];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            xo8[t] = (sx + t - x0 + 2) * 8;
-            xok[t] = !XCHK || sx + t >= 0;
+        for (int k = 0; k < 4; ++k) {
+            const int dx = (k + rx) & 3, dy = (k + ry) & 3;
+            tx[k] = kb_weight(tab, dsub(x, sx + dx), fltb);
+            ty[k] = kb_weight(tab, dsub(y, sy + dy), fltb);
+            x8[k] = Lds::xpart(xl + dx) * 8;
+            y8[k] = Lds::ypart(yl + dy) * 8;
+            z8[k] = Lds::zpart(zl + k) * 8;
+            xok[k] = !XCHK || sx + dx >= 0;
         }
 #pragma unroll
         for (int lz = 0; lz < 4; ++lz) {
-            const int zl = sz + lz - z0 + 2;
 #pragma unroll
             for (int ly = 0; ly < 4; ++ly) {
-                const int yl = sy + ly - y0 + 2;
-                const int offd = L.rowbase8(yl, zl), rd = L.rot8(yl, zl);
-                const float tyz = __fmul_rn(ty[ly], tz[lz]);
+                const int row = y8[ly] + z8[lz];
+                const float tyz = __fmul_rn(ty[ly], tz[lz]);           // TY = TABI(Y) * TZ
 #pragma unroll
                 for (int lx = 0; lx < 4; ++lx) {
-                    const float wg = __fmul_rn(tx[lx], tyz);
+                    const float wg = __fmul_rn(tx[lx], tyz);           // WG = TABI(X) * TY
                     if (xok[lx] && wg != 0.0f) {
-                        const int ob = L.at8(offd, rd, xo8[lx]);
+                        const int ob = row + x8[lx];
                         const double wd = (double)wg;
                         lds_add(L.re, ob, fixed_term(vxs, wd));
                         lds_add(L.im, ob, fixed_term(vys, wd));
@@ -756,7 +818,7 @@ __device__ __forceinline__ unsigned pack_cand(int I, int J, int s) {
 }
 
 template <int BX, int BY, int BZ, bool MIRPASS, bool XCHK, typename P, int NT, bool WGT>
-__device__ __forceinline__ void own_pass(BrickLds<BX, BY, BZ, NT, WGT>& L, const BrickJob& Jb, const int* __restrict__ lst,
+__device__ __forceinline__ void own_pass(TileLds<BX, BY, BZ, NT, WGT>& L, const BrickJob& Jb, const int* __restrict__ lst,
                                          int ns, int wave, int nwave, int x0, int y0, int z0, const Box box,
                                          const float* __restrict__ tab, P fltb, double vscale, double wscale) {
     constexpr int CX = BX - 4, CY = BY - 4, CZ = BZ - 4;
@@ -904,7 +966,9 @@ template <int BX, int BY, int BZ, typename P, int NT = TPB, bool OWN = false, bo
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(P) == 4 ? 4 : 3)))
 void insert_brick_kernel(const BrickJob* __restrict__ jobs, int njobs, int* __restrict__ next_item) {
     constexpr int NV = BX * BY * BZ;
-    __shared__ BrickLds<BX, BY, BZ, NT, WGT> L;
+    // owner-computes regions in the bank-conflict-free tile layout
+    using Lds = std::conditional_t<OWN, TileLds<BX, BY, BZ, NT, WGT>, BrickLds<BX, BY, BZ, NT, WGT>>;
+    __shared__ Lds L;
     __shared__ int jbase[MAX_JOBS + 1];
     if (threadIdx.x == 0) {
         int acc = 0;
@@ -987,15 +1051,17 @@ void insert_brick_kernel(const BrickJob* __restrict__ jobs, int njobs, int* __re
                 own_pass<BX, BY, BZ, false, false, P, NT, WGT>(L, J, lst, ns, wave, nwave, x0, y0, z0, oc, tab, fltb, vscale, wsc);
             }
         }
-        for (int s = wave; s < ns && !OWN; s += nwave) {
-            const int p = lst[s];
-            const Geom g = J.geom[p];
-            const float2* spec_p = J.spec + (size_t)(g.img - J.img0) * N * H;
-            if (x0 <= LN2) {
-                wave_project<BX, BY, BZ, true, P, NT, WGT>(L, g, spec_p, nx, N, x0, y0, z0, bx, false, bx, tab, fltb, vscale, wsc);
-                wave_project<BX, BY, BZ, true, P, NT, WGT>(L, g, spec_p, nx, N, x0, y0, z0, mb, true, bx, tab, fltb, vscale, wsc);
-            } else {
-                wave_project<BX, BY, BZ, false, P, NT, WGT>(L, g, spec_p, nx, N, x0, y0, z0, bx, false, bx, tab, fltb, vscale, wsc);
+        if constexpr (!OWN) {
+            for (int s = wave; s < ns; s += nwave) {
+                const int p = lst[s];
+                const Geom g = J.geom[p];
+                const float2* spec_p = J.spec + (size_t)(g.img - J.img0) * N * H;
+                if (x0 <= LN2) {
+                    wave_project<BX, BY, BZ, true, P, NT, WGT>(L, g, spec_p, nx, N, x0, y0, z0, bx, false, bx, tab, fltb, vscale, wsc);
+                    wave_project<BX, BY, BZ, true, P, NT, WGT>(L, g, spec_p, nx, N, x0, y0, z0, mb, true, bx, tab, fltb, vscale, wsc);
+                } else {
+                    wave_project<BX, BY, BZ, false, P, NT, WGT>(L, g, spec_p, nx, N, x0, y0, z0, bx, false, bx, tab, fltb, vscale, wsc);
+                }
             }
         }
         __syncthreads();
@@ -1004,8 +1070,8 @@ void insert_brick_kernel(const BrickJob* __restrict__ jobs, int njobs, int* __re
         [[maybe_unused]] float* __restrict__ W = half ? J.W1 : J.W0;
         [[maybe_unused]] float* __restrict__ Wc = half ? J.Wc1 : J.Wc0;   // null unless compensated weights
         for (int v = threadIdx.x; v < NV; v += blockDim.x) {
-            const int lx = v % BX, ly = (v / BX) % BY, lz = v / (BX * BY);
-            const int i = L.at(L.rowbase(ly, lz), L.rot(ly, lz), lx);
+            int lx, ly, lz;
+            const int i = L.coords(v, lx, ly, lz);
             const unsigned long long sre = L.re[i];
             if (sre == 0ull) continue;   // no tap reached the voxel (n = 0)
             const int rg = OWN ? 2 : 0;   // owner-computes: the region starts 2 voxels before the core
