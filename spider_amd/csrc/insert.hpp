@@ -832,6 +832,7 @@ __device__ __forceinline__ void own_pass(TileLds<BX, BY, BZ, NT, WGT>& L, const 
     int b0A = 0, b0B = 0;
     int pref = 0, cnt = 0, alo = 0, total = 0, q0 = 0;
     int s = 2 * wave - 2 * nwave;   // current pair
+    int sA = -1;                    // the pair whose geometry gA, gB hold (A = sA, B = sA + 1)
     unsigned pend = 0;   // this lane's pending candidate (valid for lane < npend)
     int npend = 0;       // wave-uniform
     for (;;) {
@@ -844,6 +845,7 @@ __device__ __forceinline__ void own_pass(TileLds<BX, BY, BZ, NT, WGT>& L, const 
                 const bool hasB = s + 1 < ns;
                 gA = Jb.geom[lst[s]];
                 gB = Jb.geom[lst[hasB ? s + 1 : s]];
+                sA = s;
                 int nlA, nlB;
                 box_lines(gA, box, b0A, nlA);
                 box_lines(gB, box, b0B, nlB);
@@ -945,7 +947,26 @@ __device__ __forceinline__ void own_pass(TileLds<BX, BY, BZ, NT, WGT>& L, const 
         }
         if (full || lane < npend) {
             const int I = (int)(cur & 0x7FFu), J = (int)((cur >> 11) & 0xFFFu) - 1024;
-            const Geom& gc = Jb.geom[lst[cur >> 23]];
+            // the section's geometry: the current pair's from registers; a
+            // candidate carried over from an earlier pair reloads its own
+            const int sc = (int)(cur >> 23);
+            const bool inB = sc == sA + 1;
+            Geom gc;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                gc.r[k] = inB ? gB.r[k] : gA.r[k];
+                gc.d[k] = inB ? gB.d[k] : gA.d[k];
+            }
+            gc.img = inB ? gB.img : gA.img;
+            if (sc != sA && !inB) {
+                const Geom& gl = Jb.geom[lst[sc]];
+#pragma unroll
+                for (int k = 0; k < 6; ++k) {
+                    gc.r[k] = gl.r[k];
+                    gc.d[k] = gl.d[k];
+                }
+                gc.img = gl.img;
+            }
             const float2* spec_p = Jb.spec + (size_t)(gc.img - Jb.img0) * N * H;
             own_taps<BX, BY, BZ, MIRPASS, XCHK, P, NT, WGT>(L, gc, spec_p, I, J, nx, N, x0, y0, z0, tab, fltb, vscale,
                                                             wscale);
