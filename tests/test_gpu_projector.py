@@ -46,11 +46,12 @@ def _rand_c(rng, shape):
     return (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
 
 
-@pytest.mark.parametrize("nx", [16, 25])
+@pytest.mark.parametrize("nx", [16, 25, 128])
 def test_gather_matches_oracle(mods, nx):
+    """nx = 128: the configuration-4 box (256^3 grid)."""
     bp, _ = mods
     rng = np.random.default_rng(100 + nx)
-    dm = _lib.rotations(_angles(rng, 36))
+    dm = _lib.rotations(_angles(rng, 36 if nx < 128 else 12))
     N = 2 * nx
     X = _rand_c(rng, (N, N, nx + 1))
     with bp.Backprojector(nx) as ctx:
