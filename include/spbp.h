@@ -103,6 +103,8 @@ int spbp_reset(spbp_ctx* ctx, void* stream);
  *   out_bi[i][J mod N][I] = c( sum_direct w X[t] + conj(sum_reflected w X[-t]) )
  * over the taps t, weights w and reflections ONELINE would use for that
  * sample (c = conj for a Friedel-flipped sample); samples ONELINE skips are 0.
+ * On an SPBP_GEOM_F64 context the sample positions are FP64, as that
+ * context's insertion computes them, so the transpose stays exact.
  * X: caller-owned complex64 half grid, (NX+1) N N (device);
  * out_bi: nimg x N x (NX+1) complex64 (device), ONELINE's BI layout (SPIDER
  * sign and checkerboard applied).  Rotations from angles_deg (host nimg x 3,

@@ -24,8 +24,11 @@ typedef float float4a8 __attribute__((ext_vector_type(4), aligned(8)));
 // its samples form a patch of the section and share most of their taps in
 // the caches; blockIdx.x = tile (I tiles fastest), blockIdx.y = projection.
 constexpr int GT = 16;
+// P: sample-position precision, float (the reference's geometry) or double
+// (contexts with FP64 geometry), as in the insertion kernels it transposes.
+template <typename P>
 __global__ __launch_bounds__(GT * GT) void gather_kernel(const float2* __restrict__ X, const Geom* __restrict__ geom,
-                                                         int nx, int N, const float* __restrict__ tab, float fltb,
+                                                         int nx, int N, const float* __restrict__ tab, float fltb_f,
                                                          float2* __restrict__ out) {
     const int H = nx + 1;
     const int p = blockIdx.y;
@@ -38,9 +41,10 @@ __global__ __launch_bounds__(GT * GT) void gather_kernel(const float2* __restric
     float2* o = out + ((size_t)p * N + jp) * H + I;
     if (!(I * I + J * J < (N * N) / 4) || (I == 0 && J < 0)) { *o = make_float2(0.f, 0.f); return; }
     const Geom g = geom[p];
-    float x, y, z;
+    const P fltb = fltb_of<P>(fltb_f);
+    P x, y, z;
     sample_pos(g, I, J, x, y, z);
-    const bool flip = x < 0.0f;
+    const bool flip = x < (P)0;
     if (flip) { x = -x; y = -y; z = -z; }
     const int ix = nearest_voxel(x, N), iy = nearest_voxel(y, N), iz = nearest_voxel(z, N);
     if (!in_bounds(ix, iy, iz, nx)) { *o = make_float2(0.f, 0.f); return; }
@@ -51,17 +55,17 @@ __global__ __launch_bounds__(GT * GT) void gather_kernel(const float2* __restric
         // 1.8 voxels): from i - 2 when its weight is nonzero, else from i - 1.
         // Same nonzero terms in the same order as the general loop below.
         float tx[4], ty[4], tz[4];
-        const float wx0 = kb_weight(tab, __fsub_rn(x, (float)(ix - 2)), fltb);
-        const float wy0 = kb_weight(tab, __fsub_rn(y, (float)(iy - 2)), fltb);
-        const float wz0 = kb_weight(tab, __fsub_rn(z, (float)(iz - 2)), fltb);
+        const float wx0 = kb_weight(tab, dsub(x, ix - 2), fltb);
+        const float wy0 = kb_weight(tab, dsub(y, iy - 2), fltb);
+        const float wz0 = kb_weight(tab, dsub(z, iz - 2), fltb);
         const int sx = wx0 != 0.0f ? ix - 2 : ix - 1;
         const int sy = wy0 != 0.0f ? iy - 2 : iy - 1;
         const int sz = wz0 != 0.0f ? iz - 2 : iz - 1;
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            tx[t] = (t == 0 && sx == ix - 2) ? wx0 : kb_weight(tab, __fsub_rn(x, (float)(sx + t)), fltb);
-            ty[t] = (t == 0 && sy == iy - 2) ? wy0 : kb_weight(tab, __fsub_rn(y, (float)(sy + t)), fltb);
-            tz[t] = (t == 0 && sz == iz - 2) ? wz0 : kb_weight(tab, __fsub_rn(z, (float)(sz + t)), fltb);
+            tx[t] = (t == 0 && sx == ix - 2) ? wx0 : kb_weight(tab, dsub(x, sx + t), fltb);
+            ty[t] = (t == 0 && sy == iy - 2) ? wy0 : kb_weight(tab, dsub(y, sy + t), fltb);
+            tz[t] = (t == 0 && sz == iz - 2) ? wz0 : kb_weight(tab, dsub(z, sz + t), fltb);
         }
         // each tap row is read whole (columns sx .. sx + 3 <= ix + 2 <= nx - 1,
         // inside the row) as two 16-byte loads; zero-weight taps add exact
@@ -94,9 +98,9 @@ __global__ __launch_bounds__(GT * GT) void gather_kernel(const float2* __restric
     float tx[5], ty[5], tz[5];
 #pragma unroll
     for (int l = 0; l < 5; ++l) {
-        tx[l] = kb_weight(tab, __fsub_rn(x, (float)(ix + l - LN2)), fltb);
-        ty[l] = kb_weight(tab, __fsub_rn(y, (float)(iy + l - LN2)), fltb);
-        tz[l] = kb_weight(tab, __fsub_rn(z, (float)(iz + l - LN2)), fltb);
+        tx[l] = kb_weight(tab, dsub(x, ix + l - LN2), fltb);
+        ty[l] = kb_weight(tab, dsub(y, iy + l - LN2), fltb);
+        tz[l] = kb_weight(tab, dsub(z, iz + l - LN2), fltb);
     }
     double dre = 0.0, dim = 0.0, mre = 0.0, mim = 0.0;
     for (int lz = 0; lz < 5; ++lz) {

@@ -557,9 +557,11 @@ BrickJob brick_job(const spbp_ctx* c, int img0, const Geom* geom_d, int nsec, co
 }
 
 unsigned brick_grid(const spbp_ctx* c) {
-    // workgroups resident per CU: LDS-bound (52.7 KB) for the largest brick,
-    // VGPR-bound (4 waves of 256-thread groups per SIMD) for the others
-    // (the FP64-geometry instantiations need ~165 VGPRs: 3 per CU for every shape)
+    // workgroups resident per CU: the 512-thread shapes (16x16x10, 16x16x12,
+    // 16x16x13: 61.7, 73.8 and 80.2 KB of LDS with weights) run two per CU,
+    // 16x16x8 (49.2 KB) three, the other 256-thread shapes four (VGPR-bound:
+    // 4 waves per SIMD); the FP64-geometry instantiations need ~150-170
+    // VGPRs: one 512-thread group or three 256-thread groups per CU
     if (c->shape >= 4) return (unsigned)c->ncu * ((c->flags & SPBP_GEOM_F64) ? 1u : 2u);
     return (unsigned)c->ncu * ((c->shape == 0 || (c->flags & SPBP_GEOM_F64)) ? 3u : 4u);   // 16x12x8: 36.9 KB, 4 fit
 }
@@ -1261,8 +1263,12 @@ int spbp_gather(spbp_ctx* c, const float* X, int nimg, const double* angles_deg,
     for (int p0 = 0; p0 < nimg; p0 += 65535) {
         const int b = std::min(65535, nimg - p0);
         Timed tm(c, SPBP_K_GATHER, b, st);
-        gather_kernel<<<dim3(gx, b), GT * GT, 0, st>>>((const float2*)X, c->geom_d + p0, c->nx, N, c->tab, c->fltb,
-                                                   (float2*)out_bi + (size_t)p0 * N * H);
+        if (c->flags & SPBP_GEOM_F64)
+            gather_kernel<double><<<dim3(gx, b), GT * GT, 0, st>>>((const float2*)X, c->geom_d + p0, c->nx, N, c->tab,
+                                                                   c->fltb, (float2*)out_bi + (size_t)p0 * N * H);
+        else
+            gather_kernel<float><<<dim3(gx, b), GT * GT, 0, st>>>((const float2*)X, c->geom_d + p0, c->nx, N, c->tab,
+                                                                  c->fltb, (float2*)out_bi + (size_t)p0 * N * H);
     }
     HIPCHK(c, hipGetLastError());
     return SPBP_OK;
@@ -1333,8 +1339,12 @@ int spbp_gather_images(spbp_ctx* c, const float* X, int nimg, const double* angl
             Timed tm(c, SPBP_K_GATHER, b, st);
             for (int q0 = 0; q0 < b; q0 += 65535) {
                 const int bq = std::min(65535, b - q0);
-                gather_kernel<<<dim3(gx, bq), GT * GT, 0, st>>>((const float2*)X, c->geom_d + p0 + q0, c->nx, N, c->tab,
-                                                                 c->fltb, c->spec + (size_t)q0 * N * H);
+                if (c->flags & SPBP_GEOM_F64)
+                    gather_kernel<double><<<dim3(gx, bq), GT * GT, 0, st>>>((const float2*)X, c->geom_d + p0 + q0, c->nx,
+                                                                            N, c->tab, c->fltb, c->spec + (size_t)q0 * N * H);
+                else
+                    gather_kernel<float><<<dim3(gx, bq), GT * GT, 0, st>>>((const float2*)X, c->geom_d + p0 + q0, c->nx,
+                                                                           N, c->tab, c->fltb, c->spec + (size_t)q0 * N * H);
             }
         }
         const size_t n = (size_t)b * N * H;

@@ -252,5 +252,13 @@ def read_partial(path: str):
 
 
 def _weights_name(path: str) -> str:
+    """Name of a partial volume's weights file: `_W` before the '@' of a
+    stack name (`stk@3` -> `stk_W@3`), else at the end of the name
+    (bp3f.f:204-210, 576-583); a data extension stays at the end of the file
+    name (`stk.dat@3` -> `stk_W.dat@3`, `vol.dat` -> `vol_W.dat`)."""
+    if "@" in path:
+        base, rest = path.split("@", 1)
+        root, ext = os.path.splitext(base)
+        return root + "_W" + ext + "@" + rest
     root, ext = os.path.splitext(path)
     return root + "_W" + ext

@@ -81,18 +81,23 @@ def test_scatter_matches_oracle(mods, strategy):
     assert rel(X2g, Xr) <= 1e-6
 
 
+@pytest.mark.parametrize("geom_f64", [False, True])
 @pytest.mark.parametrize("strategy", ["direct", "brick"])
-def test_adjoint_identity(mods, strategy):
+def test_adjoint_identity(mods, strategy, geom_f64):
+    """<G x, b> = <x, S b>; with FP64 geometry (FP64 rotations from the doc
+    angles, FP64 sample positions) the gather transposes that context's
+    insertion too."""
     bp, _ = mods
     nx = 32
     rng = np.random.default_rng(9)
-    dm = _lib.rotations(_angles(rng, 100))
+    ang = _angles(rng, 100)
+    rot = {"angles": ang} if geom_f64 else {"dm": _lib.rotations(ang)}
     N = 2 * nx
     X = _rand_c(rng, (N, N, nx + 1))
-    B = _rand_c(rng, (dm.shape[0], N, nx + 1))
-    with bp.Backprojector(nx, strategy=strategy) as ctx:
-        GX = ctx.gather(torch.from_numpy(X).cuda(), dm=dm).cpu().numpy().astype(np.complex128)
-        SB = ctx.scatter(torch.from_numpy(B).cuda(), dm=dm).cpu().numpy().astype(np.complex128)
+    B = _rand_c(rng, (ang.shape[0], N, nx + 1))
+    with bp.Backprojector(nx, strategy=strategy, geom_f64=geom_f64) as ctx:
+        GX = ctx.gather(torch.from_numpy(X).cuda(), **rot).cpu().numpy().astype(np.complex128)
+        SB = ctx.scatter(torch.from_numpy(B).cuda(), **rot).cpu().numpy().astype(np.complex128)
     B64, X64 = B.astype(np.complex128), X.astype(np.complex128)
     lhs = np.sum(GX.real * B64.real + GX.imag * B64.imag)
     rhs = np.sum(X64.real * SB.real + X64.imag * SB.imag)

@@ -80,3 +80,12 @@ def test_partial_volume_files(tmp_path):
     X2, W2 = spiderio.read_partial(p)
     np.testing.assert_array_equal(X2, X)
     np.testing.assert_array_equal(W2, W)
+
+
+def test_weights_file_names():
+    """_W goes before the '@' of a stack name, else at the end of the name
+    (bp3f.f:204-210, 576-583)."""
+    assert spiderio._weights_name("part.dat") == "part_W.dat"
+    assert spiderio._weights_name("dir.x/part") == "dir.x/part_W"
+    assert spiderio._weights_name("stk@3") == "stk_W@3"
+    assert spiderio._weights_name("stk.dat@12") == "stk_W.dat@12"
