@@ -120,6 +120,20 @@ int spbp_gather(spbp_ctx* ctx, const float* X, int nimg, const double* angles_de
  * bp32f.f:802-913) minus PADD2 and the forward FFT. */
 int spbp_scatter(spbp_ctx* ctx, const float* bi, int nimg, const double* angles_deg, const float* dm,
                  float* X, float* W, void* stream);
+/* spbp_scatter without a context and without allocating: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  No counterpart in the reference.
+ * nx and flags (SPBP_EXACT_PI, SPBP_GEOM_F64; others are refused) stand for
+ * the context; bi, angles_deg / dm, X and W are as for spbp_scatter, W may be
+ * NULL.  Scratch (KB table, geometry, raw spectra, brick lists, queue head,
+ * job table) lives in a file-scope device array of 64 MiB that every GPU the
+ * library runs on holds until the process ends; boxes whose per-chunk
+ * scratch does not fit (NX of 733 and above) are refused with SPBP_ERR_SIZE.  The
+ * projections go through in chunks of up to 256, and the call synchronises
+ * `stream` after each chunk (its host staging is on the stack).  Every call
+ * on one GPU shares that GPU's array: calls that overlap on one GPU race. */
+int spbp_scatter_noalloc(int nx, int flags, const float* bi, int nimg, const double* angles_deg, const float* dm,
+                         float* X, float* W, void* stream);
 /* The projector applied to a grid: for nimg rotations, the gather of X
  * (spbp_gather's BI spectra), completed to the full C2R input (conjugated,
  * checkerboard (-1)^(I + jp), column 0 by Hermitian symmetry), transformed

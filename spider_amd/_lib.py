@@ -122,6 +122,7 @@ def lib():
         L.spbp_scatter_images.argtypes = [vp, vp, C.c_int, vp, vp, vp, vp]
         L.spbp_gather_images.argtypes = [vp, vp, C.c_int, vp, vp, vp, vp]
         L.spbp_debug_fail_alloc.argtypes = [C.c_long]
+        L.spbp_scatter_noalloc.argtypes = [C.c_int, C.c_int, vp, C.c_int, vp, vp, vp, vp, vp]
         for name in ("spbp_create", "spbp_destroy", "spbp_reset", "spbp_insert", "spbp_insert_dm",
                      "spbp_insert_host", "spbp_finalize", "spbp_finalize_host", "spbp_export_partial",
                      "spbp_import_partial", "spbp_export_raw", "spbp_kernel_stats", "spbp_reset_stats",
@@ -129,7 +130,7 @@ def lib():
                      "spbp_get_plan", "spbp_set_plan", "spbp_insert_batch", "spbp_graph_create",
                      "spbp_graph_launch", "spbp_graph_destroy", "spbp_set_symmetries", "spbp_insert_rb",
                      "spbp_rtsq", "spbp_insert_rb_host", "spbp_threshold_stats", "spbp_reference_weights",
-                     "spbp_calibrate", "spbp_scatter_images", "spbp_gather_images", "spbp_debug_fail_alloc"):
+                     "spbp_calibrate", "spbp_scatter_images", "spbp_gather_images", "spbp_debug_fail_alloc", "spbp_scatter_noalloc"):
             getattr(L, name).restype = C.c_int
         _LIB = L
     return _LIB

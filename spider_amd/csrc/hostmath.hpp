@@ -88,6 +88,41 @@ void kb_table_f64_f32(int n2, std::vector<float>& tab, float& fltb_f) {
     fltb_f = (float)LTAB / (float)(LN2 + 1);
 }
 
+// The same tables into a caller's buffer of LTAB + 1 floats (no allocation):
+// the REAL evaluation, or with f64 the FP64 one rounded to fp32.
+void kb_table_into(int n2, bool f64, float* tab, float& fltb_f) {
+    if (!f64) {
+        const float alpha = 6.5f, V = (float)(5 - 1) / 2.0f / (float)n2, aaaa = 0.9f * V;
+        const float b0 = std::sqrt(alpha) * bessel_i1_real(alpha);
+        const float fltb = (float)LTAB / (float)(LN2 + 1);
+        for (int i = 0; i <= LTAB; ++i) {
+            const float s = (float)i / fltb / (float)n2;
+            float t = 0.0f;
+            if (s <= aaaa) {
+                const float q = s / aaaa;
+                const float x = std::sqrt(1.0f - q * q);
+                t = std::sqrt(alpha * x) * bessel_i1_real(alpha * x) / b0;
+            }
+            tab[i] = t;
+        }
+        fltb_f = fltb;
+        return;
+    }
+    const double alpha = 6.5, V = 4.0 / 2.0 / (double)n2, aaaa = 0.9 * V;
+    const double b0 = std::sqrt(alpha) * bessel_i1(alpha);
+    const double fltb = (double)LTAB / (double)(LN2 + 1);
+    for (int i = 0; i <= LTAB; ++i) {
+        const double s = (double)i / fltb / (double)n2;
+        double t = 0.0;
+        if (s <= aaaa) {
+            const double q = s / aaaa, x = std::sqrt(1.0 - q * q);
+            t = std::sqrt(alpha * x) * bessel_i1(alpha * x) / b0;
+        }
+        tab[i] = (float)t;
+    }
+    fltb_f = (float)LTAB / (float)(LN2 + 1);
+}
+
 // |WKB| of WINDKB2A per integer squared radius LR = 0..(L/2)^2, FP64.
 void wkb_table(int nx, int N, std::vector<float>& t) {
     const int cc = nx / 2;

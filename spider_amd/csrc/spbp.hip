@@ -31,6 +31,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -51,7 +52,9 @@
 using namespace spbp;
 
 namespace {
-thread_local std::string g_last_error;
+// last failure messages: fixed buffers, so that no error path allocates
+constexpr int ERR_LEN = 512;
+thread_local char g_last_error[ERR_LEN] = {0};
 // brick shapes of the LDS-binned strategy (voxels; 3 x 8 B per voxel in LDS)
 // {x, y, z, owner-computes}; for owner-computes bricks the LDS region, whose
 // core (each sample's owner) is 4 voxels shorter per axis
@@ -174,19 +177,19 @@ struct spbp_ctx {
     struct Rec { hipEvent_t a, b; long units; };
     std::vector<Rec> recs[7];
     std::vector<hipEvent_t> evpool;
-    std::string err;
+    char err[ERR_LEN] = {0};
 };
 
 namespace {
 
 int fail(spbp_ctx* ctx, int code, const char* fmt, ...) {
-    char buf[512];
+    char buf[ERR_LEN];
     va_list ap;
     va_start(ap, fmt);
     vsnprintf(buf, sizeof buf, fmt, ap);
     va_end(ap);
-    g_last_error = buf;
-    if (ctx) ctx->err = buf;
+    std::memcpy(g_last_error, buf, sizeof buf);
+    if (ctx) std::memcpy(ctx->err, buf, sizeof buf);
     return code;
 }
 
@@ -272,8 +275,10 @@ int refuse_captured(spbp_ctx* c, const char* what) {
 // Brick grid of shape `shape` and its per-key arrays.  On failure the
 // context holds no brick grid (c->bg.nb = 0) and the next call allocates
 // again; the shape and grid are committed only once every array exists.
-int configure_bricks(spbp_ctx* c, int shape) {
-    if ((c->flags & SPBP_GEOM_F64) && (shape == 5 || shape == 6)) shape = 4;
+// Brick grid of shape `shape` over the half grid of edge nx (FP64-geometry
+// contexts have no shared-evaluation 16x16x12 / 16x16x13 kernels: 16x16x10).
+BrickGrid grid_of_shape(int nx, int flags, int& shape) {
+    if ((flags & SPBP_GEOM_F64) && (shape == 5 || shape == 6)) shape = 4;
     const int own = SHAPES[shape][3], halo = own ? 4 : 0;
     const int bx = SHAPES[shape][0] - halo, by = SHAPES[shape][1] - halo, bz = SHAPES[shape][2] - halo;
     BrickGrid g;
@@ -281,10 +286,15 @@ int configure_bricks(spbp_ctx* c, int shape) {
     g.by = by;
     g.bz = bz;
     g.own = own;
-    g.nbx = (c->nx + bx - 1) / bx;
-    g.nby = (2 * c->nx - 1 + by - 1) / by;
-    g.nbz = (2 * c->nx - 1 + bz - 1) / bz;
+    g.nbx = (nx + bx - 1) / bx;
+    g.nby = (2 * nx - 1 + by - 1) / by;
+    g.nbz = (2 * nx - 1 + bz - 1) / bz;
     g.nb = g.nbx * g.nby * g.nbz;
+    return g;
+}
+
+int configure_bricks(spbp_ctx* c, int shape) {
+    const BrickGrid g = grid_of_shape(c->nx, c->flags, shape);
     if (c->bg.nb > 0 && g.nb == c->bg.nb && shape == c->shape) return SPBP_OK;
     if (c->graphs > 0) return refuse_captured(c, "changing the brick shape");
     c->bg.nb = 0;   // no valid grid until every array below exists
@@ -1093,7 +1103,7 @@ extern "C" {
 const char* spbp_version(void) { return "spbp 0.1 (gfx950)"; }
 
 const char* spbp_last_error(const spbp_ctx* ctx) {
-    return ctx ? ctx->err.c_str() : g_last_error.c_str();
+    return ctx ? ctx->err : g_last_error;
 }
 
 size_t spbp_grid_elems(const spbp_ctx* ctx) { return ctx ? ctx->nh : 0; }
@@ -1177,7 +1187,7 @@ int spbp_create(const spbp_opts* o, spbp_ctx** out) {
     // default plan: owner-computes bricks, 16x16x13 LDS regions around 12x12x9 cores (the largest region
     // that runs two 512-thread groups per CU), 128 projections per item: fastest measured at 128^2 into
     // 256^3 and 256^2 into 512^3; spbp_tune picks per workload
-    if ((rc = configure_bricks(c, NSHAPES - 1))) { g_last_error = c->err; return bail(rc); }
+    if ((rc = configure_bricks(c, NSHAPES - 1))) { std::memcpy(g_last_error, c->err, ERR_LEN); return bail(rc); }
     rc = spbp_reset(c, nullptr);
     if (rc) return bail(rc);
     if (hipDeviceSynchronize() != hipSuccess) return bail(fail(nullptr, SPBP_ERR_DEVICE, "zeroing accumulators failed"));
@@ -2074,6 +2084,150 @@ int spbp_reference_weights(spbp_ctx* c, int which, const long long* voxels, int 
         rc = fail(c, SPBP_ERR_DEVICE, "reference weights read-back failed");
     (void)hipFree(d);
     return rc;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ allocation-free adjoint scatter
+// spbp_scatter_noalloc keeps its scratch -- KB table, geometry, raw spectra,
+// their maxima, brick lists, queue head and job table -- in one file-scope
+// device array.  Each GPU the library runs on holds its own copy until the
+// process ends.  Host code never uses the array's name as an address: the
+// first call on a device (that device current) asks hipGetSymbolAddress for
+// the copy's device address, kept per device, and every kernel argument,
+// fill and copy uses that address.
+namespace {
+constexpr size_t SCRATCH_BYTES = (size_t)64 << 20;   // the scratch array: 64 MiB per GPU
+constexpr int SCRATCH_DEVICES = 64;                  // devices served
+constexpr int SCRATCH_CHUNK = 256;                   // projections per internal chunk at most
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+
+namespace {
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[SCRATCH_DEVICES] = {};
+
+int scratch_address(int dev, unsigned char** out) {
+    if (dev < 0 || dev >= SCRATCH_DEVICES) return fail(nullptr, SPBP_ERR_DEVICE, "device %d beyond the %d served", dev, SCRATCH_DEVICES);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(nullptr, SPBP_ERR_DEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        g_scratch_ptr[dev] = (unsigned char*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return SPBP_OK;
+}
+
+// Consecutive 256-byte-aligned pieces of the scratch array.
+struct Carve {
+    unsigned char* p;
+    size_t left;
+    template <typename T>
+    T* take(size_t n) {
+        const size_t b = (n * sizeof(T) + 255) & ~(size_t)255;
+        if (b > left) return nullptr;
+        T* r = (T*)p;
+        p += b;
+        left -= b;
+        return r;
+    }
+};
+}  // namespace
+
+extern "C" {
+
+int spbp_scatter_noalloc(int nx, int flags, const float* bi, int nimg, const double* angles_deg, const float* dm,
+                         float* X, float* W, void* stream) {
+    if (nx < 4 || nx > 1024) return fail(nullptr, SPBP_ERR_SIZE, "image size %d outside 4..1024", nx);
+    if (flags & ~(SPBP_EXACT_PI | SPBP_GEOM_F64)) return fail(nullptr, SPBP_ERR_ARG, "flags 0x%x: only SPBP_EXACT_PI and SPBP_GEOM_F64 apply", flags);
+    if (nimg < 0) return fail(nullptr, SPBP_ERR_ARG, "negative projection count %d", nimg);
+    if (nimg == 0) return SPBP_OK;
+    if (!bi || !X) return fail(nullptr, SPBP_ERR_ARG, "NULL spectrum or grid pointer");
+    if (!dm && !angles_deg) return fail(nullptr, SPBP_ERR_ANGLE, "MISSING ANGLE FOR IMAGE (no angles given)");
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return fail(nullptr, SPBP_ERR_DEVICE, "no HIP device");
+    unsigned char* base = nullptr;
+    int rc = scratch_address(dev, &base);
+    if (rc) return rc;
+    // a context on the stack whose buffers are pieces of the scratch array
+    spbp_ctx c;
+    c.nx = nx;
+    c.N = 2 * nx;
+    c.H = nx + 1;
+    c.flags = flags;
+    c.device = dev;
+    c.nh = (size_t)c.H * c.N * c.N;
+    c.strategy = SPBP_STRAT_BRICK;
+    {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0) c.ncu = n;
+    }
+    int shape = NSHAPES - 1;
+    c.bg = grid_of_shape(nx, flags, shape);
+    c.shape = shape;
+    const int N = c.N, H = c.H;
+    const size_t nk = 2 * (size_t)c.bg.nb;
+    Carve cv{base, SCRATCH_BYTES};
+    c.tab = cv.take<float>(LTAB + 1);
+    c.bcnt = cv.take<int>(nk);
+    c.boff = cv.take<int>(nk + 1);
+    c.bioff = cv.take<int>(nk + 1);
+    c.bcur = cv.take<int>(nk);
+    c.bscan = cv.take<int2>((nk + 1023) / 1024 + 1);
+    c.bnext = cv.take<int>(1);
+    BrickJob* job_d = cv.take<BrickJob>(1);
+    // per projection: geometry, raw spectrum, its maximum, list entries and work items
+    const size_t per = sizeof(Geom) + (size_t)N * H * sizeof(float2) + sizeof(float) + 2 * (size_t)c.bg.nb * sizeof(int);
+    const size_t fixed = nk * sizeof(int) + 6 * 256;   // the work-item keys' extra 2 nb, alignment
+    size_t B = 0;
+    if (c.tab && job_d && cv.left > fixed) B = std::min<size_t>((cv.left - fixed) / per, SCRATCH_CHUNK);
+    B = std::min<size_t>(B, (size_t)nimg);
+    if (B < 1)
+        return fail(nullptr, SPBP_ERR_SIZE, "NX = %d needs more than the %zu MiB scratch of spbp_scatter_noalloc",
+                    nx, SCRATCH_BYTES >> 20);
+    Geom* geom_d = cv.take<Geom>(B);
+    c.spec = cv.take<float2>(B * N * H);
+    c.vmax = cv.take<float>(B);
+    c.blist = cv.take<int>(B * c.bg.nb);
+    c.bikey = cv.take<int>(B * c.bg.nb + nk);
+    if (!geom_d || !c.spec || !c.vmax || !c.blist || !c.bikey)
+        return fail(nullptr, SPBP_ERR_SIZE, "scratch layout for NX = %d does not fit", nx);
+    c.blist_cap = B * c.bg.nb;
+    c.fwd_batch = (int)B;
+    // host staging on the stack
+    float tabh[LTAB + 1];
+    Geom gh[SCRATCH_CHUNK];
+    c.geom_h = gh;
+    const bool f64 = (flags & SPBP_GEOM_F64) != 0, exact = (flags & SPBP_EXACT_PI) != 0;
+    kb_table_into(N, f64, tabh, c.fltb);
+    hipStream_t st = (hipStream_t)stream;
+    HIPCHK(nullptr, hipMemcpyAsync(c.tab, tabh, sizeof tabh, hipMemcpyHostToDevice, st));
+    const Targets t{(float2*)X, W, nullptr, nullptr, nullptr, nullptr};
+    for (int p0 = 0; p0 < nimg; p0 += (int)B) {
+        const int b = (int)std::min<size_t>(B, (size_t)(nimg - p0));
+        for (int p = 0; p < b; ++p) {
+            const int k = geom_one(&c, p, 1, dm ? dm + 9 * (size_t)p0 : nullptr,
+                                   angles_deg ? angles_deg + 3 * (size_t)p0 : nullptr, nullptr, exact, f64);
+            if (k == 1) return fail(nullptr, SPBP_ERR_ANGLE, "MISSING ANGLE FOR IMAGE %d", p0 + p + 1);
+            if (k) return fail(nullptr, SPBP_ERR_ANGLE, "rotation of image %d is not finite and orthonormal", p0 + p + 1);
+        }
+        HIPCHK(nullptr, hipMemcpyAsync(geom_d, gh, (size_t)b * sizeof(Geom), hipMemcpyHostToDevice, st));
+        const size_t n = (size_t)b * N * H;
+        bi_to_raw_kernel<<<nblocks(n), TPB, 0, st>>>((const float2*)bi + (size_t)p0 * N * H, N, H, n, c.spec);
+        spec_absmax_kernel<<<b, TPB, 0, st>>>(c.spec, N * H, c.vmax);
+        HIPCHK(nullptr, hipGetLastError());
+        if ((rc = build_lists(&c, geom_d, b, st))) return fail(nullptr, rc, "%s", c.err);
+        const BrickJob job = brick_job(&c, 0, geom_d, b, t);
+        HIPCHK(nullptr, hipMemcpyAsync(job_d, &job, sizeof job, hipMemcpyHostToDevice, st));
+        if ((rc = launch_bricks_dev(&c, job_d, 1, c.bnext, st, W != nullptr))) return fail(nullptr, rc, "%s", c.err);
+        // the stack staging (geometry, job table, KB table) is read by the copies above
+        HIPCHK(nullptr, hipStreamSynchronize(st));
+    }
+    return SPBP_OK;
 }
 
 int spbp_threshold_stats(const spbp_ctx* c, long long* listed, long long* changed, long long* inconsistent) {
