@@ -337,9 +337,11 @@ def run_batched(args):
     form: one context after another, fp32 images, default plan, insert +
     finalise.  Batched form: every context tuned on its own images, BF16
     image storage, all insertions captured once in one HIP graph (one
-    persistent scatter launch per round of chunks) and replayed, then the
-    same finalisations.  Both forms are timed over the same work; the FSC
-    between their volumes and against the phantoms shows they agree."""
+    persistent scatter launch per round of chunks) and replayed, then one
+    batched finalisation of all 36 volumes (spbp_finalize_batch).  Both forms
+    are timed over the same work; the FSC between their volumes and against
+    the phantoms shows they agree.  The finalisations are also timed alone,
+    one context after another and batched."""
     import numpy as np
     import torch
     from spider_amd import bp as bpmod
@@ -400,7 +402,7 @@ def run_batched(args):
 
     def run_fast():
         g.launch()
-        return finalize_all(fast)
+        return bpmod.finalize_batch(fast, full=True, halves=True)
 
     run_fast()
     torch.cuda.synchronize()
@@ -410,6 +412,20 @@ def run_batched(args):
     torch.cuda.synchronize()
     t_fast = (time.perf_counter() - t0) / args.steps
     log(f"batched: batched form {t_fast:.3f} s per pass")
+
+    # the finalisations alone: one context after another vs one batched call
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / args.steps
+
+    t_fin_plain = timed(lambda: finalize_all(fast))
+    t_fin_batch = timed(lambda: bpmod.finalize_batch(fast, full=True, halves=True))
+    log(f"batched: finalisation {t_fin_plain:.4f} s one by one, {t_fin_batch:.4f} s batched")
 
     per = []
     for d, a, b in zip(vols, out_plain, out_fast):
@@ -432,6 +448,7 @@ def run_batched(args):
            "plain_proj_per_s": round(total / t_plain, 1), "speedup_vs_plain": round(t_plain / t_fast, 3),
            "plain_seconds": round(t_plain, 4), "batched_seconds": round(t_fast, 4),
            "tune_seconds": round(t_tune, 3), "capture_seconds": round(t_capture, 3),
+           "finalize_seconds_one_by_one": round(t_fin_plain, 4), "finalize_seconds_batched": round(t_fin_batch, 4),
            "plans": [{"strategy": p["strategy"], "brick": p["brick"], "brick_proj": p["brick_proj"]} for p in plans],
            "volumes": per}
     print(json.dumps(out), flush=True)

@@ -173,6 +173,26 @@ int spbp_insert_host(spbp_ctx* ctx, const void* imgs_host, int dtype, int nimg,
  * be NULL.  The accumulators are left unchanged (more inserts may follow). */
 int spbp_finalize(spbp_ctx* ctx, float* out_full, float* out_h1, float* out_h2, void* stream);
 int spbp_finalize_host(spbp_ctx* ctx, float* out_full, float* out_h1, float* out_h2, void* stream);
+/* One volume set to finalise: spbp_finalize's arguments for one context. */
+typedef struct {
+    spbp_ctx* ctx;
+    float* out_full;   /* device NX^3 of ctx's NX, or NULL */
+    float* out_h1;
+    float* out_h2;
+} spbp_final_job;
+/* Finalise the volumes of several contexts (distinct, one device, any box
+ * sizes, at most 64) as spbp_finalize would, in one pass: the NRMW2
+ * re-summation of every volume in shared phases (three host synchronisations
+ * per call, four when a band list outgrows its buffer, instead of four per
+ * volume), then per wave -- the first requested volume of every context,
+ * then the second, then the third -- one normalisation launch over all the
+ * wave's volumes, their inverse 3D transforms, and one window, shell-mean and
+ * subtraction launch over all of them.  The reference finalises one volume
+ * after another (bp32f.f:275-410).  spbp_finalize is this call with one job.
+ * With SPBP_TIMING each context records the call's re-summation and
+ * finalisation spans.  No counterpart in the reference (one operation, one
+ * volume). */
+int spbp_finalize_batch(const spbp_final_job* jobs, int njobs, void* stream);
 
 /* Partial volumes = `BP 3F P` / `BP 3F M` (bp3f.f:191-248, 531-739):
  * export writes the post-SYMPLANE0 half-grid (X interleaved complex64,

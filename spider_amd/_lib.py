@@ -61,6 +61,10 @@ class Job(C.Structure):
                 ("angles_deg", C.c_void_p), ("dm", C.c_void_p), ("half", C.c_void_p)]
 
 
+class FinalJob(C.Structure):
+    _fields_ = [("ctx", C.c_void_p), ("out_full", C.c_void_p), ("out_h1", C.c_void_p), ("out_h2", C.c_void_p)]
+
+
 SHAPES = [(16, 16, 8), (16, 8, 8), (8, 8, 8), (16, 12, 8), (16, 16, 10), (16, 16, 12), (16, 16, 13), (16, 16, 13)]
 # owner-computes bricks (SHAPES gives the LDS region; each sample is evaluated in one brick only)
 SHAPE_OWN = [False] * 7 + [True]
@@ -96,6 +100,7 @@ def lib():
         L.spbp_insert_host.argtypes = [vp, vp, C.c_int, C.c_int, vp, vp, vp]
         L.spbp_finalize.argtypes = [vp, vp, vp, vp, vp]
         L.spbp_finalize_host.argtypes = [vp, vp, vp, vp, vp]
+        L.spbp_finalize_batch.argtypes = [vp, C.c_int, vp]
         L.spbp_export_partial.argtypes = [vp, C.c_int, vp, vp, vp]
         L.spbp_import_partial.argtypes = [vp, C.c_int, vp, vp, C.c_int, vp]
         L.spbp_export_raw.argtypes = [vp, C.c_int, vp, vp, vp]
@@ -124,7 +129,7 @@ def lib():
         L.spbp_debug_fail_alloc.argtypes = [C.c_long]
         L.spbp_scatter_noalloc.argtypes = [C.c_int, C.c_int, vp, C.c_int, vp, vp, vp, vp, vp]
         for name in ("spbp_create", "spbp_destroy", "spbp_reset", "spbp_insert", "spbp_insert_dm",
-                     "spbp_insert_host", "spbp_finalize", "spbp_finalize_host", "spbp_export_partial",
+                     "spbp_insert_host", "spbp_finalize", "spbp_finalize_host", "spbp_finalize_batch", "spbp_export_partial",
                      "spbp_import_partial", "spbp_export_raw", "spbp_kernel_stats", "spbp_reset_stats",
                      "spbp_gather", "spbp_scatter", "spbp_kb_table", "spbp_rotations", "spbp_tune",
                      "spbp_get_plan", "spbp_set_plan", "spbp_insert_batch", "spbp_graph_create",

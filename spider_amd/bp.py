@@ -341,9 +341,7 @@ class Backprojector:
         return job, (imgs, ang, dmc, hp)
 
     # -------------------------------------------------------------- finalisation
-    def finalize(self, full=True, halves=False) -> dict:
-        """NX^3 volumes: 'full' (BP 3F volume / BP 32F overall volume) and, for
-        two-halves contexts, 'h1', 'h2' (first / second sample volume)."""
+    def _final_outputs(self, full, halves) -> dict:
         torch = _torch()
         nx = self.nx
         out = {}
@@ -354,6 +352,12 @@ class Backprojector:
                 raise SpbpError(_lib.ERR_ARG, "sample volumes need two_halves=True")
             out["h1"] = torch.empty((nx, nx, nx), dtype=torch.float32, device=self.device)
             out["h2"] = torch.empty((nx, nx, nx), dtype=torch.float32, device=self.device)
+        return out
+
+    def finalize(self, full=True, halves=False) -> dict:
+        """NX^3 volumes: 'full' (BP 3F volume / BP 32F overall volume) and, for
+        two-halves contexts, 'h1', 'h2' (first / second sample volume)."""
+        out = self._final_outputs(full, halves)
         ptr = lambda k: C.c_void_p(out[k].data_ptr()) if k in out else None
         self._check(_lib.lib().spbp_finalize(self._ctx, ptr("full"), ptr("h1"), ptr("h2"), _stream(self.device)))
         return out
@@ -568,6 +572,27 @@ def insert_batch(items, device=None) -> None:
     arr, keep = _jobs(items)
     _lib.check(_lib.lib().spbp_insert_batch(arr, len(arr), _stream(device)))
     return keep
+
+
+def finalize_batch(items, full=True, halves=False, device=None) -> list:
+    """Finalise several contexts (any box sizes) in one pass
+    (spbp_finalize_batch): the threshold re-summation of all volumes shares
+    its host synchronisations, and each wave of volumes -- every context's
+    first requested volume, then the second, then the third -- takes one
+    normalisation and one window launch.  items: Backprojectors (finalised
+    with `full` / `halves`) or (Backprojector, full, halves) tuples.
+    Returns one finalize()-style dict per context."""
+    ctxs, outs = [], []
+    for it in items:
+        c, f, h = it if isinstance(it, tuple) else (it, full, halves)
+        ctxs.append(c)
+        outs.append(c._final_outputs(f, h))
+    ptr = lambda o, k: o[k].data_ptr() if k in o else None
+    arr = (_lib.FinalJob * len(ctxs))(*[_lib.FinalJob(c._ctx.value, ptr(o, "full"), ptr(o, "h1"), ptr(o, "h2"))
+                                         for c, o in zip(ctxs, outs)])
+    dev = ctxs[0].device if device is None and ctxs else device
+    _lib.check(_lib.lib().spbp_finalize_batch(arr, len(arr), _stream(dev)))
+    return outs
 
 
 class BatchGraph:

@@ -10,7 +10,11 @@ struct Acc {
     const float* W[2];
     const float* Wc[2];   // compensation terms of the weights, or null
     int n;                // halves summed: 1 or 2
-    const unsigned char* mask = nullptr;   // NRMW2 decisions fixed by the reference's sums (1 zero, 2 divide), or null
+    // NRMW2 decisions fixed by the reference's sums, or null: bits mshift and
+    // mshift + 1 of each voxel's byte hold 1 (zero) or 2 (divide), 0 where the
+    // GPU sums decide; one byte serves the three volumes of a context
+    const unsigned char* mask = nullptr;
+    int mshift = 0;
 };
 
 // (j, k) of the kx = 0 plane is paired by SYMPLANE0 with (N-j, N-k) unless it
@@ -53,7 +57,7 @@ __device__ inline float2 norm_val(const Acc& a, int nx, int N, int i, int j, int
     const float s = ((i + j + k) & 1) ? -1.0f : 1.0f;
     // NRMW2 (bp3f.f:1359): divide where W > 0.1; near 0.1 the decision may
     // come from the reference's own sums
-    const unsigned char m = a.mask ? a.mask[((size_t)k * N + j) * (nx + 1) + i] : 0;
+    const unsigned char m = a.mask ? (a.mask[((size_t)k * N + j) * (nx + 1) + i] >> a.mshift) & 3 : 0;
     if (a.Wc[0]) {
         // compensated weights: threshold and division in FP64
         if (m ? m == 2 : wd > (double)0.1f)
@@ -65,30 +69,64 @@ __device__ inline float2 norm_val(const Acc& a, int nx, int N, int i, int j, int
     return make_float2(0.f, 0.f);
 }
 
-__global__ __launch_bounds__(TPB) void normalize_kernel(Acc a, int nx, int N, float2* __restrict__ out) {
-    const int H = nx + 1;
+// ------------------------------------------------------------------ batched finalisation
+// One volume of a batched finalisation: its accumulators, transform buffers,
+// output and its first block in each of the two grids one launch covers (the
+// half-grid grid of the normalisation, the NX^3 grid of the window steps).
+// A launch covers a contiguous run of jobs; block offsets are relative to the
+// run's first job.
+struct FinalJob {
+    Acc a;
+    float2* work;          // half-complex input of the inverse transform
+    const float* real;     // its N^3 output
+    float* out;            // NX^3 volume
+    const float* wkb;      // WINDKB2A divisor by squared radius
+    double* psum;          // per volume block: outer-shell sum and count
+    unsigned* pcnt;
+    float* mean;
+    int nx, N;
+    float pix;             // 1 / N^3
+    unsigned hb0, vb0;     // first block in the half-grid / volume grid
+    unsigned vnb;          // volume blocks
+};
+
+// Job of block b: the last of the run whose first block is <= b.
+template <bool VOL>
+__device__ inline int job_of_block(const FinalJob* __restrict__ jobs, int nj, unsigned b) {
+    int lo = 0, hi = nj - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((VOL ? jobs[mid].vb0 : jobs[mid].hb0) <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(TPB) void normalize_kernel(const FinalJob* __restrict__ jobs, int nj) {
+    const FinalJob& f = jobs[job_of_block<false>(jobs, nj, blockIdx.x)];
+    const int nx = f.nx, N = f.N, H = nx + 1;
     const size_t total = (size_t)H * N * N;
-    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t idx = (size_t)(blockIdx.x - f.hb0) * blockDim.x + threadIdx.x;
     if (idx >= total) return;
     const int i = (int)(idx % H);
     const size_t r = idx / H;
     const int j = (int)(r % N), k = (int)(r / N);
-    float2 v = norm_val(a, nx, N, i, j, k);
+    float2 v = norm_val(f.a, nx, N, i, j, k);
     if (i == 0 || i == nx) {
         // c2r reads the kx = 0 and kx = N/2 planes as their Hermitian part
-        const float2 m = norm_val(a, nx, N, i, (N - j) % N, (N - k) % N);
+        const float2 m = norm_val(f.a, nx, N, i, (N - j) % N, (N - k) % N);
         v = make_float2((v.x + m.x) * 0.5f, (v.y - m.y) * 0.5f);
     }
-    out[idx] = make_float2(v.x, -v.y);                 // SPIDER inverse: conjugate first
+    f.work[idx] = make_float2(v.x, -v.y);              // SPIDER inverse: conjugate first
 }
 
 // ------------------------------------------------------------------ WINDKB2A
-__global__ __launch_bounds__(TPB) void window_kernel(const float* __restrict__ real, float* __restrict__ out,
-                                                     const float* __restrict__ wkb, int nx, int N, float pix,
-                                                     double* __restrict__ psum, unsigned* __restrict__ pcnt) {
-    const int L = nx;
+__global__ __launch_bounds__(TPB) void window_kernel(const FinalJob* __restrict__ jobs, int nj) {
+    const FinalJob& f = jobs[job_of_block<true>(jobs, nj, blockIdx.x)];
+    const int L = f.nx, N = f.N;
+    const unsigned lb = blockIdx.x - f.vb0;
     const size_t total = (size_t)L * L * L;
-    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t idx = (size_t)lb * blockDim.x + threadIdx.x;
     double s = 0.0;
     unsigned c = 0;
     if (idx < total) {
@@ -97,40 +135,42 @@ __global__ __launch_bounds__(TPB) void window_kernel(const float* __restrict__ r
         const int j = (int)(t % L), k = (int)(t / L);
         const int ip = (N - L) / 2 + (L % 2);
         const int cc = L / 2;
-        float v = real[((size_t)(ip + k) * N + (ip + j)) * N + (ip + i)] * pix;   // FMRS_3 scale
+        float v = f.real[((size_t)(ip + k) * N + (ip + j)) * N + (ip + i)] * f.pix;   // FMRS_3 scale
         const int lr = (i - cc) * (i - cc) + (j - cc) * (j - cc) + (k - cc) * (k - cc);
         if (lr <= cc * cc) {
-            v = __fdiv_rn(v, wkb[lr]);
+            v = __fdiv_rn(v, f.wkb[lr]);
             if (lr >= (cc - 1) * (cc - 1)) { s = v; c = 1; }
         } else {
             v = 0.0f;
         }
-        out[idx] = v;
+        f.out[idx] = v;
     }
     block_sum2(s, c);
-    if (threadIdx.x == 0) { psum[blockIdx.x] = s; pcnt[blockIdx.x] = c; }
+    if (threadIdx.x == 0) { f.psum[lb] = s; f.pcnt[lb] = c; }
 }
 
-__global__ __launch_bounds__(TPB) void shell_mean_kernel(const double* __restrict__ psum, const unsigned* __restrict__ pcnt,
-                                                         int nb, float* __restrict__ mean) {
+// One block per job: the mean over the outer shell.
+__global__ __launch_bounds__(TPB) void shell_mean_kernel(const FinalJob* __restrict__ jobs) {
+    const FinalJob& f = jobs[blockIdx.x];
     double s = 0.0;
     unsigned c = 0;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) { s += psum[b]; c += pcnt[b]; }
+    for (unsigned b = threadIdx.x; b < f.vnb; b += blockDim.x) { s += f.psum[b]; c += f.pcnt[b]; }
     block_sum2(s, c);
-    if (threadIdx.x == 0) mean[0] = (float)(s / (double)(c ? c : 1));
+    if (threadIdx.x == 0) f.mean[0] = (float)(s / (double)(c ? c : 1));
 }
 
-__global__ __launch_bounds__(TPB) void subtract_kernel(float* __restrict__ out, const float* __restrict__ mean, int nx) {
-    const int L = nx;
+__global__ __launch_bounds__(TPB) void subtract_kernel(const FinalJob* __restrict__ jobs, int nj) {
+    const FinalJob& f = jobs[job_of_block<true>(jobs, nj, blockIdx.x)];
+    const int L = f.nx;
     const size_t total = (size_t)L * L * L;
-    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t idx = (size_t)(blockIdx.x - f.vb0) * blockDim.x + threadIdx.x;
     if (idx >= total) return;
     const int i = (int)(idx % L);
     const size_t t = idx / L;
     const int j = (int)(t % L), k = (int)(t / L);
     const int cc = L / 2;
     const int lr = (i - cc) * (i - cc) + (j - cc) * (j - cc) + (k - cc) * (k - cc);
-    if (lr <= cc * cc) out[idx] = out[idx] - mean[0];
+    if (lr <= cc * cc) f.out[idx] = f.out[idx] - f.mean[0];
 }
 
 // ------------------------------------------------------------------ partial volumes

@@ -11,7 +11,7 @@
 // (oneline.f:56-152).  For every voxel whose GPU weight lies in a narrow band
 // around 0.1, these kernels recompute that exact sequence from the context's
 // insertion history and record the reference's decision in a byte mask that
-// normalize_kernel reads:
+// normalize_kernel reads (two bits per volume of the context):
 //   1. thresh_list_kernel  -- voxels in the band (two targets each: the voxel
 //                             and, on the kx = 0 plane, its SYMPLANE0 partner);
 //   2. hit_kernel          -- (target, section) pairs whose plane passes within
@@ -142,12 +142,13 @@ __device__ inline bool plane_near(const Geom& g, int x, int y, int z) {
 
 // Block (item, chunk of RES_TPB sections): MODE 0 counts the hits of the
 // chunk into cnt[block]; MODE 1 writes them, in section order, at off[block].
+// A launch covers blocks b0 .. b0 + gridDim.x - 1.
 template <int MODE>
-__global__ __launch_bounds__(RES_TPB) void hit_kernel(const long long* __restrict__ list, long long nitems, int nchunks,
-                                                      const Geom* __restrict__ hist, int nhist, int hmask, int nx, int N,
-                                                      int* __restrict__ cnt, const long long* __restrict__ off,
-                                                      int2* __restrict__ hits) {
-    const long long b = blockIdx.x;
+__global__ __launch_bounds__(RES_TPB) void hit_kernel(long long b0, const long long* __restrict__ list, long long nitems,
+                                                      int nchunks, const Geom* __restrict__ hist, int nhist, int hmask,
+                                                      int nx, int N, int* __restrict__ cnt,
+                                                      const long long* __restrict__ off, int2* __restrict__ hits) {
+    const long long b = b0 + blockIdx.x;
     const long long it = b / nchunks;
     const int e = (int)(b % nchunks) * RES_TPB + threadIdx.x;
     int x = 0, y = 0, z = 0;
@@ -365,7 +366,7 @@ __global__ __launch_bounds__(RES_TPB) void weights_out_kernel(const long long* _
 template <typename P>
 __global__ __launch_bounds__(RES_TPB) void decide_kernel(const long long* __restrict__ list, long long nv, Acc a,
                                                          int hmask, int nx, int N, const P* __restrict__ sums,
-                                                         double tol, unsigned char* __restrict__ mask,
+                                                         double tol, unsigned char* __restrict__ mask, int shift,
                                                          unsigned long long* __restrict__ ctr) {
     const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nv) return;
@@ -382,7 +383,9 @@ __global__ __launch_bounds__(RES_TPB) void decide_kernel(const long long* __rest
         return;
     }
     const bool divide = w > (P)0.1f;
-    mask[o] = divide ? 2 : 1;
+    // listed voxels are distinct, and the volumes sharing the byte decide in
+    // launches of their own
+    mask[o] = (unsigned char)(mask[o] | ((divide ? 2 : 1) << shift));
     if ((gv > (double)0.1f) != divide) atomicAdd(&ctr[1], 1ull);
 }
 

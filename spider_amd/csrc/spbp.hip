@@ -31,6 +31,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -71,6 +72,23 @@ constexpr int CVALS[NCVALS] = {16, 32, 64, 128};   // projections per work item 
 }  // namespace
 
 // ------------------------------------------------------------------ context
+// Re-summation buffers of one finalised volume: the listed voxels, hit and
+// term counts and offsets, the terms and their sums, the counters, and a
+// pinned word area through which counts come back to the host.
+struct Res {
+    long long* list = nullptr; size_t list_cap = 0;
+    int* cnt = nullptr; size_t cnt_cap = 0;
+    long long* off = nullptr; size_t off_cap = 0;
+    long long* bsum = nullptr; size_t bsum_cap = 0;
+    int2* hits = nullptr; size_t hits_cap = 0;
+    int* tcnt = nullptr; size_t tcnt_cap = 0;
+    long long* toff = nullptr; size_t toff_cap = 0;
+    char* terms = nullptr; size_t terms_cap = 0;
+    char* sums = nullptr; size_t sums_cap = 0;
+    unsigned long long* ctr = nullptr;   // [listed, changed, inconsistent, -]
+    unsigned long long* h = nullptr;     // pinned [4]
+};
+
 struct spbp_ctx {
     int nx = 0, N = 0, H = 0, flags = 0, device = 0, batch = 0, nvol = 1;
     size_t nh = 0;
@@ -111,6 +129,12 @@ struct spbp_ctx {
     unsigned* pcnt = nullptr;
     float* mean = nullptr;
     int* flag = nullptr;
+    // batched finalisation job table (spbp_finalize_batch's first context
+    // holds the call's table): device, pinned staging, its upload
+    FinalJob* fjobs_d = nullptr;
+    FinalJob* fjobs_h = nullptr;
+    int fjobs_cap = 0;
+    hipEvent_t fjobs_evt = nullptr;
     // host staging for _host entry points
     void* stage = nullptr;
     size_t stage_cap = 0;
@@ -160,19 +184,14 @@ struct spbp_ctx {
     Geom* hist_d = nullptr;
     size_t hist_dcap = 0, hist_dn = 0;
     double* tab64 = nullptr;          // FP64 table (FP64 geometry)
-    unsigned char* tmask = nullptr;   // per voxel: 0 = GPU sums decide, 1 = zero, 2 = divide
+    // per voxel, two bits per volume slot (full, half 1, half 2): 0 = GPU sums
+    // decide, 1 = zero, 2 = divide
+    unsigned char* tmask = nullptr;
     bool tmask_dirty = true;
-    long long* rlist = nullptr; size_t rlist_cap = 0;
-    int* rcnt = nullptr; size_t rcnt_cap = 0;
-    long long* roff = nullptr; size_t roff_cap = 0;
-    long long* rbsum = nullptr; size_t rbsum_cap = 0;
-    int2* rhits = nullptr; size_t rhits_cap = 0;
-    int* rtcnt = nullptr; size_t rtcnt_cap = 0;
-    long long* rtoff = nullptr; size_t rtoff_cap = 0;
-    char* rterms = nullptr; size_t rterms_cap = 0;
-    char* rsums = nullptr; size_t rsums_cap = 0;
-    unsigned long long* rctr = nullptr;   // [listed, changed, inconsistent]
+    Res res[3];                   // re-summation buffers per volume slot
     long long res_stats[3] = {0, 0, 0};
+    int stats_slot = -1;          // slot whose counters res[].h receives (-1: res_stats holds all)
+    hipEvent_t stats_evt = nullptr;   // their read-back
     // SPBP_TIMING: (start, stop, units) per recorded launch, per kernel kind
     struct Rec { hipEvent_t a, b; long units; };
     std::vector<Rec> recs[7];
@@ -765,6 +784,8 @@ int check_images(spbp_ctx* c, const void* imgs, int dtype, int nimg, hipStream_t
 // RTSQ parameters of nimg images (in-plane angle, x shift, y shift per image)
 // into c->rprm_d, queued on st.
 int stage_rtsq(spbp_ctx* c, int nimg, const float* inplane, hipStream_t st) {
+    // the event exists before any staging buffer does
+    if (!c->rprm_evt) HIPCHK(c, hipEventCreateWithFlags(&c->rprm_evt, hipEventDisableTiming));
     if (c->rprm_cap < nimg) {
         c->rprm_cap = 0;   // committed again only once both buffers exist
         if (c->rprm_h) { (void)hipEventSynchronize(c->rprm_evt); (void)hipHostFree(c->rprm_h); c->rprm_h = nullptr; }
@@ -772,7 +793,6 @@ int stage_rtsq(spbp_ctx* c, int nimg, const float* inplane, hipStream_t st) {
         const int cap = std::max(nimg, 1024);
         HIPCHK(c, hipHostMalloc((void**)&c->rprm_h, cap * sizeof(RtsqParam), hipHostMallocDefault));
         HIPCHK(c, dmalloc(&c->rprm_d, cap * sizeof(RtsqParam)));
-        if (!c->rprm_evt) HIPCHK(c, hipEventCreateWithFlags(&c->rprm_evt, hipEventDisableTiming));
         c->rprm_cap = cap;
     } else {
         HIPCHK(c, hipEventSynchronize(c->rprm_evt));
@@ -913,60 +933,14 @@ int grow(spbp_ctx* c, T*& p, size_t& cap, size_t n, const char* what) {
 
 // out[0..n] = exclusive scan of n counts (out[n] = total), on st.
 template <typename C>
-int scan_counts(spbp_ctx* c, const C* in, long long n, long long* out, hipStream_t st) {
+int scan_counts(spbp_ctx* c, Res& r, const C* in, long long n, long long* out, hipStream_t st) {
     const int nb = (int)((n + 1023) / 1024);
-    int rc = grow(c, c->rbsum, c->rbsum_cap, (size_t)nb + 1, "scan");
+    int rc = grow(c, r.bsum, r.bsum_cap, (size_t)nb + 1, "scan");
     if (rc) return rc;
-    if (nb > 0) scan_blocks_kernel<C><<<nb, 1024, 0, st>>>(in, n, out, c->rbsum);
-    scan_sums_kernel<<<1, 1024, 0, st>>>(c->rbsum, nb);
-    if (nb > 0) scan_add_kernel<<<nb, 1024, 0, st>>>(out, n, c->rbsum, nb);
-    else HIPCHK(c, hipMemcpyAsync(out, c->rbsum, sizeof(long long), hipMemcpyDeviceToDevice, st));
-    HIPCHK(c, hipGetLastError());
-    return SPBP_OK;
-}
-
-template <typename T>
-int read_back(spbp_ctx* c, const T* d, T* h, hipStream_t st) {
-    HIPCHK(c, hipMemcpyAsync(h, d, sizeof(T), hipMemcpyDeviceToHost, st));
-    HIPCHK(c, hipStreamSynchronize(st));
-    return SPBP_OK;
-}
-
-// Steps 3-5 for positions / terms of type P (float: the reference's fp32
-// geometry and sums; double: the FP64 computation).
-template <typename P>
-int resolve_terms(spbp_ctx* c, const Acc& a, int hmask, long long nv, int nchunks, long long nhits, double* wout,
-                  hipStream_t st) {
-    const long long nitems = 2 * nv;
-    const P* tab = sizeof(P) == 4 ? (const P*)c->tab : (const P*)c->tab64;
-    const P fltb = sizeof(P) == 4 ? (P)c->fltb : (P)(4999.0 / 3.0);
-    int rc;
-    if ((rc = grow(c, c->rtcnt, c->rtcnt_cap, (size_t)nhits, "term counts")) ||
-        (rc = grow(c, c->rtoff, c->rtoff_cap, (size_t)nhits + 1, "term offsets")))
-        return rc;
-    const unsigned hb = (unsigned)((nhits + RES_TPB - 1) / RES_TPB);
-    if (nhits > 0)
-        term_kernel<P, false><<<hb, RES_TPB, 0, st>>>(c->rhits, nhits, c->rlist, c->hist_d, tab, fltb, c->nx, c->N,
-                                                      c->rtcnt, nullptr, nullptr);
-    if ((rc = scan_counts(c, c->rtcnt, nhits, c->rtoff, st))) return rc;
-    long long nterms = 0;
-    if ((rc = read_back(c, c->rtoff + nhits, &nterms, st))) return rc;
-    if ((rc = grow(c, c->rterms, c->rterms_cap, (size_t)std::max<long long>(nterms, 1) * sizeof(P), "weight terms")) ||
-        (rc = grow(c, c->rsums, c->rsums_cap, (size_t)nitems * 2 * sizeof(P), "weight sums")))
-        return rc;
-    if (nhits > 0)
-        term_kernel<P, true><<<hb, RES_TPB, 0, st>>>(c->rhits, nhits, c->rlist, c->hist_d, tab, fltb, c->nx, c->N,
-                                                     nullptr, c->rtoff, (P*)c->rterms);
-    const unsigned ib = (unsigned)((nitems + RES_TPB - 1) / RES_TPB);
-    sum_kernel<P><<<ib, RES_TPB, 0, st>>>(nitems, nchunks, c->roff, c->rtoff, (const P*)c->rterms, (P*)c->rsums);
-    const unsigned vb = (unsigned)((nv + RES_TPB - 1) / RES_TPB);
-    if (wout) {
-        weights_out_kernel<P><<<vb, RES_TPB, 0, st>>>(c->rlist, nv, hmask, c->nx, c->N, (const P*)c->rsums, wout);
-    } else {
-        c->tmask_dirty = true;
-        decide_kernel<P><<<vb, RES_TPB, 0, st>>>(c->rlist, nv, a, hmask, c->nx, c->N, (const P*)c->rsums,
-                                                 0.1 * (1.0 / 512.0), c->tmask, c->rctr);
-    }
+    if (nb > 0) scan_blocks_kernel<C><<<nb, 1024, 0, st>>>(in, n, out, r.bsum);
+    scan_sums_kernel<<<1, 1024, 0, st>>>(r.bsum, nb);
+    if (nb > 0) scan_add_kernel<<<nb, 1024, 0, st>>>(out, n, r.bsum, nb);
+    else HIPCHK(c, hipMemcpyAsync(out, r.bsum, sizeof(long long), hipMemcpyDeviceToDevice, st));
     HIPCHK(c, hipGetLastError());
     return SPBP_OK;
 }
@@ -987,111 +961,375 @@ int upload_history(spbp_ctx* c, hipStream_t st) {
     return SPBP_OK;
 }
 
-// Steps 2-5 for the nv voxels in c->rlist.
-int resolve_list(spbp_ctx* c, int which, const Acc& a, long long nv, double* wout, hipStream_t st) {
-    const size_t nhist = c->hist.size();
-    const int hmask = which < 0 ? (1 << c->nvol) - 1 : 1 << which;
-    const long long nitems = 2 * nv;
-    const int nchunks = (int)((nhist + RES_TPB - 1) / RES_TPB);
-    const long long nblk = nitems * nchunks;
-    if (nblk > (long long)INT32_MAX) return fail(c, SPBP_ERR_SIZE, "threshold re-summation too large (%lld blocks)", nblk);
-    int rc;
-    if ((rc = grow(c, c->rcnt, c->rcnt_cap, (size_t)nblk, "hit counts")) ||
-        (rc = grow(c, c->roff, c->roff_cap, (size_t)nblk + 1, "hit offsets")))
-        return rc;
-    hit_kernel<0><<<(unsigned)nblk, RES_TPB, 0, st>>>(c->rlist, nitems, nchunks, c->hist_d, (int)nhist, hmask, c->nx, c->N,
-                                                      c->rcnt, nullptr, nullptr);
-    if ((rc = scan_counts(c, c->rcnt, nblk, c->roff, st))) return rc;
-    long long nhits = 0;
-    if ((rc = read_back(c, c->roff + nblk, &nhits, st))) return rc;
-    if ((rc = grow(c, c->rhits, c->rhits_cap, (size_t)std::max<long long>(nhits, 1), "hits"))) return rc;
-    hit_kernel<1><<<(unsigned)nblk, RES_TPB, 0, st>>>(c->rlist, nitems, nchunks, c->hist_d, (int)nhist, hmask, c->nx, c->N,
-                                                      nullptr, c->roff, c->rhits);
-    HIPCHK(c, hipGetLastError());
-    return (c->flags & SPBP_GEOM_F64) ? resolve_terms<double>(c, a, hmask, nv, nchunks, nhits, wout, st)
-                                      : resolve_terms<float>(c, a, hmask, nv, nchunks, nhits, wout, st);
+// One volume on its way through a finalisation: `which` (-1 all halves, else
+// one half) of context c, with the re-summation buffers and mask bits of
+// volume slot `slot` (0 full, 1 half 1, 2 half 2).  The counts that size each
+// next re-summation phase come back through res().h[0].
+struct Round {
+    spbp_ctx* c = nullptr;
+    int which = -1, slot = 0, wave = 0;
+    float* out = nullptr;
+    Acc a{};
+    bool resolve = false;   // the reference's sums decide NRMW2's test near 0.1
+    long long nv = 0, nhits = 0, nterms = 0;
+    int nchunks = 0, hmask = 0;
+    Res& res() const { return c->res[slot]; }
+};
+
+Round make_round(spbp_ctx* c, int which, int slot) {
+    Round R;
+    R.c = c;
+    R.which = which;
+    R.slot = slot;
+    R.a = make_acc(c, which);
+    R.hmask = which < 0 ? (1 << c->nvol) - 1 : 1 << which;
+    return R;
 }
 
-// Decide NRMW2's test for the voxels near 0.1 by the reference's sums (the
-// finalisation of `which`: -1 all halves, else one half); on success with
-// *active, a.mask holds the decisions until clear_decisions.
-int resolve_threshold(spbp_ctx* c, int which, Acc& a, bool* active, long long* nv_out, hipStream_t st) {
-    *active = false;
-    *nv_out = 0;
-    c->res_stats[0] = c->res_stats[1] = c->res_stats[2] = 0;
-    if (!c->hist_ok || (c->flags & SPBP_PLAIN_THRESHOLD) || c->hist.empty()) return SPBP_OK;
-    const size_t nhist = c->hist.size();
-    if (nhist > (size_t)INT32_MAX) return SPBP_OK;
-    int rc;
-    Timed tm(c, SPBP_K_RESOLVE, 1, st);
-    if ((rc = upload_history(c, st))) return rc;
-    if (!c->tmask) {
-        HIPCHK(c, dmalloc(&c->tmask, (c->nh + 3) / 4 * 4));
-        c->tmask_dirty = true;
+int ensure_res(spbp_ctx* c, Res& r, size_t list_min) {
+    if (!r.ctr) HIPCHK(c, dmalloc(&r.ctr, 4 * sizeof(unsigned long long)));
+    if (!r.h) {
+        HIPCHK(c, hipHostMalloc((void**)&r.h, 4 * sizeof(unsigned long long), hipHostMallocDefault));
+        r.h[0] = r.h[1] = r.h[2] = r.h[3] = 0;
     }
-    if (c->tmask_dirty) {
-        fill_kernel<<<nblocks((c->nh + 3) / 4), TPB, 0, st>>>((float*)c->tmask, (c->nh + 3) / 4, 0.0f);   // zero bits
-        c->tmask_dirty = false;
+    return grow(c, r.list, r.list_cap, list_min, "threshold list");
+}
+
+// Blocks of one hit_kernel launch at most (2^30 work-items).
+constexpr long long HIT_SLICE = 1ll << 22;
+
+template <int MODE>
+int launch_hits(Round& R, long long nblk, hipStream_t st) {
+    spbp_ctx* c = R.c;
+    Res& r = R.res();
+    for (long long b0 = 0; b0 < nblk; b0 += HIT_SLICE) {
+        const unsigned nb = (unsigned)std::min(HIT_SLICE, nblk - b0);
+        hit_kernel<MODE><<<nb, RES_TPB, 0, st>>>(b0, r.list, 2 * R.nv, R.nchunks, c->hist_d, (int)c->hist.size(), R.hmask,
+                                                 c->nx, c->N, r.cnt, r.off, r.hits);
     }
-    if (!c->rctr) HIPCHK(c, dmalloc(&c->rctr, 4 * sizeof(unsigned long long)));
-    // 1. voxels in the band
-    if ((rc = grow(c, c->rlist, c->rlist_cap, 4096, "threshold list"))) return rc;
-    unsigned long long nv = 0;
-    for (int pass = 0; pass < 2; ++pass) {
-        fill_kernel<<<1, TPB, 0, st>>>((float*)c->rctr, 8, 0.0f);   // zero bits
-        thresh_list_kernel<<<nblocks(c->nh), TPB, 0, st>>>(a, c->nx, c->N, THRESH_BAND, c->rlist, (long long)c->rlist_cap,
-                                                           c->rctr);
-        HIPCHK(c, hipGetLastError());
-        if ((rc = read_back(c, c->rctr, &nv, st))) return rc;
-        if (nv <= c->rlist_cap) break;
-        if ((rc = grow(c, c->rlist, c->rlist_cap, (size_t)nv, "threshold list"))) return rc;
-    }
-    c->res_stats[0] = (long long)nv;
-    *nv_out = (long long)nv;
-    if (nv == 0) return SPBP_OK;
-    // 2-5. hits, terms, sums, decisions
-    if ((rc = resolve_list(c, which, a, (long long)nv, nullptr, st))) return rc;
-    unsigned long long ctr[4] = {0, 0, 0, 0};
-    HIPCHK(c, hipMemcpyAsync(ctr, c->rctr, sizeof ctr, hipMemcpyDeviceToHost, st));
-    HIPCHK(c, hipStreamSynchronize(st));
-    c->res_stats[1] = (long long)ctr[1];
-    c->res_stats[2] = (long long)ctr[2];
-    a.mask = c->tmask;
-    *active = true;
+    HIPCHK(c, hipGetLastError());
     return SPBP_OK;
 }
 
-void clear_decisions(spbp_ctx* c, long long nv, hipStream_t st) {
-    mask_clear_kernel<<<(unsigned)((nv + RES_TPB - 1) / RES_TPB), RES_TPB, 0, st>>>(c->rlist, nv, c->tmask);
-    if (hipGetLastError() == hipSuccess) c->tmask_dirty = false;
+template <typename P>
+const P* res_table(const spbp_ctx* c) { return sizeof(P) == 4 ? (const P*)c->tab : (const P*)c->tab64; }
+template <typename P>
+P res_fltb(const spbp_ctx* c) { return sizeof(P) == 4 ? (P)c->fltb : (P)(4999.0 / 3.0); }
+
+// Phase 1: the voxels in the band; their count to res().h[0].
+int phase_list(Round& R, hipStream_t st) {
+    spbp_ctx* c = R.c;
+    Res& r = R.res();
+    fill_kernel<<<1, TPB, 0, st>>>((float*)r.ctr, 8, 0.0f);   // zero bits
+    thresh_list_kernel<<<nblocks(c->nh), TPB, 0, st>>>(R.a, c->nx, c->N, THRESH_BAND, r.list, (long long)r.list_cap,
+                                                       r.ctr);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(r.h, r.ctr, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    return SPBP_OK;
 }
 
-int finalize_one(spbp_ctx* c, int which, float* out, hipStream_t st) {
-    const int nx = c->nx, N = c->N;
-    Acc a = make_acc(c, which);
-    bool active = false;
-    long long nlisted = 0;
-    int rc = resolve_threshold(c, which, a, &active, &nlisted, st);
-    if (rc) return rc;
-    Timed t(c, SPBP_K_FINALIZE, 1, st);
-    normalize_kernel<<<nblocks(c->nh), TPB, 0, st>>>(a, nx, N, c->work);
-    HIPCHK(c, hipGetLastError());
-    if (active) clear_decisions(c, nlisted, st);
-    FFTCHK(c, hipfftSetStream(c->inv, st));
-    FFTCHK(c, hipfftExecC2R(c->inv, (hipfftComplex*)c->work, (hipfftReal*)c->real));
-    const size_t nv = (size_t)nx * nx * nx;
-    const unsigned nb = nblocks(nv);
-    const float pix = 1.0f / (float)((double)N * N * N);   // PIX = 1.0/FLOAT(N^3) (fftw3.f:444)
-    window_kernel<<<nb, TPB, 0, st>>>(c->real, out, c->wkb, nx, N, pix, c->psum, c->pcnt);
-    shell_mean_kernel<<<1, TPB, 0, st>>>(c->psum, c->pcnt, (int)nb, c->mean);
-    subtract_kernel<<<nb, TPB, 0, st>>>(out, c->mean, nx);
-    HIPCHK(c, hipGetLastError());
-    if (c->nsym > 1) {   // real-space symmetrisation after WINDKB2A (bp32f.f:278-303)
-        if (!c->symtmp) HIPCHK(c, dmalloc(&c->symtmp, nv * sizeof(float)));
-        HIPCHK(c, hipMemcpyAsync(c->symtmp, out, nv * sizeof(float), hipMemcpyDeviceToDevice, st));
-        symvol_kernel<<<nb, TPB, 0, st>>>(c->symtmp, out, nx, c->sm_d, c->nsym);
-        HIPCHK(c, hipGetLastError());
+// Phase 2: the (target, section) hits counted per block; their total to res().h[0].
+int phase_hit_counts(Round& R, hipStream_t st) {
+    spbp_ctx* c = R.c;
+    Res& r = R.res();
+    R.nchunks = (int)((c->hist.size() + RES_TPB - 1) / RES_TPB);
+    const long long nblk = 2 * R.nv * R.nchunks;
+    if (nblk > (long long)INT32_MAX) return fail(c, SPBP_ERR_SIZE, "threshold re-summation too large (%lld blocks)", nblk);
+    int rc;
+    if ((rc = grow(c, r.cnt, r.cnt_cap, (size_t)nblk, "hit counts")) ||
+        (rc = grow(c, r.off, r.off_cap, (size_t)nblk + 1, "hit offsets")) || (rc = launch_hits<0>(R, nblk, st)) ||
+        (rc = scan_counts(c, r, r.cnt, nblk, r.off, st)))
+        return rc;
+    HIPCHK(c, hipMemcpyAsync(r.h, r.off + nblk, sizeof(long long), hipMemcpyDeviceToHost, st));
+    return SPBP_OK;
+}
+
+// Phase 3: the hits written in section order and their weight terms counted;
+// the total to res().h[0].
+template <typename P>
+int phase_term_counts(Round& R, hipStream_t st) {
+    spbp_ctx* c = R.c;
+    Res& r = R.res();
+    if (R.nhits > (long long)INT32_MAX) return fail(c, SPBP_ERR_SIZE, "threshold re-summation too large (%lld hits)", R.nhits);
+    const size_t nh1 = (size_t)std::max<long long>(R.nhits, 1);
+    int rc;
+    if ((rc = grow(c, r.hits, r.hits_cap, nh1, "hits")) || (rc = grow(c, r.tcnt, r.tcnt_cap, nh1, "term counts")) ||
+        (rc = grow(c, r.toff, r.toff_cap, (size_t)R.nhits + 1, "term offsets")) ||
+        (rc = launch_hits<1>(R, 2 * R.nv * R.nchunks, st)))
+        return rc;
+    if (R.nhits > 0)
+        term_kernel<P, false><<<(unsigned)((R.nhits + RES_TPB - 1) / RES_TPB), RES_TPB, 0, st>>>(
+            r.hits, R.nhits, r.list, c->hist_d, res_table<P>(c), res_fltb<P>(c), c->nx, c->N, r.tcnt, nullptr, nullptr);
+    if ((rc = scan_counts(c, r, r.tcnt, R.nhits, r.toff, st))) return rc;
+    HIPCHK(c, hipMemcpyAsync(r.h, r.toff + R.nhits, sizeof(long long), hipMemcpyDeviceToHost, st));
+    return SPBP_OK;
+}
+
+// Phase 4: the terms written and summed sequentially per half; the decisions
+// into the mask bits of the round's slot, or with wout the combined weights.
+template <typename P>
+int phase_decide(Round& R, double* wout, hipStream_t st) {
+    spbp_ctx* c = R.c;
+    Res& r = R.res();
+    const long long nitems = 2 * R.nv;
+    int rc;
+    if ((rc = grow(c, r.terms, r.terms_cap, (size_t)std::max<long long>(R.nterms, 1) * sizeof(P), "weight terms")) ||
+        (rc = grow(c, r.sums, r.sums_cap, (size_t)nitems * 2 * sizeof(P), "weight sums")))
+        return rc;
+    if (R.nhits > 0)
+        term_kernel<P, true><<<(unsigned)((R.nhits + RES_TPB - 1) / RES_TPB), RES_TPB, 0, st>>>(
+            r.hits, R.nhits, r.list, c->hist_d, res_table<P>(c), res_fltb<P>(c), c->nx, c->N, nullptr, r.toff,
+            (P*)r.terms);
+    sum_kernel<P><<<(unsigned)((nitems + RES_TPB - 1) / RES_TPB), RES_TPB, 0, st>>>(nitems, R.nchunks, r.off, r.toff,
+                                                                                   (const P*)r.terms, (P*)r.sums);
+    const unsigned vb = (unsigned)((R.nv + RES_TPB - 1) / RES_TPB);
+    if (wout) {
+        weights_out_kernel<P><<<vb, RES_TPB, 0, st>>>(r.list, R.nv, R.hmask, c->nx, c->N, (const P*)r.sums, wout);
+    } else {
+        c->tmask_dirty = true;
+        decide_kernel<P><<<vb, RES_TPB, 0, st>>>(r.list, R.nv, R.a, R.hmask, c->nx, c->N, (const P*)r.sums,
+                                                 0.1 * (1.0 / 512.0), c->tmask, 2 * R.slot, r.ctr);
     }
+    HIPCHK(c, hipGetLastError());
+    return SPBP_OK;
+}
+
+bool geom64(const spbp_ctx* c) { return (c->flags & SPBP_GEOM_F64) != 0; }
+
+// Phases 2-4 for rounds whose voxel lists are set (nv), with one host
+// synchronisation after each of phases 2 and 3 for all of them; wout (one
+// round) asks for the combined weights instead of decisions.
+int resolve_listed(const std::vector<Round*>& act, double* wout, hipStream_t st) {
+    int rc;
+    std::vector<Round*> run;
+    for (Round* R : act) if (R->nv > 0) run.push_back(R);
+    if (run.empty()) return SPBP_OK;
+    for (Round* R : run) if ((rc = phase_hit_counts(*R, st))) return rc;
+    HIPCHK(run[0]->c, hipStreamSynchronize(st));
+    for (Round* R : run) {
+        R->nhits = (long long)R->res().h[0];
+        rc = geom64(R->c) ? phase_term_counts<double>(*R, st) : phase_term_counts<float>(*R, st);
+        if (rc) return rc;
+    }
+    HIPCHK(run[0]->c, hipStreamSynchronize(st));
+    for (Round* R : run) {
+        R->nterms = (long long)R->res().h[0];
+        rc = geom64(R->c) ? phase_decide<double>(*R, wout, st) : phase_decide<float>(*R, wout, st);
+        if (rc) return rc;
+    }
+    return SPBP_OK;
+}
+
+// NRMW2's test decided by the reference's sums for every round with
+// R.resolve, phase by phase across all of them: three host synchronisations
+// for any number of volumes (four when a band list outgrows its buffer).
+int resolve_rounds(std::vector<Round>& rs, hipStream_t st) {
+    std::vector<Round*> act;
+    for (Round& R : rs) if (R.resolve) act.push_back(&R);
+    if (act.empty()) return SPBP_OK;
+    int rc;
+    for (Round* R : act) if ((rc = phase_list(*R, st))) return rc;
+    HIPCHK(act[0]->c, hipStreamSynchronize(st));
+    bool again = false;
+    for (Round* R : act) {
+        Res& r = R->res();
+        R->nv = (long long)r.h[0];
+        if ((size_t)R->nv > r.list_cap) {   // a list that outgrew its buffer: list again
+            if ((rc = grow(R->c, r.list, r.list_cap, (size_t)R->nv, "threshold list")) || (rc = phase_list(*R, st))) return rc;
+            again = true;
+        }
+    }
+    if (again) {
+        HIPCHK(act[0]->c, hipStreamSynchronize(st));
+        for (Round* R : act) {
+            R->nv = (long long)R->res().h[0];
+            if ((size_t)R->nv > R->res().list_cap)
+                return fail(R->c, SPBP_ERR_DEVICE, "threshold list changed between passes (%lld voxels)", R->nv);
+        }
+    }
+    return resolve_listed(act, nullptr, st);
+}
+
+// Blocks of one finalisation launch at most (2^31 work-items).
+constexpr unsigned FINAL_LAUNCH_BLOCKS = (1u << 31) / TPB;
+
+// A run of jobs one launch of each window-step kernel covers.
+struct FinalRun {
+    int j0 = 0, n = 0;
+    unsigned hb = 0, vb = 0;   // half-grid and volume blocks
+};
+
+// Records the job table's last use at scope exit, so the next call reuses the
+// table only after every launch that reads it.
+struct TableUse {
+    spbp_ctx* c;
+    hipStream_t st;
+    ~TableUse() { if (c->fjobs_evt) (void)hipEventRecord(c->fjobs_evt, st); }
+};
+
+// Normalisation, inverse transform and window steps of every round, in waves
+// (a context's buffers serve one volume per wave): per wave one normalise
+// launch over all its volumes, their inverse transforms, one window, one
+// shell-mean and one subtract launch over all of them, then SYMVOL where
+// symmetries were inserted.  The job table lives on the first round's context.
+int run_waves(std::vector<Round>& rs, hipStream_t st) {
+    spbp_ctx* t = rs[0].c;
+    int nw = 0;
+    for (const Round& R : rs) nw = std::max(nw, R.wave + 1);
+    std::vector<FinalJob> fj;
+    std::vector<std::vector<FinalRun>> runs(nw);
+    std::vector<std::vector<int>> members(nw);
+    for (int w = 0; w < nw; ++w) {
+        for (int i = 0; i < (int)rs.size(); ++i) {
+            const Round& R = rs[i];
+            if (R.wave != w) continue;
+            const spbp_ctx* c = R.c;
+            const size_t nv = (size_t)c->nx * c->nx * c->nx;
+            const unsigned hb = nblocks(c->nh), vb = nblocks(nv);
+            auto& rw = runs[w];
+            if (rw.empty() || (unsigned long long)rw.back().hb + hb > FINAL_LAUNCH_BLOCKS ||
+                (unsigned long long)rw.back().vb + vb > FINAL_LAUNCH_BLOCKS) {
+                FinalRun fr;
+                fr.j0 = (int)fj.size();
+                rw.push_back(fr);
+            }
+            FinalRun& fr = rw.back();
+            FinalJob f;
+            f.a = R.a;
+            f.work = c->work;
+            f.real = c->real;
+            f.out = R.out;
+            f.wkb = c->wkb;
+            f.psum = c->psum;
+            f.pcnt = c->pcnt;
+            f.mean = c->mean;
+            f.nx = c->nx;
+            f.N = c->N;
+            f.pix = 1.0f / (float)((double)c->N * c->N * c->N);   // PIX = 1.0/FLOAT(N^3) (fftw3.f:444)
+            f.hb0 = fr.hb;
+            f.vb0 = fr.vb;
+            f.vnb = vb;
+            fr.hb += hb;
+            fr.vb += vb;
+            ++fr.n;
+            fj.push_back(f);
+            members[w].push_back(i);
+        }
+    }
+    const int n = (int)fj.size();
+    // the event exists before any table does: it marks the table's last use
+    if (!t->fjobs_evt) HIPCHK(t, hipEventCreateWithFlags(&t->fjobs_evt, hipEventDisableTiming));
+    HIPCHK(t, hipEventSynchronize(t->fjobs_evt));   // the last call's launches have read the table
+    if (n > t->fjobs_cap) {
+        if (t->fjobs_h) { (void)hipHostFree(t->fjobs_h); t->fjobs_h = nullptr; }
+        if (t->fjobs_d) { (void)hipFree(t->fjobs_d); t->fjobs_d = nullptr; }
+        t->fjobs_cap = 0;
+        const int cap = std::max(n, 16);
+        HIPCHK(t, hipHostMalloc((void**)&t->fjobs_h, cap * sizeof(FinalJob), hipHostMallocDefault));
+        HIPCHK(t, dmalloc(&t->fjobs_d, cap * sizeof(FinalJob)));
+        t->fjobs_cap = cap;
+    }
+    std::memcpy(t->fjobs_h, fj.data(), n * sizeof(FinalJob));
+    TableUse use{t, st};
+    HIPCHK(t, hipMemcpyAsync(t->fjobs_d, t->fjobs_h, n * sizeof(FinalJob), hipMemcpyHostToDevice, st));
+    for (int w = 0; w < nw; ++w) {
+        for (const FinalRun& fr : runs[w]) normalize_kernel<<<fr.hb, TPB, 0, st>>>(t->fjobs_d + fr.j0, fr.n);
+        HIPCHK(t, hipGetLastError());
+        for (int i : members[w]) {
+            spbp_ctx* c = rs[i].c;
+            FFTCHK(c, hipfftSetStream(c->inv, st));
+            FFTCHK(c, hipfftExecC2R(c->inv, (hipfftComplex*)c->work, (hipfftReal*)c->real));
+        }
+        for (const FinalRun& fr : runs[w]) {
+            window_kernel<<<fr.vb, TPB, 0, st>>>(t->fjobs_d + fr.j0, fr.n);
+            shell_mean_kernel<<<fr.n, TPB, 0, st>>>(t->fjobs_d + fr.j0);
+            subtract_kernel<<<fr.vb, TPB, 0, st>>>(t->fjobs_d + fr.j0, fr.n);
+        }
+        HIPCHK(t, hipGetLastError());
+        for (int i : members[w]) {   // real-space symmetrisation after WINDKB2A (bp32f.f:278-303)
+            spbp_ctx* c = rs[i].c;
+            if (c->nsym <= 1) continue;
+            const size_t nv = (size_t)c->nx * c->nx * c->nx;
+            if (!c->symtmp) HIPCHK(c, dmalloc(&c->symtmp, nv * sizeof(float)));
+            HIPCHK(c, hipMemcpyAsync(c->symtmp, rs[i].out, nv * sizeof(float), hipMemcpyDeviceToDevice, st));
+            symvol_kernel<<<nblocks(nv), TPB, 0, st>>>(c->symtmp, rs[i].out, c->nx, c->sm_d, c->nsym);
+            HIPCHK(c, hipGetLastError());
+        }
+    }
+    return SPBP_OK;
+}
+
+// The requested volumes of njobs distinct contexts (validated by the caller).
+int finalize_jobs(const spbp_final_job* jobs, int njobs, hipStream_t st) {
+    int rc;
+    std::vector<Round> rs;
+    for (int j = 0; j < njobs; ++j) {
+        spbp_ctx* c = jobs[j].ctx;
+        if ((rc = ensure_final(c))) return rc;
+        // the last call's counters have reached the pinned words they share
+        if (c->stats_evt) HIPCHK(c, hipEventSynchronize(c->stats_evt));
+        c->res_stats[0] = c->res_stats[1] = c->res_stats[2] = 0;
+        c->stats_slot = -1;
+        const bool resolve = c->hist_ok && !(c->flags & SPBP_PLAIN_THRESHOLD) && !c->hist.empty() &&
+                             c->hist.size() <= (size_t)INT32_MAX;
+        float* outs[3] = {jobs[j].out_full, jobs[j].out_h1, jobs[j].out_h2};
+        int wave = 0;
+        for (int q = 0; q < 3; ++q) {
+            if (!outs[q]) continue;
+            Round R = make_round(c, q - 1, q);
+            R.out = outs[q];
+            R.resolve = resolve;
+            R.wave = wave++;
+            if (resolve && (rc = ensure_res(c, R.res(), 4096))) return rc;
+            rs.push_back(R);
+        }
+        if (wave == 0 || !resolve) continue;
+        if ((rc = upload_history(c, st))) return rc;
+        if (!c->tmask) {
+            HIPCHK(c, dmalloc(&c->tmask, (c->nh + 3) / 4 * 4));
+            c->tmask_dirty = true;
+        }
+        if (c->tmask_dirty) {
+            fill_kernel<<<nblocks((c->nh + 3) / 4), TPB, 0, st>>>((float*)c->tmask, (c->nh + 3) / 4, 0.0f);   // zero bits
+            HIPCHK(c, hipGetLastError());
+            c->tmask_dirty = false;
+        }
+    }
+    if (rs.empty()) return SPBP_OK;
+    // every timing context records the batch's spans
+    std::vector<std::unique_ptr<Timed>> span;
+    for (int j = 0; j < njobs; ++j) span.emplace_back(new Timed(jobs[j].ctx, SPBP_K_RESOLVE, 1, st));
+    if ((rc = resolve_rounds(rs, st))) return rc;
+    span.clear();
+    // the last volume of each context leaves its counters for spbp_threshold_stats
+    for (int i = 0; i < (int)rs.size(); ++i) {
+        Round& R = rs[i];
+        spbp_ctx* c = R.c;
+        if (R.resolve && R.nv > 0) {
+            R.a.mask = c->tmask;
+            R.a.mshift = 2 * R.slot;
+        }
+        if (i + 1 < (int)rs.size() && rs[i + 1].c == c) continue;
+        c->res_stats[0] = R.nv;
+        if (R.resolve && R.nv > 0) {
+            if (!c->stats_evt) HIPCHK(c, hipEventCreateWithFlags(&c->stats_evt, hipEventDisableTiming));
+            HIPCHK(c, hipMemcpyAsync(R.res().h, R.res().ctr, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+            HIPCHK(c, hipEventRecord(c->stats_evt, st));
+            c->stats_slot = R.slot;
+        }
+    }
+    for (int j = 0; j < njobs; ++j) span.emplace_back(new Timed(jobs[j].ctx, SPBP_K_FINALIZE, 1, st));
+    if ((rc = run_waves(rs, st))) return rc;
+    span.clear();
+    // clear the decisions for the next finalisation
+    bool cleared = true;
+    for (const Round& R : rs) {
+        if (!(R.resolve && R.nv > 0)) continue;
+        mask_clear_kernel<<<(unsigned)((R.nv + RES_TPB - 1) / RES_TPB), RES_TPB, 0, st>>>(R.res().list, R.nv, R.c->tmask);
+        cleared = cleared && hipGetLastError() == hipSuccess;
+    }
+    for (const Round& R : rs)
+        if (R.resolve && R.nv > 0) R.c->tmask_dirty = !cleared;
     return SPBP_OK;
 }
 
@@ -1208,9 +1446,16 @@ int spbp_destroy(spbp_ctx* c) {
     }
     void* dev_ptrs[] = {c->tab, c->padbuf, c->spec, c->vmax, c->geom_d, c->work, c->real, c->wkb, c->psum, c->pcnt, c->mean, c->flag,
                         c->bcnt, c->boff, c->bioff, c->bcur, c->bscan, c->bnext, c->blist, c->bikey, c->jobs_d,
-                        c->sm_d, c->symtmp, c->rotbuf, c->rprm_d, c->hist_d, c->tab64, c->tmask, c->rlist, c->rcnt, c->tw, c->fmean,
-                        c->roff, c->rbsum, c->rhits, c->rtcnt, c->rtoff, c->rterms, c->rsums, c->rctr};
+                        c->sm_d, c->symtmp, c->rotbuf, c->rprm_d, c->hist_d, c->tab64, c->tmask, c->tw, c->fmean, c->fjobs_d};
     for (void* p : dev_ptrs) if (p) (void)hipFree(p);
+    if (c->stats_evt) { (void)hipEventSynchronize(c->stats_evt); (void)hipEventDestroy(c->stats_evt); }
+    if (c->fjobs_evt) { (void)hipEventSynchronize(c->fjobs_evt); (void)hipEventDestroy(c->fjobs_evt); }
+    if (c->fjobs_h) (void)hipHostFree(c->fjobs_h);
+    for (Res& r : c->res) {
+        void* rp[] = {r.list, r.cnt, r.off, r.bsum, r.hits, r.tcnt, r.toff, r.terms, r.sums, r.ctr};
+        for (void* p : rp) if (p) (void)hipFree(p);
+        if (r.h) (void)hipHostFree(r.h);
+    }
     if (c->geom_h) (void)hipHostFree(c->geom_h);
     if (c->jobs_h) (void)hipHostFree(c->jobs_h);
     if (c->rprm_evt) (void)hipEventSynchronize(c->rprm_evt);
@@ -1843,15 +2088,24 @@ int spbp_insert_rb_host(spbp_ctx* c, const void* imgs_host, int dtype, int nimg,
 
 int spbp_finalize(spbp_ctx* c, float* out_full, float* out_h1, float* out_h2, void* stream) {
     if (!c) return fail(nullptr, SPBP_ERR_ARG, "NULL context");
-    if ((out_h1 || out_h2) && c->nvol < 2) return fail(c, SPBP_ERR_ARG, "sample volumes need SPBP_TWO_HALVES");
-    DeviceGuard dg(c->device);
-    int rc = ensure_final(c);
-    if (rc) return rc;
-    hipStream_t st = (hipStream_t)stream;
-    if (out_full && (rc = finalize_one(c, -1, out_full, st))) return rc;
-    if (out_h1 && (rc = finalize_one(c, 0, out_h1, st))) return rc;
-    if (out_h2 && (rc = finalize_one(c, 1, out_h2, st))) return rc;
-    return SPBP_OK;
+    spbp_final_job job{c, out_full, out_h1, out_h2};
+    return spbp_finalize_batch(&job, 1, stream);
+}
+
+int spbp_finalize_batch(const spbp_final_job* jobs, int njobs, void* stream) {
+    if (!jobs || njobs < 1 || njobs > MAX_JOBS)
+        return fail(nullptr, SPBP_ERR_ARG, "batch of %d finalisations (1..%d)", njobs, MAX_JOBS);
+    for (int j = 0; j < njobs; ++j) {
+        spbp_ctx* c = jobs[j].ctx;
+        if (!c) return fail(nullptr, SPBP_ERR_ARG, "finalisation %d has no context", j);
+        if (c->device != jobs[0].ctx->device) return fail(c, SPBP_ERR_ARG, "finalisation %d is on another device", j);
+        for (int k = 0; k < j; ++k)
+            if (jobs[k].ctx == c) return fail(c, SPBP_ERR_ARG, "finalisations %d and %d share a context", k, j);
+        if ((jobs[j].out_h1 || jobs[j].out_h2) && c->nvol < 2)
+            return fail(c, SPBP_ERR_ARG, "sample volumes need SPBP_TWO_HALVES");
+    }
+    DeviceGuard dg(jobs[0].ctx->device);
+    return finalize_jobs(jobs, njobs, (hipStream_t)stream);
 }
 
 int spbp_finalize_host(spbp_ctx* c, float* out_full, float* out_h1, float* out_h2, void* stream) {
@@ -2074,11 +2328,15 @@ int spbp_reference_weights(spbp_ctx* c, int which, const long long* voxels, int 
     DeviceGuard dg(c->device);
     hipStream_t st = (hipStream_t)stream;
     int rc;
-    if ((rc = upload_history(c, st)) || (rc = grow(c, c->rlist, c->rlist_cap, (size_t)n, "voxel list"))) return rc;
-    HIPCHK(c, hipMemcpyAsync(c->rlist, voxels, (size_t)n * sizeof(long long), hipMemcpyHostToDevice, st));
+    // slot 0's buffers; its pinned words may still receive a finalisation's counters
+    if (c->stats_evt) HIPCHK(c, hipEventSynchronize(c->stats_evt));
+    Round R = make_round(c, which, 0);
+    R.nv = n;
+    if ((rc = upload_history(c, st)) || (rc = ensure_res(c, R.res(), (size_t)n))) return rc;
+    HIPCHK(c, hipMemcpyAsync(R.res().list, voxels, (size_t)n * sizeof(long long), hipMemcpyHostToDevice, st));
     double* d = nullptr;
     HIPCHK(c, dmalloc(&d, (size_t)n * sizeof(double)));
-    rc = resolve_list(c, which, make_acc(c, which), n, d, st);
+    rc = resolve_listed({&R}, d, st);
     if (rc == SPBP_OK && (hipMemcpyAsync(w_out, d, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
                           hipStreamSynchronize(st) != hipSuccess))
         rc = fail(c, SPBP_ERR_DEVICE, "reference weights read-back failed");
@@ -2232,9 +2490,17 @@ int spbp_scatter_noalloc(int nx, int flags, const float* bi, int nimg, const dou
 
 int spbp_threshold_stats(const spbp_ctx* c, long long* listed, long long* changed, long long* inconsistent) {
     if (!c) return fail(nullptr, SPBP_ERR_ARG, "NULL context");
-    if (listed) *listed = c->res_stats[0];
-    if (changed) *changed = c->res_stats[1];
-    if (inconsistent) *inconsistent = c->res_stats[2];
+    long long v[3] = {c->res_stats[0], c->res_stats[1], c->res_stats[2]};
+    if (c->stats_slot >= 0) {   // the counters' read-back, enqueued by the finalisation
+        DeviceGuard dg(c->device);
+        if (hipEventSynchronize(c->stats_evt) != hipSuccess)
+            return fail(nullptr, SPBP_ERR_DEVICE, "threshold counters read-back failed");
+        v[1] = (long long)c->res[c->stats_slot].h[1];
+        v[2] = (long long)c->res[c->stats_slot].h[2];
+    }
+    if (listed) *listed = v[0];
+    if (changed) *changed = v[1];
+    if (inconsistent) *inconsistent = v[2];
     return SPBP_OK;
 }
 
