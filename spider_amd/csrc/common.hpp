@@ -62,6 +62,31 @@ __global__ __launch_bounds__(TPB) void finite_kernel(const T* __restrict__ p, si
 
 inline unsigned nblocks(size_t n) { return (unsigned)((n + TPB - 1) / TPB); }
 
+// q = n / d and r = n mod d for n < 2^52 and d >= 1, given inv = 1.0 / d
+// rounded to double: the product n * inv is within 1 of n / d, so one
+// correction step makes the quotient exact (no 64-bit integer division).
+__host__ __device__ inline void fast_divmod(unsigned long long n, unsigned d, double inv, unsigned long long& q,
+                                            unsigned& r) {
+    unsigned long long qq = (unsigned long long)((double)n * inv);
+    long long rr = (long long)(n - qq * d);
+    if (rr < 0) { --qq; rr += d; }
+    else if (rr >= (long long)d) { ++qq; rr -= d; }
+    q = qq;
+    r = (unsigned)rr;
+}
+
+// Half-grid index ((k N + j) H + i, H = NX + 1) -> (i, j, k).
+__host__ __device__ inline void half_coords(unsigned long long idx, int H, int N, double invH, double invN, int& i,
+                                            int& j, int& k) {
+    unsigned long long r, kk;
+    unsigned ii, jj;
+    fast_divmod(idx, (unsigned)H, invH, r, ii);
+    fast_divmod(r, (unsigned)N, invN, kk, jj);
+    i = (int)ii;
+    j = (int)jj;
+    k = (int)kk;
+}
+
 // ------------------------------------------------------------------ compensated weights
 // W += x with the rounding error of that very addition kept in Wc: the
 // returning atomic gives the exact prior value, so the error of

@@ -88,6 +88,7 @@ struct FinalJob {
     float pix;             // 1 / N^3
     unsigned hb0, vb0;     // first block in the half-grid / volume grid
     unsigned vnb;          // volume blocks
+    double invH, invN, invL;   // 1 / (NX + 1), 1 / N, 1 / NX
 };
 
 // Job of block b: the last of the run whose first block is <= b.
@@ -108,9 +109,8 @@ __global__ __launch_bounds__(TPB) void normalize_kernel(const FinalJob* __restri
     const size_t total = (size_t)H * N * N;
     const size_t idx = (size_t)(blockIdx.x - f.hb0) * blockDim.x + threadIdx.x;
     if (idx >= total) return;
-    const int i = (int)(idx % H);
-    const size_t r = idx / H;
-    const int j = (int)(r % N), k = (int)(r / N);
+    int i, j, k;
+    half_coords(idx, H, N, f.invH, f.invN, i, j, k);
     float2 v = norm_val(f.a, nx, N, i, j, k);
     if (i == 0 || i == nx) {
         // c2r reads the kx = 0 and kx = N/2 planes as their Hermitian part
@@ -130,9 +130,8 @@ __global__ __launch_bounds__(TPB) void window_kernel(const FinalJob* __restrict_
     double s = 0.0;
     unsigned c = 0;
     if (idx < total) {
-        const int i = (int)(idx % L);
-        const size_t t = idx / L;
-        const int j = (int)(t % L), k = (int)(t / L);
+        int i, j, k;
+        half_coords(idx, L, L, f.invL, f.invL, i, j, k);
         const int ip = (N - L) / 2 + (L % 2);
         const int cc = L / 2;
         float v = f.real[((size_t)(ip + k) * N + (ip + j)) * N + (ip + i)] * f.pix;   // FMRS_3 scale
@@ -165,9 +164,8 @@ __global__ __launch_bounds__(TPB) void subtract_kernel(const FinalJob* __restric
     const size_t total = (size_t)L * L * L;
     const size_t idx = (size_t)(blockIdx.x - f.vb0) * blockDim.x + threadIdx.x;
     if (idx >= total) return;
-    const int i = (int)(idx % L);
-    const size_t t = idx / L;
-    const int j = (int)(t % L), k = (int)(t / L);
+    int i, j, k;
+    half_coords(idx, L, L, f.invL, f.invL, i, j, k);
     const int cc = L / 2;
     const int lr = (i - cc) * (i - cc) + (j - cc) * (j - cc) + (k - cc) * (k - cc);
     if (lr <= cc * cc) f.out[idx] = f.out[idx] - f.mean[0];

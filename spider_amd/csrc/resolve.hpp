@@ -111,19 +111,19 @@ __device__ inline bool item_target(const long long* __restrict__ list, long long
 }
 
 // Voxels whose weight (as the NRMW2 test reads it) lies within `band` of 0.1.
-__global__ __launch_bounds__(RES_TPB) void thresh_list_kernel(Acc a, int nx, int N, double band,
+__global__ __launch_bounds__(RES_TPB) void thresh_list_kernel(Acc a, int nx, int N, double invH, double invN, double band,
                                                               long long* __restrict__ list, long long cap,
                                                               unsigned long long* __restrict__ count) {
     const int H = nx + 1;
     const size_t total = (size_t)H * N * N;
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= total) return;
-    const int i = (int)(idx % H);
-    const size_t r = idx / H;
+    int i, j, k;
+    half_coords(idx, H, N, invH, invN, i, j, k);
     float2 x;
     float w;
     double wd;
-    load_sym(a, nx, N, i, (int)(r % N), (int)(r / N), x, w, wd);
+    load_sym(a, nx, N, i, j, k, x, w, wd);
     const double v = a.Wc[0] ? wd : (double)w;
     if (fabs(v - (double)0.1f) <= band) {
         const unsigned long long s = atomicAdd(count, 1ull);

@@ -108,3 +108,39 @@ def test_cang_and_asta_live_reference():
         img = rng.standard_normal((L, L)).astype(np.float32)
         aba, dklp = R.asta_d(img, float(L // 2))
         assert oracle.asta_mean(img) == aba / dklp
+
+
+@pytest.mark.skipif(oracle.ref() is None, reason="reference routines not built here (oracle/_ref)")
+def test_restatement_speed_against_compiled_oneline():
+    """Calibration of the CPU baseline: the restatement's serial insertion
+    against the reference's own ONELINE (flang -O2, one call per row J as
+    bp32f.f:873-888 makes them) over the same spectra of 64^2 projections
+    (configuration 1's shape), one thread each; the sums must be bit-identical.
+    SPBP_CALIBRATE_NPROJ sets the count (default 150; the figure quoted for
+    the baseline used 2000).  Prints both rates and their ratio."""
+    import os
+    import time
+    from spider_amd import synth
+    R = oracle.ref()
+    nx, m = 64, int(os.environ.get("SPBP_CALIBRATE_NPROJ", "150"))
+    rng = np.random.default_rng(2026)
+    imgs = rng.standard_normal((m, nx, nx)).astype(np.float32)
+    ang = synth.uniform_angles(m, rng).astype(np.float32)
+    specs = [oracle.spectrum(im) for im in imgs]
+    dms = [R.cang(*a) for a in ang]
+    tab, fltb = R.fillbessil(2 * nx)[:2]
+    X1, W1 = oracle.empty_grid(nx)
+    X2, W2 = oracle.empty_grid(nx)
+    t0 = time.perf_counter()
+    for bi, dm in zip(specs, dms):
+        for J in range(-nx + 1, nx + 1):
+            R.oneline(J, X1, W1, bi, dm)
+    t_ref = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for bi, dm in zip(specs, dms):
+        oracle.insert(X2, W2, bi, dm, tab, fltb)
+    t_port = time.perf_counter() - t0
+    np.testing.assert_array_equal(W1, W2)
+    np.testing.assert_array_equal(X1, X2)
+    print(f"\ncalibration, {m} projections of {nx}^2: compiled ONELINE {t_ref:.2f} s ({m / t_ref:.0f} proj/s), "
+          f"restatement {t_port:.2f} s ({m / t_port:.0f} proj/s), ratio {t_port / t_ref:.2f}")
