@@ -76,6 +76,8 @@ extern "C" {
 #define SPBP_STRAT_AUTO   0   /* the library's choice                                     */
 #define SPBP_STRAT_DIRECT 1   /* one thread per sample, FP32 atomics into the half grid   */
 #define SPBP_STRAT_BRICK  2   /* LDS-binned bricks: workgroup owns a brick of the grid     */
+#define SPBP_STRAT_GATHER 3   /* voxel-driven gather: thread owns a voxel, sums in the
+                                 serial reference's order (bit-identical to it, no atomics) */
 
 typedef struct {
     int nx;      /* image edge NX; volume NX^3, padded Fourier grid N = 2 NX          */
@@ -261,6 +263,10 @@ int spbp_insert_rb_host(spbp_ctx* ctx, const void* imgs_host, int dtype, int nim
 /* RTSQ alone (scale 1): nimg NX x NX device images (fp32 / bf16) -> out
  * (device fp32), params host nimg x 3 as above.  Synchronises the stream. */
 int spbp_rtsq(const void* imgs, int dtype, int nimg, int nx, const float* params, float* out, void* stream);
+/* The same on host memory (fp32 in and out, staged through device memory;
+ * synchronous): the transformed images that RB 3F / RB 32F write under the
+ * template answer (bp32f.f:517-527, 817-827, WRTVOL of PROJ). */
+int spbp_rtsq_host(const float* imgs_host, int nimg, int nx, const float* params, float* out_host);
 /* Symmetries (BP 3F / BP 32F "SYMMETRIES DOC", bp32f.f:133-172): nsym doc
  * rows (PSI, THETA, PHI) -> matrices SM by CANG (BUILDS, builds.f:35-63).
  * With nsym > 1 every image is inserted once per DMS = MATMUL(SM_k, DM)
@@ -273,7 +279,7 @@ int spbp_rtsq(const void* imgs, int dtype, int nimg, int nx, const float* params
 int spbp_set_symmetries(spbp_ctx* ctx, int nsym, const double* angles_deg);
 /* ---- batching, tuning and graphs (no counterpart in the reference: its
  * projection loop is one OpenMP loop per operation, bp32f.f:802-913) ---- */
-/* Plan = insertion strategy (SPBP_STRAT_DIRECT / _BRICK), brick shape
+/* Plan = insertion strategy (SPBP_STRAT_DIRECT / _BRICK / _GATHER), brick shape
  * (0: 16x16x8, 1: 16x8x8, 2: 8x8x8, 3: 16x12x8, 4: 16x16x10, 5: 16x16x12,
  * 6: 16x16x13 voxels = the LDS bin, every sample evaluated in each bin its
  * taps reach; 7, the default: owner-computes bins, a 16x16x13 LDS region
@@ -283,8 +289,14 @@ int spbp_set_symmetries(spbp_ctx* ctx, int nsym, const double* angles_deg);
  * sections halves it, down to 16, so its busiest bricks still split).  spbp_tune times every plan on a sample of
  * up to 1024 of the given images (into scratch grids: the accumulators are
  * untouched) and keeps the fastest; spbp_get_plan reports the plan and, when
- * ms_per_proj is not NULL, the 33 measured ms per projection ([shape][C = 16,
- * 32, 64, 128], then direct). */
+ * ms_per_proj is not NULL, the 34 measured ms per projection ([shape][C = 16,
+ * 32, 64, 128], then direct, then gather).  The gather plan
+ * (SPBP_STRAT_GATHER) is the scatter's transpose: each thread owns a voxel
+ * and adds the terms of the chunk's sections in the serial reference's order
+ * (projections, symmetries, rows J, columns I, direct before reflected tap,
+ * bp32f.f:866-898, oneline.f:53-152), so X and W are the serial reference's
+ * fp32 sums bit for bit and identical from run to run; it does several times
+ * the arithmetic of the scatter plans. */
 int spbp_tune(spbp_ctx* ctx, const void* imgs, int dtype, int nimg, const double* angles_deg, const float* dm,
               void* stream);
 int spbp_get_plan(const spbp_ctx* ctx, int* strategy, int* shape, int* brick_proj, float* ms_per_proj);

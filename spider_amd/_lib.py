@@ -48,6 +48,7 @@ CAL_GLOBAL_F32 = 1
 STRAT_AUTO = 0
 STRAT_DIRECT = 1
 STRAT_BRICK = 2
+STRAT_GATHER = 3
 
 
 class SpbpError(RuntimeError):
@@ -114,6 +115,7 @@ def lib():
         L.spbp_insert_rb.argtypes = [vp, vp, C.c_int, C.c_int, vp, vp, vp, vp]
         L.spbp_rtsq.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp]
         L.spbp_insert_rb_host.argtypes = [vp, vp, C.c_int, C.c_int, vp, vp, vp, vp]
+        L.spbp_rtsq_host.argtypes = [vp, C.c_int, C.c_int, vp, vp]
         L.spbp_get_plan.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), vp]
         L.spbp_set_plan.argtypes = [vp, C.c_int, C.c_int, C.c_int]
         L.spbp_insert_batch.argtypes = [vp, C.c_int, vp]
@@ -134,7 +136,7 @@ def lib():
                      "spbp_gather", "spbp_scatter", "spbp_kb_table", "spbp_rotations", "spbp_tune",
                      "spbp_get_plan", "spbp_set_plan", "spbp_insert_batch", "spbp_graph_create",
                      "spbp_graph_launch", "spbp_graph_destroy", "spbp_set_symmetries", "spbp_insert_rb",
-                     "spbp_rtsq", "spbp_insert_rb_host", "spbp_threshold_stats", "spbp_reference_weights",
+                     "spbp_rtsq", "spbp_rtsq_host", "spbp_insert_rb_host", "spbp_threshold_stats", "spbp_reference_weights",
                      "spbp_calibrate", "spbp_scatter_images", "spbp_gather_images", "spbp_debug_fail_alloc", "spbp_scatter_noalloc"):
             getattr(L, name).restype = C.c_int
         _LIB = L

@@ -53,7 +53,8 @@ class Backprojector:
                 (_lib.TWOSUM_W if twosum else 0) | (_lib.CHECK_FINITE if check_finite else 0) | \
                 (_lib.TIMING if timing else 0) | (_lib.GEOM_F64 if geom_f64 else 0) | \
                 (_lib.PLAIN_THRESHOLD if plain_threshold else 0) | (_lib.HIPFFT if hipfft else 0)
-        strat = {"auto": _lib.STRAT_AUTO, "direct": _lib.STRAT_DIRECT, "brick": _lib.STRAT_BRICK}[strategy]
+        strat = {"auto": _lib.STRAT_AUTO, "direct": _lib.STRAT_DIRECT, "brick": _lib.STRAT_BRICK,
+                 "gather": _lib.STRAT_GATHER}[strategy]
         opts = _lib.Opts(int(nx), flags, self.device.index, int(batch), strat, int(brick_proj))
         ctx = C.c_void_p()
         L = _lib.lib()
@@ -318,16 +319,22 @@ class Backprojector:
 
     def plan(self) -> dict:
         st, sh, cp = C.c_int(), C.c_int(), C.c_int()
-        ms = (C.c_float * (len(_lib.SHAPES) * len(_lib.TUNE_C) + 1))()
+        nb = len(_lib.SHAPES) * len(_lib.TUNE_C)
+        ms = (C.c_float * (nb + 2))()
         self._check(_lib.lib().spbp_get_plan(self._ctx, C.byref(st), C.byref(sh), C.byref(cp), ms))
-        table = np.array(ms[:-1], np.float64).reshape(len(_lib.SHAPES), len(_lib.TUNE_C))
-        return {"strategy": {_lib.STRAT_DIRECT: "direct", _lib.STRAT_BRICK: "brick"}[st.value],
+        table = np.array(ms[:nb], np.float64).reshape(len(_lib.SHAPES), len(_lib.TUNE_C))
+        return {"strategy": {_lib.STRAT_DIRECT: "direct", _lib.STRAT_BRICK: "brick",
+                             _lib.STRAT_GATHER: "gather"}[st.value],
                 "brick": _lib.SHAPES[sh.value], "owner_computes": _lib.SHAPE_OWN[sh.value], "shape": sh.value,
                 "brick_proj": cp.value,
-                "ms_per_proj": table, "direct_ms_per_proj": float(ms[-1])}
+                "ms_per_proj": table, "direct_ms_per_proj": float(ms[nb]), "gather_ms_per_proj": float(ms[nb + 1])}
 
     def set_plan(self, strategy="brick", shape: int = 0, brick_proj: int = 0) -> None:
-        strat = {"auto": _lib.STRAT_AUTO, "direct": _lib.STRAT_DIRECT, "brick": _lib.STRAT_BRICK}[strategy]
+        """strategy: "brick" (LDS bins of brick shape `shape`), "direct"
+        (fp32 atomics per tap), "gather" (voxel-driven, the serial
+        reference's summation order) or "auto"."""
+        strat = {"auto": _lib.STRAT_AUTO, "direct": _lib.STRAT_DIRECT, "brick": _lib.STRAT_BRICK,
+                 "gather": _lib.STRAT_GATHER}[strategy]
         self._check(_lib.lib().spbp_set_plan(self._ctx, strat, int(shape), int(brick_proj)))
 
     def _job(self, imgs, angles=None, dm=None, half=None):
@@ -499,21 +506,35 @@ def bp32f_files(template, numbers, angdoc, symdoc, out_full, out_h1, out_h2, ext
     return vols
 
 
-def rb3f_files(template, numbers, angdoc, symdoc, out, ext: str = "", device=None, **kw):
+def _write_transformed(rot_out, numbers, imgs, rot, ext):
+    """The transformed-images output of `RB 3F` / `RB 32F` (bp32f.f:517-527,
+    826-827): every image after its RTSQ, under the template rot_out
+    ('*' or None: not written), with the input's file numbers."""
+    from . import spiderio
+    if rot_out in (None, "", "*"):
+        return
+    spiderio.write_images(rot_out, numbers, rtsq(imgs, rot).cpu().numpy(), ext)
+
+
+def rb3f_files(template, numbers, angdoc, symdoc, out, ext: str = "", device=None, rot_out=None, **kw):
     """`RB 3F`: as `BP 3F`, each image first rotated by its in-plane angle and
-    shifted (doc registers 6, 7, 8; RTSQ, bp32f.f:805-837)."""
+    shifted (doc registers 6, 7, 8; RTSQ, bp32f.f:805-837); rot_out is the
+    template for the transformed images (None or '*': not written)."""
     from . import spiderio
     imgs, ang, rot = _load_series(template, numbers, angdoc, symdoc, ext, device, inplane=True)
     vol = bp3f(imgs, ang, inplane=rot, symmetries=read_symmetries(symdoc, ext), **kw).cpu().numpy()
     spiderio.write_image(out, vol)
+    _write_transformed(rot_out, numbers, imgs, rot, ext)
     return vol
 
 
 def rb32f_files(template, numbers, angdoc, symdoc, out_full, out_h1, out_h2, ext: str = "", seed=None,
-                half=None, device=None, **kw):
-    """`RB 32F`: `BP 32F` with the RTSQ pre-rotation of `RB 3F`."""
+                half=None, device=None, rot_out=None, **kw):
+    """`RB 32F`: `BP 32F` with the RTSQ pre-rotation of `RB 3F` and its
+    transformed-images output."""
     from . import spiderio
     imgs, ang, rot = _load_series(template, numbers, angdoc, symdoc, ext, device, inplane=True)
+    _write_transformed(rot_out, numbers, imgs, rot, ext)
     full, h1, h2 = bp32f(imgs, ang, half=half, seed=seed, inplane=rot, symmetries=read_symmetries(symdoc, ext), **kw)
     vols = [v.cpu().numpy() for v in (full, h1, h2)]
     for path, v in zip((out_full, out_h1, out_h2), vols):
@@ -537,10 +558,16 @@ def bp3f_merge_files(partials, out, device=None, symdoc=None, ext: str = "", **k
     symmetries doc the merged volume is symmetrised by SYMVOL (bp3f.f:700-722)."""
     from . import spiderio
     torch = _torch()
+    # every shard is read and checked before any device work, so a lost,
+    # truncated or mismatched one stops the merge with nothing written (the
+    # reference stops at the file it cannot open, bp3f.f:646-685)
     pairs = [spiderio.read_partial(p) for p in partials]
     if not pairs:
         raise SpbpError(_lib.ERR_ARG, "no partial volumes")
     nx = pairs[0][1].shape[2] - 1
+    for p, (X, W) in zip(partials, pairs):
+        if W.shape != pairs[0][1].shape:
+            raise spiderio.SpiderFormatError(f"{p}: partial volume size {W.shape} differs from {pairs[0][1].shape}")
     dev = torch.device("cuda", torch.cuda.current_device() if device is None else int(device))
     with Backprojector(nx, device=dev.index, symmetries=read_symmetries(symdoc, ext), **kw) as b:
         for X, W in pairs:

@@ -46,6 +46,7 @@
 #include "finalize.hpp"
 #include "hostmath.hpp"
 #include "insert.hpp"
+#include "insert_gather.hpp"
 #include "project.hpp"
 #include "resolve.hpp"
 #include "rotate.hpp"
@@ -161,6 +162,7 @@ struct spbp_ctx {
     hipEvent_t rprm_evt = nullptr;
     float tune_ms[NSHAPES][NCVALS] = {};   // spbp_tune: ms per projection of each brick plan
     float tune_direct_ms = 0.f;
+    float tune_gather_ms = 0.f;
     BrickJob* jobs_d = nullptr;   // brick-kernel job table
     BrickJob* jobs_h = nullptr;   // pinned staging
     int jobs_cap = 0;
@@ -682,12 +684,42 @@ int launch_direct(spbp_ctx* c, int img0, const Geom* geom_d, int ng, const Targe
     return SPBP_OK;
 }
 
+// The same with the voxel-driven gather (reference-order sums).
+int launch_gather(spbp_ctx* c, int img0, const Geom* geom_d, int ng, const Targets& t, hipStream_t st, long units) {
+    Timed tm(c, SPBP_K_INSERT, units, st);
+    const dim3 grid((c->H + GBX - 1) / GBX, (c->N + GBY - 1) / GBY, (c->N + GBZ - 1) / GBZ);
+    const bool f64 = (c->flags & SPBP_GEOM_F64) != 0, wgt = t.W0 != nullptr, two = t.Wc0 != nullptr;
+#define SPBP_GATHER(P, W, T)                                                                                          \
+    insert_gather_kernel<P, W, T><<<grid, GTPB, 0, st>>>(c->spec, img0, geom_d, ng, c->nx, c->N, c->tab, c->fltb,   \
+                                                       t.X0, t.W0, t.X1, t.W1, t.Wc0, t.Wc1)
+    if (f64) {
+        if (!wgt) SPBP_GATHER(double, false, false);
+        else if (two) SPBP_GATHER(double, true, true);
+        else SPBP_GATHER(double, true, false);
+    } else {
+        if (!wgt) SPBP_GATHER(float, false, false);
+        else if (two) SPBP_GATHER(float, true, true);
+        else SPBP_GATHER(float, true, false);
+    }
+#undef SPBP_GATHER
+    HIPCHK(c, hipGetLastError());
+    return SPBP_OK;
+}
+
+// Strategies that run their own kernel per context (not the brick queue).
+bool own_kernel(const spbp_ctx* c) { return c->strategy == SPBP_STRAT_DIRECT || c->strategy == SPBP_STRAT_GATHER; }
+
+int launch_own(spbp_ctx* c, int img0, const Geom* geom_d, int ng, const Targets& t, hipStream_t st, long units) {
+    return c->strategy == SPBP_STRAT_GATHER ? launch_gather(c, img0, geom_d, ng, t, st, units)
+                                            : launch_direct(c, img0, geom_d, ng, t, st, units);
+}
+
 // ONELINE over ng sections (geometry at geom_d) of images img0.. whose raw
 // R2C spectra are in c->spec, with the context's strategy; `units` = images.
 int run_insert(spbp_ctx* c, int img0, const Geom* geom_d, int ng, const Targets& t, hipStream_t st, long units) {
     if (!c->spec || !c->vmax || ng > c->fwd_batch * c->nsym)
         return fail(c, SPBP_ERR_ARG, "chunk buffers not allocated for %d sections", ng);
-    if (c->strategy == SPBP_STRAT_DIRECT) return launch_direct(c, img0, geom_d, ng, t, st, units);
+    if (own_kernel(c)) return launch_own(c, img0, geom_d, ng, t, st, units);
     int rc = build_lists(c, geom_d, ng, st);
     if (rc) return rc;
     const BrickJob job = brick_job(c, img0, geom_d, ng, t);
@@ -1021,8 +1053,8 @@ int phase_list(Round& R, hipStream_t st) {
     spbp_ctx* c = R.c;
     Res& r = R.res();
     fill_kernel<<<1, TPB, 0, st>>>((float*)r.ctr, 8, 0.0f);   // zero bits
-    thresh_list_kernel<<<nblocks(c->nh), TPB, 0, st>>>(R.a, c->nx, c->N, THRESH_BAND, r.list, (long long)r.list_cap,
-                                                       r.ctr);
+    thresh_list_kernel<<<nblocks(c->nh), TPB, 0, st>>>(R.a, c->nx, c->N, 1.0 / (c->nx + 1), 1.0 / c->N, THRESH_BAND,
+                                                       r.list, (long long)r.list_cap, r.ctr);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(r.h, r.ctr, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
     return SPBP_OK;
@@ -1206,6 +1238,9 @@ int run_waves(std::vector<Round>& rs, hipStream_t st) {
             f.nx = c->nx;
             f.N = c->N;
             f.pix = 1.0f / (float)((double)c->N * c->N * c->N);   // PIX = 1.0/FLOAT(N^3) (fftw3.f:444)
+            f.invH = 1.0 / (c->nx + 1);
+            f.invN = 1.0 / c->N;
+            f.invL = 1.0 / c->nx;
             f.hb0 = fr.hb;
             f.vb0 = fr.vb;
             f.vnb = vb;
@@ -1375,8 +1410,8 @@ int spbp_create(const spbp_opts* o, spbp_ctx** out) {
         const size_t per = (own ? 0 : (size_t)c->N * c->N * 4) + (size_t)c->N * c->H * 8;
         c->batch = (int)std::max<size_t>(16, std::min<size_t>(32768, ((size_t)8192 << 20) / per));
     }
-    c->strategy = o->strategy == SPBP_STRAT_DIRECT ? SPBP_STRAT_DIRECT : SPBP_STRAT_BRICK;
-    if (o->strategy < 0 || o->strategy > SPBP_STRAT_BRICK) { delete c; return fail(nullptr, SPBP_ERR_ARG, "unknown strategy %d", o->strategy); }
+    c->strategy = o->strategy == SPBP_STRAT_DIRECT || o->strategy == SPBP_STRAT_GATHER ? o->strategy : SPBP_STRAT_BRICK;
+    if (o->strategy < 0 || o->strategy > SPBP_STRAT_GATHER) { delete c; return fail(nullptr, SPBP_ERR_ARG, "unknown strategy %d", o->strategy); }
     if (o->brick_proj > 128) { delete c; return fail(nullptr, SPBP_ERR_ARG, "brick_proj %d above 128", o->brick_proj); }
     c->C = o->brick_proj > 0 ? o->brick_proj : 128;
     {
@@ -1669,6 +1704,11 @@ int spbp_tune(spbp_ctx* c, const void* imgs, int dtype, int nimg, const double* 
     c->strategy = SPBP_STRAT_DIRECT;
     rc = time_plan(c, c->geom_d, b, t, st, 1, &ms);
     c->tune_direct_ms = ms / b;
+    if (rc == SPBP_OK) {
+        c->strategy = SPBP_STRAT_GATHER;
+        rc = time_plan(c, c->geom_d, b, t, st, 1, &ms);
+        c->tune_gather_ms = ms / b;
+    }
     c->strategy = SPBP_STRAT_BRICK;
     for (int sh = 0; sh < NSHAPES && rc == SPBP_OK; ++sh) {
         if ((rc = configure_bricks(c, sh)) || (rc = ensure_lists(c, c->fwd_batch))) break;
@@ -1693,7 +1733,10 @@ int spbp_tune(spbp_ctx* c, const void* imgs, int dtype, int nimg, const double* 
         for (int k = 0; k < NCVALS; ++k)
             if (c->tune_ms[sh][k] < c->tune_ms[bs][bk]) { bs = sh; bk = k; }
     c->tuned = true;
-    c->strategy = c->tune_direct_ms < c->tune_ms[bs][bk] ? SPBP_STRAT_DIRECT : SPBP_STRAT_BRICK;
+    const float best_brick = c->tune_ms[bs][bk];
+    c->strategy = SPBP_STRAT_BRICK;
+    if (c->tune_direct_ms < best_brick && c->tune_direct_ms <= c->tune_gather_ms) c->strategy = SPBP_STRAT_DIRECT;
+    else if (c->tune_gather_ms < best_brick && c->tune_gather_ms < c->tune_direct_ms) c->strategy = SPBP_STRAT_GATHER;
     c->C = CVALS[bk];
     if ((rc = configure_bricks(c, bs))) return rc;
     return ensure_lists(c, c->fwd_batch);
@@ -1709,17 +1752,18 @@ int spbp_get_plan(const spbp_ctx* c, int* strategy, int* shape, int* brick_proj,
         for (int sh = 0; sh < NSHAPES; ++sh)
             for (int k = 0; k < NCVALS; ++k) ms_per_proj[sh * NCVALS + k] = c->tuned ? c->tune_ms[sh][k] : 0.f;
         ms_per_proj[NSHAPES * NCVALS] = c->tuned ? c->tune_direct_ms : 0.f;
+        ms_per_proj[NSHAPES * NCVALS + 1] = c->tuned ? c->tune_gather_ms : 0.f;
     }
     return SPBP_OK;
 }
 
 int spbp_set_plan(spbp_ctx* c, int strategy, int shape, int brick_proj) {
     if (!c) return fail(nullptr, SPBP_ERR_ARG, "NULL context");
-    if (strategy < SPBP_STRAT_AUTO || strategy > SPBP_STRAT_BRICK) return fail(c, SPBP_ERR_ARG, "unknown strategy %d", strategy);
+    if (strategy < SPBP_STRAT_AUTO || strategy > SPBP_STRAT_GATHER) return fail(c, SPBP_ERR_ARG, "unknown strategy %d", strategy);
     if (shape < 0 || shape >= NSHAPES) return fail(c, SPBP_ERR_ARG, "brick shape %d outside 0..%d", shape, NSHAPES - 1);
     if (brick_proj < 0 || brick_proj > 128) return fail(c, SPBP_ERR_ARG, "projections per work item %d outside 0..128", brick_proj);
     DeviceGuard dg(c->device);
-    c->strategy = strategy == SPBP_STRAT_DIRECT ? SPBP_STRAT_DIRECT : SPBP_STRAT_BRICK;
+    c->strategy = strategy == SPBP_STRAT_DIRECT || strategy == SPBP_STRAT_GATHER ? strategy : SPBP_STRAT_BRICK;
     if (brick_proj > 0) c->C = brick_proj;
     int rc = configure_bricks(c, shape);
     if (rc) return rc;
@@ -1759,6 +1803,7 @@ int prepare_batch(const spbp_job* jobs, int njobs) {
                 float m = c->tune_ms[sh][0];
                 for (int k = 1; k < NCVALS; ++k) m = std::min(m, c->tune_ms[sh][k]);
                 if (c->strategy == SPBP_STRAT_DIRECT) m = c->tune_direct_ms;
+                if (c->strategy == SPBP_STRAT_GATHER) m = c->tune_gather_ms;
                 tot += (double)m * jobs[j].nimg;
             }
             if (tot < best) { best = tot; shape = sh; }
@@ -1772,7 +1817,7 @@ int prepare_batch(const spbp_job* jobs, int njobs) {
 }
 
 // Host-side job table of round r: the brick jobs of the contexts that still
-// have a chunk in that round (direct-strategy contexts run their own kernel).
+// have a chunk in that round (direct- and gather-strategy contexts run their own kernels).
 // Depends only on device addresses, so it can be built before any work runs.
 int round_table(const spbp_job* jobs, int njobs, Geom* const* geom, int r, BrickJob* table, spbp_ctx** lead) {
     int n = 0;
@@ -1780,7 +1825,7 @@ int round_table(const spbp_job* jobs, int njobs, Geom* const* geom, int r, Brick
     for (int j = 0; j < njobs; ++j) {
         spbp_ctx* c = jobs[j].ctx;
         const int p0 = r * c->fwd_batch;
-        if (p0 >= jobs[j].nimg || c->strategy == SPBP_STRAT_DIRECT) continue;
+        if (p0 >= jobs[j].nimg || own_kernel(c)) continue;
         const Targets t{c->X[0], c->W[0], c->X[1], c->W[1], c->Wc[0], c->Wc[1]};
         const int b = std::min(c->fwd_batch, jobs[j].nimg - p0);
         table[n++] = brick_job(c, p0, geom[j] + (size_t)p0 * c->nsym, b * c->nsym, t);
@@ -1812,8 +1857,8 @@ int enqueue_batch(const spbp_job* jobs, int njobs, Geom* const* geom, BrickJob* 
             if (rc) return rc;
             const Targets t{c->X[0], c->W[0], c->X[1], c->W[1], c->Wc[0], c->Wc[1]};
             const Geom* gj = geom[j] + (size_t)p0 * c->nsym;
-            if (c->strategy == SPBP_STRAT_DIRECT) {
-                if ((rc = launch_direct(c, p0, gj, b * c->nsym, t, st, b))) return rc;
+            if (own_kernel(c)) {
+                if ((rc = launch_own(c, p0, gj, b * c->nsym, t, st, b))) return rc;
                 continue;
             }
             if ((rc = build_lists(c, gj, b * c->nsym, st))) return rc;
@@ -2040,6 +2085,28 @@ int spbp_rtsq(const void* imgs, int dtype, int nimg, int nx, const float* params
     (void)hipFree(d);
     if (e != hipSuccess) return fail(nullptr, SPBP_ERR_DEVICE, "rtsq: %s", hipGetErrorString(e));
     return SPBP_OK;
+}
+
+int spbp_rtsq_host(const float* imgs_host, int nimg, int nx, const float* params, float* out_host) {
+    if (nimg < 0 || nx < 2) return fail(nullptr, SPBP_ERR_ARG, "bad image count or size");
+    if (nimg == 0) return SPBP_OK;
+    if (!imgs_host || !params || !out_host) return fail(nullptr, SPBP_ERR_ARG, "NULL pointer");
+    const size_t bytes = (size_t)nimg * nx * nx * sizeof(float);
+    float *din = nullptr, *dout = nullptr;
+    if (dmalloc(&din, bytes) != hipSuccess) return fail(nullptr, SPBP_ERR_ALLOC, "rtsq staging: %zu bytes", bytes);
+    if (dmalloc(&dout, bytes) != hipSuccess) {
+        (void)hipFree(din);
+        return fail(nullptr, SPBP_ERR_ALLOC, "rtsq staging: %zu bytes", bytes);
+    }
+    int rc = SPBP_OK;
+    if (hipMemcpy(din, imgs_host, bytes, hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(nullptr, SPBP_ERR_DEVICE, "rtsq: image upload failed");
+    if (rc == SPBP_OK) rc = spbp_rtsq(din, SPBP_F32, nimg, nx, params, dout, nullptr);
+    if (rc == SPBP_OK && hipMemcpy(out_host, dout, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = fail(nullptr, SPBP_ERR_DEVICE, "rtsq: image download failed");
+    (void)hipFree(din);
+    (void)hipFree(dout);
+    return rc;
 }
 
 }  // extern "C"

@@ -8,20 +8,23 @@
 !   BP 32F:  template, file numbers, angles doc, symmetries doc,
 !            reconstructed volume, first sample volume, second sample volume
 !   RB 3F / RB 32F: as BP 3F / BP 32F, then the template for the transformed
-!            images ('*': not written -- the only answer accepted here);
-!            registers 6, 7, 8 of the angles doc give the in-plane angle and
-!            shifts applied by RTSQ before the insertion (bp32f.f:805-837)
+!            images ('*': not written; `rot***` or `rot@***`: every image after
+!            its RTSQ, under the input's file numbers, bp32f.f:517-527,
+!            826-827); registers 6, 7, 8 of the angles doc give the in-plane
+!            angle and shifts applied by RTSQ before the insertion
+!            (bp32f.f:805-837)
 !
 ! Usage:  spbp_bp 3F|32F|RB3F|RB32F [data-extension] < answers
 ! Text after ';' on an answer line is a comment.  Images are SPIDER 2D files
-! (`name***` templates or `stack@***`), the angles doc holds PSI, THETA, PHI
+! (`name***` templates, or `stack@***` over stacks and indexed stacks), the
+! angles doc holds PSI, THETA, PHI
 ! in registers 1-3 per image number, volumes are written as SPIDER files.
 program spbp_bp
    use iso_c_binding
    use spbp
    implicit none
 
-   character(len=512) :: op, ext, template, numstr, angdoc, symdoc, outs(3), fname
+   character(len=512) :: op, ext, template, numstr, angdoc, symdoc, outs(3), fname, rottmpl
    integer, allocatable :: nums(:)
    real(c_float), allocatable, target :: imgs(:, :, :), vol(:, :, :, :)
    real(c_double), allocatable :: angles(:, :)
@@ -35,6 +38,7 @@ program spbp_bp
    integer(c_int) :: rc
    logical :: is32, isrb
    real(c_float), allocatable :: inplane(:, :)
+   real(c_float), allocatable :: rotimgs(:, :, :)
    real(c_double), allocatable :: symang(:, :)
    integer :: nsym
    real :: x
@@ -71,10 +75,8 @@ program spbp_bp
    do k = 1, nout
       call answer('OUTPUT VOLUME', outs(k))
    end do
-   if (isrb) then
-      call answer('TRANSFORMED OUTPUT IMAGES TEMPLATE', fname)
-      if (trim(fname) /= '*') call errt(203, 'TRANSFORMED OUTPUT IMAGES: ONLY * (NOT WRITTEN) IS SUPPORTED')
-   end if
+   rottmpl = '*'
+   if (isrb) call answer('TRANSFORMED OUTPUT IMAGES TEMPLATE', rottmpl)
 
    call expand_numbers(numstr, nums)
    nang = size(nums)
@@ -151,7 +153,20 @@ program spbp_bp
       rc = spbp_set_symmetries(ctx, int(nsym, c_int), symang)
       if (rc /= SPBP_OK) call errt(int(rc), trim(spbp_errmsg(ctx)))
    end if
-   if (isrb .and. is32) then
+   if (isrb .and. trim(rottmpl) /= '*') then
+      ! the transformed images are written, then inserted as they are
+      allocate (rotimgs(nx, nx, nang))
+      rc = spbp_rtsq_host(imgs, int(nang, c_int), int(nx, c_int), inplane, rotimgs)
+      if (rc /= SPBP_OK) call errt(int(rc), trim(spbp_errmsg(c_null_ptr)))
+      call write_images(rottmpl, nums, ext, rotimgs)
+      imgs = rotimgs
+      deallocate (rotimgs)
+      if (is32) then
+         rc = spbp_insert_host(ctx, imgs, SPBP_F32, int(nang, c_int), angles, c_loc(half), c_null_ptr)
+      else
+         rc = spbp_insert_host(ctx, imgs, SPBP_F32, int(nang, c_int), angles, c_null_ptr, c_null_ptr)
+      end if
+   else if (isrb .and. is32) then
       rc = spbp_insert_rb_host(ctx, imgs, SPBP_F32, int(nang, c_int), angles, inplane, c_loc(half), c_null_ptr)
    else if (isrb) then
       rc = spbp_insert_rb_host(ctx, imgs, SPBP_F32, int(nang, c_int), angles, inplane, c_null_ptr, c_null_ptr)
@@ -287,14 +302,24 @@ contains
       y = transfer(j, y)
    end function bswap
 
-   ! SPIDER 2D image (docs/image_doc.html): simple file, or image kidx of a stack.
+   elemental function bswapi(i) result(j)
+      integer, intent(in) :: i
+      integer :: j
+      j = ior(ior(ishft(iand(i, 255), 24), ishft(iand(ishft(i, -8), 255), 16)), &
+              ior(ishft(iand(ishft(i, -16), 255), 8), iand(ishft(i, -24), 255)))
+   end function bswapi
+
+   ! SPIDER 2D image (docs/image_doc.html): simple file, image kidx of a
+   ! stack, or of an indexed stack (ISTACK = -MAXINDX; INDXREC =
+   ! ceil(MAXINDX / NX) records of NX integers after the overall header give
+   ! each image number's storage slot, 0 if absent; lunsethdr.f LUNREDINDX).
    subroutine read_spider_2d(fname, kidx, img, ierr)
       character(len=*), intent(in) :: fname
       integer, intent(in) :: kidx
       real, allocatable, intent(out) :: img(:, :)
       integer, intent(out) :: ierr
       real :: h(26)
-      integer :: u, ios, nx, ny, nz, labbyt, istack, maxim
+      integer :: u, ios, nx, ny, nz, labbyt, istack, maxim, slot, indxrec
       integer(8) :: pos
       logical :: swap
       ierr = 0
@@ -321,6 +346,21 @@ contains
             return
          end if
          pos = pos + int(kidx - 1, 8) * (int(labbyt, 8) + 4_8 * nx * ny) + labbyt
+      else if (istack < 0) then
+         if (kidx < 1 .or. kidx > -istack) then
+            ierr = 102
+            close (u)
+            return
+         end if
+         read (u, pos=pos + 4_8 * (kidx - 1), iostat=ios) slot
+         if (swap) slot = bswapi(slot)
+         if (ios /= 0 .or. slot < 1) then
+            ierr = 102
+            close (u)
+            return
+         end if
+         indxrec = (-istack + nx - 1) / nx
+         pos = pos + int(indxrec, 8) * 4 * nx + int(slot - 1, 8) * (int(labbyt, 8) + 4_8 * nx * ny) + labbyt
       end if
       allocate (img(nx, ny))
       read (u, pos=pos, iostat=ios) img
@@ -328,6 +368,80 @@ contains
       if (swap) img = bswap(img)
       close (u)
    end subroutine read_spider_2d
+
+   ! SPIDER header words 1,2,3,5-10,12,13,22-24,26,27 for an NX x NY x NZ
+   ! image (with_stats: IMAMI = 1 and FMAX, FMIN, AV, SIG of v).
+   subroutine spider_header(h, nx, ny, nz, v, with_stats, istack, maxim, imgnum, irec)
+      real, allocatable, intent(out) :: h(:)
+      integer, intent(in) :: nx, ny, nz, istack, maxim, imgnum, irec
+      real, intent(in) :: v(:)
+      logical, intent(in) :: with_stats
+      integer :: labrec, labbyt
+      double precision :: av
+      labrec = 1024 / (4 * nx)
+      if (mod(1024, 4 * nx) /= 0) labrec = labrec + 1
+      labbyt = labrec * 4 * nx
+      allocate (h(labbyt / 4))
+      h = 0.0
+      h(1) = nz; h(2) = ny; h(3) = irec; h(5) = merge(3.0, 1.0, nz > 1)
+      if (with_stats) then
+         av = sum(dble(v)) / size(v)
+         h(6) = 1.0; h(7) = maxval(v); h(8) = minval(v); h(9) = real(av)
+         h(10) = real(sqrt(max(0d0, sum((dble(v) - av)**2) / size(v))))
+      end if
+      h(12) = nx; h(13) = labrec; h(22) = labbyt; h(23) = 4 * nx
+      h(24) = istack; h(26) = maxim; h(27) = imgnum
+   end subroutine spider_header
+
+   ! Images under a template, one per file number: `name***` -> one simple
+   ! image each; `stk@***` -> one stack, image k = the k-th listed number
+   ! (MAXIM = largest number; slots no number names hold IMGNUM 0).
+   subroutine write_images(tmpl, nums, ext, v)
+      character(len=*), intent(in) :: tmpl, ext
+      integer, intent(in) :: nums(:)
+      real, intent(in) :: v(:, :, :)
+      character(len=512) :: name
+      real, allocatable :: h(:), zero(:, :)
+      integer :: u, ios, k, j, kidx, nx, ny, maxim, labrec
+      nx = size(v, 1); ny = size(v, 2)
+      if (index(tmpl, '@') == 0) then
+         do k = 1, size(nums)
+            call image_name(tmpl, nums(k), ext, name, kidx)
+            call spider_header(h, nx, ny, 1, reshape(v(:, :, k), [nx * ny]), .true., 0, 0, 0, 0)
+            h(3) = h(13) + ny
+            open (newunit=u, file=name, access='stream', form='unformatted', status='replace', action='write', &
+                  iostat=ios)
+            if (ios /= 0) call errt(18, 'CANNOT WRITE IMAGE: '//trim(name))
+            write (u, iostat=ios) h, v(:, :, k)
+            if (ios /= 0) call errt(18, 'CANNOT WRITE IMAGE: '//trim(name))
+            close (u)
+         end do
+         return
+      end if
+      if (minval(nums) < 1) call errt(203, 'STACK IMAGE NUMBERS START AT 1')
+      call image_name(tmpl, nums(1), ext, name, kidx)
+      maxim = maxval(nums)
+      allocate (zero(nx, ny))
+      zero = 0.0
+      call spider_header(h, nx, ny, 1, [0.0], .false., 2, maxim, 0, 0)
+      labrec = nint(h(13))
+      h(3) = labrec + ny
+      open (newunit=u, file=name, access='stream', form='unformatted', status='replace', action='write', iostat=ios)
+      if (ios /= 0) call errt(18, 'CANNOT WRITE IMAGE STACK: '//trim(name))
+      write (u, iostat=ios) h
+      do k = 1, maxim
+         j = findloc(nums, k, dim=1)
+         if (j > 0) then
+            call spider_header(h, nx, ny, 1, reshape(v(:, :, j), [nx * ny]), .true., 0, 0, k, labrec + ny)
+            write (u, iostat=ios) h, v(:, :, j)
+         else
+            call spider_header(h, nx, ny, 1, [0.0], .false., 0, 0, 0, labrec + ny)
+            write (u, iostat=ios) h, zero
+         end if
+         if (ios /= 0) call errt(18, 'CANNOT WRITE IMAGE STACK: '//trim(name))
+      end do
+      close (u)
+   end subroutine write_images
 
    ! Simple SPIDER volume, machine byte order; header words 1,2,3,5-10,12,13,22,23.
    subroutine write_spider_vol(fname, v, ierr)

@@ -27,7 +27,7 @@ module spbp
    end type spbp_opts
 
    public :: spbp_create, spbp_destroy, spbp_insert_host, spbp_finalize_host, spbp_reset, spbp_errmsg, &
-      spbp_set_symmetries, spbp_insert_rb_host
+      spbp_set_symmetries, spbp_insert_rb_host, spbp_rtsq_host
 
    interface
       integer(c_int) function spbp_create(opts, ctx) bind(c, name='spbp_create')
@@ -71,6 +71,15 @@ module spbp
          real(c_float), intent(in) :: inplane(*)
          type(c_ptr), value :: half, stream
       end function spbp_insert_rb_host
+
+      ! RTSQ of host images (the RB transformed images); params 3 x nimg
+      integer(c_int) function spbp_rtsq_host(imgs, nimg, nx, params, out) bind(c, name='spbp_rtsq_host')
+         import :: c_int, c_float
+         real(c_float), intent(in) :: imgs(*)
+         integer(c_int), value :: nimg, nx
+         real(c_float), intent(in) :: params(*)
+         real(c_float), intent(out) :: out(*)
+      end function spbp_rtsq_host
 
       ! symmetries doc rows: 3 x nsym DOUBLE PRECISION (PSI, THETA, PHI)
       integer(c_int) function spbp_set_symmetries(ctx, nsym, angles) bind(c, name='spbp_set_symmetries')

@@ -2,7 +2,7 @@
 (docs/image_doc.html) and document files (docs/docfile.html).
 
 Only what the `BP 3F` / `BP 32F` signature needs: read simple 2D images,
-image stacks and templates (`jnk***`, `stk@***`), write simple volumes and
+image stacks, indexed stacks and templates (`jnk***`, `stk@***`), write simple volumes and
 stacks, read/write document files, and the raw Fourier partial volumes that
 `BP 3F P` writes and `BP 3F M` reads (bp3f.f:191-248, 646-685).
 Both byte orders are read; files are written in the machine's order.
@@ -52,16 +52,37 @@ def read_header(path: str) -> dict:
     raise SpiderFormatError(f"{path}: not a SPIDER image file")
 
 
+def _indexed_slot(path: str, h: dict, index: int | None) -> int:
+    """Storage slot (1-based) of image `index` in an indexed stack
+    (docs/image_doc.html): ISTACK = -MAXINDX in the overall header; after
+    it, INDXREC = ceil(MAXINDX / NX) records of NX 32-bit integers give each
+    image number's slot, 0 where the image does not exist (LUNGETLAB,
+    LUNREDINDX, lunsethdr.f)."""
+    maxindx = -h["istack"]
+    if index is None:
+        raise SpiderFormatError(f"{path}: is a stack, give an image number")
+    if not 1 <= index <= maxindx:
+        raise SpiderFormatError(f"{path}: image number {index} exceeds index limit {maxindx}")
+    rec, loc = divmod(index - 1, h["nx"])
+    idx = np.fromfile(path, dtype=np.dtype(h["byteorder"] + "i4"), count=1,
+                      offset=h["labbyt"] + rec * h["lenbyt"] + loc * 4)
+    if idx.size != 1 or idx[0] <= 0:
+        raise SpiderFormatError(f"{path}: no image {index} in the stack")
+    return int(idx[0])
+
+
 def read_image(path: str, index: int | None = None) -> np.ndarray:
-    """A simple image/volume, or image `index` (1-based) of a stack:
-    float32 array (NY, NX) for 2D or (NZ, NY, NX) for 3D."""
+    """A simple image/volume, or image `index` (1-based) of a stack or an
+    indexed stack: float32 array (NY, NX) for 2D or (NZ, NY, NX) for 3D."""
     h = read_header(path)
-    if h["istack"] < 0:
-        raise SpiderFormatError(f"{path}: indexed stacks are not supported")
     nx, ny, nz, lab = h["nx"], h["ny"], h["nz"], h["labbyt"]
     n = nx * ny * nz
     dt = np.dtype(h["byteorder"] + "f4")
-    if h["istack"] > 0:
+    if h["istack"] < 0:
+        slot = _indexed_slot(path, h, index)
+        indxrec = (-h["istack"] + nx - 1) // nx                 # INDXREC = ceil(MAXINDX / NX) records
+        off = lab + indxrec * h["lenbyt"] + (slot - 1) * (lab + n * 4) + lab
+    elif h["istack"] > 0:
         if index is None:
             raise SpiderFormatError(f"{path}: is a stack, give an image number")
         if not 1 <= index <= h["maxim"]:
@@ -122,6 +143,36 @@ def write_stack(path: str, imgs) -> None:
         for i in range(n):
             _make_header(nx, ny, 1, 1, imgs[i], imgnum=i + 1).tofile(fh)
             imgs[i].tofile(fh)
+
+
+def write_images(template: str, numbers, imgs, ext: str = "") -> None:
+    """Write images (n, NY, NX) under a template, one per file number:
+    `name***` -> one simple image per number; `stk@***` -> one stack whose
+    image k holds the k-th listed number's image (MAXIM = the largest
+    number; slots no number names carry IMGNUM 0, unused).  As the
+    transformed-images output of `RB 3F` / `RB 32F` (bp32f.f:517-527,
+    826-827)."""
+    imgs = np.ascontiguousarray(np.asarray(imgs, np.float32))
+    names = template_names(template, numbers, ext)
+    if len(names) != imgs.shape[0]:
+        raise SpiderFormatError(f"{len(names)} file numbers for {imgs.shape[0]} images")
+    if "@" not in template:
+        for (f, _), a in zip(names, imgs):
+            write_image(f, a)
+        return
+    n, ny, nx = imgs.shape
+    maxim = max(k for _, k in names)
+    if min(k for _, k in names) < 1:
+        raise SpiderFormatError("stack image numbers start at 1")
+    slot = {k: i for i, (_, k) in enumerate(names)}
+    labrec, _ = _header_len(nx)
+    zero = np.zeros((ny, nx), np.float32)
+    with open(names[0][0], "wb") as fh:
+        _make_header(nx, ny, 1, 1, None, istack=2, maxim=maxim, irec=labrec + ny).tofile(fh)
+        for k in range(1, maxim + 1):
+            a = imgs[slot[k]] if k in slot else zero
+            _make_header(nx, ny, 1, 1, a if k in slot else None, imgnum=k if k in slot else 0).tofile(fh)
+            a.tofile(fh)
 
 
 def _expand_numbers(spec) -> list[int]:

@@ -50,11 +50,46 @@ def test_driver_rejects_non_square_and_missing_angles(tmp_path):
     assert rc != 0 and "ERROR (102)" in out and "MISSING ANGLE" in out
     rc, out = run("3F", ["jnk***", "1-2", "savangbprp", "c1sym", "out"], str(tmp_path))
     assert rc != 0 and "ERROR (18)" in out and "c1sym" in out          # symmetries doc not found
-    # RB: the angles doc has no registers 6-8; a transformed-images template is not written
-    rc, out = run("RB3F", ["jnk***", "1-2", "savangbprp", "*", "out", "*"], str(tmp_path))
-    assert rc != 0 and "ERROR (102)" in out and "6-8" in out
-    rc, out = run("RB3F", ["jnk***", "1-2", "savangbprp", "*", "out", "rot***"], str(tmp_path))
-    assert rc != 0 and "ERROR (203)" in out
+    # RB: the angles doc has no registers 6-8, with or without a transformed-images template
+    for rot in ("*", "rot***"):
+        rc, out = run("RB3F", ["jnk***", "1-2", "savangbprp", "*", "out", rot], str(tmp_path))
+        assert rc != 0 and "ERROR (102)" in out and "6-8" in out
+
+
+def indexed_stack(path, imgs, slots, maxindx):
+    """An indexed stack laid out as docs/image_doc.html describes it
+    (ISTACK = -MAXINDX, index records of NX integers, then slot headers and
+    images); slots maps image number -> storage slot."""
+    ny, nx = next(iter(imgs.values())).shape
+    labrec = -(-1024 // (4 * nx))
+    labbyt = labrec * 4 * nx
+    hdr = np.zeros(labbyt // 4, np.float32)
+    hdr[[0, 1, 2, 4, 11, 12, 21, 22, 23, 25]] = [1, ny, labrec + ny, 1, nx, labrec, labbyt, 4 * nx, -maxindx,
+                                                len(slots)]
+    index = np.zeros(-(-maxindx // nx) * nx, np.int32)
+    for k, sl in slots.items():
+        index[k - 1] = sl
+    with open(path, "wb") as fh:
+        fh.write(hdr.tobytes() + index.tobytes())
+        for k, sl in sorted(slots.items(), key=lambda kv: kv[1]):
+            h = np.zeros(labbyt // 4, np.float32)
+            h[[0, 1, 4, 11, 12, 21, 22, 26]] = [1, ny, 1, nx, labrec, labbyt, 4 * nx, k]
+            fh.write(h.tobytes() + imgs[k].astype(np.float32).tobytes())
+
+
+def test_driver_reads_indexed_stacks(tmp_path):
+    """Images of an indexed stack are read (the run then stops at the device
+    here, or finishes where a GPU is present); an image number the index
+    does not hold is error 102."""
+    d = str(tmp_path)
+    img = synth.mo_t_image(16)
+    indexed_stack(os.path.join(d, "ind.dat"), {2: img, 7: img}, {7: 1, 2: 2}, maxindx=20)
+    shutil.copy(os.path.join(GOLDEN, "sav_bp_ang_doc.dat"), os.path.join(d, "savangbprp.dat"))
+    rc, out = run("3F", ["ind@**", "2,7", "savangbprp", "*", "out"], d)
+    assert "NUMBER OF IMAGES" in out and "ERROR (102)" not in out and "ERROR (18)" not in out
+    assert rc == 0 or "ERROR (202)" in out, out
+    rc, out = run("3F", ["ind@**", "2,3", "savangbprp", "*", "out"], d)
+    assert rc != 0 and "ERROR (102)" in out
 
 
 @pytest.mark.gpu
@@ -122,12 +157,12 @@ def test_driver_rb32f_and_symmetries(tmp_path):
     spiderio.write_doc(os.path.join(d, "ang.dat"),
                        {k + 1: list(ang[k]) + [k + 1, k + 1] + list(rot[k]) for k in range(m)})
     spiderio.write_doc(os.path.join(d, "sym.dat"), {1: [0.0, 0.0, 0.0], 2: [0.0, 0.0, 180.0]})
-    rc, out = run("RB32F", ["img***", f"1-{m}", "ang", "sym", "b", "e", "o", "*"], d)
+    rc, out = run("RB32F", ["img***", f"1-{m}", "ang", "sym", "b", "e", "o", "rot@***"], d)
     assert rc == 0, out
     vd = spiderio.read_image(os.path.join(d, "b.dat"))
     vp, _, _ = bp.rb32f_files(os.path.join(d, "img***"), f"1-{m}", os.path.join(d, "ang.dat"),
                               os.path.join(d, "sym.dat"), os.path.join(d, "pb.dat"), os.path.join(d, "pe.dat"),
-                              os.path.join(d, "po.dat"), ext=".dat")
+                              os.path.join(d, "po.dat"), ext=".dat", rot_out=os.path.join(d, "prot***"))
     assert np.linalg.norm(vd - vp) / np.linalg.norm(vp) < 1e-6
     doc = spiderio.read_doc(os.path.join(d, "ang.dat"))
     a = spiderio.angles_for(doc, range(1, m + 1))
@@ -135,3 +170,9 @@ def test_driver_rb32f_and_symmetries(tmp_path):
     rimg = np.stack([oracle.rtsq(imgs[k], *r[k]) for k in range(m)])
     ref = oracle.bp_sym(rimg, a, np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 180.0]]))["full"]
     assert np.linalg.norm(vp - ref) / np.linalg.norm(ref) < 1e-5
+    # the transformed images (bp32f.f:826-827): the driver's stack and the
+    # Python operation's image series both hold RTSQ's output, bit for bit
+    hd = spiderio.read_header(os.path.join(d, "rot.dat"))
+    assert hd["istack"] > 0 and hd["maxim"] == m
+    np.testing.assert_array_equal(spiderio.read_images(os.path.join(d, "rot@***"), f"1-{m}", ".dat"), rimg)
+    np.testing.assert_array_equal(spiderio.read_images(os.path.join(d, "prot***"), f"1-{m}", ".dat"), rimg)

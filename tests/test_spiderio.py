@@ -89,3 +89,60 @@ def test_weights_file_names():
     assert spiderio._weights_name("dir.x/part") == "dir.x/part_W"
     assert spiderio._weights_name("stk@3") == "stk_W@3"
     assert spiderio._weights_name("stk.dat@12") == "stk_W.dat@12"
+
+
+@pytest.mark.parametrize("byteorder", ["<", ">"])
+def test_indexed_stack(tmp_path, byteorder):
+    """An indexed stack built byte by byte from the layout in
+    docs/image_doc.html: overall header with ISTACK = -MAXINDX, then
+    INDXREC = ceil(MAXINDX / NX) records of NX 32-bit integers (image number
+    -> storage slot, 0 = absent; LUNREDINDX, lunsethdr.f), then per slot a
+    LABREC-record header and the image.  Images are stored out of order."""
+    nx, ny, maxindx = 6, 5, 15                # INDXREC = 3 records of 6 entries
+    labrec = -(-1024 // (4 * nx))             # 43 records of 24 bytes
+    labbyt, lenbyt = labrec * 4 * nx, 4 * nx
+    indxrec = -(-maxindx // nx)
+    rng = np.random.default_rng(4)
+    imgs = {3: rng.standard_normal((ny, nx)), 11: rng.standard_normal((ny, nx)), 14: rng.standard_normal((ny, nx))}
+    slots = {14: 1, 3: 2, 11: 3}              # image number -> slot
+    f4, i4 = np.dtype(byteorder + "f4"), np.dtype(byteorder + "i4")
+    hdr = np.zeros(labbyt // 4, f4)
+    hdr[0], hdr[1], hdr[2], hdr[4] = 1, ny, labrec + ny, 1            # NZ, NY, IREC, IFORM
+    hdr[11], hdr[12], hdr[21], hdr[22] = nx, labrec, labbyt, lenbyt   # NX, LABREC, LABBYT, LENBYT
+    hdr[23], hdr[25], hdr[27] = -maxindx, 3, 14                       # ISTACK, MAXIM, LASTINDX
+    index = np.zeros(indxrec * nx, i4)
+    for k, s in slots.items():
+        index[k - 1] = s
+    body = b""
+    for k, s in sorted(slots.items(), key=lambda kv: kv[1]):
+        h = np.zeros(labbyt // 4, f4)
+        h[0], h[1], h[4], h[11], h[12], h[21], h[22], h[26] = 1, ny, 1, nx, labrec, labbyt, lenbyt, k
+        body += h.tobytes() + imgs[k].astype(f4).tobytes()
+    path = str(tmp_path / "ind.dat")
+    with open(path, "wb") as fh:
+        fh.write(hdr.tobytes() + index.tobytes() + body)
+    for k in slots:
+        np.testing.assert_array_equal(spiderio.read_image(path, k), imgs[k].astype(np.float32))
+    got = spiderio.read_images(str(tmp_path / "ind@**"), "14,3,11", ext=".dat")
+    np.testing.assert_array_equal(got, np.stack([imgs[k] for k in (14, 3, 11)]).astype(np.float32))
+    with pytest.raises(spiderio.SpiderFormatError, match="no image 5"):
+        spiderio.read_image(path, 5)
+    with pytest.raises(spiderio.SpiderFormatError, match="exceeds index limit"):
+        spiderio.read_image(path, 16)
+    with pytest.raises(spiderio.SpiderFormatError, match="give an image number"):
+        spiderio.read_image(path)
+
+
+def test_write_images_series_and_sparse_stack(tmp_path):
+    """Transformed-images output: a `***` series and a stack whose image
+    numbers skip slots (unused slots carry IMGNUM 0)."""
+    rng = np.random.default_rng(9)
+    imgs = rng.standard_normal((3, 8, 8)).astype(np.float32)
+    spiderio.write_images(str(tmp_path / "r***"), "2,5,9", imgs, ext=".dat")
+    np.testing.assert_array_equal(spiderio.read_images(str(tmp_path / "r***"), [2, 5, 9], ".dat"), imgs)
+    assert os.path.exists(str(tmp_path / "r005.dat"))
+    spiderio.write_images(str(tmp_path / "s@**"), [4, 1, 6], imgs, ext=".dat")
+    h = spiderio.read_header(str(tmp_path / "s.dat"))
+    assert h["istack"] > 0 and h["maxim"] == 6
+    np.testing.assert_array_equal(spiderio.read_images(str(tmp_path / "s@**"), [4, 1, 6], ".dat"), imgs)
+    np.testing.assert_array_equal(spiderio.read_image(str(tmp_path / "s.dat"), 3), np.zeros((8, 8), np.float32))
