@@ -62,6 +62,9 @@ extern "C" {
                                    voxels whose W lies within 2^-11 of 0.1 are decided by
                                    re-summing their weights in the reference's order (see
                                    spbp_threshold_stats)                                       */
+#define SPBP_FBS          0x100 /* FBS ON (setmode.f:128,169): RB inserts pre-rotate with RTSF,
+                                   Fourier-based bicubic interpolation (rtsf.f, fbs2.f), instead of
+                                   RTSQ (bp32f.f:811-822)                                       */
 
 /* kernels reported by spbp_kernel_stats */
 #define SPBP_K_INSERT   0   /* ONELINE scatter (units: projections)       */
@@ -263,6 +266,10 @@ int spbp_insert_rb_host(spbp_ctx* ctx, const void* imgs_host, int dtype, int nim
 /* RTSQ alone (scale 1): nimg NX x NX device images (fp32 / bf16) -> out
  * (device fp32), params host nimg x 3 as above.  Synchronises the stream. */
 int spbp_rtsq(const void* imgs, int dtype, int nimg, int nx, const float* params, float* out, void* stream);
+/* RTSF alone (FBS ON; rtsf.f:106-205 with scale 1, FBS2 interpolation from
+ * the image's Fourier derivatives, fbs2.f:95-245): as spbp_rtsq. */
+int spbp_rtsf(const void* imgs, int dtype, int nimg, int nx, const float* params, float* out, void* stream);
+int spbp_rtsf_host(const float* imgs_host, int nimg, int nx, const float* params, float* out_host);
 /* The same on host memory (fp32 in and out, staged through device memory;
  * synchronous): the transformed images that RB 3F / RB 32F write under the
  * template answer (bp32f.f:517-527, 817-827, WRTVOL of PROJ). */

@@ -85,6 +85,11 @@ def lib():
         L.or_quadri.argtypes = [C.c_float, C.c_float, C.c_int, C.c_int, _f32p]
         L.or_quadri.restype = C.c_float
         L.or_rtsq.argtypes = [_f32p, _f32p, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float]
+        L.or_fbs2.argtypes = [C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, _f32p, C.c_int, _f32p, _f32p, _f32p,
+                              C.c_int]
+        L.or_fbs2.restype = C.c_float
+        L.or_fbs2_prep.argtypes = [_f32p, C.c_int, C.c_int, _f32p, _f32p, _f32p]
+        L.or_rtsf.argtypes = [_f32p, _f32p, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float]
         L.or_scatter.argtypes = [C.c_int, _f64p, C.c_int, _f32p, _f32p, C.c_float, _f64p, _f64p]
         L.or_symplane0_f32.argtypes = [C.c_int, _f32p, _f32p]
         L.or_symplane0_f64.argtypes = [C.c_int, _f64p, _f64p]
@@ -348,6 +353,32 @@ def rtsq(img, theta, shx, shy, scale=1.0):
     return out
 
 
+def fbs2(x, y, img, x1, y1, xy2, chk=True) -> np.float32:
+    """FBS2 (fbs2.f:95-160) at 1-based (x, y) of img ([y][x] fp32) with its
+    derivative images x1 = dF/dx, y1 = dF/dy, xy2 = d2F/dxdy."""
+    a = [np.ascontiguousarray(v, np.float32) for v in (img, x1, y1, xy2)]
+    ny, nx = a[0].shape
+    return np.float32(lib().or_fbs2(float(x), float(y), nx, nx, ny, a[0], nx, a[1], a[2], a[3], int(bool(chk))))
+
+
+def fbs2_prep(img):
+    """FBS2_PREP (fbs2.f:180-245): (dF/dx, dF/dy, d2F/dxdy) of img ([y][x]) by
+    the Fourier method, FP64 transforms rounded to fp32."""
+    img = np.ascontiguousarray(img, np.float32)
+    out = [np.zeros_like(img) for _ in range(3)]
+    lib().or_fbs2_prep(img, img.shape[1], img.shape[0], *out)
+    return tuple(out)
+
+
+def rtsf(img, theta, shx, shy):
+    """RTSF (rtsf.f:106-205, scale 1): rotation by theta degrees and shift
+    (shx, shy) of one image ([y][x] fp32) with FBS2 interpolation."""
+    img = np.ascontiguousarray(img, np.float32)
+    out = np.zeros_like(img)
+    lib().or_rtsf(img, out, img.shape[1], img.shape[0], float(theta), float(shx), float(shy))
+    return out
+
+
 def bp_sym(imgs, angles, sym_angles, half=None, want_xw=False):
     """FP32-faithful BP 3F / BP 32F with a symmetry document (sym_angles:
     (nsym, 3) PSI, THETA, PHI rows): every projection inserted once per
@@ -481,6 +512,17 @@ class RefLib:
         self.L.rtsq_(img.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p), i(nx), i(ny), i(nx), i(ny),
                      f(theta), f(scale), f(shx), f(shy), C.byref(irt))
         return out
+
+    # FBS2(X,Y,NXLD,NX,NY,FDATA,NXP,X1,Y1,XY2,CHKBOUND): REAL function
+    def fbs2(self, x, y, img, x1, y1, xy2, chk=True) -> np.float32:
+        self.L.fbs2_.restype = C.c_float
+        a = [np.ascontiguousarray(v, np.float32) for v in (img, x1, y1, xy2)]
+        ny, nx = a[0].shape
+        i = lambda v: C.byref(C.c_int32(v))
+        return np.float32(self.L.fbs2_(C.byref(C.c_float(x)), C.byref(C.c_float(y)), i(nx), i(nx), i(ny),
+                                       a[0].ctypes.data_as(C.c_void_p), i(nx), a[1].ctypes.data_as(C.c_void_p),
+                                       a[2].ctypes.data_as(C.c_void_p), a[3].ctypes.data_as(C.c_void_p),
+                                       C.byref(C.c_int32(1 if chk else 0))))
 
     # SYMVOL(Q1,Q2,KLX,KNX,KLY,KNY,KLZ,KNZ,RM,NSYM)
     def symvol(self, q1, sm) -> np.ndarray:

@@ -860,6 +860,152 @@ void rtsq_ref(const float* img, float* out, int nx, int ny, float theta, float s
     }
 }
 
+// ---------------------------------------------------------------- RB with FBS ON: RTSF
+// FBS2 (fbs2.f:95-160): Fourier-based bicubic interpolation of FDATA(NXP, NY)
+// at (X, Y) (1-based) from the image and its derivatives X1 = dF/dx,
+// Y1 = dF/dy, XY2 = d2F/dxdy (arrays (NXLD, NY), x fastest); cubic Hermite
+// along x on the two rows around Y, then along y.  CHKBOUND wraps the cell's
+// corners with MODULO.  REAL arithmetic in the reference's order.
+int imodulo(int a, int p) { const int r = a % p; return r < 0 ? r + p : r; }
+float fbs2_ref(float x, float y, int nxld, int nx, int ny, const float* fd, int nxp, const float* x1,
+               const float* y1, const float* xy2, bool chk) {
+    const int i = (int)std::floor(x), j = (int)std::floor(y);
+    const float dx = x - (float)i, dy = y - (float)j;
+    const float dxsqr = dx * dx, dxcub = dx * dx * dx;
+    int i2 = i, j2 = j, i3 = i + 1, j3 = j + 1;
+    if (chk) {
+        i2 = imodulo(i - 1, nx) + 1;
+        j2 = imodulo(j - 1, ny) + 1;
+        i3 = imodulo(i, nx) + 1;
+        j3 = imodulo(j, ny) + 1;
+    }
+    auto F = [&](int a, int b) { return fd[(size_t)(b - 1) * nxp + (a - 1)]; };
+    auto D = [&](const float* d, int a, int b) { return d[(size_t)(b - 1) * nxld + (a - 1)]; };
+    // A0 + A1 DX + A2 DX^2 + A3 DX^3 with A2, A3 from the values and slopes at both ends
+    auto hermite = [&](float a0, float a1, float f3, float d3) {
+        const float a2 = (3.0f * (f3 - a0) - 2.0f * a1) - d3;
+        const float a3 = (2.0f * ((-f3) + a0) + a1) + d3;
+        return ((a0 + a1 * dx) + a2 * dxsqr) + a3 * dxcub;
+    };
+    const float adx = hermite(F(i2, j2), D(x1, i2, j2), F(i3, j2), D(x1, i3, j2));
+    const float bdx = hermite(F(i2, j3), D(x1, i2, j3), F(i3, j3), D(x1, i3, j3));
+    const float dadx = hermite(D(y1, i2, j2), D(xy2, i2, j2), D(y1, i3, j2), D(xy2, i3, j2));
+    const float dbdx = hermite(D(y1, i2, j3), D(xy2, i2, j3), D(y1, i3, j3), D(xy2, i3, j3));
+    const float a2 = (3.0f * (bdx - adx) - 2.0f * dadx) - dbdx;
+    const float a3 = (2.0f * (adx - bdx) + dadx) + dbdx;
+    return ((adx + dadx * dy) + a2 * (dy * dy)) + a3 * (dy * dy * dy);
+}
+
+// FBS2_PREP (fbs2.f:180-245): the derivatives as inverse transforms of the
+// spectrum times i kx, i ky and (i kx)(i ky), kx = 2 pi I / NX for the stored
+// half spectrum I = 0 .. NX/2 and ky = 2 pi J / NY (J <= NY/2, else J - NY),
+// the factors in REAL as the reference forms them.  The inverse follows
+// FFTW's complex-to-real reading of the half spectrum: the kx = 0 and
+// kx = NX/2 columns count by their Hermitian parts.  Transforms in FP64,
+// results rounded to fp32.  Arrays NX x NY, x fastest.
+void fbs2_prep_ref(const float* img, int nx, int ny, float* x1, float* y1, float* xy2) {
+    const int h = nx / 2 + 1;
+    const FFTPlan px = make_plan(nx, -1), py = make_plan(ny, -1);
+    const FFTPlan qx = make_plan(nx, 1), qy = make_plan(ny, 1);
+    std::vector<double> a(2 * (size_t)nx * ny), wi(2 * (size_t)std::max(nx, ny)), wo(2 * (size_t)std::max(nx, ny));
+    for (size_t q = 0; q < (size_t)nx * ny; ++q) { a[2 * q] = img[q]; a[2 * q + 1] = 0.0; }
+    for (int y = 0; y < ny; ++y) fft_strided(px, a.data() + 2 * (size_t)y * nx, 1, wi.data(), wo.data());
+    for (int x = 0; x < nx; ++x) fft_strided(py, a.data() + 2 * x, nx, wi.data(), wo.data());
+    const float pi2 = (float)6.28318530717958647692;
+    const float ax = pi2 / (float)nx, ay = pi2 / (float)ny;
+    auto wxf = [&](int k) { return (double)((float)k * ax); };
+    auto wyf = [&](int k) { return (double)((float)(k <= ny / 2 ? k : k - ny) * ay); };
+    float* outs[3] = {x1, y1, xy2};
+    for (int d = 0; d < 3; ++d) {
+        // half-spectrum multipliers: i wx, i wy, -wx wy (FFTW sign: the
+        // reference's conjugations cancel on the way back)
+        std::vector<double> g(2 * (size_t)h * ny);
+        for (int ky = 0; ky < ny; ++ky)
+            for (int kx = 0; kx < h; ++kx) {
+                const double sr = a[2 * ((size_t)ky * nx + kx)], si = a[2 * ((size_t)ky * nx + kx) + 1];
+                double mr = 0.0, mi = 0.0;
+                if (d == 0) mi = wxf(kx);
+                else if (d == 1) mi = wyf(ky);
+                else mr = -wxf(kx) * wyf(ky);
+                g[2 * ((size_t)ky * h + kx)] = sr * mr - si * mi;
+                g[2 * ((size_t)ky * h + kx) + 1] = sr * mi + si * mr;
+            }
+        for (int kx = 0; kx < h; ++kx) {
+            if (!(kx == 0 || 2 * kx == nx)) continue;
+            std::vector<double> col(2 * (size_t)ny);
+            for (int ky = 0; ky < ny; ++ky) {
+                const int km = (ny - ky) % ny;
+                col[2 * ky] = 0.5 * (g[2 * ((size_t)ky * h + kx)] + g[2 * ((size_t)km * h + kx)]);
+                col[2 * ky + 1] = 0.5 * (g[2 * ((size_t)ky * h + kx) + 1] - g[2 * ((size_t)km * h + kx) + 1]);
+            }
+            for (int ky = 0; ky < ny; ++ky) {
+                g[2 * ((size_t)ky * h + kx)] = col[2 * ky];
+                g[2 * ((size_t)ky * h + kx) + 1] = col[2 * ky + 1];
+            }
+        }
+        // full Hermitian spectrum, then the inverse 2D DFT (real part), / (NX NY)
+        std::vector<double> f(2 * (size_t)nx * ny);
+        for (int ky = 0; ky < ny; ++ky)
+            for (int kx = 0; kx < nx; ++kx) {
+                double re, im;
+                if (kx < h) {
+                    re = g[2 * ((size_t)ky * h + kx)];
+                    im = g[2 * ((size_t)ky * h + kx) + 1];
+                } else {
+                    const int km = (ny - ky) % ny;
+                    re = g[2 * ((size_t)km * h + (nx - kx))];
+                    im = -g[2 * ((size_t)km * h + (nx - kx)) + 1];
+                }
+                f[2 * ((size_t)ky * nx + kx)] = re;
+                f[2 * ((size_t)ky * nx + kx) + 1] = im;
+            }
+        for (int y = 0; y < ny; ++y) fft_strided(qx, f.data() + 2 * (size_t)y * nx, 1, wi.data(), wo.data());
+        for (int x = 0; x < nx; ++x) fft_strided(qy, f.data() + 2 * x, nx, wi.data(), wo.data());
+        const double pix = 1.0 / ((double)nx * ny);
+        for (size_t q = 0; q < (size_t)nx * ny; ++q) outs[d][q] = (float)(f[2 * q] * pix);
+    }
+}
+
+// MODULO(A, P) for REAL A, P > 0: A - FLOOR(A / P) P, from the exact remainder.
+float rmodulo(float a, float p) {
+    float r = std::fmod(a, p);
+    if (r != 0.0f && (r < 0.0f) != (p < 0.0f)) r = r + p;
+    return r;
+}
+
+// RTSF (rtsf.f:106-205, scale 1): rotate by THETA degrees and shift by
+// (SHXI, SHYI) with FBS2 interpolation; the shift is taken MODULO the image
+// size and applied by wrapping the output coordinates.  REAL arithmetic; the
+// angle's cosine and sine are single-precision library values.
+void rtsf_ref(const float* img, float* out, int nx, int ny, float theta, float shxi, float shyi) {
+    std::vector<float> x1((size_t)nx * ny), y1((size_t)nx * ny), xy2((size_t)nx * ny);
+    fbs2_prep_ref(img, nx, ny, x1.data(), y1.data(), xy2.data());
+    const float pi = (float)3.14159265358979323846;
+    const float th = theta * pi / 180.0f;
+    const float cth = std::cos(th), sth = std::sin(th);
+    const int cx = nx / 2 + 1, cy = ny / 2 + 1;
+    const float shx = rmodulo(shxi, (float)nx), shy = rmodulo(shyi, (float)ny);
+    const float fy0 = (-shy) - (float)cy, fy1 = ((-shy) + (float)ny) - (float)cy, fy2 = ((-shy) - (float)ny) - (float)cy;
+    const float fx0 = (-shx) - (float)cx, fx1 = ((-shx) + (float)nx) - (float)cx, fx2 = ((-shx) - (float)nx) - (float)cx;
+    const float shypny = shy + (float)ny, shxpnx = shx + (float)nx;
+    for (int iy = 1; iy <= ny; ++iy) {
+        float fy = (float)iy + fy0;
+        if ((float)(iy - 1) < shy) fy = (float)iy + fy1;
+        if ((float)(iy - 1) >= shypny) fy = (float)iy + fy2;
+        const float ycod = cth * fy + (float)cy;
+        const float ysid = -(sth * fy) + (float)cx;
+        for (int ix = 1; ix <= nx; ++ix) {
+            float fx = (float)ix + fx0;
+            if ((float)(ix - 1) < shx) fx = (float)ix + fx1;
+            if ((float)(ix - 1) >= shxpnx) fx = (float)ix + fx2;
+            const float xold = cth * fx + ysid;
+            const float yold = sth * fx + ycod;
+            out[(size_t)(iy - 1) * nx + (ix - 1)] =
+                fbs2_ref(xold, yold, nx, nx, ny, img, nx, x1.data(), y1.data(), xy2.data(), true);
+        }
+    }
+}
+
 template <typename R> struct Rot;
 template <> struct Rot<float> {
     static void make(const double* ang, float* dm, int) { cang_ref((float)ang[2], (float)ang[1], (float)ang[0], dm); }
@@ -1022,6 +1168,16 @@ void or_sym_matmul_f32(const float* sm, const float* dm, float* out) { sym_matmu
 float or_triquad(double r, double s, double t, const float* f) { return triquad_ref(r, s, t, f); }
 void or_symvol(int nx, const float* q1, float* q2, const float* sm, int nsym) { symvol_ref(nx, q1, q2, sm, nsym); }
 float or_quadri(float x, float y, int nx, int ny, const float* f) { return quadri_ref(x, y, nx, ny, f); }
+float or_fbs2(float x, float y, int nxld, int nx, int ny, const float* fd, int nxp, const float* x1, const float* y1,
+              const float* xy2, int chk) {
+    return fbs2_ref(x, y, nxld, nx, ny, fd, nxp, x1, y1, xy2, chk != 0);
+}
+void or_fbs2_prep(const float* img, int nx, int ny, float* x1, float* y1, float* xy2) {
+    fbs2_prep_ref(img, nx, ny, x1, y1, xy2);
+}
+void or_rtsf(const float* img, float* out, int nx, int ny, float theta, float shx, float shy) {
+    rtsf_ref(img, out, nx, ny, theta, shx, shy);
+}
 void or_rtsq(const float* img, float* out, int nx, int ny, float theta, float scli, float shx, float shy) {
     rtsq_ref(img, out, nx, ny, theta, scli, shx, shy);
 }

@@ -33,6 +33,7 @@ TIMING = 0x10
 GEOM_F64 = 0x20
 PLAIN_THRESHOLD = 0x40
 HIPFFT = 0x80
+FBS = 0x100
 
 K_INSERT = 0
 K_PAD = 1
@@ -116,6 +117,8 @@ def lib():
         L.spbp_rtsq.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp]
         L.spbp_insert_rb_host.argtypes = [vp, vp, C.c_int, C.c_int, vp, vp, vp, vp]
         L.spbp_rtsq_host.argtypes = [vp, C.c_int, C.c_int, vp, vp]
+        L.spbp_rtsf.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp]
+        L.spbp_rtsf_host.argtypes = [vp, C.c_int, C.c_int, vp, vp]
         L.spbp_get_plan.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), vp]
         L.spbp_set_plan.argtypes = [vp, C.c_int, C.c_int, C.c_int]
         L.spbp_insert_batch.argtypes = [vp, C.c_int, vp]
@@ -136,7 +139,7 @@ def lib():
                      "spbp_gather", "spbp_scatter", "spbp_kb_table", "spbp_rotations", "spbp_tune",
                      "spbp_get_plan", "spbp_set_plan", "spbp_insert_batch", "spbp_graph_create",
                      "spbp_graph_launch", "spbp_graph_destroy", "spbp_set_symmetries", "spbp_insert_rb",
-                     "spbp_rtsq", "spbp_rtsq_host", "spbp_insert_rb_host", "spbp_threshold_stats", "spbp_reference_weights",
+                     "spbp_rtsq", "spbp_rtsq_host", "spbp_rtsf", "spbp_rtsf_host", "spbp_insert_rb_host", "spbp_threshold_stats", "spbp_reference_weights",
                      "spbp_calibrate", "spbp_scatter_images", "spbp_gather_images", "spbp_debug_fail_alloc", "spbp_scatter_noalloc"):
             getattr(L, name).restype = C.c_int
         _LIB = L

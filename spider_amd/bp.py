@@ -44,7 +44,7 @@ class Backprojector:
     def __init__(self, nx: int, two_halves: bool = False, exact_pi: bool = False,
                  twosum: bool = False, check_finite: bool = False, device=None, batch: int = 0,
                  timing: bool = False, strategy: str = "auto", brick_proj: int = 0, symmetries=None,
-                 geom_f64: bool = False, plain_threshold: bool = False, hipfft: bool = False):
+                 geom_f64: bool = False, plain_threshold: bool = False, hipfft: bool = False, fbs: bool = False):
         torch = _torch()
         if not torch.cuda.is_available():
             raise RuntimeError("spider_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
@@ -52,7 +52,8 @@ class Backprojector:
         flags = (_lib.TWO_HALVES if two_halves else 0) | (_lib.EXACT_PI if exact_pi else 0) | \
                 (_lib.TWOSUM_W if twosum else 0) | (_lib.CHECK_FINITE if check_finite else 0) | \
                 (_lib.TIMING if timing else 0) | (_lib.GEOM_F64 if geom_f64 else 0) | \
-                (_lib.PLAIN_THRESHOLD if plain_threshold else 0) | (_lib.HIPFFT if hipfft else 0)
+                (_lib.PLAIN_THRESHOLD if plain_threshold else 0) | (_lib.HIPFFT if hipfft else 0) | \
+                (_lib.FBS if fbs else 0)
         strat = {"auto": _lib.STRAT_AUTO, "direct": _lib.STRAT_DIRECT, "brick": _lib.STRAT_BRICK,
                  "gather": _lib.STRAT_GATHER}[strategy]
         opts = _lib.Opts(int(nx), flags, self.device.index, int(batch), strat, int(brick_proj))
@@ -411,9 +412,11 @@ class Backprojector:
         self._keep = (X, W)
 
 
-def rtsq(imgs, params):
+def rtsq(imgs, params, fbs: bool = False):
     """RTSQ (rtsq.f:56, scale 1) of device images (n, NX, NX) fp32/bf16 by
-    params (n, 3) = (angle in degrees, x shift, y shift): fp32 (n, NX, NX)."""
+    params (n, 3) = (angle in degrees, x shift, y shift): fp32 (n, NX, NX).
+    fbs: RTSF instead (FBS ON, rtsf.f:106: Fourier-based bicubic
+    interpolation)."""
     torch = _torch()
     if imgs.dim() != 3 or imgs.shape[1] != imgs.shape[2]:
         raise SpbpError(_lib.ERR_SIZE, "ONLY WORKS ON SQUARE IMAGES")
@@ -423,8 +426,9 @@ def rtsq(imgs, params):
         raise TypeError(f"unsupported image dtype {imgs.dtype}")
     p = np.ascontiguousarray(np.asarray(params, np.float32).reshape(-1, 3))
     out = torch.empty(imgs.shape, dtype=torch.float32, device=imgs.device)
-    _lib.check(_lib.lib().spbp_rtsq(C.c_void_p(imgs.data_ptr()), dt, imgs.shape[0], imgs.shape[1],
-                                    p.ctypes.data_as(C.c_void_p), C.c_void_p(out.data_ptr()), _stream(imgs.device)))
+    fn = _lib.lib().spbp_rtsf if fbs else _lib.lib().spbp_rtsq
+    _lib.check(fn(C.c_void_p(imgs.data_ptr()), dt, imgs.shape[0], imgs.shape[1],
+                  p.ctypes.data_as(C.c_void_p), C.c_void_p(out.data_ptr()), _stream(imgs.device)))
     return out
 
 
@@ -506,25 +510,26 @@ def bp32f_files(template, numbers, angdoc, symdoc, out_full, out_h1, out_h2, ext
     return vols
 
 
-def _write_transformed(rot_out, numbers, imgs, rot, ext):
+def _write_transformed(rot_out, numbers, imgs, rot, ext, fbs=False):
     """The transformed-images output of `RB 3F` / `RB 32F` (bp32f.f:517-527,
-    826-827): every image after its RTSQ, under the template rot_out
-    ('*' or None: not written), with the input's file numbers."""
+    826-827): every image after its RTSQ (RTSF with fbs), under the template
+    rot_out ('*' or None: not written), with the input's file numbers."""
     from . import spiderio
     if rot_out in (None, "", "*"):
         return
-    spiderio.write_images(rot_out, numbers, rtsq(imgs, rot).cpu().numpy(), ext)
+    spiderio.write_images(rot_out, numbers, rtsq(imgs, rot, fbs=fbs).cpu().numpy(), ext)
 
 
 def rb3f_files(template, numbers, angdoc, symdoc, out, ext: str = "", device=None, rot_out=None, **kw):
     """`RB 3F`: as `BP 3F`, each image first rotated by its in-plane angle and
-    shifted (doc registers 6, 7, 8; RTSQ, bp32f.f:805-837); rot_out is the
-    template for the transformed images (None or '*': not written)."""
+    shifted (doc registers 6, 7, 8; RTSQ, bp32f.f:805-837, or RTSF with
+    fbs=True, FBS ON); rot_out is the template for the transformed images
+    (None or '*': not written)."""
     from . import spiderio
     imgs, ang, rot = _load_series(template, numbers, angdoc, symdoc, ext, device, inplane=True)
     vol = bp3f(imgs, ang, inplane=rot, symmetries=read_symmetries(symdoc, ext), **kw).cpu().numpy()
     spiderio.write_image(out, vol)
-    _write_transformed(rot_out, numbers, imgs, rot, ext)
+    _write_transformed(rot_out, numbers, imgs, rot, ext, kw.get("fbs", False))
     return vol
 
 
@@ -534,7 +539,7 @@ def rb32f_files(template, numbers, angdoc, symdoc, out_full, out_h1, out_h2, ext
     transformed-images output."""
     from . import spiderio
     imgs, ang, rot = _load_series(template, numbers, angdoc, symdoc, ext, device, inplane=True)
-    _write_transformed(rot_out, numbers, imgs, rot, ext)
+    _write_transformed(rot_out, numbers, imgs, rot, ext, kw.get("fbs", False))
     full, h1, h2 = bp32f(imgs, ang, half=half, seed=seed, inplane=rot, symmetries=read_symmetries(symdoc, ext), **kw)
     vols = [v.cpu().numpy() for v in (full, h1, h2)]
     for path, v in zip((out_full, out_h1, out_h2), vols):

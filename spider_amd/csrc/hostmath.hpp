@@ -175,6 +175,33 @@ void rtsq_param(int nx, float theta, float shxi, float shyi, float* out4) {
     out4[3] = (float)(-cen) - shy;
 }
 
+// RTSF's per-image constants (rtsf.f:128-159, scale 1) in REAL: the angle in
+// radians as THETAT * PI / 180, its single-precision cosine and sine, the
+// shifts MODULO the size and the output-coordinate offsets.
+inline float real_modulo(float a, float p) {
+    float r = std::fmod(a, p);
+    if (r != 0.0f && (r < 0.0f) != (p < 0.0f)) r = r + p;
+    return r;
+}
+void rtsf_param(int nx, float theta, float shxi, float shyi, float* out12) {
+    const float pi = (float)3.14159265358979323846;
+    const float th = theta * pi / 180.0f;
+    const float n = (float)nx, c = (float)(nx / 2 + 1);
+    const float shx = real_modulo(shxi, n), shy = real_modulo(shyi, n);
+    out12[0] = std::cos(th);
+    out12[1] = std::sin(th);
+    out12[2] = shx;
+    out12[3] = shy;
+    out12[4] = shx + n;
+    out12[5] = shy + n;
+    out12[6] = (-shx) - c;
+    out12[7] = ((-shx) + n) - c;
+    out12[8] = ((-shx) - n) - c;
+    out12[9] = (-shy) - c;
+    out12[10] = ((-shy) + n) - c;
+    out12[11] = ((-shy) - n) - c;
+}
+
 // CANG in FP64 throughout, from FP64 angles (SPBP_GEOM_F64: no fp32 rounding
 // of the angles or of the matrix).
 void cang_d(double phi, double theta, double psi, bool exact_pi, double* dm) {

@@ -73,6 +73,19 @@ constexpr int CVALS[NCVALS] = {16, 32, 64, 128};   // projections per work item 
 }  // namespace
 
 // ------------------------------------------------------------------ context
+// RTSF scratch for sub-batches of up to `cap` images of NX^2: the R2C
+// spectra and the three derivative spectra per image, the derivative images,
+// fp32 copies of BF16 images, and R2C / C2R plans for two batch sizes (the
+// full sub-batch and one other).
+struct RtsfWork {
+    int nx = 0, cap = 0;
+    float2* spec = nullptr;   // 4 cap x NX x (NX/2+1)
+    float* der = nullptr;     // 3 cap x NX^2
+    float* img = nullptr;     // cap x NX^2
+    hipfftHandle fwd[2] = {0, 0}, inv[2] = {0, 0};
+    int pb[2] = {0, 0};
+};
+
 // Re-summation buffers of one finalised volume: the listed voxels, hit and
 // term counts and offsets, the terms and their sums, the counters, and a
 // pinned word area through which counts come back to the host.
@@ -156,6 +169,11 @@ struct spbp_ctx {
     // RB pre-rotation: rotated chunk and per-image RTSQ parameters
     float* rotbuf = nullptr;
     int rot_cap = 0;
+    RtsfParam* fprm_d = nullptr;   // SPBP_FBS: per-image RTSF constants
+    RtsfParam* fprm_h = nullptr;   // pinned
+    int fprm_cap = 0;
+    hipEvent_t fprm_evt = nullptr;
+    RtsfWork rtsf;                 // SPBP_FBS: RTSF scratch and transform plans
     RtsqParam* rprm_d = nullptr;
     RtsqParam* rprm_h = nullptr;   // pinned
     int rprm_cap = 0;
@@ -813,6 +831,146 @@ int check_images(spbp_ctx* c, const void* imgs, int dtype, int nimg, hipStream_t
     return SPBP_OK;
 }
 
+// ------------------------------------------------------------------ RTSF (FBS ON)
+void rtsf_release(RtsfWork& w) {
+    for (int k = 0; k < 2; ++k) {
+        if (w.fwd[k]) hipfftDestroy(w.fwd[k]);
+        if (w.inv[k]) hipfftDestroy(w.inv[k]);
+        w.fwd[k] = w.inv[k] = 0;
+        w.pb[k] = 0;
+    }
+    void* bufs[] = {w.spec, w.der, w.img};
+    for (void* q : bufs) if (q) (void)hipFree(q);
+    w.spec = nullptr; w.der = nullptr; w.img = nullptr;
+    w.nx = w.cap = 0;
+}
+
+// Images per RTSF sub-batch: about 256 MiB of scratch.
+int rtsf_cap(int nx, int want) {
+    const size_t per = (size_t)nx * (nx / 2 + 1) * 8 * 4 + (size_t)nx * nx * 4 * 4;
+    return (int)std::max<size_t>(1, std::min<size_t>((size_t)want, ((size_t)256 << 20) / per));
+}
+
+// Scratch for sub-batches of `cap` images (committed only once all of it exists).
+int rtsf_ensure(spbp_ctx* c, RtsfWork& w, int nx, int cap) {
+    if (w.nx == nx && w.cap >= cap && w.spec && w.der && w.img) return SPBP_OK;
+    rtsf_release(w);
+    const size_t H = nx / 2 + 1;
+    float2* spec = nullptr;
+    float *der = nullptr, *img = nullptr;
+    if (dmalloc(&spec, 4 * (size_t)cap * nx * H * sizeof(float2)) != hipSuccess ||
+        dmalloc(&der, 3 * (size_t)cap * nx * nx * sizeof(float)) != hipSuccess ||
+        dmalloc(&img, (size_t)cap * nx * nx * sizeof(float)) != hipSuccess) {
+        if (spec) (void)hipFree(spec);
+        if (der) (void)hipFree(der);
+        return fail(c, SPBP_ERR_ALLOC, "RTSF scratch for %d images of %d^2", cap, nx);
+    }
+    w.spec = spec; w.der = der; w.img = img;
+    w.nx = nx;
+    w.cap = cap;
+    return SPBP_OK;
+}
+
+// R2C / C2R plans for a sub-batch of b images: slot 0 holds the full
+// sub-batch's, slot 1 any other size.
+int rtsf_plans(spbp_ctx* c, RtsfWork& w, int b, hipfftHandle* fwd, hipfftHandle* inv) {
+    int k = b == w.pb[0] ? 0 : (b == w.pb[1] ? 1 : -1);
+    if (k < 0) {
+        k = w.pb[0] == 0 ? 0 : 1;
+        if (w.fwd[k]) hipfftDestroy(w.fwd[k]);
+        if (w.inv[k]) hipfftDestroy(w.inv[k]);
+        w.fwd[k] = w.inv[k] = 0;
+        w.pb[k] = 0;
+        int n[2] = {w.nx, w.nx};
+        if (hipfftPlanMany(&w.fwd[k], 2, n, nullptr, 1, 0, nullptr, 1, 0, HIPFFT_R2C, b) != HIPFFT_SUCCESS) {
+            w.fwd[k] = 0;
+            return fail(c, SPBP_ERR_FFT, "RTSF forward plan (%d^2 x %d)", w.nx, b);
+        }
+        if (hipfftPlanMany(&w.inv[k], 2, n, nullptr, 1, 0, nullptr, 1, 0, HIPFFT_C2R, 3 * b) != HIPFFT_SUCCESS) {
+            hipfftDestroy(w.fwd[k]);
+            w.fwd[k] = w.inv[k] = 0;
+            return fail(c, SPBP_ERR_FFT, "RTSF inverse plan (%d^2 x %d)", w.nx, 3 * b);
+        }
+        w.pb[k] = b;
+    }
+    *fwd = w.fwd[k];
+    *inv = w.inv[k];
+    return SPBP_OK;
+}
+
+__global__ __launch_bounds__(TPB) void widen_kernel(const uint16_t* __restrict__ in, size_t n, float* __restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = load_img(in, i);
+}
+
+// RTSF of nimg NX x NX images (fp32 / BF16) with per-image constants prm_d
+// (device) into out (fp32), on st, in sub-batches of w.cap images.
+int rtsf_run(spbp_ctx* c, RtsfWork& w, const void* imgs, int dtype, int nimg, int nx, const RtsfParam* prm_d,
+             float* out, hipStream_t st) {
+    const size_t npix = (size_t)nx * nx, H = nx / 2 + 1, nsp = (size_t)nx * H;
+    const size_t esz = dtype == SPBP_BF16 ? 2 : 4;
+    for (int s0 = 0; s0 < nimg; s0 += w.cap) {
+        const int b = std::min(w.cap, nimg - s0);
+        const void* src = (const char*)imgs + (size_t)s0 * npix * esz;
+        hipfftHandle fwd, inv;
+        int rc = rtsf_plans(c, w, b, &fwd, &inv);
+        if (rc) return rc;
+        // the transform reads a copy: the caller's images stay untouched whatever the FFT library does with its input
+        if (dtype == SPBP_BF16)
+            widen_kernel<<<nblocks((size_t)b * npix), TPB, 0, st>>>((const uint16_t*)src, (size_t)b * npix, w.img);
+        else
+            HIPCHK(c, hipMemcpyAsync(w.img, src, (size_t)b * npix * sizeof(float), hipMemcpyDeviceToDevice, st));
+        FFTCHK(c, hipfftSetStream(fwd, st));
+        FFTCHK(c, hipfftExecR2C(fwd, (hipfftReal*)w.img, (hipfftComplex*)w.spec));
+        float2* dspec = w.spec + (size_t)b * nsp;
+        fbs_deriv_kernel<<<nblocks((size_t)b * nsp), TPB, 0, st>>>(w.spec, nx, b, dspec);
+        HIPCHK(c, hipGetLastError());
+        FFTCHK(c, hipfftSetStream(inv, st));
+        FFTCHK(c, hipfftExecC2R(inv, (hipfftComplex*)dspec, (hipfftReal*)w.der));
+        const dim3 grid(nblocks(npix), b);
+        if (dtype == SPBP_BF16)
+            rtsf_kernel<uint16_t><<<grid, TPB, 0, st>>>((const uint16_t*)src, w.der, prm_d + s0, nx, out + (size_t)s0 * npix);
+        else
+            rtsf_kernel<float><<<grid, TPB, 0, st>>>((const float*)src, w.der, prm_d + s0, nx, out + (size_t)s0 * npix);
+        HIPCHK(c, hipGetLastError());
+    }
+    return SPBP_OK;
+}
+
+// RTSF constants of nimg images (checked finite) into h (host).
+int rtsf_params(spbp_ctx* c, int nx, int nimg, const float* inplane, RtsfParam* h) {
+    for (int p = 0; p < nimg; ++p) {
+        const float* q = inplane + 3 * (size_t)p;
+        if (!std::isfinite(q[0]) || !std::isfinite(q[1]) || !std::isfinite(q[2]))
+            return fail(c, SPBP_ERR_ANGLE, "MISSING ANGLE FOR IMAGE %d (in-plane angle or shift)", p + 1);
+        float v[12];
+        rtsf_param(nx, q[0], q[1], q[2], v);
+        h[p] = RtsfParam{v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8], v[9], v[10], v[11]};
+    }
+    return SPBP_OK;
+}
+
+// SPBP_FBS: the RTSF constants of nimg images into c->fprm_d, queued on st.
+int stage_rtsf(spbp_ctx* c, int nimg, const float* inplane, hipStream_t st) {
+    if (!c->fprm_evt) HIPCHK(c, hipEventCreateWithFlags(&c->fprm_evt, hipEventDisableTiming));
+    if (c->fprm_cap < nimg) {
+        c->fprm_cap = 0;   // committed again only once both buffers exist
+        if (c->fprm_h) { (void)hipEventSynchronize(c->fprm_evt); (void)hipHostFree(c->fprm_h); c->fprm_h = nullptr; }
+        if (c->fprm_d) { (void)hipFree(c->fprm_d); c->fprm_d = nullptr; }
+        const int cap = std::max(nimg, 1024);
+        HIPCHK(c, hipHostMalloc((void**)&c->fprm_h, cap * sizeof(RtsfParam), hipHostMallocDefault));
+        HIPCHK(c, dmalloc(&c->fprm_d, cap * sizeof(RtsfParam)));
+        c->fprm_cap = cap;
+    } else {
+        HIPCHK(c, hipEventSynchronize(c->fprm_evt));
+    }
+    int rc = rtsf_params(c, c->nx, nimg, inplane, c->fprm_h);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->fprm_d, c->fprm_h, nimg * sizeof(RtsfParam), hipMemcpyHostToDevice, st));
+    HIPCHK(c, hipEventRecord(c->fprm_evt, st));
+    return SPBP_OK;
+}
+
 // RTSQ parameters of nimg images (in-plane angle, x shift, y shift per image)
 // into c->rprm_d, queued on st.
 int stage_rtsq(spbp_ctx* c, int nimg, const float* inplane, hipStream_t st) {
@@ -857,8 +1015,10 @@ int insert_impl(spbp_ctx* c, const void* imgs, int dtype, int nimg, const float*
     int rc = check_images(c, imgs, dtype, nimg, st);
     if (rc) return rc;
     if ((rc = stage_geom(c, nimg, dm_in, ang, half, st))) return rc;
-    if (inplane && (rc = stage_rtsq(c, nimg, inplane, st))) return rc;
+    const bool fbs = inplane && (c->flags & SPBP_FBS);
+    if (inplane && (rc = fbs ? stage_rtsf(c, nimg, inplane, st) : stage_rtsq(c, nimg, inplane, st))) return rc;
     if ((rc = ensure_chunk(c, std::min(c->batch, nimg)))) return rc;
+    if (fbs && (rc = rtsf_ensure(c, c->rtsf, c->nx, rtsf_cap(c->nx, c->fwd_batch)))) return rc;
     const size_t esz = dtype == SPBP_BF16 ? 2 : 4;
     const size_t npix = (size_t)c->nx * c->nx;
     if (inplane && c->rot_cap < c->fwd_batch) {
@@ -873,7 +1033,10 @@ int insert_impl(spbp_ctx* c, const void* imgs, int dtype, int nimg, const float*
     for (int p0 = 0; p0 < nimg; p0 += c->fwd_batch) {
         const int b = std::min(c->fwd_batch, nimg - p0);
         const char* src = (const char*)imgs + (size_t)p0 * npix * esz;
-        if (inplane) {   // RTSQ before PADD2 (bp32f.f:805-837)
+        if (fbs) {   // RTSF before PADD2 (bp32f.f:811-816)
+            if ((rc = rtsf_run(c, c->rtsf, src, dtype, b, c->nx, c->fprm_d + p0, c->rotbuf, st))) return rc;
+            if ((rc = pad_fft(c, c->rotbuf, SPBP_F32, b, st))) return rc;
+        } else if (inplane) {   // RTSQ before PADD2 (bp32f.f:818-822)
             const dim3 grid(nblocks(npix), b);
             if (dtype == SPBP_BF16)
                 rtsq_kernel<uint16_t><<<grid, TPB, 0, st>>>((const uint16_t*)src, c->rprm_d + p0, c->nx, c->rotbuf);
@@ -1495,6 +1658,10 @@ int spbp_destroy(spbp_ctx* c) {
     if (c->jobs_h) (void)hipHostFree(c->jobs_h);
     if (c->rprm_evt) (void)hipEventSynchronize(c->rprm_evt);
     if (c->rprm_h) (void)hipHostFree(c->rprm_h);
+    if (c->fprm_evt) { (void)hipEventSynchronize(c->fprm_evt); (void)hipEventDestroy(c->fprm_evt); }
+    if (c->fprm_h) (void)hipHostFree(c->fprm_h);
+    if (c->fprm_d) (void)hipFree(c->fprm_d);
+    rtsf_release(c->rtsf);
     if (c->rprm_evt) (void)hipEventDestroy(c->rprm_evt);
     if (c->stage) (void)hipHostFree(c->stage);
     if (c->fwd) hipfftDestroy(c->fwd);
@@ -2104,6 +2271,50 @@ int spbp_rtsq_host(const float* imgs_host, int nimg, int nx, const float* params
     if (rc == SPBP_OK) rc = spbp_rtsq(din, SPBP_F32, nimg, nx, params, dout, nullptr);
     if (rc == SPBP_OK && hipMemcpy(out_host, dout, bytes, hipMemcpyDeviceToHost) != hipSuccess)
         rc = fail(nullptr, SPBP_ERR_DEVICE, "rtsq: image download failed");
+    (void)hipFree(din);
+    (void)hipFree(dout);
+    return rc;
+}
+
+int spbp_rtsf(const void* imgs, int dtype, int nimg, int nx, const float* params, float* out, void* stream) {
+    if (nimg < 0 || nx < 2) return fail(nullptr, SPBP_ERR_ARG, "bad image count or size");
+    if (nimg == 0) return SPBP_OK;
+    if (!imgs || !params || !out) return fail(nullptr, SPBP_ERR_ARG, "NULL pointer");
+    if (dtype != SPBP_F32 && dtype != SPBP_BF16) return fail(nullptr, SPBP_ERR_ARG, "unknown image type %d", dtype);
+    std::vector<RtsfParam> h(nimg);
+    int rc = rtsf_params(nullptr, nx, nimg, params, h.data());
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    RtsfWork w;
+    RtsfParam* d = nullptr;
+    if (dmalloc(&d, nimg * sizeof(RtsfParam)) != hipSuccess) return fail(nullptr, SPBP_ERR_ALLOC, "RTSF parameters");
+    rc = rtsf_ensure(nullptr, w, nx, rtsf_cap(nx, nimg));
+    if (rc == SPBP_OK && hipMemcpyAsync(d, h.data(), nimg * sizeof(RtsfParam), hipMemcpyHostToDevice, st) != hipSuccess)
+        rc = fail(nullptr, SPBP_ERR_DEVICE, "RTSF parameter upload failed");
+    if (rc == SPBP_OK) rc = rtsf_run(nullptr, w, imgs, dtype, nimg, nx, d, out, st);
+    if (rc == SPBP_OK && hipStreamSynchronize(st) != hipSuccess) rc = fail(nullptr, SPBP_ERR_DEVICE, "RTSF failed");
+    rtsf_release(w);
+    (void)hipFree(d);
+    return rc;
+}
+
+int spbp_rtsf_host(const float* imgs_host, int nimg, int nx, const float* params, float* out_host) {
+    if (nimg < 0 || nx < 2) return fail(nullptr, SPBP_ERR_ARG, "bad image count or size");
+    if (nimg == 0) return SPBP_OK;
+    if (!imgs_host || !params || !out_host) return fail(nullptr, SPBP_ERR_ARG, "NULL pointer");
+    const size_t bytes = (size_t)nimg * nx * nx * sizeof(float);
+    float *din = nullptr, *dout = nullptr;
+    if (dmalloc(&din, bytes) != hipSuccess) return fail(nullptr, SPBP_ERR_ALLOC, "rtsf staging: %zu bytes", bytes);
+    if (dmalloc(&dout, bytes) != hipSuccess) {
+        (void)hipFree(din);
+        return fail(nullptr, SPBP_ERR_ALLOC, "rtsf staging: %zu bytes", bytes);
+    }
+    int rc = SPBP_OK;
+    if (hipMemcpy(din, imgs_host, bytes, hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(nullptr, SPBP_ERR_DEVICE, "rtsf: image upload failed");
+    if (rc == SPBP_OK) rc = spbp_rtsf(din, SPBP_F32, nimg, nx, params, dout, nullptr);
+    if (rc == SPBP_OK && hipMemcpy(out_host, dout, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = fail(nullptr, SPBP_ERR_DEVICE, "rtsf: image download failed");
     (void)hipFree(din);
     (void)hipFree(dout);
     return rc;

@@ -14,7 +14,9 @@
 !            angle and shifts applied by RTSQ before the insertion
 !            (bp32f.f:805-837)
 !
-! Usage:  spbp_bp 3F|32F|RB3F|RB32F [data-extension] < answers
+! Usage:  spbp_bp 3F|32F|RB3F|RB32F [data-extension] [FBS] < answers
+! FBS: the RB pre-rotation by RTSF (Fourier-based bicubic interpolation), as
+! after SPIDER's `MD` `FBS ON` (setmode.f:128,169; bp32f.f:811-816).
 ! Text after ';' on an answer line is a comment.  Images are SPIDER 2D files
 ! (`name***` templates, or `stack@***` over stacks and indexed stacks), the
 ! angles doc holds PSI, THETA, PHI
@@ -36,7 +38,7 @@ program spbp_bp
    type(c_ptr) :: ctx
    integer :: nang, nx, ny, k, ierr, nout, kidx, i, j, iord
    integer(c_int) :: rc
-   logical :: is32, isrb
+   logical :: is32, isrb, fbs
    real(c_float), allocatable :: inplane(:, :)
    real(c_float), allocatable :: rotimgs(:, :, :)
    real(c_double), allocatable :: symang(:, :)
@@ -47,6 +49,11 @@ program spbp_bp
    call get_command_argument(1, op)
    ext = ' '
    if (command_argument_count() >= 2) call get_command_argument(2, ext)
+   fbs = .false.
+   if (command_argument_count() >= 3) then
+      call get_command_argument(3, fname)
+      fbs = trim(fname) == 'FBS' .or. trim(fname) == 'fbs'
+   end if
    isrb = .false.
    select case (trim(op))
    case ('3F', '3f', 'BP 3F')
@@ -64,7 +71,7 @@ program spbp_bp
       isrb = .true.
       nout = 3
    case default
-      write (*, '(A)') ' usage: spbp_bp 3F|32F|RB3F|RB32F [data-extension] < answers'
+      write (*, '(A)') ' usage: spbp_bp 3F|32F|RB3F|RB32F [data-extension] [FBS] < answers'
       stop 2
    end select
 
@@ -145,7 +152,7 @@ program spbp_bp
    end if
 
    opts%nx = nx
-   opts%flags = merge(SPBP_TWO_HALVES, 0, is32)
+   opts%flags = ior(merge(SPBP_TWO_HALVES, 0, is32), merge(SPBP_FBS, 0, fbs))
    opts%device = -1
    rc = spbp_create(opts, ctx)
    if (rc /= SPBP_OK) call errt(int(rc), trim(spbp_errmsg(c_null_ptr)))
@@ -156,7 +163,11 @@ program spbp_bp
    if (isrb .and. trim(rottmpl) /= '*') then
       ! the transformed images are written, then inserted as they are
       allocate (rotimgs(nx, nx, nang))
-      rc = spbp_rtsq_host(imgs, int(nang, c_int), int(nx, c_int), inplane, rotimgs)
+      if (fbs) then
+         rc = spbp_rtsf_host(imgs, int(nang, c_int), int(nx, c_int), inplane, rotimgs)
+      else
+         rc = spbp_rtsq_host(imgs, int(nang, c_int), int(nx, c_int), inplane, rotimgs)
+      end if
       if (rc /= SPBP_OK) call errt(int(rc), trim(spbp_errmsg(c_null_ptr)))
       call write_images(rottmpl, nums, ext, rotimgs)
       imgs = rotimgs

@@ -14,8 +14,10 @@ module spbp
       SPBP_ERR_NONFINITE = 205
    integer(c_int), parameter, public :: SPBP_F32 = 0, SPBP_BF16 = 1
    integer(c_int), parameter, public :: SPBP_TWO_HALVES = 1, SPBP_EXACT_PI = 2, SPBP_TWOSUM_W = 4, &
-      SPBP_CHECK_FINITE = 8, SPBP_TIMING = 16, SPBP_GEOM_F64 = 32, SPBP_PLAIN_THRESHOLD = 64, SPBP_HIPFFT = 128
-   integer(c_int), parameter, public :: SPBP_STRAT_AUTO = 0, SPBP_STRAT_DIRECT = 1, SPBP_STRAT_BRICK = 2
+      SPBP_CHECK_FINITE = 8, SPBP_TIMING = 16, SPBP_GEOM_F64 = 32, SPBP_PLAIN_THRESHOLD = 64, SPBP_HIPFFT = 128, &
+      SPBP_FBS = 256
+   integer(c_int), parameter, public :: SPBP_STRAT_AUTO = 0, SPBP_STRAT_DIRECT = 1, SPBP_STRAT_BRICK = 2, &
+      SPBP_STRAT_GATHER = 3
 
    type, bind(c), public :: spbp_opts
       integer(c_int) :: nx = 0
@@ -27,7 +29,7 @@ module spbp
    end type spbp_opts
 
    public :: spbp_create, spbp_destroy, spbp_insert_host, spbp_finalize_host, spbp_reset, spbp_errmsg, &
-      spbp_set_symmetries, spbp_insert_rb_host, spbp_rtsq_host
+      spbp_set_symmetries, spbp_insert_rb_host, spbp_rtsq_host, spbp_rtsf_host
 
    interface
       integer(c_int) function spbp_create(opts, ctx) bind(c, name='spbp_create')
@@ -80,6 +82,15 @@ module spbp
          real(c_float), intent(in) :: params(*)
          real(c_float), intent(out) :: out(*)
       end function spbp_rtsq_host
+
+      ! RTSF of host images (FBS ON)
+      integer(c_int) function spbp_rtsf_host(imgs, nimg, nx, params, out) bind(c, name='spbp_rtsf_host')
+         import :: c_int, c_float
+         real(c_float), intent(in) :: imgs(*)
+         integer(c_int), value :: nimg, nx
+         real(c_float), intent(in) :: params(*)
+         real(c_float), intent(out) :: out(*)
+      end function spbp_rtsf_host
 
       ! symmetries doc rows: 3 x nsym DOUBLE PRECISION (PSI, THETA, PHI)
       integer(c_int) function spbp_set_symmetries(ctx, nsym, angles) bind(c, name='spbp_set_symmetries')
