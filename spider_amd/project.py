@@ -273,3 +273,91 @@ def cgls(proj: Projector, images, iters: int = 30, x0=None, callback=None):
             if callback is not None:
                 callback(it + 1, x)
         return x, hist
+
+
+class DensityPreconditioner:
+    """M ~ (A^T A)^-1 built from the sampling density of the projector's
+    rotations: M r = D crop(irfftn(rfftn(pad(D r)) / max(W, floor))), W the
+    post-SYMPLANE0 weights the insertion accumulates for those rotations
+    (their value does not depend on the images), D the kernel's transform.
+    A^T A is close to (1/D) crop(F^-1 rho F pad(v / D)) with rho the density,
+    so one application of M to A^T y is a gridding reconstruction; M is
+    symmetric positive definite (pad is crop's transpose, the multiplier is
+    real, non-negative and Hermitian-symmetric).  As NRMW2 does, voxels whose
+    W does not exceed `threshold` (0.1, bp3f.f:1359) carry no estimate: there
+    X / W is dominated by kernel tails, and M is semi-definite, so CG works in
+    the well-sampled subspace.  threshold=None keeps every voxel, with W
+    clamped below at `floor` times max W (M then positive definite)."""
+
+    def __init__(self, proj: Projector, floor: float = 1e-3, batch: int = 4096, threshold=0.1):
+        torch = _torch()
+        nx, n = proj.nx, proj.nproj
+        with _bp.Backprojector(nx, device=proj.device.index) as b:
+            zeros = torch.zeros((min(batch, n), nx, nx), dtype=torch.float32, device=proj.device)
+            for i0 in range(0, n, batch):
+                i1 = min(n, i0 + batch)
+                b.insert_rotations(zeros[: i1 - i0], proj.dm[i0:i1])
+            _, W = b.export_partial()
+        W = W.to(torch.float32)
+        self.inv = 1.0 / torch.clamp(W, min=float(floor) * float(W.max()))
+        if threshold is not None:
+            # NRMW2's rule (bp3f.f:1359): voxels with W <= threshold carry no estimate;
+            # M is then positive semi-definite and CG stays in the well-sampled subspace
+            self.inv = torch.where(W > float(threshold), self.inv, torch.zeros_like(self.inv))
+        self.proj = proj
+
+    def __call__(self, r):
+        torch = _torch()
+        P = self.proj
+        p0, p1 = P.ip, P.N - P.ip - P.nx
+        u = torch.nn.functional.pad(r * P.D, (p0, p1, p0, p1, p0, p1))
+        g = torch.fft.irfftn(torch.fft.rfftn(u) * self.inv, s=(P.N,) * 3)
+        return g[P.ip:P.ip + P.nx, P.ip:P.ip + P.nx, P.ip:P.ip + P.nx] * P.D
+
+
+def pcg(proj: Projector, images, iters: int = 30, precond=None, damp: float = 0.0, x0=None, callback=None):
+    """Preconditioned conjugate gradients on the (damped) normal equations
+    (A^T A + damp^2 I) x = A^T y: min ||A x - y||^2 + damp^2 ||x||^2, with
+    an SPD preconditioner M ~ (A^T A)^-1 (DensityPreconditioner; None: the
+    identity, i.e. plain CG on the normal equations).  Returns (x, history)
+    with the data residual norm and the seconds per iteration."""
+    torch = _torch()
+    d2 = float(damp) ** 2
+    M = precond if precond is not None else (lambda r: r)
+
+    def dot(a, b):
+        return float(torch.sum(a.double() * b.double()))
+
+    with torch.no_grad():
+        y = images.to(torch.float32)
+        x = torch.zeros((proj.nx,) * 3, dtype=torch.float32, device=proj.device) if x0 is None else x0.clone()
+        r = proj.adjoint(y - proj.apply(x) if x0 is not None else y)
+        if x0 is not None and d2:
+            r.sub_(x, alpha=d2)
+        z = M(r)
+        p = z.clone()
+        rz = dot(r, z)
+        hist = []
+        for it in range(iters):
+            t0 = time.perf_counter()
+            Ap = proj.apply(p)
+            q = proj.adjoint(Ap)
+            if d2:
+                q.add_(p, alpha=d2)
+            pq = dot(p, q)
+            if pq <= 0.0 or rz == 0.0:
+                break
+            alpha = rz / pq
+            x.add_(p, alpha=alpha)
+            r.sub_(q, alpha=alpha)
+            z = M(r)
+            rz_new = dot(r, z)
+            p.mul_(rz_new / rz).add_(z)
+            rz = rz_new
+            torch.cuda.synchronize(proj.device)
+            hist.append({"iter": it + 1, "normal_residual": math.sqrt(max(dot(r, r), 0.0)),
+                         "seconds": time.perf_counter() - t0})
+            if callback is not None:
+                callback(it + 1, x)
+        return x, hist
+
