@@ -449,7 +449,10 @@ def run_batched(args):
            "plain_seconds": round(t_plain, 4), "batched_seconds": round(t_fast, 4),
            "tune_seconds": round(t_tune, 3), "capture_seconds": round(t_capture, 3),
            "finalize_seconds_one_by_one": round(t_fin_plain, 4), "finalize_seconds_batched": round(t_fin_batch, 4),
-           "plans": [{"strategy": p["strategy"], "brick": p["brick"], "brick_proj": p["brick_proj"]} for p in plans],
+           "plans": [{"strategy": p["strategy"], "brick": p["brick"], "brick_proj": p["brick_proj"],
+                      "best_brick_ms_per_proj": round(float(np.min(p["ms_per_proj"])), 6),
+                      "direct_ms_per_proj": round(p["direct_ms_per_proj"], 6),
+                      "gather_ms_per_proj": round(p["gather_ms_per_proj"], 6)} for p in plans],
            "volumes": per}
     print(json.dumps(out), flush=True)
 
