@@ -138,9 +138,17 @@ __device__ inline void brick_origin(const BrickGrid& bg, int b, int nx, int& x0,
     z0 = -nx + 1 + k * bg.bz;
 }
 
+// The list builders read, per section, only its plane's normal DM(7:9) and its
+// half: packed into 16 bytes (x, y, z, half) so that the (brick, section)
+// tests read a sixth of the section records' bytes.
+__global__ __launch_bounds__(TPB) void plane_pack_kernel(const Geom* __restrict__ geom, int n, float4* __restrict__ pln) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) pln[p] = make_float4(geom[p].r[6], geom[p].r[7], geom[p].r[8], geom[p].half ? 1.0f : 0.0f);
+}
+
 // Does the plane through the origin with normal n (DM(7:9)) pass through the
 // brick's expanded box, inside the sample disk of radius NX?
-__device__ inline bool brick_hit(const BrickGrid& bg, int b, int nx, const Geom& g) {
+__device__ inline bool brick_hit(const BrickGrid& bg, int b, int nx, const float4& g) {
     int x0, y0, z0;
     brick_origin(bg, b, nx, x0, y0, z0);
     // owner-computes: samples whose nearest voxel is in the core (within 0.5
@@ -149,8 +157,8 @@ __device__ inline bool brick_hit(const BrickGrid& bg, int b, int nx, const Geom&
     const float m = bg.own ? (x0 == 0 ? 2.52f : 0.52f) : BRICK_MARGIN;
     const float hx = 0.5f * (bg.bx - 1) + m, hy = 0.5f * (bg.by - 1) + m, hz = 0.5f * (bg.bz - 1) + m;
     const float cx = x0 + 0.5f * (bg.bx - 1), cy = y0 + 0.5f * (bg.by - 1), cz = z0 + 0.5f * (bg.bz - 1);
-    const float d = fabsf(g.r[6] * cx + g.r[7] * cy + g.r[8] * cz);
-    const float reach = hx * fabsf(g.r[6]) + hy * fabsf(g.r[7]) + hz * fabsf(g.r[8]) + 0.02f;
+    const float d = fabsf(g.x * cx + g.y * cy + g.z * cz);
+    const float reach = hx * fabsf(g.x) + hy * fabsf(g.y) + hz * fabsf(g.z) + 0.02f;
     if (d > reach) return false;
     const float qx = fmaxf(fabsf(cx) - hx, 0.f), qy = fmaxf(fabsf(cy) - hy, 0.f), qz = fmaxf(fabsf(cz) - hz, 0.f);
     const float r = (float)nx + 0.02f;
@@ -158,14 +166,14 @@ __device__ inline bool brick_hit(const BrickGrid& bg, int b, int nx, const Geom&
 }
 
 // Count (brick, half) hits.  Block: one brick x 1024 projections.
-__global__ __launch_bounds__(TPB) void brick_count_kernel(const Geom* __restrict__ geom, int nproj, BrickGrid bg, int nx,
+__global__ __launch_bounds__(TPB) void brick_count_kernel(const float4* __restrict__ pln, int nproj, BrickGrid bg, int nx,
                                                           int* __restrict__ cnt) {
     const int b = blockIdx.x;
     const int p0 = blockIdx.y * 1024;
     unsigned c0 = 0, c1 = 0;
     for (int p = p0 + threadIdx.x; p < min(nproj, p0 + 1024); p += TPB) {
-        const Geom g = geom[p];
-        if (brick_hit(bg, b, nx, g)) { if (g.half) ++c1; else ++c0; }
+        const float4 g = pln[p];
+        if (brick_hit(bg, b, nx, g)) { if (g.w != 0.0f) ++c1; else ++c0; }
     }
     double d0 = c0;
     block_sum2(d0, c1);
@@ -243,7 +251,7 @@ __global__ __launch_bounds__(1024) void key_scan_add_kernel(int nkeys, const int
 // Write the projection lists of every (brick, half) key.  Block: one brick x
 // 1024 projections; each half's hits of a round of TPB projections take one
 // cursor reservation per block (wave ballots give the ranks).
-__global__ __launch_bounds__(TPB) void brick_fill_kernel(const Geom* __restrict__ geom, int nproj, BrickGrid bg, int nx,
+__global__ __launch_bounds__(TPB) void brick_fill_kernel(const float4* __restrict__ pln, int nproj, BrickGrid bg, int nx,
                                                          const int* __restrict__ off, int* __restrict__ cursor,
                                                          int* __restrict__ list) {
     __shared__ int wc[2][TPB / 64];
@@ -256,8 +264,8 @@ __global__ __launch_bounds__(TPB) void brick_fill_kernel(const Geom* __restrict_
         const int p = r + threadIdx.x;
         bool h0 = false, h1 = false;
         if (p < nproj) {
-            const Geom& g = geom[p];
-            if (brick_hit(bg, b, nx, g)) { h1 = g.half != 0; h0 = !h1; }
+            const float4 g = pln[p];
+            if (brick_hit(bg, b, nx, g)) { h1 = g.w != 0.0f; h0 = !h1; }
         }
         const unsigned long long m0 = __ballot(h0), m1 = __ballot(h1);
         if (lane == 0) { wc[0][w] = __popcll(m0); wc[1][w] = __popcll(m1); }

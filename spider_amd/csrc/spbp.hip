@@ -193,6 +193,8 @@ struct spbp_ctx {
     int* bcur = nullptr;    // [2 nb] fill cursors
     int2* bscan = nullptr;  // [ceil(2 nb / 1024)] key-scan block totals
     int* bnext = nullptr;   // work-queue head
+    float4* bpln = nullptr; // per section: plane normal and half, for the list builders
+    size_t bpln_cap = 0;
     int* blist = nullptr;   // projection lists
     int* bikey = nullptr;   // work item -> key
     size_t blist_cap = 0;
@@ -363,15 +365,20 @@ int ensure_lists(spbp_ctx* c, int B) {
         if (rc) return rc;
     }
     const size_t cap = (size_t)B * c->nsym * c->bg.nb;   // B images, nsym sections each
-    if (c->blist && c->bikey && c->blist_cap >= cap) return SPBP_OK;
+    const size_t ns = (size_t)B * c->nsym;
+    if (c->blist && c->bikey && c->blist_cap >= cap && c->bpln && c->bpln_cap >= ns) return SPBP_OK;
     if (c->graphs > 0) return refuse_captured(c, "growing the brick lists");
     c->blist_cap = 0;
+    c->bpln_cap = 0;
     if (c->blist) { (void)hipFree(c->blist); c->blist = nullptr; }
     if (c->bikey) { (void)hipFree(c->bikey); c->bikey = nullptr; }
+    if (c->bpln) { (void)hipFree(c->bpln); c->bpln = nullptr; }
     HIPCHK(c, dmalloc(&c->blist, cap * sizeof(int)));
     // items <= sum over keys of ceil(cnt / C) <= pairs + keys
     HIPCHK(c, dmalloc(&c->bikey, (cap + 2 * (size_t)c->bg.nb) * sizeof(int)));
+    HIPCHK(c, dmalloc(&c->bpln, ns * sizeof(float4)));
     c->blist_cap = cap;
+    c->bpln_cap = ns;
     return SPBP_OK;
 }
 
@@ -669,18 +676,20 @@ int launch_bricks(spbp_ctx* c, const BrickJob* jh, int n, int* counter, hipStrea
 // Per-(brick, half) projection lists of b projections (geometry at geom_d);
 // also zeroes c's queue head.
 int build_lists(spbp_ctx* c, const Geom* geom_d, int b, hipStream_t st) {
-    if (c->bg.nb <= 0 || !c->bcnt || !c->blist || !c->bikey || c->blist_cap < (size_t)b * c->bg.nb)
+    if (c->bg.nb <= 0 || !c->bcnt || !c->blist || !c->bikey || c->blist_cap < (size_t)b * c->bg.nb || !c->bpln ||
+        c->bpln_cap < (size_t)b)
         return fail(c, SPBP_ERR_ARG, "brick lists not allocated for %d sections", b);
     const int nkeys = 2 * c->bg.nb;
     const dim3 lgrid(c->bg.nb, (b + 1023) / 1024);
     Timed tm(c, SPBP_K_BIN, b, st);
     fill_kernel<<<nblocks(nkeys), TPB, 0, st>>>((float*)c->bcnt, nkeys, 0.0f);   // zero bits
-    brick_count_kernel<<<lgrid, TPB, 0, st>>>(geom_d, b, c->bg, c->nx, c->bcnt);
+    plane_pack_kernel<<<nblocks(b), TPB, 0, st>>>(geom_d, b, c->bpln);
+    brick_count_kernel<<<lgrid, TPB, 0, st>>>(c->bpln, b, c->bg, c->nx, c->bcnt);
     const int kb = (nkeys + 1023) / 1024;
     key_scan_blocks_kernel<<<kb, 1024, 0, st>>>(c->bcnt, nkeys, item_proj(c->C, b), c->boff, c->bioff, c->bscan);
     key_scan_sums_kernel<<<1, 1024, 0, st>>>(c->bscan, kb, nkeys, c->boff, c->bioff, c->bnext);
     key_scan_add_kernel<<<kb, 1024, 0, st>>>(nkeys, c->bscan, c->boff, c->bioff, c->bcur);
-    brick_fill_kernel<<<lgrid, TPB, 0, st>>>(geom_d, b, c->bg, c->nx, c->boff, c->bcur, c->blist);
+    brick_fill_kernel<<<lgrid, TPB, 0, st>>>(c->bpln, b, c->bg, c->nx, c->boff, c->bcur, c->blist);
     brick_itemkey_kernel<<<nblocks(nkeys), TPB, 0, st>>>(c->bioff, nkeys, c->bikey);
     HIPCHK(c, hipGetLastError());
     return SPBP_OK;
@@ -1643,7 +1652,7 @@ int spbp_destroy(spbp_ctx* c) {
         if (c->Wc[h]) (void)hipFree(c->Wc[h]);
     }
     void* dev_ptrs[] = {c->tab, c->padbuf, c->spec, c->vmax, c->geom_d, c->work, c->real, c->wkb, c->psum, c->pcnt, c->mean, c->flag,
-                        c->bcnt, c->boff, c->bioff, c->bcur, c->bscan, c->bnext, c->blist, c->bikey, c->jobs_d,
+                        c->bcnt, c->boff, c->bioff, c->bcur, c->bscan, c->bnext, c->blist, c->bikey, c->bpln, c->jobs_d,
                         c->sm_d, c->symtmp, c->rotbuf, c->rprm_d, c->hist_d, c->tab64, c->tmask, c->tw, c->fmean, c->fjobs_d};
     for (void* p : dev_ptrs) if (p) (void)hipFree(p);
     if (c->stats_evt) { (void)hipEventSynchronize(c->stats_evt); (void)hipEventDestroy(c->stats_evt); }
@@ -2717,8 +2726,9 @@ int spbp_scatter_noalloc(int nx, int flags, const float* bi, int nimg, const dou
     c.bnext = cv.take<int>(1);
     BrickJob* job_d = cv.take<BrickJob>(1);
     // per projection: geometry, raw spectrum, its maximum, list entries and work items
-    const size_t per = sizeof(Geom) + (size_t)N * H * sizeof(float2) + sizeof(float) + 2 * (size_t)c.bg.nb * sizeof(int);
-    const size_t fixed = nk * sizeof(int) + 6 * 256;   // the work-item keys' extra 2 nb, alignment
+    const size_t per = sizeof(Geom) + sizeof(float4) + (size_t)N * H * sizeof(float2) + sizeof(float) +
+                       2 * (size_t)c.bg.nb * sizeof(int);
+    const size_t fixed = nk * sizeof(int) + 7 * 256;   // the work-item keys' extra 2 nb, alignment
     size_t B = 0;
     if (c.tab && job_d && cv.left > fixed) B = std::min<size_t>((cv.left - fixed) / per, SCRATCH_CHUNK);
     B = std::min<size_t>(B, (size_t)nimg);
@@ -2730,9 +2740,11 @@ int spbp_scatter_noalloc(int nx, int flags, const float* bi, int nimg, const dou
     c.vmax = cv.take<float>(B);
     c.blist = cv.take<int>(B * c.bg.nb);
     c.bikey = cv.take<int>(B * c.bg.nb + nk);
-    if (!geom_d || !c.spec || !c.vmax || !c.blist || !c.bikey)
+    c.bpln = cv.take<float4>(B);
+    if (!geom_d || !c.spec || !c.vmax || !c.blist || !c.bikey || !c.bpln)
         return fail(nullptr, SPBP_ERR_SIZE, "scratch layout for NX = %d does not fit", nx);
     c.blist_cap = B * c.bg.nb;
+    c.bpln_cap = B;
     c.fwd_batch = (int)B;
     // host staging on the stack
     float tabh[LTAB + 1];
