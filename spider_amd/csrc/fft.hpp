@@ -24,12 +24,22 @@
 
 namespace spbp {
 
-constexpr int FFT_ELEMS = 4096;             // complex values one block transforms at a time
+#ifndef SPBP_FFT_ROW_ELEMS
+#define SPBP_FFT_ROW_ELEMS 2048
+#endif
+constexpr int FFT_ELEMS = 4096;             // complex values a columns-pass block transforms at a time
 constexpr int FFT_LDS = FFT_ELEMS + 64;     // plus padding of the column layout
 constexpr int FFT_BATCH = 8;                // global loads in flight per thread
+// The rows pass transforms fewer values per block, so that more blocks share
+// a CU and their loads overlap: measured per 20,000 images of 128² (8,000 of
+// 256²), rows pass 1.78 ms (2.70) with 2048 values per block against 2.42
+// (3.30) with 4096.  The columns pass keeps 4096: its row segments are then
+// 16 (8) columns wide, and 2048 made it 2.45 → 3.38 ms (5.57 → 7.91).
+constexpr int FFT_ROW_ELEMS = SPBP_FFT_ROW_ELEMS;
 
-// Dynamic LDS of the transform kernels: the data, then the N twiddles.
-inline size_t fft_lds_bytes(int N) { return (size_t)(FFT_LDS + N) * sizeof(float2); }
+// Dynamic LDS of the transform kernels (E values per block): the data, then
+// the N twiddles.
+inline size_t fft_lds_bytes(int N, int E = FFT_ELEMS) { return (size_t)(E + 64 + N) * sizeof(float2); }
 __device__ inline void load_twiddles(float2* stw, const float2* __restrict__ tw, int N) {
     for (int i = threadIdx.x; i < N; i += blockDim.x) stw[i] = tw[i];
 }
@@ -43,13 +53,14 @@ __device__ inline float2 c_sub(float2 a, float2 b) { return make_float2(__fsub_r
 
 // Forward DFTs of M sequences of length N = 2^L in LDS (sequence m at
 // buf + m S), in place: every pass reads all its inputs into registers,
-// synchronises, then writes.  M N <= FFT_ELEMS.  Ends synchronised.  The
-// per-thread loops have fixed trip counts so that their arrays stay in
-// registers.
+// synchronises, then writes.  M N <= E.  Ends synchronised.  The per-thread
+// loops have fixed trip counts so that their arrays stay in registers.
+template <int E>
 __device__ inline void lds_fft(float2* buf, int M, int N, int L, int S, const float2* __restrict__ tw) {
+    static_assert(E >= 4 * TPB, "a radix-4 pass gives every thread at least one butterfly slot");
     int Ns = 1;
     if (L & 1) {   // radix-2 pass
-        constexpr int U = FFT_ELEMS / 2 / TPB;
+        constexpr int U = E / 2 / TPB;
         const int half = N >> 1, nb = M * half;
         float2 a[U], b[U];
         int dst[U];
@@ -74,7 +85,7 @@ __device__ inline void lds_fft(float2* buf, int M, int N, int L, int S, const fl
         Ns = 2;
     }
     for (; Ns < N; Ns <<= 2) {   // radix-4 passes
-        constexpr int U = FFT_ELEMS / 4 / TPB;
+        constexpr int U = E / 4 / TPB;
         const int q = N >> 2, nb = M * q, step = N / (4 * Ns);
         float2 x0[U], x1[U], x2[U], x3[U];
         int dst[U];
@@ -145,7 +156,7 @@ __global__ __launch_bounds__(TPB) void pad_mean_kernel(const T* __restrict__ img
     if (threadIdx.x == 0) { mean[b] = s / (double)cnt; vmax[b] = 0.0f; }
 }
 
-// Rows pass.  Block per (image, group of FFT_ELEMS / N row pairs); the
+// Rows pass.  Block per (image, group of FFT_ROW_ELEMS / N row pairs); the
 // spectrum's rows IP .. IP + NX - 1 receive the row transforms of the padded
 // image (PADD2: image minus the FP64 mean, bp3f.f:1316-1345).
 template <typename T>
@@ -154,13 +165,13 @@ __global__ __launch_bounds__(TPB) void row_fft_kernel(const T* __restrict__ imgs
                                                       float2* __restrict__ spec) {
     extern __shared__ float2 lds[];
     float2* buf = lds;
-    float2* stw = lds + FFT_LDS;
+    float2* stw = lds + FFT_ROW_ELEMS + 64;
     load_twiddles(stw, tw, N);
     const int b = blockIdx.y;
     const T* img = imgs + (size_t)b * nx * nx;
     const double qs = mean[b];
     const int H = nx + 1, ip = (N - nx) / 2 + (nx % 2);
-    const int pairs = nx / 2, M = min(FFT_ELEMS / N, pairs);
+    const int pairs = nx / 2, M = min(FFT_ROW_ELEMS / N, pairs);
     const int p0 = blockIdx.x * M, mc = min(M, pairs - p0);
     const int total = mc * N;
     for (int i0 = threadIdx.x; i0 < total; i0 += FFT_BATCH * TPB) {
@@ -181,7 +192,7 @@ __global__ __launch_bounds__(TPB) void row_fft_kernel(const T* __restrict__ imgs
             if (i0 + u * TPB < total) buf[i0 + u * TPB] = v[u];
     }
     __syncthreads();
-    lds_fft(buf, mc, N, L, N, stw);
+    lds_fft<FFT_ROW_ELEMS>(buf, mc, N, L, N, stw);
     // Z = FFT(x_r + i x_{r+1}): X_r[k] = (Z[k] + conj Z[N-k]) / 2,
     // X_{r+1}[k] = -i (Z[k] - conj Z[N-k]) / 2
     float2* out = spec + (size_t)b * N * H;
@@ -238,7 +249,7 @@ __global__ __launch_bounds__(TPB) void col_fft_kernel(int nx, int N, int L, cons
         if (m < cw) buf[m * S + (n < ip ? n : n + nx)] = make_float2(0.f, 0.f);
     }
     __syncthreads();
-    lds_fft(buf, cw, N, L, S, stw);
+    lds_fft<FFT_ELEMS>(buf, cw, N, L, S, stw);
     float mx = 0.0f;
     for (int idx = threadIdx.x; idx < cp * N; idx += blockDim.x) {
         const int n = idx >> lc, m = idx & (cp - 1);

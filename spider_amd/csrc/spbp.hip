@@ -759,8 +759,8 @@ int run_insert(spbp_ctx* c, int img0, const Geom* geom_d, int ng, const Targets&
 // (power-of-two N); c->vmax receives each spectrum's largest |Re|, |Im| and
 // must be zero on entry.
 int own_fft(spbp_ctx* c, const void* src, int dtype, int b, hipStream_t st) {
-    const size_t lds = fft_lds_bytes(c->N);
-    const int M = std::min(FFT_ELEMS / c->N, c->nx / 2);
+    const size_t lds = fft_lds_bytes(c->N), rlds = fft_lds_bytes(c->N, FFT_ROW_ELEMS);
+    const int M = std::min(FFT_ROW_ELEMS / c->N, c->nx / 2);   // >= 1: own transforms need N <= FFT_ROW_ELEMS
     const int cw = FFT_ELEMS / c->N;
     const size_t npix = (size_t)c->nx * c->nx, nspec = (size_t)c->N * c->H;
     const size_t esz = dtype == SPBP_BF16 ? 2 : 4;
@@ -775,10 +775,10 @@ int own_fft(spbp_ctx* c, const void* src, int dtype, int b, hipStream_t st) {
             Timed tm(c, SPBP_K_PAD, n, st);
             const dim3 rgrid((c->nx / 2 + M - 1) / M, n);
             if (dtype == SPBP_BF16)
-                row_fft_kernel<uint16_t><<<rgrid, TPB, lds, st>>>((const uint16_t*)im, c->nx, c->N, c->fft_log2, c->tw,
+                row_fft_kernel<uint16_t><<<rgrid, TPB, rlds, st>>>((const uint16_t*)im, c->nx, c->N, c->fft_log2, c->tw,
                                                                   c->fmean + s0, sp);
             else
-                row_fft_kernel<float><<<rgrid, TPB, lds, st>>>((const float*)im, c->nx, c->N, c->fft_log2, c->tw,
+                row_fft_kernel<float><<<rgrid, TPB, rlds, st>>>((const float*)im, c->nx, c->N, c->fft_log2, c->tw,
                                                                c->fmean + s0, sp);
         }
         {
@@ -1578,7 +1578,8 @@ int spbp_create(const spbp_opts* o, spbp_ctx** out) {
         // list builds and fewer persistent-kernel tails: 20,000 projections
         // of 128^2 go through one launch (481.8k proj/s for a BP 32F job,
         // 473.5k in three launches of 2 GiB chunks).
-        const bool own = (c->N & (c->N - 1)) == 0 && c->N >= 8 && c->N <= 2048 && !(o->flags & SPBP_HIPFFT);
+        const bool own = (c->N & (c->N - 1)) == 0 && c->N >= 8 && c->N <= std::min(2048, FFT_ROW_ELEMS) &&
+                         !(o->flags & SPBP_HIPFFT);
         const size_t per = (own ? 0 : (size_t)c->N * c->N * 4) + (size_t)c->N * c->H * 8;
         c->batch = (int)std::max<size_t>(16, std::min<size_t>(32768, ((size_t)8192 << 20) / per));
     }
@@ -1616,7 +1617,7 @@ int spbp_create(const spbp_opts* o, spbp_ctx** out) {
         hipEventCreateWithFlags(&c->jobs_evt, hipEventDisableTiming) != hipSuccess)
         return bail(fail(nullptr, SPBP_ERR_DEVICE, "context set-up failed"));
     if (dmalloc(&c->bnext, sizeof(int)) != hipSuccess) return bail(fail(nullptr, SPBP_ERR_ALLOC, "brick queue"));
-    if ((c->N & (c->N - 1)) == 0 && c->N >= 8 && c->N <= 2048 && !(o->flags & SPBP_HIPFFT)) {
+    if ((c->N & (c->N - 1)) == 0 && c->N >= 8 && c->N <= std::min(2048, FFT_ROW_ELEMS) && !(o->flags & SPBP_HIPFFT)) {
         int L = 0;
         while ((1 << L) < c->N) ++L;
         std::vector<float2> t(c->N);
