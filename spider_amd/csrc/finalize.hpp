@@ -103,21 +103,35 @@ __device__ inline int job_of_block(const FinalJob* __restrict__ jobs, int nj, un
     return lo;
 }
 
+// Half-grid voxels per thread of normalize_kernel: a block's job lookup and
+// job record (a serial chain of scalar loads) are paid once per 4 TPB voxels,
+// and the voxels' loads are all issued before the first store.
+constexpr int NORM_VPT = 4;
+
 __global__ __launch_bounds__(TPB) void normalize_kernel(const FinalJob* __restrict__ jobs, int nj) {
     const FinalJob& f = jobs[job_of_block<false>(jobs, nj, blockIdx.x)];
     const int nx = f.nx, N = f.N, H = nx + 1;
     const size_t total = (size_t)H * N * N;
-    const size_t idx = (size_t)(blockIdx.x - f.hb0) * blockDim.x + threadIdx.x;
-    if (idx >= total) return;
-    int i, j, k;
-    half_coords(idx, H, N, f.invH, f.invN, i, j, k);
-    float2 v = norm_val(f.a, nx, N, i, j, k);
-    if (i == 0 || i == nx) {
-        // c2r reads the kx = 0 and kx = N/2 planes as their Hermitian part
-        const float2 m = norm_val(f.a, nx, N, i, (N - j) % N, (N - k) % N);
-        v = make_float2((v.x + m.x) * 0.5f, (v.y - m.y) * 0.5f);
+    const size_t base = (size_t)(blockIdx.x - f.hb0) * (TPB * NORM_VPT) + threadIdx.x;
+    float2 v[NORM_VPT];
+#pragma unroll
+    for (int u = 0; u < NORM_VPT; ++u) {
+        const size_t idx = base + (size_t)u * TPB;
+        if (idx >= total) continue;
+        int i, j, k;
+        half_coords(idx, H, N, f.invH, f.invN, i, j, k);
+        v[u] = norm_val(f.a, nx, N, i, j, k);
+        if (i == 0 || i == nx) {
+            // c2r reads the kx = 0 and kx = N/2 planes as their Hermitian part
+            const float2 m = norm_val(f.a, nx, N, i, (N - j) % N, (N - k) % N);
+            v[u] = make_float2((v[u].x + m.x) * 0.5f, (v[u].y - m.y) * 0.5f);
+        }
     }
-    f.work[idx] = make_float2(v.x, -v.y);              // SPIDER inverse: conjugate first
+#pragma unroll
+    for (int u = 0; u < NORM_VPT; ++u) {
+        const size_t idx = base + (size_t)u * TPB;
+        if (idx < total) f.work[idx] = make_float2(v[u].x, -v[u].y);   // SPIDER inverse: conjugate first
+    }
 }
 
 // ------------------------------------------------------------------ WINDKB2A

@@ -1392,7 +1392,7 @@ int run_waves(std::vector<Round>& rs, hipStream_t st) {
             if (R.wave != w) continue;
             const spbp_ctx* c = R.c;
             const size_t nv = (size_t)c->nx * c->nx * c->nx;
-            const unsigned hb = nblocks(c->nh), vb = nblocks(nv);
+            const unsigned hb = (unsigned)((c->nh + (size_t)TPB * NORM_VPT - 1) / ((size_t)TPB * NORM_VPT)), vb = nblocks(nv);
             auto& rw = runs[w];
             if (rw.empty() || (unsigned long long)rw.back().hb + hb > FINAL_LAUNCH_BLOCKS ||
                 (unsigned long long)rw.back().vb + vb > FINAL_LAUNCH_BLOCKS) {
