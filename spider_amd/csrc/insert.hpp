@@ -146,43 +146,126 @@ __global__ __launch_bounds__(TPB) void plane_pack_kernel(const Geom* __restrict_
     if (p < n) pln[p] = make_float4(geom[p].r[6], geom[p].r[7], geom[p].r[8], geom[p].half ? 1.0f : 0.0f);
 }
 
-// Does the plane through the origin with normal n (DM(7:9)) pass through the
-// brick's expanded box, inside the sample disk of radius NX?
-__device__ inline bool brick_hit(const BrickGrid& bg, int b, int nx, const float4& g) {
-    int x0, y0, z0;
-    brick_origin(bg, b, nx, x0, y0, z0);
-    // owner-computes: samples whose nearest voxel is in the core (within 0.5
-    // of it), and at kx = 0 the mirrored samples within 2.5 of -core (a plane
-    // through the origin crosses -B exactly when it crosses B)
-    const float m = bg.own ? (x0 == 0 ? 2.52f : 0.52f) : BRICK_MARGIN;
-    const float hx = 0.5f * (bg.bx - 1) + m, hy = 0.5f * (bg.by - 1) + m, hz = 0.5f * (bg.bz - 1) + m;
-    const float cx = x0 + 0.5f * (bg.bx - 1), cy = y0 + 0.5f * (bg.by - 1), cz = z0 + 0.5f * (bg.bz - 1);
-    const float d = fabsf(g.x * cx + g.y * cy + g.z * cz);
-    const float reach = hx * fabsf(g.x) + hy * fabsf(g.y) + hz * fabsf(g.z) + 0.02f;
-    if (d > reach) return false;
-    const float qx = fmaxf(fabsf(cx) - hx, 0.f), qy = fmaxf(fabsf(cy) - hy, 0.f), qz = fmaxf(fabsf(cz) - hz, 0.f);
+// A section's plane (through the origin, normal n = DM(7:9)) feeds a brick
+// when it passes through the brick's expanded box and that box reaches into
+// the sample disk of radius NX.  Owner-computes bricks expand their core by
+// 0.52 (samples whose nearest voxel lies in the core), and at kx = 0 by 2.52
+// (the mirrored samples within 2.5 of -core: a plane through the origin
+// crosses -B exactly when it crosses B); shared-evaluation bricks by the
+// table's support.
+//
+// The list builders take bricks in groups of 2 x 2 x 2 ("superbricks"): a
+// section is tested against the group's box first, and against its member
+// bricks only where that box is hit.  The group's box holds every member's
+// expanded box (with the largest member margin, plus 0.05 for rounding), so
+// the first test never drops a pair the brick test keeps.  The members'
+// boxes, and the disk condition, which does not depend on the section, are
+// set up once per block.
+struct SuperBrick {
+    int b[8];       // member bricks, -1 past the grid's edge or wholly outside the sample disk
+    float mc[8][3]; // member expanded boxes: centres and half extents
+    float mh[8][3];
+    float cx, cy, cz, hx, hy, hz;
+};
+
+__device__ __forceinline__ SuperBrick superbrick(const BrickGrid& bg, int s, int nx) {
+    const int nsx = (bg.nbx + 1) / 2, nsy = (bg.nby + 1) / 2;
+    const int si = s % nsx, sj = (s / nsx) % nsy, sk = s / (nsx * nsy);
+    SuperBrick g;
     const float r = (float)nx + 0.02f;
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {   // unrolled: the arrays stay in registers
+        const int i = 2 * si + (d & 1), j = 2 * sj + ((d >> 1) & 1), k = 2 * sk + (d >> 2);
+        g.b[d] = (i < bg.nbx && j < bg.nby && k < bg.nbz) ? i + bg.nbx * (j + bg.nby * k) : -1;
+        // the member's expanded box and its disk condition
+        const int x0 = i * bg.bx, y0 = -nx + 1 + j * bg.by, z0 = -nx + 1 + k * bg.bz;
+        const float m = bg.own ? (x0 == 0 ? 2.52f : 0.52f) : BRICK_MARGIN;
+        const float hx = 0.5f * (bg.bx - 1) + m, hy = 0.5f * (bg.by - 1) + m, hz = 0.5f * (bg.bz - 1) + m;
+        const float cx = x0 + 0.5f * (bg.bx - 1), cy = y0 + 0.5f * (bg.by - 1), cz = z0 + 0.5f * (bg.bz - 1);
+        const float qx = fmaxf(fabsf(cx) - hx, 0.f), qy = fmaxf(fabsf(cy) - hy, 0.f), qz = fmaxf(fabsf(cz) - hz, 0.f);
+        if (!(qx * qx + qy * qy + qz * qz <= r * r)) g.b[d] = -1;
+        g.mc[d][0] = cx;
+        g.mc[d][1] = cy;
+        g.mc[d][2] = cz;
+        g.mh[d][0] = hx;
+        g.mh[d][1] = hy;
+        g.mh[d][2] = hz;
+    }
+    const int i1 = min(2 * si + 1, bg.nbx - 1), j1 = min(2 * sj + 1, bg.nby - 1), k1 = min(2 * sk + 1, bg.nbz - 1);
+    const float m = (bg.own ? (si == 0 ? 2.52f : 0.52f) : BRICK_MARGIN) + 0.05f;
+    const float xlo = 2 * si * bg.bx - m, xhi = i1 * bg.bx + bg.bx - 1 + m;
+    const float ylo = -nx + 1 + 2 * sj * bg.by - m, yhi = -nx + 1 + j1 * bg.by + bg.by - 1 + m;
+    const float zlo = -nx + 1 + 2 * sk * bg.bz - m, zhi = -nx + 1 + k1 * bg.bz + bg.bz - 1 + m;
+    g.cx = 0.5f * (xlo + xhi);
+    g.cy = 0.5f * (ylo + yhi);
+    g.cz = 0.5f * (zlo + zhi);
+    g.hx = 0.5f * (xhi - xlo);
+    g.hy = 0.5f * (yhi - ylo);
+    g.hz = 0.5f * (zhi - zlo);
+    return g;
+}
+
+// Member d of superbrick s for a thread-dependent d, recomputed (indexing
+// the record's array with it would move the whole record to scratch
+// memory); -1 past the grid's edge.
+__device__ __forceinline__ int member(const BrickGrid& bg, int s, int d) {
+    const int nsx = (bg.nbx + 1) / 2, nsy = (bg.nby + 1) / 2;
+    const int i = 2 * (s % nsx) + (d & 1), j = 2 * ((s / nsx) % nsy) + ((d >> 1) & 1), k = 2 * (s / (nsx * nsy)) + (d >> 2);
+    return (i < bg.nbx && j < bg.nby && k < bg.nbz) ? i + bg.nbx * (j + bg.nby * k) : -1;
+}
+
+// Does the plane pass through member d's expanded box (its disk condition
+// already applied)?
+__device__ __forceinline__ bool member_hit(const SuperBrick& sb, int d, const float4& g) {
+    const float dd = fabsf(g.x * sb.mc[d][0] + g.y * sb.mc[d][1] + g.z * sb.mc[d][2]);
+    const float reach = sb.mh[d][0] * fabsf(g.x) + sb.mh[d][1] * fabsf(g.y) + sb.mh[d][2] * fabsf(g.z) + 0.02f;
+    return dd <= reach;
+}
+
+// The group test: both conditions on the group's box.
+__device__ __forceinline__ bool superbrick_hit(const SuperBrick& sb, int nx, const float4& g) {
+    const float d = fabsf(g.x * sb.cx + g.y * sb.cy + g.z * sb.cz);
+    const float reach = sb.hx * fabsf(g.x) + sb.hy * fabsf(g.y) + sb.hz * fabsf(g.z) + 0.05f;
+    if (d > reach) return false;
+    const float qx = fmaxf(fabsf(sb.cx) - sb.hx, 0.f), qy = fmaxf(fabsf(sb.cy) - sb.hy, 0.f),
+                qz = fmaxf(fabsf(sb.cz) - sb.hz, 0.f);
+    const float r = (float)nx + 0.1f;
     return qx * qx + qy * qy + qz * qz <= r * r;
 }
 
-// Count (brick, half) hits.  Block: one brick x 1024 projections.
+// Count (brick, half) hits.  Block: one superbrick x 1024 projections.
 __global__ __launch_bounds__(TPB) void brick_count_kernel(const float4* __restrict__ pln, int nproj, BrickGrid bg, int nx,
                                                           int* __restrict__ cnt) {
-    const int b = blockIdx.x;
+    __shared__ int sc[TPB / 64][16];
+    const SuperBrick sb = superbrick(bg, blockIdx.x, nx);
     const int p0 = blockIdx.y * 1024;
-    unsigned c0 = 0, c1 = 0;
+    unsigned c[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) c[k] = 0;
     for (int p = p0 + threadIdx.x; p < min(nproj, p0 + 1024); p += TPB) {
         const float4 g = pln[p];
-        if (brick_hit(bg, b, nx, g)) { if (g.w != 0.0f) ++c1; else ++c0; }
+        if (!superbrick_hit(sb, nx, g)) continue;
+        const bool h1 = g.w != 0.0f;
+#pragma unroll
+        for (int d = 0; d < 8; ++d)
+            if (sb.b[d] >= 0 && member_hit(sb, d, g)) {
+                if (h1) ++c[8 + d];
+                else ++c[d];
+            }
     }
-    double d0 = c0;
-    block_sum2(d0, c1);
-    __shared__ unsigned sc0;
-    if (threadIdx.x == 0) sc0 = (unsigned)d0;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const unsigned v = wave_sum(c[k]);
+        if (lane == 0) sc[w][k] = (int)v;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        if (sc0) atomicAdd(&cnt[b], (int)sc0);
-        if (c1) atomicAdd(&cnt[b + bg.nb], (int)c1);
+    if (threadIdx.x < 16) {
+        const int d = threadIdx.x & 7, h = threadIdx.x >> 3;
+        int t = 0;
+        for (int k = 0; k < TPB / 64; ++k) t += sc[k][threadIdx.x];
+        const int b = member(bg, blockIdx.x, d);   // a member outside the disk counted nothing
+        if (t && b >= 0) atomicAdd(&cnt[b + h * bg.nb], t);
     }
 }
 
@@ -248,40 +331,63 @@ __global__ __launch_bounds__(1024) void key_scan_add_kernel(int nkeys, const int
     cursor[k] = 0;
 }
 
-// Write the projection lists of every (brick, half) key.  Block: one brick x
-// 1024 projections; each half's hits of a round of TPB projections take one
-// cursor reservation per block (wave ballots give the ranks).
+// Write the projection lists of every (brick, half) key.  Block: one
+// superbrick x 1024 projections; each key's hits of a round of TPB
+// projections take one cursor reservation per block (wave ballots give the
+// ranks).
 __global__ __launch_bounds__(TPB) void brick_fill_kernel(const float4* __restrict__ pln, int nproj, BrickGrid bg, int nx,
                                                          const int* __restrict__ off, int* __restrict__ cursor,
                                                          int* __restrict__ list) {
-    __shared__ int wc[2][TPB / 64];
-    __shared__ int base[2];
-    const int b = blockIdx.x;
+    __shared__ int wc[16][TPB / 64];
+    __shared__ int base[16];
+    const SuperBrick sb = superbrick(bg, blockIdx.x, nx);
     const int p0 = blockIdx.y * 1024;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const unsigned long long below = (1ull << lane) - 1ull;
     for (int r = p0; r < min(nproj, p0 + 1024); r += TPB) {
         const int p = r + threadIdx.x;
-        bool h0 = false, h1 = false;
+        unsigned hits = 0;   // bit d: member d hit; this section's half is h
+        int h = 0;
         if (p < nproj) {
             const float4 g = pln[p];
-            if (brick_hit(bg, b, nx, g)) { h1 = g.w != 0.0f; h0 = !h1; }
+            h = g.w != 0.0f ? 1 : 0;
+            if (superbrick_hit(sb, nx, g)) {
+#pragma unroll
+                for (int d = 0; d < 8; ++d)
+                    if (sb.b[d] >= 0 && member_hit(sb, d, g)) hits |= 1u << d;
+            }
         }
-        const unsigned long long m0 = __ballot(h0), m1 = __ballot(h1);
-        if (lane == 0) { wc[0][w] = __popcll(m0); wc[1][w] = __popcll(m1); }
+        unsigned long long m[8];
+        const bool any = __ballot(hits != 0) != 0ull;
+        if (any) {
+#pragma unroll
+            for (int d = 0; d < 8; ++d) {
+                const unsigned long long m0 = __ballot((hits >> d & 1u) && h == 0), m1 = __ballot((hits >> d & 1u) && h == 1);
+                m[d] = h ? m1 : m0;
+                if (lane == 0) { wc[d][w] = __popcll(m0); wc[8 + d][w] = __popcll(m1); }
+            }
+        } else if (lane == 0) {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) wc[k][w] = 0;
+        }
         __syncthreads();
-        if (threadIdx.x < 2) {
+        if (threadIdx.x < 16) {
+            const int d = threadIdx.x & 7, hh = threadIdx.x >> 3;
             int t = 0;
             for (int k = 0; k < TPB / 64; ++k) t += wc[threadIdx.x][k];
-            base[threadIdx.x] = t ? atomicAdd(&cursor[b + threadIdx.x * bg.nb], t) : 0;
+            const int b = member(bg, blockIdx.x, d);
+            base[threadIdx.x] = (t && b >= 0) ? atomicAdd(&cursor[b + hh * bg.nb], t) : 0;
         }
         __syncthreads();
-        if (h0 || h1) {
-            const int hh = h1 ? 1 : 0;
-            int rank = __popcll((h1 ? m1 : m0) & below);
-            for (int k = 0; k < w; ++k) rank += wc[hh][k];
-            const int key = b + hh * bg.nb;
-            list[off[key] + base[hh] + rank] = p;
+        if (hits) {
+#pragma unroll
+            for (int d = 0; d < 8; ++d) {
+                if (!(hits >> d & 1u)) continue;
+                const int key = sb.b[d] + h * bg.nb;
+                int rank = __popcll(m[d] & below);
+                for (int k = 0; k < w; ++k) rank += wc[h * 8 + d][k];
+                list[off[key] + base[h * 8 + d] + rank] = p;
+            }
         }
         __syncthreads();
     }

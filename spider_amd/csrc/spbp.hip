@@ -680,7 +680,8 @@ int build_lists(spbp_ctx* c, const Geom* geom_d, int b, hipStream_t st) {
         c->bpln_cap < (size_t)b)
         return fail(c, SPBP_ERR_ARG, "brick lists not allocated for %d sections", b);
     const int nkeys = 2 * c->bg.nb;
-    const dim3 lgrid(c->bg.nb, (b + 1023) / 1024);
+    const int nsb = ((c->bg.nbx + 1) / 2) * ((c->bg.nby + 1) / 2) * ((c->bg.nbz + 1) / 2);   // superbricks
+    const dim3 lgrid(nsb, (b + 1023) / 1024);
     Timed tm(c, SPBP_K_BIN, b, st);
     fill_kernel<<<nblocks(nkeys), TPB, 0, st>>>((float*)c->bcnt, nkeys, 0.0f);   // zero bits
     plane_pack_kernel<<<nblocks(b), TPB, 0, st>>>(geom_d, b, c->bpln);
