@@ -49,9 +49,16 @@ __device__ inline void block_sum2(double& s, unsigned& c) {
 }
 
 
+// p[0 .. n) = v, four values per thread (one 16-byte store when p is
+// 16-byte aligned).  Launched by fill(): a kernel, not hipMemsetAsync, so that
+// it may be captured in a graph.
 __global__ __launch_bounds__(TPB) void fill_kernel(float* __restrict__ p, size_t n, float v) {
-    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx < n) p[idx] = v;
+    const size_t i = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    if (i + 4 <= n && ((uintptr_t)p & 15) == 0) {
+        *reinterpret_cast<float4*>(p + i) = make_float4(v, v, v, v);
+    } else {
+        for (size_t k = i; k < n && k < i + 4; ++k) p[k] = v;
+    }
 }
 
 template <typename T>
@@ -61,6 +68,9 @@ __global__ __launch_bounds__(TPB) void finite_kernel(const T* __restrict__ p, si
 }
 
 inline unsigned nblocks(size_t n) { return (unsigned)((n + TPB - 1) / TPB); }
+inline void fill(float* p, size_t n, float v, hipStream_t st) {
+    if (n) fill_kernel<<<nblocks((n + 3) / 4), TPB, 0, st>>>(p, n, v);
+}
 
 // q = n / d and r = n mod d for n < 2^52 and d >= 1, given inv = 1.0 / d
 // rounded to double: the product n * inv is within 1 of n / d, so one

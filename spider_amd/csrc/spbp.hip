@@ -683,7 +683,7 @@ int build_lists(spbp_ctx* c, const Geom* geom_d, int b, hipStream_t st) {
     const int nsb = ((c->bg.nbx + 1) / 2) * ((c->bg.nby + 1) / 2) * ((c->bg.nbz + 1) / 2);   // superbricks
     const dim3 lgrid(nsb, (b + 1023) / 1024);
     Timed tm(c, SPBP_K_BIN, b, st);
-    fill_kernel<<<nblocks(nkeys), TPB, 0, st>>>((float*)c->bcnt, nkeys, 0.0f);   // zero bits
+    fill((float*)c->bcnt, nkeys, 0.0f, st);   // zero bits
     plane_pack_kernel<<<nblocks(b), TPB, 0, st>>>(geom_d, b, c->bpln);
     brick_count_kernel<<<lgrid, TPB, 0, st>>>(c->bpln, b, c->bg, c->nx, c->bcnt);
     const int kb = (nkeys + 1023) / 1024;
@@ -832,7 +832,7 @@ int check_images(spbp_ctx* c, const void* imgs, int dtype, int nimg, hipStream_t
     if (!(c->flags & SPBP_CHECK_FINITE)) return SPBP_OK;
     if (c->capturing) return fail(c, SPBP_ERR_ARG, "SPBP_CHECK_FINITE cannot be used in a captured batch");
     const size_t n = (size_t)nimg * c->nx * c->nx;
-    fill_kernel<<<1, TPB, 0, st>>>((float*)c->flag, 1, 0.0f);   // zero bits
+    fill((float*)c->flag, 1, 0.0f, st);   // zero bits
     if (dtype == SPBP_BF16)
         finite_kernel<uint16_t><<<nblocks(n), TPB, 0, st>>>((const uint16_t*)imgs, n, c->flag);
     else
@@ -1228,7 +1228,7 @@ P res_fltb(const spbp_ctx* c) { return sizeof(P) == 4 ? (P)c->fltb : (P)(4999.0 
 int phase_list(Round& R, hipStream_t st) {
     spbp_ctx* c = R.c;
     Res& r = R.res();
-    fill_kernel<<<1, TPB, 0, st>>>((float*)r.ctr, 8, 0.0f);   // zero bits
+    fill((float*)r.ctr, 8, 0.0f, st);   // zero bits
     thresh_list_kernel<<<nblocks(c->nh), TPB, 0, st>>>(R.a, c->nx, c->N, 1.0 / (c->nx + 1), 1.0 / c->N, THRESH_BAND,
                                                        r.list, (long long)r.list_cap, r.ctr);
     HIPCHK(c, hipGetLastError());
@@ -1501,7 +1501,7 @@ int finalize_jobs(const spbp_final_job* jobs, int njobs, hipStream_t st) {
             c->tmask_dirty = true;
         }
         if (c->tmask_dirty) {
-            fill_kernel<<<nblocks((c->nh + 3) / 4), TPB, 0, st>>>((float*)c->tmask, (c->nh + 3) / 4, 0.0f);   // zero bits
+            fill((float*)c->tmask, (c->nh + 3) / 4, 0.0f, st);   // zero bits
             HIPCHK(c, hipGetLastError());
             c->tmask_dirty = false;
         }
@@ -1700,9 +1700,9 @@ int spbp_reset(spbp_ctx* c, void* stream) {
     c->filled = false;
     hipStream_t st = (hipStream_t)stream;
     for (int h = 0; h < c->nvol; ++h) {
-        fill_kernel<<<nblocks(2 * c->nh), TPB, 0, st>>>((float*)c->X[h], 2 * c->nh, 0.0f);
-        fill_kernel<<<nblocks(c->nh), TPB, 0, st>>>(c->W[h], c->nh, 0.0f);
-        if (c->Wc[h]) fill_kernel<<<nblocks(c->nh), TPB, 0, st>>>(c->Wc[h], c->nh, 0.0f);
+        fill((float*)c->X[h], 2 * c->nh, 0.0f, st);
+        fill(c->W[h], c->nh, 0.0f, st);
+        if (c->Wc[h]) fill(c->Wc[h], c->nh, 0.0f, st);
     }
     HIPCHK(c, hipGetLastError());
     return SPBP_OK;
@@ -1849,8 +1849,8 @@ int spbp_scatter_images(spbp_ctx* c, const float* imgs, int nimg, const double* 
         const int b = std::min(c->fwd_batch, nimg - p0);
         const size_t n = (size_t)b * N * H;
         // no PADD2 mean: zero means (all-zero bits are 0.0) and zero maxima
-        fill_kernel<<<nblocks(2 * (size_t)b), TPB, 0, st>>>((float*)c->fmean, 2 * (size_t)b, 0.0f);
-        fill_kernel<<<nblocks(b), TPB, 0, st>>>(c->vmax, b, 0.0f);
+        fill((float*)c->fmean, 2 * (size_t)b, 0.0f, st);
+        fill(c->vmax, b, 0.0f, st);
         HIPCHK(c, hipGetLastError());
         if ((rc = own_fft(c, imgs + (size_t)p0 * npix, SPBP_F32, b, st))) return rc;
         adjoint_spectrum_kernel<<<nblocks(n), TPB, 0, st>>>(c->spec, N, H, c->nx, n);
@@ -2192,9 +2192,9 @@ int spbp_graph_create(const spbp_job* jobs, int njobs, int reset_first, spbp_gra
             for (int j = 0; j < njobs && rc == SPBP_OK; ++j) {
                 spbp_ctx* c = jobs[j].ctx;
                 for (int h = 0; h < c->nvol; ++h) {
-                    fill_kernel<<<nblocks(2 * c->nh), TPB, 0, g->stream>>>((float*)c->X[h], 2 * c->nh, 0.0f);
-                    fill_kernel<<<nblocks(c->nh), TPB, 0, g->stream>>>(c->W[h], c->nh, 0.0f);
-                    if (c->Wc[h]) fill_kernel<<<nblocks(c->nh), TPB, 0, g->stream>>>(c->Wc[h], c->nh, 0.0f);
+                    fill((float*)c->X[h], 2 * c->nh, 0.0f, g->stream);
+                    fill(c->W[h], c->nh, 0.0f, g->stream);
+                    if (c->Wc[h]) fill(c->Wc[h], c->nh, 0.0f, g->stream);
                 }
             }
         if (rc == SPBP_OK) rc = enqueue_batch(jobs, njobs, g->geom.data(), g->tables.data(), g->stream);
@@ -2532,7 +2532,7 @@ int spbp_calibrate(int what, size_t bytes, double* rate, void* stream) {
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = fail(nullptr, SPBP_ERR_DEVICE, "events");
     float ms = 0.f;
     if (rc == SPBP_OK && what == SPBP_CAL_GLOBAL_F32) {
-        fill_kernel<<<(unsigned)((n + TPB - 1) / TPB), TPB, 0, st>>>((float*)buf, n, 0.f);
+        fill((float*)buf, n, 0.f, st);
         cal_global_kernel<<<gblocks, TPB, 0, st>>>((float*)buf, n);   // warm-up
     } else if (rc == SPBP_OK) {
         cal_lds_kernel<LNT><<<lblocks, LNT, 0, st>>>((unsigned long long*)buf);
@@ -2595,7 +2595,7 @@ int spbp_import_partial(spbp_ctx* c, int half, const float* X, const float* W, i
     if (half < 0 || half >= c->nvol) return fail(c, SPBP_ERR_ARG, "no half %d", half);
     DeviceGuard dg(c->device);
     hipStream_t st = (hipStream_t)stream;
-    if (!accumulate && c->Wc[half]) fill_kernel<<<nblocks(c->nh), TPB, 0, st>>>(c->Wc[half], c->nh, 0.0f);
+    if (!accumulate && c->Wc[half]) fill(c->Wc[half], c->nh, 0.0f, st);
     import_kernel<<<nblocks(c->nh), TPB, 0, st>>>((const float2*)X, W, c->nx, c->N, c->X[half], c->W[half], accumulate);
     HIPCHK(c, hipGetLastError());
     // the accumulators now hold sums the insertion history does not describe
