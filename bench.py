@@ -8,8 +8,10 @@ overall volume and the two sample volumes (SYMPLANE0 + normalise + 3D inverse
 FFT + window, bp32f.f:275-410).  Default workload: 20,000 projections of
 128x128 fp32 into the 256^3 padded grid (the recipe: 60-blob phantom, uniform
 orientations, SNR 0.1); the same line carries a `geom_f64` record (the job
-with FP64 sample geometry) and a `config5` record (100,000 projections of
-256x256 stored as BF16 into the 512^3 grid with compensated weights).
+with FP64 sample geometry), a `host_images` record (the job with the images
+handed over in host memory: the PCIe-inclusive rate, never `value`) and a
+`config5` record (100,000 projections of 256x256 stored as BF16 into the
+512^3 grid with compensated weights).
 
 Prints one JSON line.  `roofline` is for the insertion kernel: algorithmic
 bytes per projection B = 4 NX^2 + 24 U(NX) (image read + read/write of the
@@ -331,6 +333,50 @@ def run_throughput(args, config, nproj, steps, warmup, op="bp32f", bf16=False, t
     return out
 
 
+def run_host_images(args, steps=3, warmup=1):
+    """The config-2 BP 32F job with the images handed over in host memory
+    (pageable, fp32) through spbp_insert_host: each step uploads them over
+    PCIe, in sub-chunks that overlap the insertion.  Reported beside `value`,
+    never as it."""
+    import numpy as np
+    import torch
+    from spider_amd import synth
+    from spider_amd import bp as bpmod
+
+    nproj = args.nproj or synth.CONFIGS[2]["nproj"]
+    ang, _, imgs = synth.make_dataset(2, nproj=nproj, device="cuda")
+    host = np.ascontiguousarray(imgs.cpu().numpy())
+    del imgs
+    half = bpmod.random_halves(nproj, seed=synth.config_seed(2))
+    nx = host.shape[-1]
+
+    def timed(src):
+        with bpmod.Backprojector(nx, two_halves=True) as b:
+            def step():
+                b.reset()
+                b.insert_host(src, ang, half)
+                return b.finalize(full=True, halves=True)
+            for _ in range(warmup):
+                step()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                step()
+            torch.cuda.synchronize()
+            return time.perf_counter() - t0
+
+    dt = timed(host)
+    pinned = torch.from_numpy(host).pin_memory()   # a caller's page-locked buffer: asynchronous DMA
+    dtp = timed(pinned)
+    del pinned
+    return {"value": round(nproj * steps / dt, 2), "unit": "projections/s", "ms_per_step": round(dt / steps * 1e3, 3),
+            "steps": steps, "bytes_uploaded_per_step": int(host.nbytes),
+            "pinned": {"value": round(nproj * steps / dtp, 2), "ms_per_step": round(dtp / steps * 1e3, 3)},
+            "note": ("the config-2 BP 32F job with the images in host memory (spbp_insert_host), pageable "
+                     "(`value`) and page-locked (`pinned`): PCIe-inclusive, uploads overlapped with insertion "
+                     "in ~256 MiB sub-chunks; not the line's `value`")}
+
+
 def run_batched(args):
     """Twelve BP 32F reconstructions with box sizes from {64, 96, 128, 192} and
     U[500, 30000] projections each (seeded blob phantoms, SNR 0.1).  Plain
@@ -628,6 +674,9 @@ def main():
                            geom_f64=True, parity_proj=0, cpu=False, calib=False, tune=False, extras=False)
         out["geom_f64"] = {k: g[k] for k in ("value", "unit", "ms_per_step", "steps", "config")}
         out["geom_f64"]["rel_l2_vs_fp64"] = out.get("rel_l2_vs_fp64_with_geom_f64")
+        log("bench: host images (PCIe-inclusive)")
+        torch.cuda.empty_cache()
+        out["host_images"] = run_host_images(args)
         if not args.no_config5:
             torch.cuda.empty_cache()
             log("bench: config 5")

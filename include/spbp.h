@@ -167,7 +167,12 @@ int spbp_insert(spbp_ctx* ctx, const void* imgs, int dtype, int nimg,
  * rows 1-2 of each must be orthonormal to 1e-4 (SPBP_ERR_ANGLE otherwise). */
 int spbp_insert_dm(spbp_ctx* ctx, const void* imgs, int dtype, int nimg,
                    const float* dm, const int8_t* half, void* stream);
-/* Same with host-resident images (staged through pinned memory). */
+/* Same with host-resident images: uploaded in sub-chunks of about 256 MiB
+ * (at most the context's batch) into two device buffers by turns, on a
+ * stream of the library's own, so that each upload overlaps the insertion
+ * of the sub-chunk before; returns when every image is inserted.  Page-locked
+ * images upload by asynchronous DMA, pageable ones through the runtime's
+ * staging. */
 int spbp_insert_host(spbp_ctx* ctx, const void* imgs_host, int dtype, int nimg,
                      const double* angles_deg, const int8_t* half, void* stream);
 

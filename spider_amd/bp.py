@@ -177,6 +177,47 @@ class Backprojector:
                                               hptr, _stream(self.device)))
         self._keep = (imgs, ang, hp, ip)
 
+    def insert_host(self, imgs, angles, half=None, inplane=None) -> None:
+        """insert() for images in host memory: a numpy float32 array or a CPU
+        torch tensor (float32 or bfloat16) of shape (n, NX, NX), through
+        spbp_insert_host / spbp_insert_rb_host.  The library uploads them in
+        sub-chunks that overlap the insertion; the call returns when they
+        are inserted."""
+        torch = _torch()
+        if isinstance(imgs, np.ndarray):
+            imgs = torch.from_numpy(np.ascontiguousarray(imgs, dtype=np.float32))
+        if not isinstance(imgs, torch.Tensor) or imgs.device.type != "cpu":
+            raise TypeError("host images must be a numpy array or a CPU torch tensor")
+        if imgs.dim() == 2:
+            imgs = imgs[None]
+        if imgs.dim() != 3 or imgs.shape[1] != imgs.shape[2]:
+            raise SpbpError(_lib.ERR_SIZE, "ONLY WORKS ON SQUARE IMAGES")
+        if imgs.shape[1] != self.nx:
+            raise SpbpError(_lib.ERR_SIZE, f"image size {imgs.shape[1]} != context NX {self.nx}")
+        if imgs.dtype == torch.float32:
+            dt = _lib.F32
+        elif imgs.dtype == torch.bfloat16:
+            dt = _lib.BF16
+        else:
+            raise TypeError(f"unsupported image dtype {imgs.dtype}")
+        imgs = imgs.contiguous()
+        n = imgs.shape[0]
+        ang = np.ascontiguousarray(np.asarray(angles, np.float64).reshape(-1, 3))
+        if ang.shape[0] != n:
+            raise SpbpError(_lib.ERR_ANGLE, f"MISSING ANGLE FOR IMAGE: {ang.shape[0]} angles for {n} images")
+        hp = None if half is None else np.ascontiguousarray(np.asarray(half).astype(np.int8).reshape(n))
+        hptr = None if hp is None else hp.ctypes.data_as(C.c_void_p)
+        if inplane is None:
+            self._check(_lib.lib().spbp_insert_host(self._ctx, C.c_void_p(imgs.data_ptr()), dt, n,
+                                                    ang.ctypes.data_as(C.c_void_p), hptr, _stream(self.device)))
+            return
+        ip = np.ascontiguousarray(np.asarray(inplane, np.float32).reshape(-1, 3))
+        if ip.shape[0] != n:
+            raise SpbpError(_lib.ERR_ANGLE, f"MISSING ANGLE FOR IMAGE: {ip.shape[0]} in-plane entries for {n} images")
+        self._check(_lib.lib().spbp_insert_rb_host(self._ctx, C.c_void_p(imgs.data_ptr()), dt, n,
+                                                   ang.ctypes.data_as(C.c_void_p), ip.ctypes.data_as(C.c_void_p),
+                                                   hptr, _stream(self.device)))
+
     def insert_rotations(self, imgs, dm, half=None) -> None:
         """Insert projections with precomputed CANG matrices (n, 9) fp32."""
         imgs, dt = self._images(imgs)

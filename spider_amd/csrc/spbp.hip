@@ -2339,6 +2339,11 @@ int spbp_rtsf_host(const float* imgs_host, int nimg, int nx, const float* params
 namespace {
 // Host images staged chunk by chunk through one device buffer, each chunk
 // inserted as spbp_insert (or spbp_insert_rb with inplane) would.
+// Host images: uploaded in sub-chunks of about 256 MiB (at most c->batch
+// images) into two device buffers by turns, on a stream of their own, so
+// that the upload of one sub-chunk overlaps the insertion of the one before
+// (the insertion of sub-chunk k waits for its upload; the upload of k + 2
+// waits for that insertion to be done with the buffer).
 int insert_host_impl(spbp_ctx* c, const void* imgs_host, int dtype, int nimg, const double* angles_deg,
                      const float* inplane, const int8_t* half, hipStream_t st) {
     if (!c) return fail(nullptr, SPBP_ERR_ARG, "NULL context");
@@ -2347,20 +2352,46 @@ int insert_host_impl(spbp_ctx* c, const void* imgs_host, int dtype, int nimg, co
     DeviceGuard dg(c->device);
     const size_t esz = dtype == SPBP_BF16 ? 2 : 4;
     const size_t per = (size_t)c->nx * c->nx * esz;
-    const int chunk = std::max(1, std::min(nimg, c->batch));
-    void* dimg = nullptr;
-    HIPCHK(c, dmalloc(&dimg, per * chunk));
+    const int sub = (int)std::max<size_t>(1, std::min<size_t>({(size_t)nimg, (size_t)std::max(1, c->batch),
+                                                                ((size_t)256 << 20) / per}));
+    const int nbuf = nimg > sub ? 2 : 1;
+    void* dimg[2] = {nullptr, nullptr};
+    hipStream_t cs = nullptr;
+    hipEvent_t up[2] = {nullptr, nullptr}, used[2] = {nullptr, nullptr};
     int rc = SPBP_OK;
-    for (int p0 = 0; p0 < nimg && rc == SPBP_OK; p0 += chunk) {
-        const int b = std::min(chunk, nimg - p0);
-        hipError_t e = hipMemcpyAsync(dimg, (const char*)imgs_host + per * p0, per * b, hipMemcpyHostToDevice, st);
-        if (e != hipSuccess) { rc = fail(c, SPBP_ERR_DEVICE, "image upload: %s", hipGetErrorString(e)); break; }
-        rc = insert_impl(c, dimg, dtype, b, nullptr, angles_deg ? angles_deg + 3 * (size_t)p0 : nullptr,
-                         half ? half + p0 : nullptr, st, inplane ? inplane + 3 * (size_t)p0 : nullptr);
-        e = hipStreamSynchronize(st);
-        if (rc == SPBP_OK && e != hipSuccess) rc = fail(c, SPBP_ERR_DEVICE, "insert: %s", hipGetErrorString(e));
+    auto release = [&]() {
+        (void)hipStreamSynchronize(st);
+        if (cs) { (void)hipStreamSynchronize(cs); (void)hipStreamDestroy(cs); }
+        for (int k = 0; k < 2; ++k) {
+            if (up[k]) (void)hipEventDestroy(up[k]);
+            if (used[k]) (void)hipEventDestroy(used[k]);
+            if (dimg[k]) (void)hipFree(dimg[k]);
+        }
+    };
+    for (int k = 0; k < nbuf && rc == SPBP_OK; ++k) {
+        if (dmalloc(&dimg[k], per * sub) != hipSuccess) rc = fail(c, SPBP_ERR_ALLOC, "image upload buffer");
+        else if (hipEventCreateWithFlags(&up[k], hipEventDisableTiming) != hipSuccess ||
+                 hipEventCreateWithFlags(&used[k], hipEventDisableTiming) != hipSuccess)
+            rc = fail(c, SPBP_ERR_DEVICE, "upload events");
     }
-    (void)hipFree(dimg);
+    if (rc == SPBP_OK && hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) {
+        cs = nullptr;
+        rc = fail(c, SPBP_ERR_DEVICE, "upload stream");
+    }
+    for (int p0 = 0, k = 0; p0 < nimg && rc == SPBP_OK; p0 += sub, ++k) {
+        const int b = std::min(sub, nimg - p0), s = k % nbuf;
+        hipError_t e = k >= nbuf ? hipStreamWaitEvent(cs, used[s], 0) : hipSuccess;
+        if (e == hipSuccess) e = hipMemcpyAsync(dimg[s], (const char*)imgs_host + per * p0, per * b, hipMemcpyHostToDevice, cs);
+        if (e == hipSuccess) e = hipEventRecord(up[s], cs);
+        if (e == hipSuccess) e = hipStreamWaitEvent(st, up[s], 0);
+        if (e != hipSuccess) { rc = fail(c, SPBP_ERR_DEVICE, "image upload: %s", hipGetErrorString(e)); break; }
+        rc = insert_impl(c, dimg[s], dtype, b, nullptr, angles_deg ? angles_deg + 3 * (size_t)p0 : nullptr,
+                         half ? half + p0 : nullptr, st, inplane ? inplane + 3 * (size_t)p0 : nullptr);
+        if (rc == SPBP_OK && hipEventRecord(used[s], st) != hipSuccess) rc = fail(c, SPBP_ERR_DEVICE, "upload events");
+    }
+    const hipError_t e = hipStreamSynchronize(st);
+    if (rc == SPBP_OK && e != hipSuccess) rc = fail(c, SPBP_ERR_DEVICE, "insert: %s", hipGetErrorString(e));
+    release();
     return rc;
 }
 }  // namespace
