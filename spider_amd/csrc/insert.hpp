@@ -484,6 +484,7 @@ struct TileLds {
     }
     unsigned long long re[NS], im[NS], w[WGT ? NS : 1];
     int item;
+    int pnext[2];   // next section pair of the item's direct and mirror passes
     float vmax[NT / 64];
 };
 
@@ -910,8 +911,11 @@ struct BrickJob {
 };
 constexpr int MAX_JOBS = 64;
 
-// One pass of a wave over its sections of a work item (s = 2 wave, 2 wave +
-// 1, then every 2 nwave).  Two sections, A and B, share each candidate round:
+// One pass of a wave over its sections of a work item: the waves take the
+// item's sections in pairs (s, s + 1) from a counter in LDS (L.pnext, one per
+// pass, zeroed with the item), so that they finish together whatever the
+// pairs' work; the sums are exact, so which wave takes which pair does not
+// change them.  Two sections, A and B, share each candidate round:
 // lanes 0..31 hold A's lattice lines and lanes 32..63 B's (without B, its
 // lines are empty).  nlA, nlB <= 32: a rotation's section crosses at most the
 // box diagonal + 2 lines, which the kernel's static_assert bounds for its
@@ -933,7 +937,7 @@ __device__ __forceinline__ unsigned pack_cand(int I, int J, int s) {
 
 template <int BX, int BY, int BZ, bool MIRPASS, bool XCHK, typename P, int NT, bool WGT>
 __device__ __forceinline__ void own_pass(TileLds<BX, BY, BZ, NT, WGT>& L, const BrickJob& Jb, const int* __restrict__ lst,
-                                         int ns, int wave, int nwave, int x0, int y0, int z0, const Box box,
+                                         int ns, int x0, int y0, int z0, const Box box,
                                          const float* __restrict__ tab, P fltb, double vscale, double wscale) {
     constexpr int CX = BX - 4, CY = BY - 4, CZ = BZ - 4;
     const int nx = Jb.nx, N = Jb.N, H = nx + 1;
@@ -945,7 +949,7 @@ __device__ __forceinline__ void own_pass(TileLds<BX, BY, BZ, NT, WGT>& L, const 
     Geom gA, gB;
     int b0A = 0, b0B = 0;
     int pref = 0, cnt = 0, alo = 0, total = 0, q0 = 0;
-    int s = 2 * wave - 2 * nwave;   // current pair
+    int s = -2;                     // current pair
     int sA = -1;                    // the pair whose geometry gA, gB hold (A = sA, B = sA + 1)
     unsigned pend = 0;   // this lane's pending candidate (valid for lane < npend)
     int npend = 0;       // wave-uniform
@@ -954,7 +958,9 @@ __device__ __forceinline__ void own_pass(TileLds<BX, BY, BZ, NT, WGT>& L, const 
         bool full = false;
         for (;;) {
             if (q0 >= total) {   // next pair of sections: their lattice lines across the box
-                s += 2 * nwave;
+                int nxt = 0;
+                if (lane == 0) nxt = atomicAdd(&L.pnext[MIRPASS ? 1 : 0], 1);
+                s = 2 * __shfl(nxt, 0, 64);
                 if (s >= ns) break;
                 const bool hasB = s + 1 < ns;
                 gA = Jb.geom[lst[s]];
@@ -1140,6 +1146,9 @@ void insert_brick_kernel(const BrickJob* __restrict__ jobs, int njobs, int* __re
             L.im[i] = 0ull;
             if constexpr (WGT) L.w[i] = 0ull;
         }
+        if constexpr (OWN) {
+            if (threadIdx.x == 0) { L.pnext[0] = 0; L.pnext[1] = 0; }
+        }
         // fixed-point unit of the item: 2^-VBITS of its largest spectrum
         // magnitude (rounded up to a power of two), so every term is below
         // 2^VBITS units and a voxel's sum over the item below 2^47
@@ -1180,10 +1189,10 @@ void insert_brick_kernel(const BrickJob* __restrict__ jobs, int njobs, int* __re
                               1.52 * 1.52 + (BY + 0.02) * (BY + 0.02) + (BZ + 0.02) * (BZ + 0.02) <= 900.0,
                           "owner-computes region too large for 32 lines per projection");
             if (x0 == 0) {
-                own_pass<BX, BY, BZ, false, true, P, NT, WGT>(L, J, lst, ns, wave, nwave, x0, y0, z0, oc, tab, fltb, vscale, wsc);
-                own_pass<BX, BY, BZ, true, false, P, NT, WGT>(L, J, lst, ns, wave, nwave, x0, y0, z0, om, tab, fltb, vscale, wsc);
+                own_pass<BX, BY, BZ, false, true, P, NT, WGT>(L, J, lst, ns, x0, y0, z0, oc, tab, fltb, vscale, wsc);
+                own_pass<BX, BY, BZ, true, false, P, NT, WGT>(L, J, lst, ns, x0, y0, z0, om, tab, fltb, vscale, wsc);
             } else {
-                own_pass<BX, BY, BZ, false, false, P, NT, WGT>(L, J, lst, ns, wave, nwave, x0, y0, z0, oc, tab, fltb, vscale, wsc);
+                own_pass<BX, BY, BZ, false, false, P, NT, WGT>(L, J, lst, ns, x0, y0, z0, oc, tab, fltb, vscale, wsc);
             }
         }
         if constexpr (!OWN) {
