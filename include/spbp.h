@@ -171,8 +171,10 @@ int spbp_insert_dm(spbp_ctx* ctx, const void* imgs, int dtype, int nimg,
  * (at most the context's batch) into two device buffers by turns, on a
  * stream of the library's own, so that each upload overlaps the insertion
  * of the sub-chunk before; returns when every image is inserted.  Page-locked
- * images upload by asynchronous DMA, pageable ones through the runtime's
- * staging. */
+ * images upload by asynchronous DMA; pageable ones are first copied by host
+ * threads into two page-locked staging buffers the context keeps (2 x the
+ * sub-chunk size), the copy of one sub-chunk overlapping the upload and
+ * insertion of the one before. */
 int spbp_insert_host(spbp_ctx* ctx, const void* imgs_host, int dtype, int nimg,
                      const double* angles_deg, const int8_t* half, void* stream);
 

@@ -149,9 +149,10 @@ struct spbp_ctx {
     FinalJob* fjobs_h = nullptr;
     int fjobs_cap = 0;
     hipEvent_t fjobs_evt = nullptr;
-    // host staging for _host entry points
-    void* stage = nullptr;
-    size_t stage_cap = 0;
+    // page-locked staging of spbp_insert_host's pageable images: two buffers
+    // of hstage_cap bytes, filled by host threads while the other uploads
+    void* hstage[2] = {nullptr, nullptr};
+    size_t hstage_cap = 0;
     // brick strategy
     int strategy = SPBP_STRAT_BRICK;
     int shape = 3;   // index into SHAPES
@@ -1677,7 +1678,8 @@ int spbp_destroy(spbp_ctx* c) {
     if (c->fprm_d) (void)hipFree(c->fprm_d);
     rtsf_release(c->rtsf);
     if (c->rprm_evt) (void)hipEventDestroy(c->rprm_evt);
-    if (c->stage) (void)hipHostFree(c->stage);
+    for (void* h : c->hstage)
+        if (h) (void)hipHostFree(h);
     if (c->fwd) hipfftDestroy(c->fwd);
     if (c->fwd_tail) hipfftDestroy(c->fwd_tail);
     if (c->inv) hipfftDestroy(c->inv);
@@ -2339,11 +2341,42 @@ int spbp_rtsf_host(const float* imgs_host, int nimg, int nx, const float* params
 namespace {
 // Host images staged chunk by chunk through one device buffer, each chunk
 // inserted as spbp_insert (or spbp_insert_rb with inplane) would.
+// Is p in page-locked host memory?  (A failed query is pageable memory; its
+// error is cleared.)
+bool host_pinned(const void* p) {
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeHost;
+}
+
+// dst[0 .. n) = src[0 .. n) on up to 8 host threads (contiguous slices).
+void parallel_copy(void* dst, const void* src, size_t n) {
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int nt = (int)std::min<size_t>(std::min(hw, 8u), std::max<size_t>(1, n >> 22));   // >= 4 MiB each
+    auto work = [&](int t) {
+        const size_t a = n * t / nt, b = n * (t + 1) / nt;
+        std::memcpy((char*)dst + a, (const char*)src + a, b - a);
+    };
+    std::vector<std::thread> th;
+    th.reserve(nt - 1);
+    for (int t = 1; t < nt; ++t) th.emplace_back(work, t);
+    work(0);
+    for (auto& x : th) x.join();
+}
+
 // Host images: uploaded in sub-chunks of about 256 MiB (at most c->batch
 // images) into two device buffers by turns, on a stream of their own, so
 // that the upload of one sub-chunk overlaps the insertion of the one before
 // (the insertion of sub-chunk k waits for its upload; the upload of k + 2
-// waits for that insertion to be done with the buffer).
+// waits for that insertion to be done with the buffer).  Pageable images
+// are first copied by host threads into one of two page-locked staging
+// buffers (kept by the context), so that the upload is an asynchronous DMA
+// and the copy of sub-chunk k + 1 overlaps the upload and insertion of k;
+// when page-locked memory cannot be had, the runtime's pageable copies
+// serve instead.
 int insert_host_impl(spbp_ctx* c, const void* imgs_host, int dtype, int nimg, const double* angles_deg,
                      const float* inplane, const int8_t* half, hipStream_t st) {
     if (!c) return fail(nullptr, SPBP_ERR_ARG, "NULL context");
@@ -2355,6 +2388,23 @@ int insert_host_impl(spbp_ctx* c, const void* imgs_host, int dtype, int nimg, co
     const int sub = (int)std::max<size_t>(1, std::min<size_t>({(size_t)nimg, (size_t)std::max(1, c->batch),
                                                                 ((size_t)256 << 20) / per}));
     const int nbuf = nimg > sub ? 2 : 1;
+    bool staged = !host_pinned(imgs_host);
+    if (staged && c->hstage_cap < per * sub) {   // grow the page-locked staging (kept for later calls)
+        for (void*& h : c->hstage)
+            if (h) { (void)hipHostFree(h); h = nullptr; }
+        c->hstage_cap = 0;
+        bool ok = true;
+        for (void*& h : c->hstage)
+            if (ok && hipHostMalloc(&h, per * sub, hipHostMallocDefault) != hipSuccess) { h = nullptr; ok = false; }
+        if (ok) {
+            c->hstage_cap = per * sub;
+        } else {   // no page-locked memory: the runtime's pageable copies instead
+            (void)hipGetLastError();
+            for (void*& h : c->hstage)
+                if (h) { (void)hipHostFree(h); h = nullptr; }
+            staged = false;
+        }
+    }
     void* dimg[2] = {nullptr, nullptr};
     hipStream_t cs = nullptr;
     hipEvent_t up[2] = {nullptr, nullptr}, used[2] = {nullptr, nullptr};
@@ -2380,8 +2430,18 @@ int insert_host_impl(spbp_ctx* c, const void* imgs_host, int dtype, int nimg, co
     }
     for (int p0 = 0, k = 0; p0 < nimg && rc == SPBP_OK; p0 += sub, ++k) {
         const int b = std::min(sub, nimg - p0), s = k % nbuf;
-        hipError_t e = k >= nbuf ? hipStreamWaitEvent(cs, used[s], 0) : hipSuccess;
-        if (e == hipSuccess) e = hipMemcpyAsync(dimg[s], (const char*)imgs_host + per * p0, per * b, hipMemcpyHostToDevice, cs);
+        const char* src = (const char*)imgs_host + per * p0;
+        hipError_t e = hipSuccess;
+        if (staged) {
+            // the staging buffer is free once its previous upload (sub-chunk k - 2) is done
+            if (k >= 2) e = hipEventSynchronize(up[s]);
+            if (e == hipSuccess) {
+                parallel_copy(c->hstage[s], src, per * b);
+                src = (const char*)c->hstage[s];
+            }
+        }
+        if (e == hipSuccess && k >= nbuf) e = hipStreamWaitEvent(cs, used[s], 0);
+        if (e == hipSuccess) e = hipMemcpyAsync(dimg[s], src, per * b, hipMemcpyHostToDevice, cs);
         if (e == hipSuccess) e = hipEventRecord(up[s], cs);
         if (e == hipSuccess) e = hipStreamWaitEvent(st, up[s], 0);
         if (e != hipSuccess) { rc = fail(c, SPBP_ERR_DEVICE, "image upload: %s", hipGetErrorString(e)); break; }

@@ -1,14 +1,17 @@
 """GPU checks of insertion from host memory (spbp_insert_host,
 spbp_insert_rb_host): the images are uploaded in sub-chunks on a stream of
-their own, into two device buffers by turns, overlapping the insertion.
+their own, into two device buffers by turns, overlapping the insertion;
+pageable images go through the context's two page-locked staging buffers,
+page-locked ones straight to the device.
 
 Bars: with the gather plan, whose sums are the serial reference's whatever
 the chunking, the host path gives the device path's accumulators bit for bit
-(fp32 and BF16 images, one and two halves, RB in-plane transforms, several
-sub-chunks so that both buffers are reused); with the default plan the
-finalised volumes agree to rel-L2 1e-6 and meet the FP32-faithful oracle; a
-failed upload-buffer allocation returns SPBP_ERR_ALLOC and leaves the context
-usable.
+(fp32 and BF16 images, pageable and page-locked, one and two halves, RB
+in-plane transforms, several sub-chunks so that both buffers are reused, and
+a second call on the same context reusing the staging); with the default
+plan the finalised volumes agree to rel-L2 1e-6 and meet the FP32-faithful
+oracle; a failed upload-buffer allocation returns SPBP_ERR_ALLOC and leaves
+the context usable.
 """
 import numpy as np
 import pytest
@@ -52,21 +55,27 @@ def raw(b, halves):
 
 @pytest.mark.parametrize("bf16", [False, True])
 @pytest.mark.parametrize("halves", [False, True])
-def test_host_images_are_the_device_path_bit_for_bit(bpmod, bf16, halves):
+@pytest.mark.parametrize("pinned", [False, True])
+def test_host_images_are_the_device_path_bit_for_bit(bpmod, bf16, halves, pinned):
     nx, m = 24, 70
     imgs, ang, _ = dataset(nx, m, 11 + 2 * bf16 + halves)
     half = bpmod.random_halves(m, seed=3) if halves else None
     host = torch.from_numpy(imgs)
     if bf16:
         host = host.to(torch.bfloat16)
+    if pinned:
+        host = host.pin_memory()
     out = {}
     for where in ("device", "host"):
-        # batch 16: five sub-chunks, both upload buffers used twice
+        # batch 16: five sub-chunks, both upload buffers used twice; the host
+        # path in two calls, the second reusing the context's staging
         with bpmod.Backprojector(nx, strategy="gather", two_halves=halves, batch=16) as b:
             if where == "device":
                 b.insert(host.cuda(), ang, half)
             else:
-                b.insert_host(host, ang, half)
+                k = 37
+                b.insert_host(host[:k], ang[:k], None if half is None else half[:k])
+                b.insert_host(host[k:], ang[k:], None if half is None else half[k:])
             out[where] = raw(b, halves)
     for (Xd, Wd), (Xh, Wh) in zip(out["device"], out["host"]):
         np.testing.assert_array_equal(Wh, Wd)
