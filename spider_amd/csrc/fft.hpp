@@ -180,6 +180,14 @@ __global__ __launch_bounds__(TPB) void pad_mean_kernel(const T* __restrict__ img
     if (threadIdx.x == 0) { mean[b] = s / (double)cnt; vmax[b] = 0.0f; }
 }
 
+// Four consecutive pixels (an aligned 16- or 8-byte load) as fp32.
+__device__ inline float4 load_img4(const float* p) { return *reinterpret_cast<const float4*>(p); }
+__device__ inline float4 load_img4(const uint16_t* p) {
+    const uint2 w = *reinterpret_cast<const uint2*>(p);   // pixel 0 in the low half of w.x
+    return make_float4(__uint_as_float(w.x << 16), __uint_as_float(w.x & 0xFFFF0000u), __uint_as_float(w.y << 16),
+                       __uint_as_float(w.y & 0xFFFF0000u));
+}
+
 // Rows pass.  Block per (image, group of FFT_ROW_ELEMS / N row pairs); the
 // spectrum's rows IP .. IP + NX - 1 receive the row transforms of the padded
 // image (PADD2: image minus the FP64 mean, bp3f.f:1316-1345).
@@ -198,22 +206,58 @@ __global__ __launch_bounds__(TPB) void row_fft_kernel(const T* __restrict__ imgs
     const int pairs = nx / 2, M = min(FFT_ROW_ELEMS / N, pairs);
     const int p0 = blockIdx.x * M, mc = min(M, pairs - p0);
     const int total = mc * N;
-    for (int i0 = threadIdx.x; i0 < total; i0 += FFT_BATCH * TPB) {
-        float2 v[FFT_BATCH];
+    if ((nx & 3) == 0 && ((uintptr_t)img & (4 * sizeof(T) - 1)) == 0) {
+        // four pixels of each of the pair's rows per load (N = 2 NX: the
+        // padding is NX columns per sequence, zeroed separately)
+        const int lq = L - 3, nq = 1 << lq;   // quads per row: NX / 4
+        const int totq = mc * nq;
+        for (int i = threadIdx.x; i < mc * nx; i += TPB) {
+            const int m = i >> (L - 1), n = i & (nx - 1);
+            buf[m * N + (n < ip ? n : n + nx)] = make_float2(0.f, 0.f);
+        }
+        constexpr int BQ = FFT_BATCH / 2;
+        for (int q0 = threadIdx.x; q0 < totq; q0 += BQ * TPB) {
+            float4 a[BQ], c[BQ];
 #pragma unroll
-        for (int u = 0; u < FFT_BATCH; ++u) {
-            const int idx = i0 + u * TPB;
-            const int m = idx >> L, x = (idx & (N - 1)) - ip;
-            const int r = 2 * (p0 + m);
-            v[u] = make_float2(0.f, 0.f);
-            if (idx < total && (unsigned)x < (unsigned)nx) {
-                v[u].x = (float)((double)load_img(img, (size_t)r * nx + x) - qs);
-                v[u].y = (float)((double)load_img(img, (size_t)(r + 1) * nx + x) - qs);
+            for (int u = 0; u < BQ; ++u) {
+                const int q = q0 + u * TPB;
+                const int m = q >> lq, xq = q & (nq - 1);
+                const int r = 2 * (p0 + m);
+                if (q < totq) {
+                    a[u] = load_img4(img + (size_t)r * nx + 4 * xq);
+                    c[u] = load_img4(img + (size_t)(r + 1) * nx + 4 * xq);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < BQ; ++u) {
+                const int q = q0 + u * TPB;
+                if (q >= totq) continue;
+                const int m = q >> lq, xq = q & (nq - 1);
+                float2* o = buf + m * N + ip + 4 * xq;
+                o[0] = make_float2((float)((double)a[u].x - qs), (float)((double)c[u].x - qs));
+                o[1] = make_float2((float)((double)a[u].y - qs), (float)((double)c[u].y - qs));
+                o[2] = make_float2((float)((double)a[u].z - qs), (float)((double)c[u].z - qs));
+                o[3] = make_float2((float)((double)a[u].w - qs), (float)((double)c[u].w - qs));
             }
         }
+    } else {
+        for (int i0 = threadIdx.x; i0 < total; i0 += FFT_BATCH * TPB) {
+            float2 v[FFT_BATCH];
 #pragma unroll
-        for (int u = 0; u < FFT_BATCH; ++u)
-            if (i0 + u * TPB < total) buf[i0 + u * TPB] = v[u];
+            for (int u = 0; u < FFT_BATCH; ++u) {
+                const int idx = i0 + u * TPB;
+                const int m = idx >> L, x = (idx & (N - 1)) - ip;
+                const int r = 2 * (p0 + m);
+                v[u] = make_float2(0.f, 0.f);
+                if (idx < total && (unsigned)x < (unsigned)nx) {
+                    v[u].x = (float)((double)load_img(img, (size_t)r * nx + x) - qs);
+                    v[u].y = (float)((double)load_img(img, (size_t)(r + 1) * nx + x) - qs);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < FFT_BATCH; ++u)
+                if (i0 + u * TPB < total) buf[i0 + u * TPB] = v[u];
+        }
     }
     __syncthreads();
     lds_fft<FFT_ROW_ELEMS>(buf, mc, N, L, N, stw);
