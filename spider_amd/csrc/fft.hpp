@@ -36,6 +36,10 @@ constexpr int FFT_BATCH = 8;                // global loads in flight per thread
 // (3.30) with 4096.  The columns pass keeps 4096: its row segments are then
 // 16 (8) columns wide, and 2048 made it 2.45 → 3.38 ms (5.57 → 7.91).
 constexpr int FFT_ROW_ELEMS = SPBP_FFT_ROW_ELEMS;
+// Threads of a columns-pass block: 512 (eight waves over the same LDS slab)
+// hide its loads better than 256 (2.38 -> 2.31 ms per 20,000 images of 128^2,
+// 5.43 -> 5.14 ms per 8,000 of 256^2).
+constexpr int FFT_COL_NT = 512;
 
 // Dynamic LDS of the transform kernels (E values per block): the data, then
 // the N twiddles.
@@ -52,21 +56,22 @@ __device__ inline float2 c_add(float2 a, float2 b) { return make_float2(__fadd_r
 __device__ inline float2 c_sub(float2 a, float2 b) { return make_float2(__fsub_rn(a.x, b.x), __fsub_rn(a.y, b.y)); }
 
 // Forward DFTs of M sequences of length N = 2^L in LDS (sequence m at
-// buf + m S), in place: every pass reads all its inputs into registers,
-// synchronises, then writes.  M N <= E.  Ends synchronised.  The per-thread
-// loops have fixed trip counts so that their arrays stay in registers.
-template <int E>
+// buf + m S), in place, by a block of NT threads: every pass reads all its
+// inputs into registers, synchronises, then writes.  M N <= E.  Ends
+// synchronised.  The per-thread loops have fixed trip counts so that their
+// arrays stay in registers.
+template <int E, int NT = TPB>
 __device__ inline void lds_fft(float2* buf, int M, int N, int L, int S, const float2* __restrict__ tw) {
-    static_assert(E >= 4 * TPB, "a radix-4 pass gives every thread at least one butterfly slot");
+    static_assert(E >= 4 * NT, "a radix-4 pass gives every thread at least one butterfly slot");
     int Ns = 1;
     if (L & 1) {   // radix-2 pass
-        constexpr int U = E / 2 / TPB;
+        constexpr int U = E / 2 / NT;
         const int half = N >> 1, nb = M * half;
         float2 a[U], b[U];
         int dst[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int t = threadIdx.x + u * TPB;
+            const int t = threadIdx.x + u * NT;
             dst[u] = -1;
             if (t < nb) {
                 const int m = t >> (L - 1), j = t & (half - 1);
@@ -85,13 +90,13 @@ __device__ inline void lds_fft(float2* buf, int M, int N, int L, int S, const fl
         Ns = 2;
     }
     for (; Ns < N; Ns <<= 2) {   // radix-4 passes
-        constexpr int U = E / 4 / TPB;
+        constexpr int U = E / 4 / NT;
         const int q = N >> 2, nb = M * q, step = N / (4 * Ns);
         float2 x0[U], x1[U], x2[U], x3[U];
         int dst[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int t = threadIdx.x + u * TPB;
+            const int t = threadIdx.x + u * NT;
             dst[u] = -1;
             if (t < nb) {
                 const int m = t >> (L - 2), j = t & (q - 1);
@@ -279,12 +284,12 @@ __global__ __launch_bounds__(TPB) void row_fft_kernel(const T* __restrict__ imgs
 // Columns pass.  Block per (image, slab of FFT_ELEMS / N columns): the
 // columns' transforms over all N rows, from rows IP .. IP + NX - 1; the
 // image's largest |Re|, |Im| into vmax (zeroed by pad_mean_kernel).
-__global__ __launch_bounds__(TPB) void col_fft_kernel(int nx, int N, int L, const float2* __restrict__ tw,
+__global__ __launch_bounds__(FFT_COL_NT) void col_fft_kernel(int nx, int N, int L, const float2* __restrict__ tw,
                                                       float2* __restrict__ spec, float* __restrict__ vmax) {
     extern __shared__ float2 lds[];
     float2* buf = lds;
     float2* stw = lds + FFT_LDS;
-    __shared__ float wmax[TPB / 64];
+    __shared__ float wmax[FFT_COL_NT / 64];
     load_twiddles(stw, tw, N);
     const int b = blockIdx.y;
     const int H = nx + 1, ip = (N - nx) / 2 + (nx % 2);
@@ -297,17 +302,17 @@ __global__ __launch_bounds__(TPB) void col_fft_kernel(int nx, int N, int L, cons
     while ((1 << lc) < cw) ++lc;
     const int cp = 1 << lc;
     const int total = cp * nx;   // only rows IP .. IP + NX - 1 are nonzero
-    for (int i0 = threadIdx.x; i0 < total; i0 += FFT_BATCH * TPB) {
+    for (int i0 = threadIdx.x; i0 < total; i0 += FFT_BATCH * FFT_COL_NT) {
         float2 v[FFT_BATCH];
 #pragma unroll
         for (int u = 0; u < FFT_BATCH; ++u) {
-            const int idx = i0 + u * TPB;
+            const int idx = i0 + u * FFT_COL_NT;
             const int n = idx >> lc, m = idx & (cp - 1);
             v[u] = (idx < total && m < cw) ? sp[(size_t)(ip + n) * H + m] : make_float2(0.f, 0.f);
         }
 #pragma unroll
         for (int u = 0; u < FFT_BATCH; ++u) {
-            const int idx = i0 + u * TPB;
+            const int idx = i0 + u * FFT_COL_NT;
             const int n = idx >> lc, m = idx & (cp - 1);
             if (idx < total && m < cw) buf[m * S + ip + n] = v[u];
         }
@@ -317,7 +322,7 @@ __global__ __launch_bounds__(TPB) void col_fft_kernel(int nx, int N, int L, cons
         if (m < cw) buf[m * S + (n < ip ? n : n + nx)] = make_float2(0.f, 0.f);
     }
     __syncthreads();
-    lds_fft<FFT_ELEMS>(buf, cw, N, L, S, stw);
+    lds_fft<FFT_ELEMS, FFT_COL_NT>(buf, cw, N, L, S, stw);
     float mx = 0.0f;
     for (int idx = threadIdx.x; idx < cp * N; idx += blockDim.x) {
         const int n = idx >> lc, m = idx & (cp - 1);
@@ -330,7 +335,7 @@ __global__ __launch_bounds__(TPB) void col_fft_kernel(int nx, int N, int L, cons
     if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mx;
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int k = 1; k < TPB / 64; ++k) mx = fmaxf(mx, wmax[k]);
+        for (int k = 1; k < FFT_COL_NT / 64; ++k) mx = fmaxf(mx, wmax[k]);
         atomicMax((unsigned*)&vmax[b], __float_as_uint(mx));   // non-negative floats order as their bits
     }
 }

@@ -789,7 +789,7 @@ int own_fft(spbp_ctx* c, const void* src, int dtype, int b, hipStream_t st) {
         }
         {
             Timed tm(c, SPBP_K_FFT2, n, st);
-            col_fft_kernel<<<dim3((c->H + cw - 1) / cw, n), TPB, lds, st>>>(c->nx, c->N, c->fft_log2, c->tw, sp,
+            col_fft_kernel<<<dim3((c->H + cw - 1) / cw, n), FFT_COL_NT, lds, st>>>(c->nx, c->N, c->fft_log2, c->tw, sp,
                                                                            c->vmax + s0);
         }
     }
