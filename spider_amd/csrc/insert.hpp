@@ -493,7 +493,8 @@ struct TileLds {
 #endif
 // power of two <= 64.  Measured at 128^2 into 256^3 (insertion ms per 20k
 // projections): 1 (consecutive lanes, consecutive candidates) 55.0, 2 53.2,
-// 4 53.6, 8 54.2, 16 54.3
+// 4 53.6, 8 54.2, 16 54.3; with the current kernel (compaction, tile layout,
+// counter-assigned section pairs) 1, 2 and 4 are within 0.3% (22.74-22.78)
 constexpr int LANE_SPREAD = SPBP_LANE_SPREAD;
 // Fixed-point accumulation in LDS.  A term t (an integer, |t| < 2^51) is
 // encoded as the bit pattern of the double 1.5 * 2^52 + t, which is the
