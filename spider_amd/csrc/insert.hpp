@@ -160,7 +160,8 @@ __global__ __launch_bounds__(TPB) void plane_pack_kernel(const Geom* __restrict_
 // expanded box (with the largest member margin, plus 0.05 for rounding), so
 // the first test never drops a pair the brick test keeps.  The members'
 // boxes, and the disk condition, which does not depend on the section, are
-// set up once per block.
+// set up once per block.  (Host-callable as well, so that they can be
+// checked on the CPU.)
 struct SuperBrick {
     int b[8];       // member bricks, -1 past the grid's edge or wholly outside the sample disk
     float mc[8][3]; // member expanded boxes: centres and half extents
@@ -168,7 +169,7 @@ struct SuperBrick {
     float cx, cy, cz, hx, hy, hz;
 };
 
-__device__ __forceinline__ SuperBrick superbrick(const BrickGrid& bg, int s, int nx) {
+__host__ __device__ __forceinline__ SuperBrick superbrick(const BrickGrid& bg, int s, int nx) {
     const int nsx = (bg.nbx + 1) / 2, nsy = (bg.nby + 1) / 2;
     const int si = s % nsx, sj = (s / nsx) % nsy, sk = s / (nsx * nsy);
     SuperBrick g;
@@ -208,7 +209,7 @@ __device__ __forceinline__ SuperBrick superbrick(const BrickGrid& bg, int s, int
 // Member d of superbrick s for a thread-dependent d, recomputed (indexing
 // the record's array with it would move the whole record to scratch
 // memory); -1 past the grid's edge.
-__device__ __forceinline__ int member(const BrickGrid& bg, int s, int d) {
+__host__ __device__ __forceinline__ int member(const BrickGrid& bg, int s, int d) {
     const int nsx = (bg.nbx + 1) / 2, nsy = (bg.nby + 1) / 2;
     const int i = 2 * (s % nsx) + (d & 1), j = 2 * ((s / nsx) % nsy) + ((d >> 1) & 1), k = 2 * (s / (nsx * nsy)) + (d >> 2);
     return (i < bg.nbx && j < bg.nby && k < bg.nbz) ? i + bg.nbx * (j + bg.nby * k) : -1;
@@ -216,14 +217,14 @@ __device__ __forceinline__ int member(const BrickGrid& bg, int s, int d) {
 
 // Does the plane pass through member d's expanded box (its disk condition
 // already applied)?
-__device__ __forceinline__ bool member_hit(const SuperBrick& sb, int d, const float4& g) {
+__host__ __device__ __forceinline__ bool member_hit(const SuperBrick& sb, int d, const float4& g) {
     const float dd = fabsf(g.x * sb.mc[d][0] + g.y * sb.mc[d][1] + g.z * sb.mc[d][2]);
     const float reach = sb.mh[d][0] * fabsf(g.x) + sb.mh[d][1] * fabsf(g.y) + sb.mh[d][2] * fabsf(g.z) + 0.02f;
     return dd <= reach;
 }
 
 // The group test: both conditions on the group's box.
-__device__ __forceinline__ bool superbrick_hit(const SuperBrick& sb, int nx, const float4& g) {
+__host__ __device__ __forceinline__ bool superbrick_hit(const SuperBrick& sb, int nx, const float4& g) {
     const float d = fabsf(g.x * sb.cx + g.y * sb.cy + g.z * sb.cz);
     const float reach = sb.hx * fabsf(g.x) + sb.hy * fabsf(g.y) + sb.hz * fabsf(g.z) + 0.05f;
     if (d > reach) return false;
