@@ -174,7 +174,8 @@ int spbp_insert_dm(spbp_ctx* ctx, const void* imgs, int dtype, int nimg,
  * images upload by asynchronous DMA; pageable ones are first copied by host
  * threads into two page-locked staging buffers the context keeps (2 x the
  * sub-chunk size), the copy of one sub-chunk overlapping the upload and
- * insertion of the one before. */
+ * insertion of the one before.  Device or managed memory is copied as it
+ * is. */
 int spbp_insert_host(spbp_ctx* ctx, const void* imgs_host, int dtype, int nimg,
                      const double* angles_deg, const int8_t* half, void* stream);
 

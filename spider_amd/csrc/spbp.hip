@@ -2342,15 +2342,16 @@ int spbp_rtsf_host(const float* imgs_host, int nimg, int nx, const float* params
 namespace {
 // Host images staged chunk by chunk through one device buffer, each chunk
 // inserted as spbp_insert (or spbp_insert_rb with inplane) would.
-// Is p in page-locked host memory?  (A failed query is pageable memory; its
-// error is cleared.)
-bool host_pinned(const void* p) {
+// Is p in pageable host memory, unknown to the runtime?  Page-locked host,
+// device and managed memory are copied as they are.  (A failed query is
+// pageable memory; its error is cleared.)
+bool host_pageable(const void* p) {
     hipPointerAttribute_t a;
     if (hipPointerGetAttributes(&a, p) != hipSuccess) {
         (void)hipGetLastError();
-        return false;
+        return true;
     }
-    return a.type == hipMemoryTypeHost;
+    return a.type == hipMemoryTypeUnregistered;
 }
 
 // dst[0 .. n) = src[0 .. n) on up to 8 host threads (contiguous slices).
@@ -2377,7 +2378,8 @@ void parallel_copy(void* dst, const void* src, size_t n) {
 // buffers (kept by the context), so that the upload is an asynchronous DMA
 // and the copy of sub-chunk k + 1 overlaps the upload and insertion of k;
 // when page-locked memory cannot be had, the runtime's pageable copies
-// serve instead.
+// serve instead.  Page-locked, device and managed images are copied as they
+// are (the runtime tells the direction).
 int insert_host_impl(spbp_ctx* c, const void* imgs_host, int dtype, int nimg, const double* angles_deg,
                      const float* inplane, const int8_t* half, hipStream_t st) {
     if (!c) return fail(nullptr, SPBP_ERR_ARG, "NULL context");
@@ -2389,7 +2391,8 @@ int insert_host_impl(spbp_ctx* c, const void* imgs_host, int dtype, int nimg, co
     const int sub = (int)std::max<size_t>(1, std::min<size_t>({(size_t)nimg, (size_t)std::max(1, c->batch),
                                                                 ((size_t)256 << 20) / per}));
     const int nbuf = nimg > sub ? 2 : 1;
-    bool staged = !host_pinned(imgs_host);
+    const bool pageable = host_pageable(imgs_host);
+    bool staged = pageable;
     if (staged && c->hstage_cap < per * sub) {   // grow the page-locked staging (kept for later calls)
         for (void*& h : c->hstage)
             if (h) { (void)hipHostFree(h); h = nullptr; }
@@ -2442,7 +2445,8 @@ int insert_host_impl(spbp_ctx* c, const void* imgs_host, int dtype, int nimg, co
             }
         }
         if (e == hipSuccess && k >= nbuf) e = hipStreamWaitEvent(cs, used[s], 0);
-        if (e == hipSuccess) e = hipMemcpyAsync(dimg[s], src, per * b, hipMemcpyHostToDevice, cs);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(dimg[s], src, per * b, pageable ? hipMemcpyHostToDevice : hipMemcpyDefault, cs);
         if (e == hipSuccess) e = hipEventRecord(up[s], cs);
         if (e == hipSuccess) e = hipStreamWaitEvent(st, up[s], 0);
         if (e != hipSuccess) { rc = fail(c, SPBP_ERR_DEVICE, "image upload: %s", hipGetErrorString(e)); break; }

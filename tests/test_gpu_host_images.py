@@ -8,7 +8,8 @@ Bars: with the gather plan, whose sums are the serial reference's whatever
 the chunking, the host path gives the device path's accumulators bit for bit
 (fp32 and BF16 images, pageable and page-locked, one and two halves, RB
 in-plane transforms, several sub-chunks so that both buffers are reused, and
-a second call on the same context reusing the staging); with the default
+a second call on the same context reusing the staging; device-resident
+images handed to the host entry point are copied as they are); with the default
 plan the finalised volumes agree to rel-L2 1e-6 and meet the FP32-faithful
 oracle; a failed upload-buffer allocation returns SPBP_ERR_ALLOC and leaves
 the context usable.
@@ -80,6 +81,29 @@ def test_host_images_are_the_device_path_bit_for_bit(bpmod, bf16, halves, pinned
     for (Xd, Wd), (Xh, Wh) in zip(out["device"], out["host"]):
         np.testing.assert_array_equal(Wh, Wd)
         np.testing.assert_array_equal(Xh.view(np.uint64), Xd.view(np.uint64))
+
+
+def test_insert_host_takes_device_memory_too(bpmod):
+    """Device-resident images handed to spbp_insert_host are copied as they
+    are (not staged through host memory): the same accumulators."""
+    from spider_amd import _lib
+    import ctypes as C
+    nx, m = 20, 40
+    imgs, ang, _ = dataset(nx, m, 61)
+    dev = torch.from_numpy(imgs).cuda()
+    out = {}
+    for where in ("insert", "insert_host"):
+        with bpmod.Backprojector(nx, strategy="gather", batch=16) as b:
+            if where == "insert":
+                b.insert(dev, ang)
+            else:
+                a = np.ascontiguousarray(ang, dtype=np.float64)
+                rc = _lib.lib().spbp_insert_host(b._ctx, C.c_void_p(dev.data_ptr()), _lib.F32, m,
+                                                 a.ctypes.data_as(C.c_void_p), None, bpmod._stream(b.device))
+                assert rc == 0, rc
+            out[where] = raw(b, False)
+    np.testing.assert_array_equal(out["insert_host"][0][1], out["insert"][0][1])
+    np.testing.assert_array_equal(out["insert_host"][0][0].view(np.uint64), out["insert"][0][0].view(np.uint64))
 
 
 def test_host_rb_images_are_the_device_path_bit_for_bit(bpmod):
